@@ -1,0 +1,625 @@
+// C ABI and host glue: context lifetime, the sweep driver, parallel
+// tempering cadence, cone adaptation, observables and the energy series.
+// Host C++; every HIP status is checked and turned into a PYRO_E* code.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+#include "pyro_internal.h"
+
+namespace pyro {
+
+static thread_local std::string g_error;
+
+int set_error(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return code;
+}
+
+static int device_guard(pyro_ctx *ctx) {
+  PYRO_HIP(hipSetDevice(ctx->device));
+  return 0;
+}
+
+static void drop_graph(pyro_ctx *ctx) {
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  ctx->graph_exec = nullptr;
+  ctx->graph = nullptr;
+  ctx->graph_sweeps = 0;
+}
+
+// Recompute the full energy and make it the anchor of the incremental one.
+static int reanchor(pyro_ctx *ctx) {
+  PYRO_TRY(launch_observables(ctx));
+  PYRO_TRY(launch_set_anchor(ctx));
+  return 0;
+}
+
+// One sweep as kernel launches (also the body that gets captured).
+static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
+  for (int p = 0; p < ctx->n_metro; ++p)
+    for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_metropolis_phase(ctx, c, p, nullptr));
+  for (int o = 0; o < ctx->n_or; ++o)
+    for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_overrelax_phase(ctx, c));
+  if (ctx->pt_every > 0 && (sweep_index + 1u) % (uint32_t)ctx->pt_every == 0u) {
+    PYRO_TRY(launch_observables(ctx));
+    PYRO_TRY(launch_pt_swap(ctx, 0, 0, 1));
+  }
+  PYRO_TRY(launch_end_sweep(ctx));
+  return 0;
+}
+
+static void account_sweeps(pyro_ctx *ctx, uint32_t n) {
+  const uint64_t per = (uint64_t)ctx->n_metro * 16ull * (uint64_t)ctx->ncell;
+  for (auto &p : ctx->h_proposed) p += per * n;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (ctx->pt_every > 0 && (ctx->h_sweep + 1u) % (uint32_t)ctx->pt_every == 0u) ++ctx->h_pt_round;
+    ++ctx->h_sweep;
+  }
+}
+
+}  // namespace pyro
+
+using namespace pyro;
+
+extern "C" {
+
+const char *pyro_version(void) { return "pyro-mi355x 0.1 (gfx950)"; }
+
+const char *pyro_last_error(void) { return g_error.c_str(); }
+
+int pyro_links16(int8_t out[16][6][4]) {
+  if (!out) return set_error(PYRO_EINVAL, "pyro_links16: null output");
+  static const Link t[16][6] = PYRO_LINKS16_INIT;
+  for (int s = 0; s < 16; ++s)
+    for (int n = 0; n < 6; ++n) {
+      out[s][n][0] = t[s][n].dx;
+      out[s][n][1] = t[s][n].dy;
+      out[s][n][2] = t[s][n].dz;
+      out[s][n][3] = t[s][n].site;
+    }
+  return 0;
+}
+
+int pyro_independent_sets(int8_t out[24][4]) {
+  if (!out) return set_error(PYRO_EINVAL, "pyro_independent_sets: null output");
+  static const int8_t t[24][4] = PYRO_INDEPENDENT_SETS_INIT;
+  std::memcpy(out, t, sizeof t);
+  return 0;
+}
+
+int pyro_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  if (!ctr || !key || !out) return set_error(PYRO_EINVAL, "pyro_philox4x32_10: null argument");
+  u32x4 c = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  u32x4 o = philox4x32_10(c, key[0], key[1]);
+  out[0] = o.x;
+  out[1] = o.y;
+  out[2] = o.z;
+  out[3] = o.w;
+  return 0;
+}
+
+int pyro_create(const pyro_config *cfg, pyro_ctx **out) {
+  if (!cfg || !out) return set_error(PYRO_EINVAL, "pyro_create: null argument");
+  *out = nullptr;
+  if (cfg->L < 2 || cfg->L > 256)
+    return set_error(PYRO_EINVAL, "pyro_create: L=%d outside [2, 256]", cfg->L);
+  if (cfg->n_replicas < 1 || cfg->n_replicas > 65535)
+    return set_error(PYRO_EINVAL, "pyro_create: n_replicas=%d outside [1, 65535]", cfg->n_replicas);
+  if (!cfg->temperatures) return set_error(PYRO_EINVAL, "pyro_create: null temperatures");
+  for (int r = 0; r < cfg->n_replicas; ++r)
+    if (!(cfg->temperatures[r] > 0.0) || !std::isfinite(cfg->temperatures[r]))
+      return set_error(PYRO_EINVAL, "pyro_create: temperature[%d]=%g not positive", r,
+                       cfg->temperatures[r]);
+  if (cfg->n_metro < 0 || cfg->n_or < 0 || cfg->pt_every < 0)
+    return set_error(PYRO_EINVAL, "pyro_create: negative pass counts");
+  if (!(cfg->cos_cone >= -1.0f && cfg->cos_cone < 1.0f))
+    return set_error(PYRO_EINVAL, "pyro_create: cos_cone=%g outside [-1, 1)", cfg->cos_cone);
+  const size_t ncell = (size_t)cfg->L * cfg->L * cfg->L;
+  const size_t nfloat = (size_t)cfg->n_replicas * 48 * ncell;
+  if (ncell * 16 > (size_t)INT32_MAX)
+    return set_error(PYRO_EINVAL, "pyro_create: L=%d too large for 32-bit site indices", cfg->L);
+
+  int ndev = 0;
+  PYRO_HIP(hipGetDeviceCount(&ndev));
+  if (cfg->device < 0 || cfg->device >= ndev)
+    return set_error(PYRO_EINVAL, "pyro_create: device %d of %d", cfg->device, ndev);
+  PYRO_HIP(hipSetDevice(cfg->device));
+
+  pyro_ctx *ctx = new pyro_ctx();
+  ctx->L = cfg->L;
+  ctx->R = cfg->n_replicas;
+  ctx->ncell = (int)ncell;
+  ctx->device = cfg->device;
+  ctx->seed = cfg->seed;
+  ctx->jeff = (float)(cfg->J * cfg->spin_len2);
+  ctx->n_metro = cfg->n_metro;
+  ctx->n_or = cfg->n_or;
+  ctx->pt_every = cfg->pt_every;
+  ctx->obs_blocks = (ctx->ncell + kObsBlock - 1) / kObsBlock;
+  ctx->h_proposed.assign(ctx->R, 0);
+
+  auto fail = [&](int rc) {
+    pyro_destroy(ctx);
+    return rc;
+  };
+  const int R = ctx->R;
+#define ALLOC(ptr, count)                                                                  \
+  do {                                                                                     \
+    if (hipMalloc((void **)&(ptr), sizeof(*(ptr)) * (size_t)(count)) != hipSuccess)        \
+      return fail(set_error(PYRO_ENOMEM, "pyro_create: hipMalloc of %zu bytes for %s failed", \
+                            sizeof(*(ptr)) * (size_t)(count), #ptr));                      \
+  } while (0)
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_error(PYRO_EHIP, "pyro_create: stream creation failed"));
+  ALLOC(ctx->d_spins, nfloat);
+  ALLOC(ctx->d_temps, R);
+  ALLOC(ctx->d_beta, R);
+  ALLOC(ctx->d_cosc, R);
+  ALLOC(ctx->d_replica_at, R);
+  ALLOC(ctx->d_acc, R);
+  ALLOC(ctx->d_dE, R);
+  ALLOC(ctx->d_anchor, R);
+  ALLOC(ctx->d_partials, (size_t)R * ctx->obs_blocks * kObs);
+  ALLOC(ctx->d_obs, (size_t)R * kObs);
+  ALLOC(ctx->d_ctl, kCtlWords);
+#undef ALLOC
+
+  std::vector<uint32_t> ctl(kCtlWords, 0u);
+  std::vector<float> cosc(R, cfg->cos_cone);
+  if (hipMemcpy(ctx->d_ctl, ctl.data(), sizeof(uint32_t) * kCtlWords, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemset(ctx->d_acc, 0, sizeof(unsigned long long) * R) != hipSuccess ||
+      hipMemset(ctx->d_dE, 0, sizeof(double) * R) != hipSuccess ||
+      hipMemset(ctx->d_anchor, 0, sizeof(double) * R) != hipSuccess ||
+      hipMemcpy(ctx->d_cosc, cosc.data(), sizeof(float) * R, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(set_error(PYRO_EHIP, "pyro_create: initial upload failed"));
+  int rc = pyro_set_temperatures(ctx, cfg->temperatures);
+  if (rc) return fail(rc);
+  rc = pyro_init_random_spins(ctx);
+  if (rc) return fail(rc);
+  *out = ctx;
+  return 0;
+}
+
+int pyro_destroy(pyro_ctx *ctx) {
+  if (!ctx) return 0;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  drop_graph(ctx);
+  void *ptrs[] = {ctx->d_spins, ctx->d_temps, ctx->d_beta,     ctx->d_cosc, ctx->d_replica_at,
+                  ctx->d_acc,   ctx->d_dE,    ctx->d_anchor,   ctx->d_partials, ctx->d_obs,
+                  ctx->d_ctl,   ctx->d_series};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+int pyro_init_random_spins(pyro_ctx *ctx) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  PYRO_TRY(device_guard(ctx));
+  PYRO_TRY(launch_init_spins(ctx));
+  PYRO_TRY(reanchor(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int pyro_set_spins(pyro_ctx *ctx, const float *soa) {
+  if (!ctx || !soa) return set_error(PYRO_EINVAL, "pyro_set_spins: null argument");
+  PYRO_TRY(device_guard(ctx));
+  const size_t n = (size_t)ctx->R * 48 * ctx->ncell;
+  PYRO_HIP(hipMemcpyAsync(ctx->d_spins, soa, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  PYRO_TRY(reanchor(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int pyro_get_spins(pyro_ctx *ctx, float *soa) {
+  if (!ctx || !soa) return set_error(PYRO_EINVAL, "pyro_get_spins: null argument");
+  PYRO_TRY(device_guard(ctx));
+  const size_t n = (size_t)ctx->R * 48 * ctx->ncell;
+  PYRO_HIP(hipMemcpyAsync(soa, ctx->d_spins, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int pyro_spins_device_ptr(pyro_ctx *ctx, float **dev_ptr, size_t *n_floats) {
+  if (!ctx || !dev_ptr || !n_floats) return set_error(PYRO_EINVAL, "pyro_spins_device_ptr: null");
+  *dev_ptr = ctx->d_spins;
+  *n_floats = (size_t)ctx->R * 48 * ctx->ncell;
+  return 0;
+}
+
+int pyro_set_temperatures(pyro_ctx *ctx, const double *T) {
+  if (!ctx || !T) return set_error(PYRO_EINVAL, "pyro_set_temperatures: null argument");
+  PYRO_TRY(device_guard(ctx));
+  const int R = ctx->R;
+  std::vector<float> beta(R);
+  std::vector<int> order(R);
+  for (int r = 0; r < R; ++r) {
+    if (!(T[r] > 0.0) || !std::isfinite(T[r]))
+      return set_error(PYRO_EINVAL, "pyro_set_temperatures: T[%d]=%g not positive", r, T[r]);
+    beta[r] = (float)(1.0 / T[r]);
+  }
+  std::iota(order.begin(), order.end(), 0);
+  // Descending temperature, ties by replica index (gsl_sort_largest_index order).
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return T[a] > T[b]; });
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  PYRO_HIP(hipMemcpy(ctx->d_temps, T, sizeof(double) * R, hipMemcpyHostToDevice));
+  PYRO_HIP(hipMemcpy(ctx->d_beta, beta.data(), sizeof(float) * R, hipMemcpyHostToDevice));
+  PYRO_HIP(hipMemcpy(ctx->d_replica_at, order.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int pyro_get_temperatures(pyro_ctx *ctx, double *T, int *replica_at) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  if (T) PYRO_HIP(hipMemcpy(T, ctx->d_temps, sizeof(double) * ctx->R, hipMemcpyDeviceToHost));
+  if (replica_at)
+    PYRO_HIP(hipMemcpy(replica_at, ctx->d_replica_at, sizeof(int) * ctx->R, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pyro_set_cos_cone(pyro_ctx *ctx, const float *cos_cone) {
+  if (!ctx || !cos_cone) return set_error(PYRO_EINVAL, "pyro_set_cos_cone: null argument");
+  for (int r = 0; r < ctx->R; ++r)
+    if (!(cos_cone[r] >= -1.0f && cos_cone[r] < 1.0f))
+      return set_error(PYRO_EINVAL, "pyro_set_cos_cone: cos_cone[%d]=%g outside [-1, 1)", r,
+                       cos_cone[r]);
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  PYRO_HIP(hipMemcpy(ctx->d_cosc, cos_cone, sizeof(float) * ctx->R, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int pyro_get_cos_cone(pyro_ctx *ctx, float *cos_cone) {
+  if (!ctx || !cos_cone) return set_error(PYRO_EINVAL, "pyro_get_cos_cone: null argument");
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  PYRO_HIP(hipMemcpy(cos_cone, ctx->d_cosc, sizeof(float) * ctx->R, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pyro_metropolis_phase(pyro_ctx *ctx, int colour, uint32_t sweep, int pass) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (colour < 0 || colour > 3) return set_error(PYRO_EINVAL, "colour %d outside 0..3", colour);
+  if (pass < 0 || pass > 255) return set_error(PYRO_EINVAL, "pass %d outside 0..255", pass);
+  PYRO_TRY(device_guard(ctx));
+  // The kernel reads the sweep index from the control words.
+  PYRO_TRY(launch_set_sweep(ctx, sweep, ctx->h_pt_round));
+  PYRO_TRY(launch_metropolis_phase(ctx, colour, pass, nullptr));
+  PYRO_TRY(launch_set_sweep(ctx, ctx->h_sweep, ctx->h_pt_round));
+  for (auto &p : ctx->h_proposed) p += 4ull * (uint64_t)ctx->ncell;
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int pyro_overrelax_phase(pyro_ctx *ctx, int colour) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (colour < 0 || colour > 3) return set_error(PYRO_EINVAL, "colour %d outside 0..3", colour);
+  PYRO_TRY(device_guard(ctx));
+  PYRO_TRY(launch_overrelax_phase(ctx, colour));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int pyro_sweep(pyro_ctx *ctx, int n_sweeps, int use_graph) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (n_sweeps < 0) return set_error(PYRO_EINVAL, "n_sweeps %d < 0", n_sweeps);
+  PYRO_TRY(device_guard(ctx));
+  const uint32_t block = ctx->pt_every > 0 ? (uint32_t)ctx->pt_every : 1u;
+  uint32_t left = (uint32_t)n_sweeps;
+  if (!use_graph) {
+    while (left > 0) {
+      PYRO_TRY(enqueue_sweep(ctx, ctx->h_sweep));
+      account_sweeps(ctx, 1);
+      --left;
+    }
+  }
+  // A captured block always starts on a multiple of the exchange cadence, so
+  // the swap lands after the same sweeps as in eager mode.
+  while (left > 0 && ctx->h_sweep % block != 0u) {
+    PYRO_TRY(enqueue_sweep(ctx, ctx->h_sweep));
+    account_sweeps(ctx, 1);
+    --left;
+  }
+  if (left >= block) {
+    if (!ctx->graph_exec || ctx->graph_sweeps != (int)block) {
+      drop_graph(ctx);
+      PYRO_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      int rc = 0;
+      for (uint32_t i = 0; i < block && rc == 0; ++i) rc = enqueue_sweep(ctx, i);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return set_error(PYRO_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+      ctx->graph = g;
+      PYRO_HIP(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
+      ctx->graph_sweeps = (int)block;
+    }
+    while (left >= block) {
+      PYRO_HIP(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+      account_sweeps(ctx, block);
+      left -= block;
+    }
+  }
+  while (left > 0) {
+    PYRO_TRY(enqueue_sweep(ctx, ctx->h_sweep));
+    account_sweeps(ctx, 1);
+    --left;
+  }
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int pyro_get_sweep(pyro_ctx *ctx, uint32_t *sweep, uint32_t *pt_round) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  PYRO_TRY(device_guard(ctx));
+  uint32_t ctl[kCtlWords];
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  PYRO_HIP(hipMemcpy(ctl, ctx->d_ctl, sizeof ctl, hipMemcpyDeviceToHost));
+  if (ctl[kCtlSweep] != ctx->h_sweep || ctl[kCtlPtRound] != ctx->h_pt_round)
+    return set_error(PYRO_ESTATE, "sweep counters diverged: device %u/%u host %u/%u", ctl[kCtlSweep],
+                     ctl[kCtlPtRound], ctx->h_sweep, ctx->h_pt_round);
+  if (sweep) *sweep = ctl[kCtlSweep];
+  if (pt_round) *pt_round = ctl[kCtlPtRound];
+  return 0;
+}
+
+int pyro_set_sweep(pyro_ctx *ctx, uint32_t sweep, uint32_t pt_round) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  PYRO_TRY(device_guard(ctx));
+  PYRO_TRY(launch_set_sweep(ctx, sweep, pt_round));
+  ctx->h_sweep = sweep;
+  ctx->h_pt_round = pt_round;
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int pyro_pt_swap(pyro_ctx *ctx, int parity, uint32_t round, int *n_accepted) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (parity != 0 && parity != 1) return set_error(PYRO_EINVAL, "parity %d not 0/1", parity);
+  PYRO_TRY(device_guard(ctx));
+  uint32_t before = 0, after = 0;
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  PYRO_HIP(hipMemcpy(&before, ctx->d_ctl + kCtlSwapsAcc, sizeof before, hipMemcpyDeviceToHost));
+  PYRO_TRY(launch_observables(ctx));
+  PYRO_TRY(launch_pt_swap(ctx, parity, round, 0));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  PYRO_HIP(hipMemcpy(&after, ctx->d_ctl + kCtlSwapsAcc, sizeof after, hipMemcpyDeviceToHost));
+  if (n_accepted) *n_accepted = (int)(after - before);
+  return 0;
+}
+
+static int fetch_obs(pyro_ctx *ctx, std::vector<double> &obs) {
+  PYRO_TRY(device_guard(ctx));
+  PYRO_TRY(launch_observables(ctx));
+  obs.resize((size_t)ctx->R * kObs);
+  PYRO_HIP(hipMemcpyAsync(obs.data(), ctx->d_obs, sizeof(double) * obs.size(), hipMemcpyDeviceToHost,
+                          ctx->stream));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int pyro_energy(pyro_ctx *ctx, double *E) {
+  if (!ctx || !E) return set_error(PYRO_EINVAL, "pyro_energy: null argument");
+  std::vector<double> obs;
+  PYRO_TRY(fetch_obs(ctx, obs));
+  for (int r = 0; r < ctx->R; ++r) E[r] = obs[(size_t)r * kObs];
+  return 0;
+}
+
+int pyro_sublattice_sums(pyro_ctx *ctx, double *sub12) {
+  if (!ctx || !sub12) return set_error(PYRO_EINVAL, "pyro_sublattice_sums: null argument");
+  std::vector<double> obs;
+  PYRO_TRY(fetch_obs(ctx, obs));
+  for (int r = 0; r < ctx->R; ++r)
+    for (int q = 0; q < 12; ++q) sub12[(size_t)r * 12 + q] = obs[(size_t)r * kObs + 1 + q];
+  return 0;
+}
+
+int pyro_magnetization(pyro_ctx *ctx, double *M3) {
+  if (!ctx || !M3) return set_error(PYRO_EINVAL, "pyro_magnetization: null argument");
+  std::vector<double> obs;
+  PYRO_TRY(fetch_obs(ctx, obs));
+  for (int r = 0; r < ctx->R; ++r)
+    for (int k = 0; k < 3; ++k) {
+      const double *s = &obs[(size_t)r * kObs + 1];
+      M3[(size_t)r * 3 + k] = ((s[k] + s[3 + k]) + s[6 + k]) + s[9 + k];
+    }
+  return 0;
+}
+
+// q = 0 order parameters from the sublattice sums s_b: the twelve rows of
+// Measurement (Pyrochlore.cu:580-624) -- A2 (1), E (2), T1 (3), ferro (3),
+// T2 (3) -- divided by N so the ideal ordered state has modulus 1.
+int pyro_fold_order_params(const double *sub12, int R, double n_spins, double *op12) {
+  if (!sub12 || !op12 || R < 0 || !(n_spins > 0.0))
+    return set_error(PYRO_EINVAL, "pyro_fold_order_params: bad argument");
+  const double a = 0.5773502691896258, b = 0.816496580927726, c = 0.4082482904638631,
+               d = 0.7071067811865475;
+  for (int r = 0; r < R; ++r) {
+    const double *s = sub12 + (size_t)r * 12;
+    const double *s0 = s, *s1 = s + 3, *s2 = s + 6, *s3 = s + 9;
+    double m[12];
+    m[0] = a * (s0[0] + s0[1] + s0[2]) + a * (-s1[0] - s1[1] + s1[2]) +
+           a * (s2[0] - s2[1] - s2[2]) + a * (-s3[0] + s3[1] - s3[2]);
+    m[1] = (-b * s0[0] + c * s0[1] + c * s0[2]) + (b * s1[0] - c * s1[1] + c * s1[2]) +
+           (-b * s2[0] - c * s2[1] - c * s2[2]) + (b * s3[0] + c * s3[1] - c * s3[2]);
+    m[2] = d * (-s0[1] + s0[2] + s1[1] + s1[2] + s2[1] - s2[2] - s3[1] - s3[2]);
+    m[3] = d * (-s0[1] + s0[2] - s1[1] - s1[2] + s2[1] - s2[2] + s3[1] + s3[2]);
+    m[4] = d * (s0[0] - s0[2] + s1[0] + s1[2] - s2[0] - s2[2] - s3[0] + s3[2]);
+    m[5] = d * (-s0[0] + s0[1] + s1[0] - s1[1] + s2[0] + s2[1] - s3[0] - s3[1]);
+    m[6] = s0[0] + s1[0] + s2[0] + s3[0];
+    m[7] = s0[1] + s1[1] + s2[1] + s3[1];
+    m[8] = s0[2] + s1[2] + s2[2] + s3[2];
+    m[9] = d * (-s0[1] - s0[2] - s1[1] + s1[2] + s2[1] + s2[2] + s3[1] - s3[2]);
+    m[10] = d * (-s0[0] - s0[2] - s1[0] + s1[2] + s2[0] - s2[2] + s3[0] + s3[2]);
+    m[11] = d * (-s0[0] - s0[1] + s1[0] + s1[1] + s2[0] - s2[1] - s3[0] + s3[1]);
+    for (int q = 0; q < 12; ++q) op12[(size_t)r * 12 + q] = m[q] / n_spins;
+  }
+  return 0;
+}
+
+int pyro_order_params(pyro_ctx *ctx, double *op12) {
+  if (!ctx || !op12) return set_error(PYRO_EINVAL, "pyro_order_params: null argument");
+  std::vector<double> sub((size_t)ctx->R * 12);
+  PYRO_TRY(pyro_sublattice_sums(ctx, sub.data()));
+  return pyro_fold_order_params(sub.data(), ctx->R, 16.0 * ctx->ncell, op12);
+}
+
+int pyro_energy_incremental(pyro_ctx *ctx, double *E) {
+  if (!ctx || !E) return set_error(PYRO_EINVAL, "pyro_energy_incremental: null argument");
+  PYRO_TRY(device_guard(ctx));
+  std::vector<double> a(ctx->R), d(ctx->R);
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  PYRO_HIP(hipMemcpy(a.data(), ctx->d_anchor, sizeof(double) * ctx->R, hipMemcpyDeviceToHost));
+  PYRO_HIP(hipMemcpy(d.data(), ctx->d_dE, sizeof(double) * ctx->R, hipMemcpyDeviceToHost));
+  for (int r = 0; r < ctx->R; ++r) E[r] = a[r] + d[r];
+  return 0;
+}
+
+int pyro_acceptance(pyro_ctx *ctx, uint64_t *accepted, uint64_t *proposed) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  if (accepted)
+    PYRO_HIP(hipMemcpy(accepted, ctx->d_acc, sizeof(uint64_t) * ctx->R, hipMemcpyDeviceToHost));
+  if (proposed) std::memcpy(proposed, ctx->h_proposed.data(), sizeof(uint64_t) * ctx->R);
+  return 0;
+}
+
+int pyro_reset_acceptance(pyro_ctx *ctx) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  PYRO_HIP(hipMemset(ctx->d_acc, 0, sizeof(unsigned long long) * ctx->R));
+  std::fill(ctx->h_proposed.begin(), ctx->h_proposed.end(), 0ull);
+  return 0;
+}
+
+int pyro_adapt_cone(pyro_ctx *ctx, double target, int rounds, int sweeps_per_round) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (!(target > 0.0 && target < 1.0) || rounds < 0 || sweeps_per_round < 1)
+    return set_error(PYRO_EINVAL, "pyro_adapt_cone: bad target/rounds/sweeps");
+  if (ctx->n_metro < 1) return set_error(PYRO_ESTATE, "pyro_adapt_cone: n_metro = 0");
+  const int R = ctx->R;
+  std::vector<float> cosc(R);
+  std::vector<uint64_t> acc(R), prop(R);
+  for (int it = 0; it < rounds; ++it) {
+    PYRO_TRY(pyro_reset_acceptance(ctx));
+    PYRO_TRY(pyro_sweep(ctx, sweeps_per_round, 1));
+    PYRO_TRY(pyro_acceptance(ctx, acc.data(), prop.data()));
+    PYRO_TRY(pyro_get_cos_cone(ctx, cosc.data()));
+    for (int r = 0; r < R; ++r) {
+      const double ratio = prop[r] ? (double)acc[r] / (double)prop[r] : 0.0;
+      double theta = std::acos(std::max(-1.0, std::min(1.0, (double)cosc[r])));
+      theta *= (ratio < target) ? 0.9 : 1.0 / 0.9;
+      theta = std::min(theta, M_PI);
+      theta = std::max(theta, 1e-4);
+      float c = (float)std::cos(theta);
+      cosc[r] = std::max(-1.0f, std::min(c, 0.99999994f));
+    }
+    PYRO_TRY(pyro_set_cos_cone(ctx, cosc.data()));
+  }
+  PYRO_TRY(pyro_reset_acceptance(ctx));
+  return 0;
+}
+
+int pyro_record_start(pyro_ctx *ctx, int capacity) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (capacity < 0) return set_error(PYRO_EINVAL, "capacity %d < 0", capacity);
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  if (capacity > ctx->record_cap) {
+    drop_graph(ctx);  // the captured end-of-sweep kernel holds the old pointer
+    if (ctx->d_series) PYRO_HIP(hipFree(ctx->d_series));
+    ctx->d_series = nullptr;
+    ctx->record_cap = 0;
+    PYRO_HIP(hipMalloc((void **)&ctx->d_series, sizeof(double) * (size_t)capacity * ctx->R));
+    ctx->record_cap = capacity;
+  }
+  uint32_t w[3] = {0u, capacity > 0 ? 1u : 0u, (uint32_t)capacity};
+  PYRO_HIP(hipMemcpy(ctx->d_ctl + kCtlRecordRow, w, sizeof w, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int pyro_record_fetch(pyro_ctx *ctx, double *rows, int *n_rows) {
+  if (!ctx || !n_rows) return set_error(PYRO_EINVAL, "pyro_record_fetch: null argument");
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  uint32_t row = 0;
+  PYRO_HIP(hipMemcpy(&row, ctx->d_ctl + kCtlRecordRow, sizeof row, hipMemcpyDeviceToHost));
+  *n_rows = (int)row;
+  if (rows && row > 0)
+    PYRO_HIP(hipMemcpy(rows, ctx->d_series, sizeof(double) * (size_t)row * ctx->R,
+                       hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, double *span_ms) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if ((kind != 0 && kind != 1) || n_launches < 1 || n_launches > 100000)
+    return set_error(PYRO_EINVAL, "pyro_time_phase: kind %d / n_launches %d", kind, n_launches);
+  PYRO_TRY(device_guard(ctx));
+  std::vector<hipEvent_t> ev(2 * (size_t)n_launches, nullptr);
+  int rc = 0;
+  for (auto &e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      rc = set_error(PYRO_EHIP, "pyro_time_phase: event creation failed");
+      break;
+    }
+  for (int i = 0; i < n_launches && rc == 0; ++i) {
+    if (hipEventRecord(ev[2 * i], ctx->stream) != hipSuccess) {
+      rc = set_error(PYRO_EHIP, "pyro_time_phase: event record failed");
+      break;
+    }
+    rc = kind == 0 ? launch_metropolis_phase(ctx, i & 3, 0, nullptr)
+                   : launch_overrelax_phase(ctx, i & 3);
+    if (rc == 0 && hipEventRecord(ev[2 * i + 1], ctx->stream) != hipSuccess)
+      rc = set_error(PYRO_EHIP, "pyro_time_phase: event record failed");
+  }
+  if (rc == 0 && hipStreamSynchronize(ctx->stream) != hipSuccess)
+    rc = set_error(PYRO_EHIP, "pyro_time_phase: synchronise failed");
+  double sum = 0.0;
+  float span = 0.0f;
+  for (int i = 0; i < n_launches && rc == 0; ++i) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) != hipSuccess)
+      rc = set_error(PYRO_EHIP, "pyro_time_phase: elapsed time failed");
+    sum += ms;
+  }
+  if (rc == 0 && hipEventElapsedTime(&span, ev[0], ev[2 * (size_t)n_launches - 1]) != hipSuccess)
+    rc = set_error(PYRO_EHIP, "pyro_time_phase: elapsed time failed");
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (rc) return rc;
+  if (kind == 0)
+    for (auto &p : ctx->h_proposed) p += 4ull * (uint64_t)ctx->ncell * (uint64_t)n_launches;
+  if (avg_ms) *avg_ms = sum / n_launches;
+  if (span_ms) *span_ms = span;
+  return 0;
+}
+
+int pyro_sync(pyro_ctx *ctx) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+}  // extern "C"

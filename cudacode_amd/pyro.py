@@ -1,0 +1,316 @@
+"""Host-side interface over the C ABI of libpyro.so (include/pyro.h).
+
+The class mirrors the reference's host entry points for the Monte Carlo step
+(taillefu/cudacode): ``Mc`` (Pyrochlore.cu:74), ``MCMultipleTemperaturesPerSystem``
+(Pyrochlore.cu:829), ``OverRelaxationByPiStep`` (Pyrochlore.cu:975),
+``TotalEnergyv2`` (Math.cu:309), ``Magnetization`` (Pyrochlore.cu:453),
+``DoMeasurement`` (Pyrochlore.cu:1081), ``TrySwapingTemperatures``
+(MonteCarlo.cu:105), ``GenerateRandomVectorsOnDevice`` (MonteCarlo.cu:275),
+``CalculateRatio`` (MonteCarlo.cu:346) and ``CalculateSolidAngle``
+(MonteCarlo.cu:371). Errors raise :class:`PyroError` carrying the library's
+message instead of the reference's printf-and-continue.
+
+All compute runs in the gfx950 library; this module only moves arguments.
+There is no CPU fallback: without the built library the import fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpyro.so")
+
+# Entry points of include/pyro.h, in declaration order.
+EXPORTS = [
+    "pyro_version", "pyro_last_error", "pyro_create", "pyro_destroy", "pyro_links16",
+    "pyro_independent_sets", "pyro_philox4x32_10", "pyro_init_random_spins", "pyro_set_spins",
+    "pyro_get_spins", "pyro_spins_device_ptr", "pyro_set_temperatures", "pyro_get_temperatures",
+    "pyro_set_cos_cone", "pyro_get_cos_cone", "pyro_metropolis_phase", "pyro_overrelax_phase",
+    "pyro_sweep", "pyro_get_sweep", "pyro_set_sweep", "pyro_pt_swap", "pyro_energy",
+    "pyro_magnetization", "pyro_order_params", "pyro_sublattice_sums", "pyro_fold_order_params",
+    "pyro_energy_incremental",
+    "pyro_acceptance", "pyro_reset_acceptance", "pyro_adapt_cone", "pyro_record_start",
+    "pyro_record_fetch", "pyro_time_phase", "pyro_sync",
+]
+
+
+class PyroError(RuntimeError):
+    pass
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("L", C.c_int),
+        ("n_replicas", C.c_int),
+        ("temperatures", C.POINTER(C.c_double)),
+        ("seed", C.c_uint64),
+        ("J", C.c_double),
+        ("spin_len2", C.c_double),
+        ("n_metro", C.c_int),
+        ("n_or", C.c_int),
+        ("pt_every", C.c_int),
+        ("cos_cone", C.c_float),
+        ("device", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libpyro.so. Raises if it has not been built: there is no fallback."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing; build it with `python -m cudacode_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.pyro_version.restype = C.c_char_p
+        L.pyro_last_error.restype = C.c_char_p
+        L.pyro_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+        L.pyro_destroy.argtypes = [vp]
+        L.pyro_links16.argtypes = [vp]
+        L.pyro_independent_sets.argtypes = [vp]
+        L.pyro_philox4x32_10.argtypes = [vp, vp, vp]
+        L.pyro_init_random_spins.argtypes = [vp]
+        L.pyro_set_spins.argtypes = [vp, vp]
+        L.pyro_get_spins.argtypes = [vp, vp]
+        L.pyro_spins_device_ptr.argtypes = [vp, C.POINTER(vp), C.POINTER(C.c_size_t)]
+        L.pyro_set_temperatures.argtypes = [vp, vp]
+        L.pyro_get_temperatures.argtypes = [vp, vp, vp]
+        L.pyro_set_cos_cone.argtypes = [vp, vp]
+        L.pyro_get_cos_cone.argtypes = [vp, vp]
+        L.pyro_metropolis_phase.argtypes = [vp, C.c_int, C.c_uint32, C.c_int]
+        L.pyro_overrelax_phase.argtypes = [vp, C.c_int]
+        L.pyro_sweep.argtypes = [vp, C.c_int, C.c_int]
+        L.pyro_get_sweep.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.pyro_set_sweep.argtypes = [vp, C.c_uint32, C.c_uint32]
+        L.pyro_pt_swap.argtypes = [vp, C.c_int, C.c_uint32, C.POINTER(C.c_int)]
+        for name in ("pyro_energy", "pyro_magnetization", "pyro_order_params",
+                     "pyro_sublattice_sums", "pyro_energy_incremental"):
+            getattr(L, name).argtypes = [vp, vp]
+        L.pyro_fold_order_params.argtypes = [vp, C.c_int, C.c_double, vp]
+        L.pyro_acceptance.argtypes = [vp, vp, vp]
+        L.pyro_reset_acceptance.argtypes = [vp]
+        L.pyro_adapt_cone.argtypes = [vp, C.c_double, C.c_int, C.c_int]
+        L.pyro_record_start.argtypes = [vp, C.c_int]
+        L.pyro_record_fetch.argtypes = [vp, vp, C.POINTER(C.c_int)]
+        L.pyro_time_phase.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]
+        L.pyro_sync.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().pyro_last_error().decode(errors="replace")
+        raise PyroError(f"pyro error {rc}: {msg}")
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def links16() -> np.ndarray:
+    out = np.zeros((16, 6, 4), np.int8)
+    _check(lib().pyro_links16(_p(out)))
+    return out
+
+
+def independent_sets() -> np.ndarray:
+    out = np.zeros((24, 4), np.int8)
+    _check(lib().pyro_independent_sets(_p(out)))
+    return out
+
+
+def philox4x32_10(ctr, key) -> np.ndarray:
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    _check(lib().pyro_philox4x32_10(_p(c), _p(k), _p(out)))
+    return out
+
+
+def fold_order_params(sub12, n_spins: float) -> np.ndarray:
+    """Order parameters (R, 12) from sublattice sums (R, 12); host arithmetic only."""
+    s = np.ascontiguousarray(sub12, np.float64).reshape(-1, 12)
+    out = np.empty_like(s)
+    _check(lib().pyro_fold_order_params(_p(s), s.shape[0], float(n_spins), _p(out)))
+    return out
+
+
+def geometric_ladder(t_min: float, t_max: float, n: int) -> np.ndarray:
+    """n temperatures T_k = t_min (t_max/t_min)^(k/(n-1))."""
+    if n == 1:
+        return np.array([t_min], np.float64)
+    k = np.arange(n, dtype=np.float64)
+    return t_min * (t_max / t_min) ** (k / (n - 1))
+
+
+class Pyrochlore:
+    """One GPU's resident ensemble of R replicas of an L^3-cell pyrochlore lattice."""
+
+    def __init__(self, L: int, temperatures, seed: int = 56081376, J: float = 1.0,
+                 spin_len2: float = 1.0, n_metro: int = 1, n_or: int = 0, pt_every: int = 0,
+                 cos_cone: float = -1.0, device: int = 0):
+        self._ctx = C.c_void_p()
+        self._temps_init = np.ascontiguousarray(temperatures, np.float64)
+        self.L, self.R = int(L), len(self._temps_init)
+        self.ncell = self.L ** 3
+        self.N = 16 * self.ncell
+        self.seed, self.J, self.spin_len2 = int(seed), float(J), float(spin_len2)
+        self.n_metro, self.n_or, self.pt_every = int(n_metro), int(n_or), int(pt_every)
+        cfg = Config(self.L, self.R,
+                     self._temps_init.ctypes.data_as(C.POINTER(C.c_double)),
+                     self.seed & 0xFFFFFFFFFFFFFFFF, self.J, self.spin_len2, self.n_metro,
+                     self.n_or, self.pt_every, float(cos_cone), int(device))
+        _check(lib().pyro_create(C.byref(cfg), C.byref(self._ctx)))
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self) -> None:
+        if self._ctx:
+            lib().pyro_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- spins --------------------------------------------------------------
+    @property
+    def n_floats(self) -> int:
+        return self.R * 48 * self.ncell
+
+    def get_spins(self) -> np.ndarray:
+        out = np.empty(self.n_floats, np.float32)
+        _check(lib().pyro_get_spins(self._ctx, _p(out)))
+        return out
+
+    def set_spins(self, soa) -> None:
+        a = np.ascontiguousarray(soa, np.float32).ravel()
+        if a.size != self.n_floats:
+            raise PyroError(f"set_spins: expected {self.n_floats} floats, got {a.size}")
+        _check(lib().pyro_set_spins(self._ctx, _p(a)))
+
+    def GenerateRandomVectorsOnDevice(self) -> None:
+        _check(lib().pyro_init_random_spins(self._ctx))
+
+    def device_ptr(self):
+        p, n = C.c_void_p(), C.c_size_t()
+        _check(lib().pyro_spins_device_ptr(self._ctx, C.byref(p), C.byref(n)))
+        return p.value, n.value
+
+    # -- temperatures -------------------------------------------------------
+    def temperatures(self):
+        T = np.empty(self.R, np.float64)
+        at = np.empty(self.R, np.int32)
+        _check(lib().pyro_get_temperatures(self._ctx, _p(T), _p(at)))
+        return T, at
+
+    def set_temperatures(self, T) -> None:
+        a = np.ascontiguousarray(T, np.float64)
+        _check(lib().pyro_set_temperatures(self._ctx, _p(a)))
+
+    def cos_cone(self) -> np.ndarray:
+        c = np.empty(self.R, np.float32)
+        _check(lib().pyro_get_cos_cone(self._ctx, _p(c)))
+        return c
+
+    def set_cos_cone(self, c) -> None:
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(c, np.float32), (self.R,)))
+        _check(lib().pyro_set_cos_cone(self._ctx, _p(a)))
+
+    # -- the step -----------------------------------------------------------
+    def MCMultipleTemperaturesPerSystem(self, colour: int, sweep: int, pass_: int = 0) -> None:
+        _check(lib().pyro_metropolis_phase(self._ctx, colour, sweep, pass_))
+
+    def OverRelaxationByPiStep(self, colour: int) -> None:
+        _check(lib().pyro_overrelax_phase(self._ctx, colour))
+
+    def Mc(self, steps: int, use_graph: bool = True) -> None:
+        _check(lib().pyro_sweep(self._ctx, int(steps), 1 if use_graph else 0))
+
+    def sweep_counters(self):
+        s, r = C.c_uint32(), C.c_uint32()
+        _check(lib().pyro_get_sweep(self._ctx, C.byref(s), C.byref(r)))
+        return s.value, r.value
+
+    def set_sweep_counters(self, sweep: int, pt_round: int) -> None:
+        _check(lib().pyro_set_sweep(self._ctx, sweep, pt_round))
+
+    def TrySwapingTemperatures(self, parity: int, round_: int) -> int:
+        n = C.c_int()
+        _check(lib().pyro_pt_swap(self._ctx, parity, round_, C.byref(n)))
+        return n.value
+
+    # -- observables --------------------------------------------------------
+    def _vec(self, fn, width):
+        out = np.empty(self.R * width, np.float64)
+        _check(fn(self._ctx, _p(out)))
+        return out.reshape(self.R, width) if width > 1 else out
+
+    def TotalEnergyv2(self) -> np.ndarray:
+        return self._vec(lib().pyro_energy, 1)
+
+    def energy_incremental(self) -> np.ndarray:
+        return self._vec(lib().pyro_energy_incremental, 1)
+
+    def Magnetization(self) -> np.ndarray:
+        return self._vec(lib().pyro_magnetization, 3)
+
+    def DoMeasurement(self) -> np.ndarray:
+        return self._vec(lib().pyro_order_params, 12)
+
+    def sublattice_sums(self) -> np.ndarray:
+        return self._vec(lib().pyro_sublattice_sums, 12)
+
+    def acceptance(self):
+        a = np.empty(self.R, np.uint64)
+        p = np.empty(self.R, np.uint64)
+        _check(lib().pyro_acceptance(self._ctx, _p(a), _p(p)))
+        return a, p
+
+    def reset_acceptance(self) -> None:
+        _check(lib().pyro_reset_acceptance(self._ctx))
+
+    def CalculateRatio(self) -> np.ndarray:
+        a, p = self.acceptance()
+        return a / np.maximum(p, 1)
+
+    def CalculateSolidAngle(self, target: float = 0.5, rounds: int = 20,
+                            sweeps_per_round: int = 10) -> np.ndarray:
+        _check(lib().pyro_adapt_cone(self._ctx, target, rounds, sweeps_per_round))
+        return self.cos_cone()
+
+    # -- energy series ------------------------------------------------------
+    def record_start(self, capacity: int) -> None:
+        _check(lib().pyro_record_start(self._ctx, int(capacity)))
+
+    def record_fetch(self) -> np.ndarray:
+        n = C.c_int()
+        _check(lib().pyro_record_fetch(self._ctx, None, C.byref(n)))
+        out = np.empty(max(n.value, 1) * self.R, np.float64)
+        _check(lib().pyro_record_fetch(self._ctx, _p(out), C.byref(n)))
+        return out[: n.value * self.R].reshape(n.value, self.R)
+
+    def time_phase(self, kind: int, n_launches: int):
+        """(mean ms per launch, first-to-last ms) of n back-to-back phase launches
+        (kind 0 Metropolis, 1 over-relaxation), timed with HIP events on the ctx stream."""
+        avg, span = C.c_double(), C.c_double()
+        _check(lib().pyro_time_phase(self._ctx, kind, n_launches, C.byref(avg), C.byref(span)))
+        return avg.value, span.value
+
+    def sync(self) -> None:
+        _check(lib().pyro_sync(self._ctx))

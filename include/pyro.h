@@ -1,0 +1,164 @@
+/* pyro.h -- C ABI of the MI355X-native pyrochlore Heisenberg Monte Carlo step.
+ *
+ * The reference (taillefu/cudacode) has no plugin or FFI layer: its boundary is
+ * a set of host C++ functions inside one unity build that take a
+ * std::vector<dev>& (Pyrochlore.cu, MonteCarlo.cu, Math.cu). Each entry point
+ * below replaces one of them; the comment names the reference function and
+ * its file:line.
+ *
+ * Conventions
+ *   - Every function returns int: 0 on success, a negative PYRO_E* code on
+ *     failure; pyro_last_error() returns a thread-local message. Nothing calls
+ *     exit(). (The reference ignores CUDA status codes, printf's and goes on,
+ *     or calls exit(): MonteCarlo.cu:5-7,221-224, Math.cu:676-680.)
+ *   - One pyro_ctx per GPU; a ctx is not thread-safe. Calls enqueue on the
+ *     ctx's HIP stream; functions that return host data synchronise it.
+ *   - Spins are FP32 unit vectors, layout [replica][x,y,z][site 0..15][cell],
+ *     cell = (z*L + y)*L + x: the reference SoA (Math.cu:93,
+ *     Pyrochlore.cu:843-845). All replicas stay resident in HBM.
+ *   - Energy H = J * spin_len2 * sum_<ij> s_i.s_j. spin_len2 = 1 is the unit
+ *     scale; spin_len2 = 0.25 is the reference's spin-1/2 scale
+ *     (Pyrochlore.cu:254-262,388; T_ref = T / 4).
+ */
+#ifndef PYRO_H
+#define PYRO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYRO_OK 0
+#define PYRO_EINVAL (-1)   /* bad argument */
+#define PYRO_EHIP (-2)     /* HIP runtime error */
+#define PYRO_ENOMEM (-3)   /* device or host allocation failed */
+#define PYRO_ESTATE (-4)   /* call not valid in the ctx's current state */
+
+typedef struct pyro_ctx pyro_ctx;
+
+typedef struct pyro_config {
+  int L;                      /* cubic cells per side; N = 16 L^3 spins per replica */
+  int n_replicas;             /* R, all resident on the one GPU */
+  const double *temperatures; /* R temperatures, replica r starts at temperatures[r] */
+  uint64_t seed;              /* Philox key */
+  double J;                   /* exchange, > 0 antiferromagnetic */
+  double spin_len2;           /* 1 (unit scale) or 0.25 (reference scale) */
+  int n_metro;                /* Metropolis passes per sweep */
+  int n_or;                   /* over-relaxation passes per sweep */
+  int pt_every;               /* sweeps between parallel-tempering swaps, 0 = off */
+  float cos_cone;             /* initial cosine of the proposal cone, -1 = whole sphere */
+  int device;                 /* HIP device ordinal */
+} pyro_config;
+
+/* Version and error text. */
+const char *pyro_version(void);
+const char *pyro_last_error(void);
+
+/* Create / destroy a ctx: allocates all device state. Replaces InitializeDevice
+ * (missing glue, called at Pyrochlore.cu:1251) and InitializeCouplings3D
+ * (Pyrochlore.cu:643-711). */
+int pyro_create(const pyro_config *cfg, pyro_ctx **out);
+int pyro_destroy(pyro_ctx *ctx);
+
+/* Lattice tables: links16 (Pyrochlore.cu:152-167) and IndependentSubLattices
+ * (Pyrochlore.cu:36-43), as (dx, dy, dz, site) and 24 sets of 4 sites. */
+int pyro_links16(int8_t out[16][6][4]);
+int pyro_independent_sets(int8_t out[24][4]);
+
+/* Philox4x32-10 block function (replaces the cuRAND fill, MonteCarlo.cu:3-13,
+ * and the MKL MT19937 stream, MonteCarlo.cu:116). Host-side. */
+int pyro_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Spins. init: uniform on the sphere from the Philox stream
+ * (GenerateRandomVectorsOnDevice, MonteCarlo.cu:257-293; Generate3DVectorList,
+ * Math.cu:132-143). set/get copy R*48*L^3 floats host<->device
+ * (CopyResultsOnGPU / CopyResultsOnCPU, missing glue, Pyrochlore.cu:1530,1766).
+ * device_ptr exposes the resident buffer (no copy). All reset the energy anchor. */
+int pyro_init_random_spins(pyro_ctx *ctx);
+int pyro_set_spins(pyro_ctx *ctx, const float *soa);
+int pyro_get_spins(pyro_ctx *ctx, float *soa);
+int pyro_spins_device_ptr(pyro_ctx *ctx, float **dev_ptr, size_t *n_floats);
+
+/* Temperatures and proposal cones per replica (the host vectors the reference
+ * swaps in TrySwapingTemperatures and copies with cudaMemcpyAsync,
+ * MonteCarlo.cu:94-98,207-210). replica_at[rung] lists replicas by descending
+ * temperature (the gsl_sort_largest_index order, MonteCarlo.cu:120). */
+int pyro_set_temperatures(pyro_ctx *ctx, const double *T);
+int pyro_get_temperatures(pyro_ctx *ctx, double *T, int *replica_at);
+int pyro_set_cos_cone(pyro_ctx *ctx, const float *cos_cone);
+int pyro_get_cos_cone(pyro_ctx *ctx, float *cos_cone);
+
+/* Single colour phases (colour = site % 4 in 0..3). sweep and pass key the
+ * Philox counters. MCMultipleTemperaturesPerSystem (Pyrochlore.cu:829-885)
+ * and OverRelaxationByPiStep (Pyrochlore.cu:975-1007). */
+int pyro_metropolis_phase(pyro_ctx *ctx, int colour, uint32_t sweep, int pass);
+int pyro_overrelax_phase(pyro_ctx *ctx, int colour);
+
+/* n_sweeps full sweeps (n_metro Metropolis + n_or over-relaxation passes, each
+ * over colours 0..3, and a parallel-tempering swap after every pt_every-th
+ * sweep). Replaces Mc (Pyrochlore.cu:74-133) driven by MonteCarlo /
+ * ParallelTempering (MonteCarlo.cu:176-228,429-470), without Mc's 1000-sweep
+ * cap. use_graph != 0 replays a captured hipGraph per block of sweeps. */
+int pyro_sweep(pyro_ctx *ctx, int n_sweeps, int use_graph);
+int pyro_get_sweep(pyro_ctx *ctx, uint32_t *sweep, uint32_t *pt_round);
+int pyro_set_sweep(pyro_ctx *ctx, uint32_t sweep, uint32_t pt_round);
+
+/* One parallel-tempering exchange with the given parity on the current
+ * energies (TrySwapingTemperatures, MonteCarlo.cu:105-172). round keys the
+ * Philox draw. */
+int pyro_pt_swap(pyro_ctx *ctx, int parity, uint32_t round, int *n_accepted);
+
+/* Observables, FP64, per replica (replica order). energy: R doubles
+ * (TotalEnergyv2, Math.cu:309-330); magnetization: R*3 (Magnetization,
+ * Pyrochlore.cu:453-466, g = 1); order_params: R*12 q=0 components
+ * normalised so the ideal ordered state has modulus 1 (Measurement +
+ * DoMeasurement, Pyrochlore.cu:526-641,1081-1092); sublattice: R*12 raw
+ * sublattice sums (b, xyz). */
+int pyro_energy(pyro_ctx *ctx, double *E);
+int pyro_magnetization(pyro_ctx *ctx, double *M3);
+int pyro_order_params(pyro_ctx *ctx, double *op12);
+int pyro_sublattice_sums(pyro_ctx *ctx, double *sub12);
+
+/* Host-only fold of R sublattice-sum rows (R*12) into the 12 order
+ * parameters (R*12), divided by n_spins: the arithmetic of pyro_order_params
+ * without the device reduction. */
+int pyro_fold_order_params(const double *sub12, int R, double n_spins, double *op12);
+
+/* Incremental FP64 energy: the last full energy plus every accepted change
+ * since (no reference counterpart; checked against pyro_energy). */
+int pyro_energy_incremental(pyro_ctx *ctx, double *E);
+
+/* Metropolis acceptance counters since the last reset (CalculateRatio,
+ * MonteCarlo.cu:346-367; the reference kernels never fill them). */
+int pyro_acceptance(pyro_ctx *ctx, uint64_t *accepted, uint64_t *proposed);
+int pyro_reset_acceptance(pyro_ctx *ctx);
+
+/* Cone adaptation toward a target acceptance (CalculateSolidAngle,
+ * MonteCarlo.cu:371-426, which shrinks the angle by 0.9 per round): rounds of
+ * sweeps_per_round sweeps, the half-angle of each replica scaled by 0.9 or
+ * 1/0.9 toward `target`. Thermalisation only. */
+int pyro_adapt_cone(pyro_ctx *ctx, double target, int rounds, int sweeps_per_round);
+
+/* Energy time series, one row per sweep in rung order (incremental energy of
+ * the replica holding each rung), for autocorrelation analysis. capacity rows
+ * are kept; recording stops when full. */
+int pyro_record_start(pyro_ctx *ctx, int capacity);
+int pyro_record_fetch(pyro_ctx *ctx, double *rows, int *n_rows);
+
+/* Time n_launches back-to-back colour-phase launches of one kind (0 =
+ * Metropolis, 1 = over-relaxation; colours cycle 0..3) with a HIP event pair
+ * around each launch on the ctx stream. avg_ms: mean event-to-event time per
+ * launch; span_ms: first to last event. The phases run for real (the state
+ * advances). The reference's counterpart is the ENABLE_TIMING cudaEvent block
+ * around Mc (MonteCarlo.cu:36-63). */
+int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, double *span_ms);
+
+/* Synchronise the ctx stream. */
+int pyro_sync(pyro_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,180 @@
+"""ctypes binding of the CPU oracle (oracle/libpyro_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py as the checker. The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpyro_oracle.so")
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_i8p = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_links16.argtypes = [_i8p]
+        L.orc_independent_sets.argtypes = [_i8p]
+        L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        L.orc_exp_neg.argtypes = [C.c_float]
+        L.orc_exp_neg.restype = C.c_float
+        L.orc_sincospi.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_cone_proposal.argtypes = [_f32p, C.c_float, C.c_float, C.c_float, _f32p]
+        L.orc_overrelax.argtypes = [_f32p, _f32p, _f32p]
+        L.orc_metropolis_accept.argtypes = [_f32p, _f32p, _f32p, C.c_float, C.c_float]
+        L.orc_metropolis_accept.restype = C.c_int
+        L.orc_init_spins.argtypes = [_f32p, C.c_int, C.c_int, C.c_uint64]
+        L.orc_metropolis_phase.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int,
+                                           _f32p, _f32p, C.c_float, C.c_uint64, C.c_void_p,
+                                           C.c_void_p]
+        L.orc_overrelax_phase.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_void_p]
+        L.orc_observables.argtypes = [_f32p, C.c_int, C.c_int, C.c_float, _f64p]
+        L.orc_pt_swap.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint64, _f64p, _f64p, _f32p,
+                                  _f32p, _i32p]
+        L.orc_pt_swap.restype = C.c_int
+        L.orc_run.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int,
+                              C.c_int, C.POINTER(C.c_uint32), _f64p, _f32p, _f32p, _i32p,
+                              C.c_float, C.c_uint64, C.c_void_p, C.c_void_p,
+                              C.POINTER(C.c_int)]
+        L.orc_num_threads.restype = C.c_int
+        L.orc_set_num_threads.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def links16() -> np.ndarray:
+    out = np.zeros((16, 6, 4), np.int8)
+    lib().orc_links16(out)
+    return out
+
+
+def independent_sets() -> np.ndarray:
+    out = np.zeros((24, 4), np.int8)
+    lib().orc_independent_sets(out)
+    return out
+
+
+def philox(ctr, key) -> np.ndarray:
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(np.asarray(ctr, np.uint32), np.asarray(key, np.uint32), out)
+    return out
+
+
+def exp_neg(x: float) -> float:
+    return float(lib().orc_exp_neg(x))
+
+
+def sincospi(a: float):
+    s, c = C.c_float(), C.c_float()
+    lib().orc_sincospi(a, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def cone_proposal(s, cos_cone, u_cos, u_phi) -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    lib().orc_cone_proposal(np.asarray(s, np.float32), cos_cone, u_cos, u_phi, out)
+    return out
+
+
+def overrelax(h, s) -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    lib().orc_overrelax(np.asarray(h, np.float32), np.asarray(s, np.float32), out)
+    return out
+
+
+def metropolis_accept(h, s, sp, beta, u) -> bool:
+    return bool(lib().orc_metropolis_accept(np.asarray(h, np.float32), np.asarray(s, np.float32),
+                                            np.asarray(sp, np.float32), beta, u))
+
+
+def init_spins(L: int, R: int, seed: int) -> np.ndarray:
+    s = np.zeros(R * 48 * L ** 3, np.float32)
+    lib().orc_init_spins(s, L, R, seed)
+    return s
+
+
+def metropolis_phase(spins, L, R, colour, sweep, pass_, beta, cosc, jeff, seed, acc=None, dE=None):
+    lib().orc_metropolis_phase(spins, L, R, colour, sweep, pass_, np.asarray(beta, np.float32),
+                               np.asarray(cosc, np.float32), jeff, seed, _ptr(acc), _ptr(dE))
+
+
+def overrelax_phase(spins, L, R, colour, jeff, dE=None):
+    lib().orc_overrelax_phase(spins, L, R, colour, jeff, _ptr(dE))
+
+
+def observables(spins, L, R, jeff) -> np.ndarray:
+    out = np.zeros(R * 13, np.float64)
+    lib().orc_observables(spins, L, R, jeff, out)
+    return out.reshape(R, 13)
+
+
+def pt_swap(R, parity, rnd, seed, energy, temps, beta, cosc, replica_at) -> int:
+    return lib().orc_pt_swap(R, parity, rnd, seed, np.ascontiguousarray(energy, np.float64), temps,
+                             beta, cosc, replica_at)
+
+
+class OracleRun:
+    """Whole-run state mirrored from the GPU ctx (spins, temperatures, rungs)."""
+
+    def __init__(self, L, temps, seed, J=1.0, spin_len2=1.0, n_metro=1, n_or=0, pt_every=0,
+                 cos_cone=-1.0):
+        self.L, self.R = L, len(temps)
+        self.seed = seed
+        self.jeff = np.float32(J * spin_len2)
+        self.n_metro, self.n_or, self.pt_every = n_metro, n_or, pt_every
+        self.temps = np.array(temps, np.float64)
+        self.beta = (1.0 / self.temps).astype(np.float32)
+        self.cosc = np.full(self.R, cos_cone, np.float32)
+        self.replica_at = np.argsort(-self.temps, kind="stable").astype(np.int32)
+        self.spins = init_spins(L, self.R, seed)
+        self.sweep = 0
+        self.pt_round = C.c_uint32(0)
+        self.accepted = np.zeros(self.R, np.uint64)
+        self.dE = np.zeros(self.R, np.float64)
+        self.swaps = C.c_int(0)
+        self.anchor = observables(self.spins, L, self.R, self.jeff)[:, 0].copy()
+
+    def run(self, n):
+        lib().orc_run(self.spins, self.L, self.R, n, self.sweep, self.n_metro, self.n_or,
+                      self.pt_every, C.byref(self.pt_round), self.temps, self.beta, self.cosc,
+                      self.replica_at, self.jeff, self.seed, _ptr(self.accepted), _ptr(self.dE),
+                      C.byref(self.swaps))
+        self.sweep += n
+
+    def observables(self):
+        return observables(self.spins, self.L, self.R, self.jeff)
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+def set_num_threads(n: int) -> None:
+    lib().orc_set_num_threads(n)

@@ -1,0 +1,82 @@
+/* CPU oracle for the pyrochlore Heisenberg Monte Carlo step.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of the reference's
+ * algorithm (taillefu/cudacode: Pyrochlore.cu, MonteCarlo.cu, Math.cu), used by
+ * the tests and by the CPU-baseline leg of bench.py as the checker. The product
+ * (cudacode_amd/libpyro.so) never links, loads or calls it.
+ *
+ * Conventions (shared with the GPU path so a trajectory is reproducible bit
+ * for bit from the same seed):
+ *   spins   FP32, layout [replica][component x,y,z][site 0..15][cell],
+ *           cell = (z*L + y)*L + x  (the reference SoA, Math.cu:93,
+ *           Pyrochlore.cu:843-845)
+ *   energy  H = jeff * sum_<ij> s_i.s_j, jeff = J * spin_len2; local field
+ *           h_i = -jeff * sum_j s_j
+ *   RNG     Philox4x32-10, key = (seed lo, seed hi), counter =
+ *           (cell, site | pass<<8 | kind<<24, replica, sweep)
+ *
+ * Parity status per function is listed in oracle/README.md.
+ */
+#ifndef PYRO_ORACLE_H
+#define PYRO_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void orc_links16(int8_t out[16][6][4]);
+void orc_independent_sets(int8_t out[24][4]);
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+float orc_exp_neg(float x);
+void orc_sincospi(float a, float *s, float *c);
+void orc_cone_proposal(const float s[3], float cos_cone, float u_cos, float u_phi, float out[3]);
+void orc_overrelax(const float h[3], const float s[3], float out[3]);
+int orc_metropolis_accept(const float h[3], const float s[3], const float sp[3], float beta,
+                          float u);
+
+/* Uniform spins on the sphere from the Philox stream (kind = init). */
+void orc_init_spins(float *spins, int L, int R, uint64_t seed);
+
+/* One colour phase of Metropolis (cone proposal) on every replica.
+ * beta[r], cos_cone[r] per replica. accepted[r] += accepts; dE[r] += FP64
+ * energy change. Either of accepted / dE may be NULL. */
+void orc_metropolis_phase(float *spins, int L, int R, int colour, uint32_t sweep, int pass,
+                          const float *beta, const float *cos_cone, float jeff, uint64_t seed,
+                          uint64_t *accepted, double *dE);
+
+/* One colour phase of over-relaxation on every replica. dE may be NULL. */
+void orc_overrelax_phase(float *spins, int L, int R, int colour, float jeff, double *dE);
+
+/* Per-replica FP64 observables: out[r*13 + 0] = energy, out[r*13 + 1 + 3*b + k]
+ * = sum over the N/4 spins of sublattice b (= site % 4) of component k. */
+void orc_observables(const float *spins, int L, int R, float jeff, double *out);
+
+/* Parallel-tempering exchange (MonteCarlo.cu:105-172). Rungs are ordered by
+ * descending temperature; replica_at[rung] is the replica holding that rung.
+ * Pairs (c, c+1) for c even (parity 0) or odd (parity 1) are proposed and
+ * accepted with min(1, exp((1/T1 - 1/T2)(E1 - E2))); on acceptance the
+ * temperatures (here: the rung assignment, beta and the cone) are exchanged.
+ * temps[r] is replica r's temperature. Returns the number accepted. */
+int orc_pt_swap(int R, int parity, uint32_t round, uint64_t seed, const double *energy,
+                double *temps, float *beta, float *cos_cone, int *replica_at);
+
+/* Full sweeps: n_metro Metropolis passes + n_or over-relaxation passes, each
+ * over colours 0..3; after every pt_every sweeps (if > 0) a swap with parity
+ * alternating from the stored round counter. sweep0 is the first sweep index;
+ * *pt_round is advanced. energy series optional (NULL): if given, after every
+ * sweep the full FP64 energy of each replica in rung order is appended. */
+void orc_run(float *spins, int L, int R, int n_sweeps, uint32_t sweep0, int n_metro, int n_or,
+             int pt_every, uint32_t *pt_round, double *temps, float *beta, float *cos_cone,
+             int *replica_at, float jeff, uint64_t seed, uint64_t *accepted, double *dE,
+             int *n_swaps_accepted);
+
+/* Threads used by the OpenMP loops (1 when built without OpenMP). */
+int orc_num_threads(void);
+void orc_set_num_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,262 @@
+"""GPU parity: the gfx950 kernels against the CPU oracle on the same Philox
+stream, through the C ABI.
+
+Bar: spins bit-identical where the arithmetic is (single phases, init), and
+within 1e-5 after 100 sweeps (they are expected to stay identical: both sides
+evaluate the same float expressions in the same order); integer counts exact;
+FP64 energies to 1e-12 relative (different summation order); at full size,
+size-independent properties (unit spins, incremental == recomputed energy,
+over-relaxation energy conservation).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0x5EED1234ABCD
+
+
+def say(msg):
+    print(f"[gpu-parity] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def gpu(L, temps, **kw):
+    from cudacode_amd import pyro
+    say(f"create L={L} R={len(temps)} {kw}")
+    return pyro.Pyrochlore(L, temps, seed=kw.pop("seed", SEED), **kw)
+
+
+def orc(L, temps, **kw):
+    return oracle.OracleRun(L, temps, seed=kw.pop("seed", SEED), **kw)
+
+
+@pytest.mark.parametrize("L,R", [(4, 3), (5, 2), (16, 2)])
+def test_init_spins_bitwise(L, R):
+    with gpu(L, [1.0] * R) as g:
+        s = g.get_spins()
+    o = oracle.init_spins(L, R, SEED)
+    assert np.array_equal(s, o)
+
+
+@pytest.mark.parametrize("L", [4, 5])
+def test_single_phases_bitwise(L):
+    temps = [0.05, 0.4, 2.5]
+    cones = np.array([0.95, 0.3, -1.0], np.float32)
+    with gpu(L, temps, cos_cone=-1.0) as g:
+        g.set_cos_cone(cones)
+        o = orc(L, temps)
+        o.cosc[:] = cones
+        acc = np.zeros(3, np.uint64)
+        dE = np.zeros(3)
+        for step, colour in enumerate([0, 1, 2, 3, 2, 0]):
+            say(f"metropolis L={L} colour={colour}")
+            g.MCMultipleTemperaturesPerSystem(colour, sweep=7 + step, pass_=step % 2)
+            oracle.metropolis_phase(o.spins, L, 3, colour, 7 + step, step % 2, o.beta, o.cosc,
+                                    o.jeff, SEED, acc, dE)
+            assert np.array_equal(g.get_spins(), o.spins), f"metropolis colour {colour}"
+            say(f"overrelax L={L} colour={colour}")
+            g.OverRelaxationByPiStep(colour)
+            oracle.overrelax_phase(o.spins, L, 3, colour, o.jeff, dE)
+            assert np.array_equal(g.get_spins(), o.spins), f"overrelax colour {colour}"
+        a, p = g.acceptance()
+        assert np.array_equal(a, acc)
+        assert np.all(p == 6 * 4 * L ** 3)
+        e_inc = g.energy_incremental()
+        assert np.allclose(e_inc - (o.anchor + dE), 0, atol=1e-9 * 16 * L ** 3)
+
+
+def test_c1_100_sweeps_spins_agree():
+    """L=4, one replica, T=0.5, 1 Metropolis (cone) + 5 over-relaxation per sweep."""
+    L, temps = 4, [0.5]
+    with gpu(L, temps, n_metro=1, n_or=5, cos_cone=0.2) as g:
+        o = orc(L, temps, n_metro=1, n_or=5, cos_cone=0.2)
+        for chunk in range(4):
+            say(f"c1 sweeps chunk {chunk}")
+            g.Mc(25, use_graph=(chunk % 2 == 0))
+            o.run(25)
+            d = np.abs(g.get_spins() - o.spins).max()
+            assert d <= 1e-5, (chunk, d)
+        assert np.array_equal(g.get_spins(), o.spins)
+        a, _ = g.acceptance()
+        assert np.array_equal(a, o.accepted)
+        assert g.sweep_counters() == (100, 0)
+
+
+def test_c2_shape_100_sweeps_spins_agree_with_pt():
+    """L=16 (the c2 lattice) with 4 replicas on a geometric ladder in [0.01, 1],
+    1 Metropolis + 10 over-relaxation per sweep, swaps every 10 sweeps."""
+    from cudacode_amd.pyro import geometric_ladder
+    L = 16
+    temps = geometric_ladder(0.01, 1.0, 4)
+    kw = dict(n_metro=1, n_or=10, pt_every=10, cos_cone=0.5)
+    with gpu(L, temps, **kw) as g:
+        o = orc(L, temps, **kw)
+        say("c2-shape 100 sweeps on GPU (graph)")
+        g.Mc(100)
+        say("c2-shape 100 sweeps on oracle")
+        o.run(100)
+        d = np.abs(g.get_spins() - o.spins).max()
+        assert d <= 1e-5, d
+        T, at = g.temperatures()
+        assert np.array_equal(at, o.replica_at)
+        assert np.array_equal(T, o.temps)
+        assert g.sweep_counters() == (100, 10)
+
+
+def test_parallel_tempering_trajectory():
+    L = 3
+    temps = [0.1 * 1.5 ** k for k in range(8)]
+    kw = dict(n_metro=1, n_or=2, pt_every=2, cos_cone=0.0)
+    with gpu(L, temps, **kw) as g:
+        o = orc(L, temps, **kw)
+        say("pt 40 sweeps")
+        g.Mc(13, use_graph=False)
+        g.Mc(27, use_graph=True)
+        o.run(40)
+        assert np.array_equal(g.get_spins(), o.spins)
+        T, at = g.temperatures()
+        assert np.array_equal(at, o.replica_at)
+        assert np.array_equal(T, o.temps)
+        assert np.array_equal(g.cos_cone(), o.cosc)
+        assert o.swaps.value > 0
+
+
+def test_explicit_swap_matches_oracle():
+    L = 3
+    temps = [0.3, 0.35, 0.4, 0.45, 0.5]
+    with gpu(L, temps, n_metro=1, n_or=1) as g:
+        o = orc(L, temps, n_metro=1, n_or=1)
+        g.Mc(5, use_graph=False)
+        o.run(5)
+        for rnd in range(6):
+            say(f"swap round {rnd}")
+            n = g.TrySwapingTemperatures(rnd % 2, rnd)
+            e = o.observables()[:, 0]
+            m = oracle.pt_swap(o.R, rnd % 2, rnd, SEED, e, o.temps, o.beta, o.cosc, o.replica_at)
+            assert n == m
+        T, at = g.temperatures()
+        assert np.array_equal(at, o.replica_at)
+
+
+@pytest.mark.parametrize("L,R", [(4, 2), (7, 3)])
+def test_observables_against_oracle(L, R):
+    with gpu(L, [0.2 * (r + 1) for r in range(R)], n_metro=1, n_or=1, cos_cone=0.5) as g:
+        g.Mc(10)
+        s = g.get_spins()
+        o = oracle.observables(s, L, R, 1.0)
+        E = g.TotalEnergyv2()
+        assert np.allclose(E, o[:, 0], rtol=1e-12, atol=1e-10)
+        sub = g.sublattice_sums()
+        assert np.allclose(sub, o[:, 1:], rtol=1e-12, atol=1e-10)
+        M = g.Magnetization()
+        assert np.allclose(M, o[:, 1:].reshape(R, 4, 3).sum(axis=1), atol=1e-9)
+        assert np.allclose(g.energy_incremental(), E, rtol=1e-11, atol=1e-9)
+
+
+def test_closed_form_states_on_gpu():
+    L = 4
+    N = 16 * L ** 3
+    axes = np.array([[-1, -1, -1], [1, 1, -1], [-1, 1, 1], [1, -1, 1]]) / np.sqrt(3)
+    s = np.zeros((1, 3, 16, L ** 3), np.float32)
+    for site in range(16):
+        s[0, :, site, :] = axes[site % 4][:, None]
+    with gpu(L, [1.0], spin_len2=0.25) as g:
+        g.set_spins(s)
+        assert g.TotalEnergyv2()[0] == pytest.approx(-0.25 * N, rel=1e-6)
+        m = g.DoMeasurement()[0]
+        assert abs(abs(m[0]) - 1) < 1e-6 and np.allclose(m[1:], 0, atol=1e-6)
+        s[:] = 0
+        s[0, 0] = 1.0
+        g.set_spins(s)
+        assert g.TotalEnergyv2()[0] == pytest.approx(0.75 * N, rel=1e-12)
+        assert np.allclose(g.DoMeasurement()[0, 6:9], [1, 0, 0])
+
+
+def test_graph_and_eager_identical():
+    L = 8
+    temps = [0.1, 0.3, 0.9]
+    kw = dict(n_metro=1, n_or=4, pt_every=3, cos_cone=0.4)
+    with gpu(L, temps, **kw) as a, gpu(L, temps, **kw) as b:
+        a.Mc(30, use_graph=True)
+        b.Mc(30, use_graph=False)
+        assert np.array_equal(a.get_spins(), b.get_spins())
+        assert np.array_equal(a.temperatures()[1], b.temperatures()[1])
+
+
+def test_energy_series_recording():
+    L = 4
+    temps = [0.2, 0.5, 1.0]
+    with gpu(L, temps, n_metro=1, n_or=2, pt_every=5, cos_cone=0.3) as g:
+        g.record_start(50)
+        g.Mc(20)
+        rows = g.record_fetch()
+        assert rows.shape == (20, 3)
+        e = g.energy_incremental()
+        _, at = g.temperatures()
+        assert np.array_equal(rows[-1], e[at])
+        assert np.allclose(g.TotalEnergyv2()[at], rows[-1], rtol=1e-11)
+
+
+def test_cone_adaptation_reaches_target():
+    L = 4
+    temps = [0.05, 0.5, 5.0]
+    with gpu(L, temps, n_metro=1, n_or=1, cos_cone=-1.0) as g:
+        g.CalculateSolidAngle(target=0.5, rounds=30, sweeps_per_round=5)
+        g.Mc(40)
+        r = g.CalculateRatio()
+        assert np.all(np.abs(r[:2] - 0.5) < 0.12), r
+        assert r[2] > 0.4  # at high T even the whole sphere is accepted often
+
+
+def test_bad_arguments_raise():
+    from cudacode_amd import pyro
+    with gpu(4, [1.0]) as g:
+        with pytest.raises(pyro.PyroError, match="colour"):
+            g.MCMultipleTemperaturesPerSystem(4, 0)
+        with pytest.raises(pyro.PyroError, match="expected"):
+            g.set_spins(np.zeros(10, np.float32))
+        with pytest.raises(pyro.PyroError, match="cos_cone"):
+            g.set_cos_cone([1.0])
+
+
+def test_full_size_c2_properties():
+    """The c2 workload itself: L=16 x 64 replicas, T in [0.01, 1], 1 + 10 passes,
+    swaps every 10 sweeps. After 50 sweeps every spin is a unit vector and the
+    incremental FP64 energy equals the recomputed one."""
+    from cudacode_amd.pyro import geometric_ladder
+    temps = geometric_ladder(0.01, 1.0, 64)
+    with gpu(16, temps, n_metro=1, n_or=10, pt_every=10, cos_cone=0.5) as g:
+        g.Mc(50)
+        s = g.get_spins().reshape(64, 3, -1).astype(np.float64)
+        n = np.sqrt((s ** 2).sum(axis=1))
+        assert np.abs(n - 1).max() < 1e-6
+        E = g.TotalEnergyv2()
+        assert np.allclose(g.energy_incremental(), E, rtol=1e-10, atol=1e-8)
+        # low-T replicas are far below the high-T ones
+        T, at = g.temperatures()
+        e_per = E / (16 * 16 ** 3)
+        assert e_per[at[-1]] < -0.9 and e_per[at[0]] > e_per[at[-1]]
+
+
+def test_overrelaxation_only_conserves_energy_full_size():
+    with gpu(16, [1.0] * 8, n_metro=0, n_or=10) as g:
+        e0 = g.TotalEnergyv2()
+        g.Mc(20)
+        e1 = g.TotalEnergyv2()
+        assert np.allclose(e0, e1, rtol=0, atol=65536 * 2e-6)
+        assert np.allclose(g.energy_incremental(), e1, rtol=1e-11, atol=1e-8)
+
+
+def test_large_lattice_properties():
+    """L=64 (the c5 lattice, 4,194,304 spins per replica), 4 replicas."""
+    with gpu(64, [0.02, 0.08, 0.2, 0.5], n_metro=1, n_or=3, pt_every=2, cos_cone=0.5) as g:
+        g.Mc(4)
+        s = g.get_spins().reshape(4, 3, -1)
+        n = np.sqrt((s.astype(np.float64) ** 2).sum(axis=1))
+        assert np.abs(n - 1).max() < 1e-6
+        assert np.allclose(g.energy_incremental(), g.TotalEnergyv2(), rtol=1e-10)

@@ -1,0 +1,186 @@
+"""Oracle dynamics pinned by closed forms and invariants of the model.
+
+H = jeff * sum_<ij> s_i.s_j on the pyrochlore lattice. Closed forms used:
+  ferromagnetic state: 6 bonds per tetrahedron, E/N = 3 jeff;
+  all-in-all-out (any state with zero spin sum on every tetrahedron): E/N = -jeff;
+  high temperature: E/N = -jeff^2 / T to leading order (3 bonds per spin,
+      <s_i.s_j> = -beta jeff / 3);
+  low temperature: C/N -> 3/4 (Moessner & Chalker, PRL 80, 2929 (1998)), so
+      E/N = -jeff + (3/4) T.
+Invariants: incremental FP64 energy equals a full recompute; over-relaxation
+conserves the energy; a colour phase touches only its own sites and does not
+depend on the order or thread count of its updates; runs are resumable.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle
+
+AIAO_AXES = np.array([[-1, -1, -1], [1, 1, -1], [-1, 1, 1], [1, -1, 1]], np.float64) / np.sqrt(3)
+
+
+def state(L, R, fn):
+    """spins[r][comp][site][cell] with spin(site) = fn(site)."""
+    nc = L ** 3
+    s = np.zeros((R, 3, 16, nc), np.float32)
+    for site in range(16):
+        v = np.asarray(fn(site), np.float32)
+        s[:, :, site, :] = v[None, :, None]
+    return s.ravel()
+
+
+@pytest.mark.parametrize("jeff", [1.0, 0.25])
+@pytest.mark.parametrize("L", [2, 3, 4])
+def test_ferro_and_aiao_energies(L, jeff):
+    N = 16 * L ** 3
+    ferro = state(L, 1, lambda site: [1, 0, 0])
+    obs = oracle.observables(ferro, L, 1, jeff)
+    assert obs[0, 0] == pytest.approx(3 * jeff * N, rel=1e-12)
+    assert obs[0, 1:4] == pytest.approx([N / 4, 0, 0])
+    aiao = state(L, 1, lambda site: AIAO_AXES[site % 4])
+    obs = oracle.observables(aiao, L, 1, jeff)
+    assert obs[0, 0] == pytest.approx(-jeff * N, rel=1e-6)
+
+
+def fresh(L, R, seed=7):
+    return oracle.init_spins(L, R, seed)
+
+
+def test_init_spins_unit_and_isotropic():
+    L, R = 4, 3
+    s = fresh(L, R).reshape(R, 3, -1).astype(np.float64)
+    n = np.sqrt((s ** 2).sum(axis=1))
+    assert np.all(np.abs(n - 1) < 3e-7)
+    m = s.mean(axis=2)
+    assert np.all(np.abs(m) < 5 / np.sqrt(s.shape[2] * 3))
+
+
+def test_phase_touches_only_its_colour():
+    L, R = 3, 2
+    s0 = fresh(L, R)
+    for colour in range(4):
+        s = s0.copy()
+        oracle.metropolis_phase(s, L, R, colour, 5, 0, [2.0, 0.5], [-1.0, 0.3], 1.0, 99)
+        d = (s != s0).reshape(R, 3, 16, -1).any(axis=(0, 1, 3))
+        assert d[colour::4].any()
+        assert not np.delete(d, np.arange(colour, 16, 4)).any()
+
+
+def test_phase_independent_of_thread_count():
+    L, R = 4, 3
+    ref = fresh(L, R)
+    outs = []
+    for nt in (1, 3, 8):
+        oracle.set_num_threads(nt)
+        s = ref.copy()
+        for c in range(4):
+            oracle.metropolis_phase(s, L, R, c, 1, 0, [1.0, 2.0, 3.0], [-1.0] * 3, 1.0, 11)
+            oracle.overrelax_phase(s, L, R, c, 1.0)
+        outs.append(s)
+    oracle.set_num_threads(0)
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_incremental_energy_matches_recompute():
+    L = 3
+    run = oracle.OracleRun(L, [0.3, 1.0, 4.0], seed=5, n_metro=1, n_or=3, cos_cone=0.5)
+    run.run(50)
+    e_full = run.observables()[:, 0]
+    assert np.allclose(run.anchor + run.dE, e_full, rtol=1e-11, atol=1e-9)
+    assert run.accepted.sum() > 0
+
+
+def test_overrelaxation_conserves_energy():
+    L, R = 4, 2
+    s = fresh(L, R)
+    e0 = oracle.observables(s, L, R, 1.0)[:, 0]
+    for _ in range(20):
+        for c in range(4):
+            oracle.overrelax_phase(s, L, R, c, 1.0)
+    e1 = oracle.observables(s, L, R, 1.0)[:, 0]
+    assert np.allclose(e0, e1, rtol=0, atol=16 * L ** 3 * 1e-6)
+    assert not np.array_equal(s, fresh(L, R))
+
+
+def test_run_is_resumable():
+    """Counter-based RNG: 30 sweeps equal 10 + 20 sweeps, bit for bit, with PT."""
+    temps = [0.2, 0.4, 0.8, 1.6]
+    a = oracle.OracleRun(2, temps, seed=3, n_metro=1, n_or=2, pt_every=5, cos_cone=0.2)
+    b = oracle.OracleRun(2, temps, seed=3, n_metro=1, n_or=2, pt_every=5, cos_cone=0.2)
+    a.run(30)
+    b.run(10)
+    b.run(20)
+    assert np.array_equal(a.spins, b.spins)
+    assert np.array_equal(a.replica_at, b.replica_at)
+    assert a.pt_round.value == b.pt_round.value == 6
+
+
+def test_pt_swap_detailed_balance():
+    """Two replicas with fixed energies: the swap is taken with probability
+    min(1, exp((1/T1 - 1/T2)(E1 - E2))) (MonteCarlo.cu:131), independent of
+    which replica currently holds which rung."""
+    T = np.array([2.0, 1.0])
+    E = np.array([-10.0, -9.0])  # the hotter replica holds the lower energy
+    p_expect = min(1.0, np.exp((1 / 2.0 - 1 / 1.0) * (-10.0 + 9.0)))
+    assert p_expect == 1.0
+    E = np.array([-9.0, -10.0])
+    p_expect = np.exp((1 / 2.0 - 1 / 1.0) * (-9.0 + 10.0))
+    n, acc = 20000, 0
+    for rnd in range(n):
+        temps = T.copy()
+        beta = (1 / temps).astype(np.float32)
+        cosc = np.array([0.1, 0.2], np.float32)
+        at = np.array([0, 1], np.int32)
+        a = oracle.pt_swap(2, 0, rnd, 17, E, temps, beta, cosc, at)
+        acc += a
+        if a:
+            assert list(temps) == [1.0, 2.0] and list(at) == [1, 0]
+            assert list(cosc) == pytest.approx([0.2, 0.1])
+            assert beta[0] == np.float32(1.0)
+    p = acc / n
+    assert abs(p - p_expect) < 5 * np.sqrt(p_expect * (1 - p_expect) / n)
+
+
+def test_pt_pairs_follow_parity():
+    R = 6
+    temps = np.array([1.0, 2.0, 3.0, 4.0, 5.0, 6.0])
+    at = np.argsort(-temps).astype(np.int32)
+    E = np.zeros(R)  # every proposal accepted (weight 1)
+    beta = (1 / temps).astype(np.float32)
+    cosc = np.zeros(R, np.float32)
+    assert oracle.pt_swap(R, 0, 0, 1, E, temps, beta, cosc, at) == 3
+    assert list(at) == [4, 5, 2, 3, 0, 1]
+    assert oracle.pt_swap(R, 1, 1, 1, E, temps, beta, cosc, at) == 2
+    assert list(at) == [4, 2, 5, 0, 3, 1]
+    # the temperature held by each replica follows its rung
+    for rung, rep in enumerate(at):
+        assert temps[rep] == 6.0 - rung
+
+
+def equilibrium_energy(L, T, sweeps, therm, start, cos_cone, n_or=3, seed=21):
+    run = oracle.OracleRun(L, [T], seed=seed, n_metro=1, n_or=n_or, cos_cone=cos_cone)
+    if start is not None:
+        run.spins[:] = start
+        run.anchor = run.observables()[:, 0].copy()
+    run.run(therm)
+    es = []
+    for _ in range(sweeps):
+        run.run(1)
+        es.append(run.observables()[0, 0])
+    es = np.array(es) / (16 * L ** 3)
+    bm = es.reshape(20, -1).mean(axis=1)
+    return es.mean(), bm.std(ddof=1) / np.sqrt(len(bm))
+
+
+def test_high_temperature_energy():
+    T = 40.0
+    e, err = equilibrium_energy(2, T, 4000, 200, None, -1.0, n_or=0)
+    assert abs(e - (-1.0 / T)) < 4 * err + 3e-4, (e, err)
+
+
+def test_low_temperature_specific_heat_three_quarters():
+    L, T = 2, 0.01
+    aiao = state(L, 1, lambda site: AIAO_AXES[site % 4])
+    e, err = equilibrium_energy(L, T, 2000, 500, aiao, 0.995, n_or=5)
+    c_est = (e + 1.0) / T
+    assert abs(c_est - 0.75) < 4 * err / T + 0.04, (c_est, err / T)
