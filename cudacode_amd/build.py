@@ -17,6 +17,7 @@ LIB = os.path.join(HERE, "libpyro.so")
 
 SOURCES = [
     "pyro_kernels.hip",
+    "pyro_learned.hip",
     "pyro_capi.hip",
 ]
 

@@ -47,6 +47,8 @@ static int reanchor(pyro_ctx *ctx) {
 static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
   for (int p = 0; p < ctx->n_metro; ++p)
     for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_metropolis_phase(ctx, c, p, nullptr));
+  for (int p = 0; p < ctx->n_learned; ++p)
+    for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_learned_phase(ctx, c, p));
   for (int o = 0; o < ctx->n_or; ++o)
     for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_overrelax_phase(ctx, c));
   if (ctx->pt_every > 0 && (sweep_index + 1u) % (uint32_t)ctx->pt_every == 0u) {
@@ -60,6 +62,8 @@ static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
 static void account_sweeps(pyro_ctx *ctx, uint32_t n) {
   const uint64_t per = (uint64_t)ctx->n_metro * 16ull * (uint64_t)ctx->ncell;
   for (auto &p : ctx->h_proposed) p += per * n;
+  const uint64_t per_l = (uint64_t)ctx->n_learned * 16ull * (uint64_t)ctx->ncell;
+  for (auto &p : ctx->h_proposed_learned) p += per_l * n;
   for (uint32_t i = 0; i < n; ++i) {
     if (ctx->pt_every > 0 && (ctx->h_sweep + 1u) % (uint32_t)ctx->pt_every == 0u) ++ctx->h_pt_round;
     ++ctx->h_sweep;
@@ -195,6 +199,7 @@ int pyro_destroy(pyro_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   drop_graph(ctx);
+  mlp_free(ctx);
   void *ptrs[] = {ctx->d_spins, ctx->d_temps, ctx->d_beta,     ctx->d_cosc, ctx->d_replica_at,
                   ctx->d_acc,   ctx->d_dE,    ctx->d_anchor,   ctx->d_partials, ctx->d_obs,
                   ctx->d_ctl,   ctx->d_series};
@@ -509,6 +514,32 @@ int pyro_reset_acceptance(pyro_ctx *ctx) {
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   PYRO_HIP(hipMemset(ctx->d_acc, 0, sizeof(unsigned long long) * ctx->R));
   std::fill(ctx->h_proposed.begin(), ctx->h_proposed.end(), 0ull);
+  if (ctx->d_acc_learned)
+    PYRO_HIP(hipMemset(ctx->d_acc_learned, 0, sizeof(unsigned long long) * ctx->R));
+  std::fill(ctx->h_proposed_learned.begin(), ctx->h_proposed_learned.end(), 0ull);
+  return 0;
+}
+
+int pyro_set_learned_passes(pyro_ctx *ctx, int n_learned) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (n_learned < 0 || n_learned > 255) return set_error(PYRO_EINVAL, "n_learned %d", n_learned);
+  if (n_learned > 0 && !ctx->mlp_ready)
+    return set_error(PYRO_ESTATE, "pyro_set_learned_passes: set or train the network first");
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  drop_graph(ctx);  // the captured sweep has the old schedule
+  ctx->n_learned = n_learned;
+  return 0;
+}
+
+int pyro_set_passes(pyro_ctx *ctx, int n_metro, int n_or) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (n_metro < 0 || n_or < 0 || n_metro > 255) return set_error(PYRO_EINVAL, "pass counts");
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  drop_graph(ctx);
+  ctx->n_metro = n_metro;
+  ctx->n_or = n_or;
   return 0;
 }
 

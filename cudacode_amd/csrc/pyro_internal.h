@@ -30,6 +30,27 @@ constexpr int kPhaseBlock = 256;   // 4 waves: one per site of the colour
 constexpr int kPhaseCells = 64;    // cells per block (one per lane)
 constexpr int kObsBlock = 256;     // cells per block in the observable pass
 
+// Learned proposal network 6 -> 64 -> 64 -> 4 (ReLU), packed FP32 parameters
+// W1[64][6] b1[64] W2[64][64] b2[64] W3[4][64] b3[4].
+constexpr int kMlpIn = 6, kMlpH = 64, kMlpOut = 4;
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kMlpH * kMlpIn;
+constexpr int kOffW2 = kOffB1 + kMlpH;
+constexpr int kOffB2 = kOffW2 + kMlpH * kMlpH;
+constexpr int kOffW3 = kOffB2 + kMlpH;
+constexpr int kOffB3 = kOffW3 + kMlpOut * kMlpH;
+constexpr int kMlpParams = kOffB3 + kMlpOut;  // 4868
+// bf16 operand images: W1|b1 as [64][8] (column 6 = bias, 7 = 0), W2 [64][64],
+// W3 padded to [32][64] (rows 4..31 zero).
+constexpr int kPackP1 = 0;
+constexpr int kPackP2 = kPackP1 + 64 * 8;
+constexpr int kPackP3 = kPackP2 + 64 * 64;
+constexpr int kPackSize = kPackP3 + 32 * 64;
+// Per-sample training buffers, [unit][batch]: x(6) a1(64) a2(64) d3(4) d2(64) d1(64) loss(1)
+constexpr int kTrainRows = 6 + 64 + 64 + 4 + 64 + 64 + 1;
+constexpr float kLogKappaMin = -4.60517019f;  // ln 0.01
+constexpr float kLogKappaMax = 8.31776617f;   // ln 4096
+
 }  // namespace pyro
 
 struct pyro_ctx {
@@ -58,6 +79,21 @@ struct pyro_ctx {
   int obs_blocks = 0;
   int record_cap = 0;
 
+  // learned proposal
+  int n_learned = 0;                     // learned passes per sweep (after Metropolis)
+  float *d_mlp_w = nullptr;              // packed FP32 master weights (kMlpParams)
+  float *d_mlp_m = nullptr, *d_mlp_v = nullptr;  // Adam moments
+  uint16_t *d_mlp_pack = nullptr;        // bf16 operand images for the MFMA forward
+  float *d_grad = nullptr;               // kMlpParams gradient + 1 loss slot
+  float *d_env = nullptr;                // environments (u = beta h, s), 6 floats each
+  int env_cap = 0, env_n = 0;
+  float *d_train = nullptr;              // per-sample activations / deltas of a batch
+  int train_cap = 0;
+  int adam_t = 0;
+  bool mlp_ready = false;
+  unsigned long long *d_acc_learned = nullptr;  // R, accepted learned proposals
+  std::vector<uint64_t> h_proposed_learned;
+
   // host mirrors
   uint32_t h_sweep = 0, h_pt_round = 0;
   std::vector<uint64_t> h_proposed;  // per replica, Metropolis proposals since reset
@@ -81,6 +117,12 @@ int launch_pt_swap(pyro_ctx *ctx, int parity, uint32_t round, int from_ctl);
 int launch_end_sweep(pyro_ctx *ctx);             // record row, advance sweep counter
 int launch_set_anchor(pyro_ctx *ctx);            // d_anchor = d_obs energy, d_dE = 0
 int launch_set_sweep(pyro_ctx *ctx, uint32_t sweep, uint32_t round);
+
+// Learned proposal (pyro_learned.hip).
+int launch_learned_phase(pyro_ctx *ctx, int colour, int pass);
+int mlp_alloc(pyro_ctx *ctx);
+int mlp_pack(pyro_ctx *ctx);
+void mlp_free(pyro_ctx *ctx);
 
 }  // namespace pyro
 

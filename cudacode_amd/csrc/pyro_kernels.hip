@@ -30,12 +30,16 @@ __device__ __forceinline__ double wave_sum(double v) {
 // proposal (MCMultipleTemperaturesPerSystem, Pyrochlore.cu:829-885).
 // KIND 1: over-relaxation by pi (OverRelaxationByPiStep,
 // Pyrochlore.cu:975-1007). grid = (ceil(ncell/64), R), block = 256.
-template <int KIND>
+// LT > 0 fixes the lattice size at compile time (index arithmetic by
+// constants); LT = 0 reads it from L.
+template <int KIND, int LT>
 __global__ __launch_bounds__(kPhaseBlock) void k_phase(
-    float *__restrict__ spins, int L, int ncell, int colour, int pass,
+    float *__restrict__ spins, int L_rt, int ncell_rt, int colour, int pass,
     const float *__restrict__ beta, const float *__restrict__ cosc,
     const uint32_t *__restrict__ ctl, uint32_t key0, uint32_t key1, float jeff,
     unsigned long long *__restrict__ acc, double *__restrict__ dE) {
+  const int L = LT > 0 ? LT : L_rt;
+  const int ncell = LT > 0 ? LT * LT * LT : ncell_rt;
   const int r = blockIdx.y;
   const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -62,7 +66,7 @@ __global__ __launch_bounds__(kPhaseBlock) void k_phase(
       const int x1 = wrap(x + lk.dx, L);
       const int y1 = wrap(y + lk.dy, L);
       const int z1 = wrap(z + lk.dz, L);
-      const size_t j = (size_t)lk.site * ncell + (size_t)((z1 * L + y1) * L + x1);
+      const int j = lk.site * ncell + (z1 * L + y1) * L + x1;
       const float a = Sx[j], b = Sy[j], c = Sz[j];
       fx = fx + a;
       fy = fy + b;
@@ -71,7 +75,7 @@ __global__ __launch_bounds__(kPhaseBlock) void k_phase(
       gy += (double)b;
       gz += (double)c;
     }
-    const size_t i = (size_t)site * ncell + cell;
+    const int i = site * ncell + cell;
     const f3 s = {Sx[i], Sy[i], Sz[i]};
     f3 sp;
     bool write;
@@ -313,20 +317,37 @@ int launch_init_spins(pyro_ctx *ctx) {
   return 0;
 }
 
-int launch_metropolis_phase(pyro_ctx *ctx, int colour, int pass, const uint32_t *) {
+template <int KIND, int LT>
+static void launch_phase_t(pyro_ctx *ctx, int colour, int pass) {
   dim3 grid((ctx->ncell + kPhaseCells - 1) / kPhaseCells, ctx->R);
-  k_phase<0><<<grid, kPhaseBlock, 0, ctx->stream>>>(
+  k_phase<KIND, LT><<<grid, kPhaseBlock, 0, ctx->stream>>>(
       ctx->d_spins, ctx->L, ctx->ncell, colour, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl,
       key_lo(ctx), key_hi(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE);
+}
+
+// Lattice sizes of the benchmark configurations get a compile-time copy.
+template <int KIND>
+static void launch_phase(pyro_ctx *ctx, int colour, int pass) {
+  switch (ctx->L) {
+    case 4: launch_phase_t<KIND, 4>(ctx, colour, pass); break;
+    case 8: launch_phase_t<KIND, 8>(ctx, colour, pass); break;
+    case 12: launch_phase_t<KIND, 12>(ctx, colour, pass); break;
+    case 16: launch_phase_t<KIND, 16>(ctx, colour, pass); break;
+    case 24: launch_phase_t<KIND, 24>(ctx, colour, pass); break;
+    case 32: launch_phase_t<KIND, 32>(ctx, colour, pass); break;
+    case 64: launch_phase_t<KIND, 64>(ctx, colour, pass); break;
+    default: launch_phase_t<KIND, 0>(ctx, colour, pass); break;
+  }
+}
+
+int launch_metropolis_phase(pyro_ctx *ctx, int colour, int pass, const uint32_t *) {
+  launch_phase<0>(ctx, colour, pass);
   PYRO_HIP(hipGetLastError());
   return 0;
 }
 
 int launch_overrelax_phase(pyro_ctx *ctx, int colour) {
-  dim3 grid((ctx->ncell + kPhaseCells - 1) / kPhaseCells, ctx->R);
-  k_phase<1><<<grid, kPhaseBlock, 0, ctx->stream>>>(
-      ctx->d_spins, ctx->L, ctx->ncell, colour, 0, ctx->d_beta, ctx->d_cosc, ctx->d_ctl,
-      key_lo(ctx), key_hi(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE);
+  launch_phase<1>(ctx, colour, 0);
   PYRO_HIP(hipGetLastError());
   return 0;
 }

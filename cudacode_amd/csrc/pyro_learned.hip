@@ -1,0 +1,855 @@
+// Learned local proposal: a 6 -> 64 -> 64 -> 4 ReLU network maps the local
+// environment (u = beta*h, s) to a von Mises-Fisher proposal vMF(mu, kappa)
+// for the new spin; an exact Metropolis-Hastings test keeps the Boltzmann
+// distribution stationary. No reference counterpart (the reference's moves are
+// Pyrochlore.cu:829-885 and 975-1007); this file is the build's addition.
+//
+// Inference runs on BF16 MFMA with FP32 accumulation. The network is evaluated
+// transposed, D = W * X^T (rows = units, columns = the 64 samples of a wave),
+// with v_mfma_f32_32x32x8_bf16: the accumulator layout of one layer (lane l
+// holds column l%32, rows 8b + 4(l/32) + r) is exactly the B-operand layout
+// of the next layer's 8-wide k-steps, so activations never leave registers.
+// Training (forward, backward, Adam) runs in FP32 on the vector units.
+#include <cmath>
+#include <cstring>
+
+#include "pyro_internal.h"
+
+namespace pyro {
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+enum : uint32_t { kKindTrain = 6u, kKindEnv = 7u };
+
+__constant__ Link c_links_learned[16][6] = PYRO_LINKS16_INIT;
+
+__host__ __device__ __forceinline__ uint32_t bf16_bits(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return u >> 16;
+}
+
+__device__ __forceinline__ uint32_t pack2(float lo, float hi) {
+  return bf16_bits(lo) | (bf16_bits(hi) << 16);
+}
+
+__device__ __forceinline__ s16x4 frag(uint32_t lo, uint32_t hi) {
+  u32x2 v = {lo, hi};
+  return __builtin_bit_cast(s16x4, v);
+}
+
+__device__ __forceinline__ float relu(float x) { return fmaxf(x, 0.0f); }
+
+// Weight fragments held in registers by every wave.
+struct MlpW {
+  s16x4 a1[2];      // [row tile]
+  s16x4 a2[2][8];   // [row tile][k step]
+  s16x4 a3[8];      // [k step]
+  float b2[2][16];  // bias of the rows this lane's accumulators hold
+  float b3[4];
+};
+
+__device__ __forceinline__ s16x4 load_frag(const uint16_t *__restrict__ p) {
+  const u32x2 v = *reinterpret_cast<const u32x2 *>(p);
+  return __builtin_bit_cast(s16x4, v);
+}
+
+__device__ __forceinline__ void mlp_load(const uint16_t *__restrict__ pack,
+                                         const float *__restrict__ w, MlpW &W) {
+  const int lane = threadIdx.x & 63, h = lane >> 5, n = lane & 31;
+#pragma unroll
+  for (int tj = 0; tj < 2; ++tj) W.a1[tj] = load_frag(pack + kPackP1 + (32 * tj + n) * 8 + 4 * h);
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      W.a2[ti][t] = load_frag(pack + kPackP2 + (32 * ti + n) * 64 + 8 * t + 4 * h);
+#pragma unroll
+  for (int t = 0; t < 8; ++t) W.a3[t] = load_frag(pack + kPackP3 + n * 64 + 8 * t + 4 * h);
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) W.b2[ti][v] = w[kOffB2 + 32 * ti + 8 * (v >> 2) + 4 * h + (v & 3)];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) W.b3[q] = w[kOffB3 + q];
+}
+
+// Forward pass of the 64 samples of a wave (one per lane). Every lane of the
+// wave must call it. x: this lane's 6 features; o: this lane's 4 outputs.
+__device__ __forceinline__ void mlp_forward(const MlpW &W, const float x[6], float o[4]) {
+  const int lane = threadIdx.x & 63, h = lane >> 5, n = lane & 31;
+  const uint32_t p0 = pack2(x[0], x[1]), p1 = pack2(x[2], x[3]), p2 = pack2(x[4], x[5]);
+  const uint32_t p3 = 0x3F80u;  // bf16(1.0) low, 0 high: the bias column
+  s16x4 b1[2];
+#pragma unroll
+  for (int tn = 0; tn < 2; ++tn) {
+    const int src = 32 * tn + n;
+    const uint32_t q0 = __shfl(p0, src), q1 = __shfl(p1, src), q2 = __shfl(p2, src);
+    b1[tn] = h ? frag(q2, p3) : frag(q0, q1);
+  }
+  const f32x16 zero = {};
+  s16x4 b2[2][8];
+#pragma unroll
+  for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+    for (int tn = 0; tn < 2; ++tn) {
+      const f32x16 d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W.a1[tj], b1[tn], zero, 0, 0, 0);
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        b2[tn][4 * tj + b] = frag(pack2(relu(d[4 * b]), relu(d[4 * b + 1])),
+                                  pack2(relu(d[4 * b + 2]), relu(d[4 * b + 3])));
+    }
+  s16x4 b3[2][8];
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tn = 0; tn < 2; ++tn) {
+      f32x16 acc = zero;
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W.a2[ti][t], b2[tn][t], acc, 0, 0, 0);
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        b3[tn][4 * ti + b] =
+            frag(pack2(relu(acc[4 * b] + W.b2[ti][4 * b]), relu(acc[4 * b + 1] + W.b2[ti][4 * b + 1])),
+                 pack2(relu(acc[4 * b + 2] + W.b2[ti][4 * b + 2]),
+                       relu(acc[4 * b + 3] + W.b2[ti][4 * b + 3])));
+    }
+  f32x16 d3[2];
+#pragma unroll
+  for (int tn = 0; tn < 2; ++tn) {
+    f32x16 acc = zero;
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W.a3[t], b3[tn][t], acc, 0, 0, 0);
+    d3[tn] = acc;
+  }
+  // Rows 0..3 of column n sit in lane n (upper half zero-padded rows elsewhere).
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float other = __shfl(d3[1][q], n);
+    o[q] = (h ? other : d3[0][q]) + W.b3[q];
+  }
+}
+
+__device__ __forceinline__ void features(f3 u, f3 s, float x[6]) {
+  const float r = sqrtf(u.x * u.x + u.y * u.y + u.z * u.z);
+  const float g = r > 0.0f ? log1pf(r) / r : 1.0f;
+  x[0] = u.x * g;
+  x[1] = u.y * g;
+  x[2] = u.z * g;
+  x[3] = s.x;
+  x[4] = s.y;
+  x[5] = s.z;
+}
+
+struct Vmf { f3 mu; float kappa, log_norm; };
+
+__device__ __forceinline__ Vmf decode(const float o[4]) {
+  Vmf v;
+  const float nn = sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+  if (nn > 0.0f) {
+    v.mu = {o[0] / nn, o[1] / nn, o[2] / nn};
+  } else {
+    v.mu = {0.0f, 0.0f, 1.0f};
+  }
+  const float lk = fminf(fmaxf(o[3], kLogKappaMin), kLogKappaMax);
+  v.kappa = expf(lk);
+  // log of kappa / (2 pi (1 - exp(-2 kappa))); the exp(-kappa) factor of the
+  // normaliser is carried by the -kappa |mu - y|^2 / 2 term of the density.
+  v.log_norm = lk - 1.83787706641f - logf(-expm1f(-2.0f * v.kappa));
+  return v;
+}
+
+// log q(y) = log C'(kappa) - kappa |mu - y|^2 / 2: the vMF log density written
+// so that y near mu loses no precision.
+__device__ __forceinline__ float vmf_log_density(const Vmf &v, f3 y) {
+  const float dx = v.mu.x - y.x, dy = v.mu.y - y.y, dz = v.mu.z - y.z;
+  return v.log_norm - 0.5f * v.kappa * (dx * dx + dy * dy + dz * dz);
+}
+
+// vMF draw: cosine to the axis by the inverse CDF (Wood 1994), uniform
+// azimuth, carried onto mu by the Householder reflection through the pole on
+// mu's side.
+__device__ __forceinline__ f3 vmf_sample(const Vmf &v, float u1, float u2) {
+  const float k = v.kappa;
+  const float p = 1.0f - u1;  // (0, 1]
+  float w = 1.0f + log1pf((1.0f - p) * expm1f(-2.0f * k)) / k;
+  w = fminf(1.0f, fmaxf(-1.0f, w));
+  const float st = sqrtf(fmaxf(0.0f, 1.0f - w * w));
+  float ps, pc;
+  sincospi(2.0f * u2, &ps, &pc);
+  const f3 m = v.mu;
+  const bool up = m.z >= 0.0f;
+  const f3 r = {st * pc, st * ps, up ? w : -w};
+  const f3 N = {m.x, m.y, up ? m.z + 1.0f : m.z - 1.0f};
+  const float d = up ? 1.0f + m.z : 1.0f - m.z;
+  const float scal = (N.x * r.x + N.y * r.y + N.z * r.z) / d;
+  f3 o = {N.x * scal - r.x, N.y * scal - r.y, N.z * scal - r.z};
+  return normalise(o);
+}
+
+// log of the Metropolis-Hastings ratio for s -> sp at fixed local field h:
+// beta h.(sp - s) + log q(s | u, sp) - log q(sp | u, s).
+__device__ __forceinline__ float mh_log_ratio(f3 h, float beta, f3 s, f3 sp, const Vmf &fwd,
+                                              const Vmf &rev) {
+  const float de = (h.x * sp.x + h.y * sp.y + h.z * sp.z) - (h.x * s.x + h.y * s.y + h.z * s.z);
+  return beta * de + vmf_log_density(rev, s) - vmf_log_density(fwd, sp);
+}
+
+__device__ __forceinline__ int wrap_l(int v, int L) { return v < 0 ? v + L : (v >= L ? v - L : v); }
+
+// One colour phase of learned proposals on every replica; same geometry as
+// the Metropolis phase (block = 4 waves = the 4 sites of the colour, lane =
+// cell). grid = (ceil(ncell/64), R).
+template <int LT>
+__global__ __launch_bounds__(kPhaseBlock) void k_learned_phase(
+    float *__restrict__ spins, int L_rt, int ncell_rt, int colour, int pass,
+    const float *__restrict__ beta, const uint32_t *__restrict__ ctl, uint32_t key0,
+    uint32_t key1, float jeff, const uint16_t *__restrict__ pack, const float *__restrict__ w,
+    unsigned long long *__restrict__ acc, double *__restrict__ dE) {
+  const int L = LT > 0 ? LT : L_rt;
+  const int ncell = LT > 0 ? LT * LT * LT : ncell_rt;
+  const int r = blockIdx.y;
+  const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int cell = blockIdx.x * kPhaseCells + lane;
+  const bool valid = cell < ncell;
+  const int site = colour + 4 * k;
+  const size_t plane = (size_t)16 * ncell;
+  float *Sx = spins + (size_t)r * 3 * plane;
+  float *Sy = Sx + plane;
+  float *Sz = Sy + plane;
+
+  MlpW W;
+  mlp_load(pack, w, W);
+
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  f3 s = {0.0f, 0.0f, 1.0f};
+  int i = 0;
+  if (valid) {
+    const int LL = L * L;
+    const int z = cell / LL;
+    const int rem = cell - z * LL;
+    const int y = rem / L;
+    const int x = rem - y * L;
+#pragma unroll
+    for (int nb = 0; nb < 6; ++nb) {
+      const Link lk = c_links_learned[site][nb];
+      const int j = lk.site * ncell + (wrap_l(z + lk.dz, L) * L + wrap_l(y + lk.dy, L)) * L +
+                    wrap_l(x + lk.dx, L);
+      const float a = Sx[j], b = Sy[j], c = Sz[j];
+      fx = fx + a;
+      fy = fy + b;
+      fz = fz + c;
+      gx += (double)a;
+      gy += (double)b;
+      gz += (double)c;
+    }
+    i = site * ncell + cell;
+    s = {Sx[i], Sy[i], Sz[i]};
+  }
+  const float bt = beta[r];
+  const f3 hf = {-jeff * fx, -jeff * fy, -jeff * fz};
+  const f3 u = {bt * hf.x, bt * hf.y, bt * hf.z};
+  float x[6], o[4];
+  features(u, s, x);
+  mlp_forward(W, x, o);
+  const Vmf fwd = decode(o);
+  const u32x4 rnd = philox4x32_10(
+      rng_counter((uint32_t)cell, (uint32_t)site, (uint32_t)pass, kKindLearned, (uint32_t)r,
+                  ctl[kCtlSweep]),
+      key0, key1);
+  const f3 sp = vmf_sample(fwd, u01(rnd.x), u01(rnd.y));
+  features(u, sp, x);
+  mlp_forward(W, x, o);
+  const Vmf rev = decode(o);
+  const float la = mh_log_ratio(hf, bt, s, sp, fwd, rev);
+  const bool accept = valid && (u01(rnd.z) < expf(fminf(la, 0.0f)));
+  double de = 0.0;
+  if (accept) {
+    de = (double)jeff * (((double)sp.x - (double)s.x) * gx + ((double)sp.y - (double)s.y) * gy +
+                         ((double)sp.z - (double)s.z) * gz);
+    Sx[i] = sp.x;
+    Sy[i] = sp.y;
+    Sz[i] = sp.z;
+  }
+  __shared__ double s_de[4];
+  __shared__ unsigned s_acc[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) de += __shfl_xor(de, off, 64);
+  const unsigned nacc = (unsigned)__popcll(__ballot(accept));
+  if (lane == 0) {
+    s_de[k] = de;
+    s_acc[k] = nacc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double t = (s_de[0] + s_de[1]) + (s_de[2] + s_de[3]);
+    if (t != 0.0) atomicAdd(&dE[r], t);
+    const unsigned a = s_acc[0] + s_acc[1] + s_acc[2] + s_acc[3];
+    if (a) atomicAdd(&acc[r], (unsigned long long)a);
+  }
+}
+
+// Batched evaluation through the same device code as the phase kernel.
+// rows: n x 9 floats (u, s, sp). out: n x 9 floats
+// (o(s) [4], o(sp) [4], log MH ratio at h = u, beta = 1). grid = ceil(n/64), block = 64.
+__global__ __launch_bounds__(64) void k_mlp_eval(const float *__restrict__ rows, int n,
+                                                 const uint16_t *__restrict__ pack,
+                                                 const float *__restrict__ w,
+                                                 float *__restrict__ out) {
+  const int idx = blockIdx.x * 64 + threadIdx.x;
+  const bool valid = idx < n;
+  MlpW W;
+  mlp_load(pack, w, W);
+  f3 u = {0.0f, 0.0f, 0.0f}, s = {0.0f, 0.0f, 1.0f}, sp = {0.0f, 0.0f, 1.0f};
+  if (valid) {
+    const float *q = rows + (size_t)idx * 9;
+    u = {q[0], q[1], q[2]};
+    s = {q[3], q[4], q[5]};
+    sp = {q[6], q[7], q[8]};
+  }
+  float x[6], o1[4], o2[4];
+  features(u, s, x);
+  mlp_forward(W, x, o1);
+  features(u, sp, x);
+  mlp_forward(W, x, o2);
+  const Vmf fwd = decode(o1), rev = decode(o2);
+  const float la = mh_log_ratio(u, 1.0f, s, sp, fwd, rev);
+  if (valid) {
+    float *p = out + (size_t)idx * 9;
+    for (int q = 0; q < 4; ++q) {
+      p[q] = o1[q];
+      p[4 + q] = o2[q];
+    }
+    p[8] = la;
+  }
+}
+
+// vMF draws through the device sampler (for the statistical tests):
+// rows n x 4 (mu, kappa) -> out n x 3, uniforms from Philox(key, idx).
+__global__ void k_vmf_draw(const float *__restrict__ rows, int n, uint32_t key0, uint32_t key1,
+                           float *__restrict__ out) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const float *q = rows + (size_t)idx * 4;
+  Vmf v;
+  v.mu = {q[0], q[1], q[2]};
+  v.kappa = q[3];
+  v.log_norm = 0.0f;
+  const u32x4 rnd = philox4x32_10(rng_counter((uint32_t)idx, 0, 0, kKindLearned, 0, 0), key0, key1);
+  const f3 y = vmf_sample(v, u01(rnd.x), u01(rnd.y));
+  out[(size_t)idx * 3 + 0] = y.x;
+  out[(size_t)idx * 3 + 1] = y.y;
+  out[(size_t)idx * 3 + 2] = y.z;
+}
+
+// Environments (u = beta h, s) at Philox-chosen (replica, cell, site) of the
+// current configuration, appended at row offset `base`. grid = ceil(n/256).
+__global__ void k_gather_env(const float *__restrict__ spins, int L, int ncell, int R,
+                             int per_replica, uint32_t draw, const float *__restrict__ beta,
+                             float jeff, uint32_t key0, uint32_t key1, float *__restrict__ env,
+                             int base, int cap) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= per_replica * R || base + idx >= cap) return;
+  const int r = idx / per_replica;
+  const u32x4 rnd = philox4x32_10(rng_counter((uint32_t)idx, 0, 0, kKindEnv, 0, draw), key0, key1);
+  const int cell = (int)(rnd.x % (uint32_t)ncell);
+  const int site = (int)(rnd.y & 15u);
+  const size_t plane = (size_t)16 * ncell;
+  const float *Sx = spins + (size_t)r * 3 * plane;
+  const float *Sy = Sx + plane;
+  const float *Sz = Sy + plane;
+  const int LL = L * L;
+  const int z = cell / LL, rem = cell - z * LL, y = rem / L, x = rem - y * L;
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  for (int nb = 0; nb < 6; ++nb) {
+    const Link lk = c_links_learned[site][nb];
+    const int j = lk.site * ncell + (wrap_l(z + lk.dz, L) * L + wrap_l(y + lk.dy, L)) * L +
+                  wrap_l(x + lk.dx, L);
+    fx = fx + Sx[j];
+    fy = fy + Sy[j];
+    fz = fz + Sz[j];
+  }
+  const int i = site * ncell + cell;
+  const float bt = beta[r];
+  float *e = env + (size_t)(base + idx) * 6;
+  e[0] = bt * (-jeff * fx);
+  e[1] = bt * (-jeff * fy);
+  e[2] = bt * (-jeff * fz);
+  e[3] = Sx[i];
+  e[4] = Sy[i];
+  e[5] = Sz[i];
+}
+
+// ---------------------------------------------------------------------------
+// Training, FP32 on the vector units.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double langevin(double k) {
+  if (k < 1e-3) return k / 3.0 - k * k * k / 45.0;
+  return 1.0 / tanh(k) - 1.0 / k;
+}
+__device__ __forceinline__ double langevin_d(double k) {
+  if (k < 1e-3) return 1.0 / 3.0 - k * k / 15.0;
+  if (k > 350.0) return 1.0 / (k * k);
+  const double sh = sinh(k);
+  return 1.0 / (k * k) - 1.0 / (sh * sh);
+}
+__device__ __forceinline__ double log_norm_d(double k) {
+  if (k < 1e-6) return -log(4.0 * M_PI);
+  return log(k) - log(2.0 * M_PI) - k - log(-expm1(-2.0 * k));
+}
+
+// Per-sample forward, loss and backward. Sample n of the batch reads env row
+// `rowsel` (explicit list) or a Philox-chosen row. Writes the training rows
+// [unit][batch] consumed by k_train_reduce. grid = ceil(B/64), block = 64.
+__global__ __launch_bounds__(64) void k_train_samples(const float *__restrict__ env, int n_env,
+                                                      const int *__restrict__ rowsel, int B,
+                                                      uint32_t step, uint32_t key0, uint32_t key1,
+                                                      const float *__restrict__ w,
+                                                      float *__restrict__ T) {
+  const int n = blockIdx.x * 64 + threadIdx.x;
+  if (n >= B) return;
+  int row;
+  if (rowsel) {
+    row = rowsel[n];
+  } else {
+    const u32x4 rnd = philox4x32_10(rng_counter((uint32_t)n, 0, 0, kKindTrain, 0, step), key0, key1);
+    row = (int)(rnd.x % (uint32_t)n_env);
+  }
+  const float *e = env + (size_t)row * 6;
+  const f3 u = {e[0], e[1], e[2]};
+  const f3 s = {e[3], e[4], e[5]};
+  float x[6];
+  features(u, s, x);
+  float *Tx = T, *Ta1 = T + 6 * (size_t)B, *Ta2 = Ta1 + 64 * (size_t)B, *Td3 = Ta2 + 64 * (size_t)B,
+        *Td2 = Td3 + 4 * (size_t)B, *Td1 = Td2 + 64 * (size_t)B, *Tl = Td1 + 64 * (size_t)B;
+  for (int k = 0; k < 6; ++k) Tx[(size_t)k * B + n] = x[k];
+  float a1[64], a2[64];
+  for (int j = 0; j < 64; ++j) {
+    float t = w[kOffB1 + j];
+    for (int k = 0; k < 6; ++k) t += w[kOffW1 + j * 6 + k] * x[k];
+    a1[j] = relu(t);
+    Ta1[(size_t)j * B + n] = a1[j];
+  }
+  for (int j = 0; j < 64; ++j) {
+    float t = w[kOffB2 + j];
+    for (int k = 0; k < 64; ++k) t += w[kOffW2 + j * 64 + k] * a1[k];
+    a2[j] = relu(t);
+    Ta2[(size_t)j * B + n] = a2[j];
+  }
+  float o[4];
+  for (int q = 0; q < 4; ++q) {
+    float t = w[kOffB3 + q];
+    for (int k = 0; k < 64; ++k) t += w[kOffW3 + q * 64 + k] * a2[k];
+    o[q] = t;
+  }
+  // Reverse KL to vMF(u/|u|, |u|) and its gradient wrt o, in FP64.
+  const double on = sqrt((double)o[0] * o[0] + (double)o[1] * o[1] + (double)o[2] * o[2]);
+  double mu[3] = {0.0, 0.0, 1.0};
+  if (on > 0.0) {
+    mu[0] = o[0] / on;
+    mu[1] = o[1] / on;
+    mu[2] = o[2] / on;
+  }
+  double lk = o[3];
+  const bool clamped = lk < (double)kLogKappaMin || lk > (double)kLogKappaMax;
+  lk = fmin(fmax(lk, (double)kLogKappaMin), (double)kLogKappaMax);
+  const double kap = exp(lk);
+  const double ks = sqrt((double)u.x * u.x + (double)u.y * u.y + (double)u.z * u.z);
+  double hat[3] = {0.0, 0.0, 0.0};
+  if (ks > 0.0) {
+    hat[0] = u.x / ks;
+    hat[1] = u.y / ks;
+    hat[2] = u.z / ks;
+  }
+  const double c = mu[0] * hat[0] + mu[1] * hat[1] + mu[2] * hat[2];
+  const double A = langevin(kap);
+  const double kl = log_norm_d(kap) - log_norm_d(ks) + A * (kap - ks * c);
+  float go[4];
+  go[3] = clamped ? 0.0f : (float)(langevin_d(kap) * (kap - ks * c) * kap);
+  if (on > 0.0) {
+    const double gm[3] = {-A * ks * hat[0], -A * ks * hat[1], -A * ks * hat[2]};
+    const double proj = gm[0] * mu[0] + gm[1] * mu[1] + gm[2] * mu[2];
+    for (int q = 0; q < 3; ++q) go[q] = (float)((gm[q] - proj * mu[q]) / on);
+  } else {
+    go[0] = go[1] = go[2] = 0.0f;
+  }
+  Tl[n] = (float)kl;
+  for (int q = 0; q < 4; ++q) Td3[(size_t)q * B + n] = go[q];
+  // d2 in place of a2, then d1 in place of a1 (a1 is already stored).
+  for (int k = 0; k < 64; ++k) {
+    float t = 0.0f;
+    for (int q = 0; q < 4; ++q) t += w[kOffW3 + q * 64 + k] * go[q];
+    a2[k] = a2[k] > 0.0f ? t : 0.0f;
+    Td2[(size_t)k * B + n] = a2[k];
+  }
+  for (int k = 0; k < 64; ++k) {
+    float t = 0.0f;
+    for (int j = 0; j < 64; ++j) t += w[kOffW2 + j * 64 + k] * a2[j];
+    a1[k] = a1[k] > 0.0f ? t : 0.0f;
+    Td1[(size_t)k * B + n] = a1[k];
+  }
+}
+
+// Batch-mean gradient, one parameter per thread, fixed summation order
+// (FP64 accumulation). grad[kMlpParams] receives the mean loss.
+__global__ void k_train_reduce(const float *__restrict__ T, int B, float *__restrict__ grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > kMlpParams) return;
+  const float *Tx = T, *Ta1 = T + 6 * (size_t)B, *Ta2 = Ta1 + 64 * (size_t)B,
+              *Td3 = Ta2 + 64 * (size_t)B, *Td2 = Td3 + 4 * (size_t)B, *Td1 = Td2 + 64 * (size_t)B,
+              *Tl = Td1 + 64 * (size_t)B;
+  const float *d = nullptr, *a = nullptr;
+  if (p < kOffB1) {
+    d = Td1 + (size_t)(p / 6) * B;
+    a = Tx + (size_t)(p % 6) * B;
+  } else if (p < kOffW2) {
+    d = Td1 + (size_t)(p - kOffB1) * B;
+  } else if (p < kOffB2) {
+    const int q = p - kOffW2;
+    d = Td2 + (size_t)(q / 64) * B;
+    a = Ta1 + (size_t)(q % 64) * B;
+  } else if (p < kOffW3) {
+    d = Td2 + (size_t)(p - kOffB2) * B;
+  } else if (p < kOffB3) {
+    const int q = p - kOffW3;
+    d = Td3 + (size_t)(q / 64) * B;
+    a = Ta2 + (size_t)(q % 64) * B;
+  } else if (p < kMlpParams) {
+    d = Td3 + (size_t)(p - kOffB3) * B;
+  } else {
+    d = Tl;
+  }
+  double t = 0.0;
+  if (a) {
+    for (int n = 0; n < B; ++n) t += (double)d[n] * (double)a[n];
+  } else {
+    for (int n = 0; n < B; ++n) t += (double)d[n];
+  }
+  grad[p] = (float)(t / B);
+}
+
+__global__ void k_adam(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
+                       const float *__restrict__ g, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kMlpParams) return;
+  const float gp = g[p];
+  const float mp = b1 * m[p] + (1.0f - b1) * gp;
+  const float vp = b2 * v[p] + (1.0f - b2) * gp * gp;
+  m[p] = mp;
+  v[p] = vp;
+  w[p] -= lr * (mp / c1) / (sqrtf(vp / c2) + eps);
+}
+
+__global__ void k_mlp_pack(const float *__restrict__ w, uint16_t *__restrict__ pack) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kPackSize) return;
+  float val = 0.0f;
+  if (p < kPackP2) {
+    const int row = p / 8, col = p % 8;
+    val = col < 6 ? w[kOffW1 + row * 6 + col] : (col == 6 ? w[kOffB1 + row] : 0.0f);
+  } else if (p < kPackP3) {
+    val = w[kOffW2 + (p - kPackP2)];
+  } else {
+    const int q = p - kPackP3, row = q / 64, col = q % 64;
+    val = row < 4 ? w[kOffW3 + row * 64 + col] : 0.0f;
+  }
+  pack[p] = (uint16_t)bf16_bits(val);
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+static inline uint32_t klo(const pyro_ctx *c) { return (uint32_t)c->seed; }
+static inline uint32_t khi(const pyro_ctx *c) { return (uint32_t)(c->seed >> 32); }
+
+int mlp_alloc(pyro_ctx *ctx) {
+  if (ctx->d_mlp_w) return 0;
+  const size_t nb = sizeof(float) * kMlpParams;
+  if (hipMalloc((void **)&ctx->d_mlp_w, nb) != hipSuccess ||
+      hipMalloc((void **)&ctx->d_mlp_m, nb) != hipSuccess ||
+      hipMalloc((void **)&ctx->d_mlp_v, nb) != hipSuccess ||
+      hipMalloc((void **)&ctx->d_grad, nb + sizeof(float)) != hipSuccess ||
+      hipMalloc((void **)&ctx->d_mlp_pack, sizeof(uint16_t) * kPackSize) != hipSuccess ||
+      hipMalloc((void **)&ctx->d_acc_learned, sizeof(unsigned long long) * ctx->R) != hipSuccess)
+    return set_error(PYRO_ENOMEM, "learned proposal: device allocation failed");
+  PYRO_HIP(hipMemset(ctx->d_mlp_w, 0, nb));
+  PYRO_HIP(hipMemset(ctx->d_mlp_m, 0, nb));
+  PYRO_HIP(hipMemset(ctx->d_mlp_v, 0, nb));
+  PYRO_HIP(hipMemset(ctx->d_acc_learned, 0, sizeof(unsigned long long) * ctx->R));
+  ctx->h_proposed_learned.assign(ctx->R, 0);
+  return 0;
+}
+
+void mlp_free(pyro_ctx *ctx) {
+  void *ptrs[] = {ctx->d_mlp_w, ctx->d_mlp_m, ctx->d_mlp_v, ctx->d_grad, ctx->d_mlp_pack,
+                  ctx->d_env, ctx->d_train, ctx->d_acc_learned};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  ctx->d_mlp_w = ctx->d_mlp_m = ctx->d_mlp_v = ctx->d_grad = ctx->d_env = ctx->d_train = nullptr;
+  ctx->d_mlp_pack = nullptr;
+  ctx->d_acc_learned = nullptr;
+}
+
+int mlp_pack(pyro_ctx *ctx) {
+  k_mlp_pack<<<(kPackSize + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_mlp_w, ctx->d_mlp_pack);
+  PYRO_HIP(hipGetLastError());
+  ctx->mlp_ready = true;
+  return 0;
+}
+
+template <int LT>
+static void launch_learned_t(pyro_ctx *ctx, int colour, int pass) {
+  dim3 grid((ctx->ncell + kPhaseCells - 1) / kPhaseCells, ctx->R);
+  k_learned_phase<LT><<<grid, kPhaseBlock, 0, ctx->stream>>>(
+      ctx->d_spins, ctx->L, ctx->ncell, colour, pass, ctx->d_beta, ctx->d_ctl, klo(ctx), khi(ctx),
+      ctx->jeff, ctx->d_mlp_pack, ctx->d_mlp_w, ctx->d_acc_learned, ctx->d_dE);
+}
+
+int launch_learned_phase(pyro_ctx *ctx, int colour, int pass) {
+  if (!ctx->mlp_ready) return set_error(PYRO_ESTATE, "learned proposal: no weights set");
+  switch (ctx->L) {
+    case 8: launch_learned_t<8>(ctx, colour, pass); break;
+    case 16: launch_learned_t<16>(ctx, colour, pass); break;
+    case 64: launch_learned_t<64>(ctx, colour, pass); break;
+    default: launch_learned_t<0>(ctx, colour, pass); break;
+  }
+  PYRO_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace pyro
+
+using namespace pyro;
+
+extern "C" {
+
+int pyro_mlp_set_weights(pyro_ctx *ctx, const float *w) {
+  if (!ctx || !w) return set_error(PYRO_EINVAL, "pyro_mlp_set_weights: null argument");
+  for (int p = 0; p < kMlpParams; ++p)
+    if (!std::isfinite(w[p])) return set_error(PYRO_EINVAL, "pyro_mlp_set_weights: w[%d] not finite", p);
+  PYRO_HIP(hipSetDevice(ctx->device));
+  PYRO_TRY(mlp_alloc(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  PYRO_HIP(hipMemcpy(ctx->d_mlp_w, w, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+  PYRO_HIP(hipMemset(ctx->d_mlp_m, 0, sizeof(float) * kMlpParams));
+  PYRO_HIP(hipMemset(ctx->d_mlp_v, 0, sizeof(float) * kMlpParams));
+  ctx->adam_t = 0;
+  PYRO_TRY(mlp_pack(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int pyro_mlp_get_weights(pyro_ctx *ctx, float *w) {
+  if (!ctx || !w) return set_error(PYRO_EINVAL, "pyro_mlp_get_weights: null argument");
+  if (!ctx->d_mlp_w) return set_error(PYRO_ESTATE, "pyro_mlp_get_weights: no weights");
+  PYRO_HIP(hipSetDevice(ctx->device));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  PYRO_HIP(hipMemcpy(w, ctx->d_mlp_w, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pyro_mlp_eval(pyro_ctx *ctx, const float *rows9, int n, float *out9) {
+  if (!ctx || !rows9 || !out9 || n < 1) return set_error(PYRO_EINVAL, "pyro_mlp_eval: bad argument");
+  if (!ctx->mlp_ready) return set_error(PYRO_ESTATE, "pyro_mlp_eval: no weights set");
+  PYRO_HIP(hipSetDevice(ctx->device));
+  float *d_in = nullptr, *d_out = nullptr;
+  const size_t nb = sizeof(float) * 9 * (size_t)n;
+  if (hipMalloc((void **)&d_in, nb) != hipSuccess || hipMalloc((void **)&d_out, nb) != hipSuccess) {
+    if (d_in) (void)hipFree(d_in);
+    return set_error(PYRO_ENOMEM, "pyro_mlp_eval: allocation failed");
+  }
+  int rc = 0;
+  if (hipMemcpyAsync(d_in, rows9, nb, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    rc = set_error(PYRO_EHIP, "pyro_mlp_eval: upload failed");
+  if (!rc) {
+    k_mlp_eval<<<(n + 63) / 64, 64, 0, ctx->stream>>>(d_in, n, ctx->d_mlp_pack, ctx->d_mlp_w, d_out);
+    if (hipGetLastError() != hipSuccess) rc = set_error(PYRO_EHIP, "pyro_mlp_eval: launch failed");
+  }
+  if (!rc && hipMemcpyAsync(out9, d_out, nb, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+    rc = set_error(PYRO_EHIP, "pyro_mlp_eval: download failed");
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess)
+    rc = set_error(PYRO_EHIP, "pyro_mlp_eval: synchronise failed");
+  (void)hipFree(d_in);
+  (void)hipFree(d_out);
+  return rc;
+}
+
+int pyro_vmf_draw(pyro_ctx *ctx, const float *rows4, int n, float *out3) {
+  if (!ctx || !rows4 || !out3 || n < 1) return set_error(PYRO_EINVAL, "pyro_vmf_draw: bad argument");
+  PYRO_HIP(hipSetDevice(ctx->device));
+  float *d_in = nullptr, *d_out = nullptr;
+  if (hipMalloc((void **)&d_in, sizeof(float) * 4 * (size_t)n) != hipSuccess ||
+      hipMalloc((void **)&d_out, sizeof(float) * 3 * (size_t)n) != hipSuccess) {
+    if (d_in) (void)hipFree(d_in);
+    return set_error(PYRO_ENOMEM, "pyro_vmf_draw: allocation failed");
+  }
+  int rc = 0;
+  if (hipMemcpyAsync(d_in, rows4, sizeof(float) * 4 * (size_t)n, hipMemcpyHostToDevice, ctx->stream) !=
+      hipSuccess)
+    rc = set_error(PYRO_EHIP, "pyro_vmf_draw: upload failed");
+  if (!rc) {
+    k_vmf_draw<<<(n + 255) / 256, 256, 0, ctx->stream>>>(d_in, n, klo(ctx), khi(ctx), d_out);
+    if (hipGetLastError() != hipSuccess) rc = set_error(PYRO_EHIP, "pyro_vmf_draw: launch failed");
+  }
+  if (!rc && hipMemcpyAsync(out3, d_out, sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToHost,
+                            ctx->stream) != hipSuccess)
+    rc = set_error(PYRO_EHIP, "pyro_vmf_draw: download failed");
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess)
+    rc = set_error(PYRO_EHIP, "pyro_vmf_draw: synchronise failed");
+  (void)hipFree(d_in);
+  (void)hipFree(d_out);
+  return rc;
+}
+
+static int ensure_train(pyro_ctx *ctx, int B) {
+  if (ctx->train_cap >= B) return 0;
+  if (ctx->d_train) PYRO_HIP(hipFree(ctx->d_train));
+  ctx->d_train = nullptr;
+  ctx->train_cap = 0;
+  PYRO_HIP(hipMalloc((void **)&ctx->d_train, sizeof(float) * kTrainRows * (size_t)B));
+  ctx->train_cap = B;
+  return 0;
+}
+
+int pyro_mlp_loss_grad(pyro_ctx *ctx, const float *env6, int n, double *loss, float *grad) {
+  if (!ctx || !env6 || n < 1 || n > (1 << 20))
+    return set_error(PYRO_EINVAL, "pyro_mlp_loss_grad: bad argument");
+  if (!ctx->d_mlp_w) return set_error(PYRO_ESTATE, "pyro_mlp_loss_grad: no weights set");
+  PYRO_HIP(hipSetDevice(ctx->device));
+  PYRO_TRY(ensure_train(ctx, n));
+  float *d_env = nullptr;
+  int *d_sel = nullptr;
+  if (hipMalloc((void **)&d_env, sizeof(float) * 6 * (size_t)n) != hipSuccess ||
+      hipMalloc((void **)&d_sel, sizeof(int) * (size_t)n) != hipSuccess) {
+    if (d_env) (void)hipFree(d_env);
+    return set_error(PYRO_ENOMEM, "pyro_mlp_loss_grad: allocation failed");
+  }
+  std::vector<int> sel(n);
+  for (int i = 0; i < n; ++i) sel[i] = i;
+  std::vector<float> g(kMlpParams + 1);
+  int rc = 0;
+  if (hipMemcpy(d_env, env6, sizeof(float) * 6 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_sel, sel.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
+    rc = set_error(PYRO_EHIP, "pyro_mlp_loss_grad: upload failed");
+  if (!rc) {
+    k_train_samples<<<(n + 63) / 64, 64, 0, ctx->stream>>>(d_env, n, d_sel, n, 0, klo(ctx), khi(ctx),
+                                                          ctx->d_mlp_w, ctx->d_train);
+    k_train_reduce<<<(kMlpParams + 1 + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_train, n, ctx->d_grad);
+    if (hipGetLastError() != hipSuccess) rc = set_error(PYRO_EHIP, "pyro_mlp_loss_grad: launch failed");
+  }
+  if (!rc && (hipStreamSynchronize(ctx->stream) != hipSuccess ||
+              hipMemcpy(g.data(), ctx->d_grad, sizeof(float) * (kMlpParams + 1), hipMemcpyDeviceToHost) !=
+                  hipSuccess))
+    rc = set_error(PYRO_EHIP, "pyro_mlp_loss_grad: download failed");
+  (void)hipFree(d_env);
+  (void)hipFree(d_sel);
+  if (rc) return rc;
+  if (grad) std::memcpy(grad, g.data(), sizeof(float) * kMlpParams);
+  if (loss) *loss = g[kMlpParams];
+  return 0;
+}
+
+int pyro_gather_environments(pyro_ctx *ctx, int per_replica, uint32_t draw, int capacity, int *n_total) {
+  if (!ctx || per_replica < 1 || capacity < 1) return set_error(PYRO_EINVAL, "pyro_gather_environments: bad argument");
+  PYRO_HIP(hipSetDevice(ctx->device));
+  if (capacity > ctx->env_cap) {
+    PYRO_HIP(hipStreamSynchronize(ctx->stream));
+    float *p = nullptr;
+    PYRO_HIP(hipMalloc((void **)&p, sizeof(float) * 6 * (size_t)capacity));
+    if (ctx->d_env && ctx->env_n > 0)
+      PYRO_HIP(hipMemcpy(p, ctx->d_env, sizeof(float) * 6 * (size_t)ctx->env_n, hipMemcpyDeviceToDevice));
+    if (ctx->d_env) PYRO_HIP(hipFree(ctx->d_env));
+    ctx->d_env = p;
+    ctx->env_cap = capacity;
+  }
+  const int want = per_replica * ctx->R;
+  const int take = std::min(want, ctx->env_cap - ctx->env_n);
+  if (take > 0) {
+    k_gather_env<<<(want + 255) / 256, 256, 0, ctx->stream>>>(
+        ctx->d_spins, ctx->L, ctx->ncell, ctx->R, per_replica, draw, ctx->d_beta, ctx->jeff, klo(ctx),
+        khi(ctx), ctx->d_env, ctx->env_n, ctx->env_cap);
+    PYRO_HIP(hipGetLastError());
+    ctx->env_n += take;
+  }
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  if (n_total) *n_total = ctx->env_n;
+  return 0;
+}
+
+int pyro_get_environments(pyro_ctx *ctx, float *env6, int *n) {
+  if (!ctx || !n) return set_error(PYRO_EINVAL, "pyro_get_environments: null argument");
+  PYRO_HIP(hipSetDevice(ctx->device));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  *n = ctx->env_n;
+  if (env6 && ctx->env_n > 0)
+    PYRO_HIP(hipMemcpy(env6, ctx->d_env, sizeof(float) * 6 * (size_t)ctx->env_n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pyro_clear_environments(pyro_ctx *ctx) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  ctx->env_n = 0;
+  return 0;
+}
+
+int pyro_mlp_train(pyro_ctx *ctx, int steps, int batch, float lr, float lr_final, float *loss_hist) {
+  if (!ctx || steps < 0 || batch < 1 || batch > (1 << 20) || !(lr > 0.0f) || !(lr_final > 0.0f))
+    return set_error(PYRO_EINVAL, "pyro_mlp_train: bad argument");
+  if (!ctx->d_mlp_w) return set_error(PYRO_ESTATE, "pyro_mlp_train: no initial weights");
+  if (ctx->env_n < 1) return set_error(PYRO_ESTATE, "pyro_mlp_train: no environments gathered");
+  PYRO_HIP(hipSetDevice(ctx->device));
+  PYRO_TRY(ensure_train(ctx, batch));
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int s = 0; s < steps; ++s) {
+    const int t = ++ctx->adam_t;
+    const double frac = steps > 1 ? (double)s / (steps - 1) : 0.0;
+    const float lr_s = (float)(lr * std::pow((double)lr_final / lr, frac));  // geometric decay
+    k_train_samples<<<(batch + 63) / 64, 64, 0, ctx->stream>>>(ctx->d_env, ctx->env_n, nullptr, batch,
+                                                              (uint32_t)t, klo(ctx), khi(ctx),
+                                                              ctx->d_mlp_w, ctx->d_train);
+    k_train_reduce<<<(kMlpParams + 1 + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_train, batch, ctx->d_grad);
+    k_adam<<<(kMlpParams + 255) / 256, 256, 0, ctx->stream>>>(
+        ctx->d_mlp_w, ctx->d_mlp_m, ctx->d_mlp_v, ctx->d_grad, lr_s, b1, b2, eps,
+        (float)(1.0 - std::pow((double)b1, t)), (float)(1.0 - std::pow((double)b2, t)));
+    PYRO_HIP(hipGetLastError());
+    if (loss_hist)
+      PYRO_HIP(hipMemcpyAsync(loss_hist + s, ctx->d_grad + kMlpParams, sizeof(float),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  }
+  PYRO_TRY(mlp_pack(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int pyro_learned_phase(pyro_ctx *ctx, int colour, uint32_t sweep, int pass) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (colour < 0 || colour > 3) return set_error(PYRO_EINVAL, "colour %d outside 0..3", colour);
+  if (pass < 0 || pass > 255) return set_error(PYRO_EINVAL, "pass %d outside 0..255", pass);
+  PYRO_HIP(hipSetDevice(ctx->device));
+  PYRO_TRY(launch_set_sweep(ctx, sweep, ctx->h_pt_round));
+  PYRO_TRY(launch_learned_phase(ctx, colour, pass));
+  PYRO_TRY(launch_set_sweep(ctx, ctx->h_sweep, ctx->h_pt_round));
+  for (auto &p : ctx->h_proposed_learned) p += 4ull * (uint64_t)ctx->ncell;
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int pyro_acceptance_learned(pyro_ctx *ctx, uint64_t *accepted, uint64_t *proposed) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (!ctx->d_acc_learned) return set_error(PYRO_ESTATE, "no learned proposal set up");
+  PYRO_HIP(hipSetDevice(ctx->device));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  if (accepted)
+    PYRO_HIP(hipMemcpy(accepted, ctx->d_acc_learned, sizeof(uint64_t) * ctx->R, hipMemcpyDeviceToHost));
+  if (proposed) std::memcpy(proposed, ctx->h_proposed_learned.data(), sizeof(uint64_t) * ctx->R);
+  return 0;
+}
+
+}  // extern "C"

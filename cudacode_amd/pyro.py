@@ -33,8 +33,13 @@ EXPORTS = [
     "pyro_magnetization", "pyro_order_params", "pyro_sublattice_sums", "pyro_fold_order_params",
     "pyro_energy_incremental",
     "pyro_acceptance", "pyro_reset_acceptance", "pyro_adapt_cone", "pyro_record_start",
-    "pyro_record_fetch", "pyro_time_phase", "pyro_sync",
+    "pyro_record_fetch", "pyro_set_passes", "pyro_mlp_set_weights", "pyro_mlp_get_weights",
+    "pyro_mlp_eval", "pyro_vmf_draw", "pyro_mlp_loss_grad", "pyro_gather_environments",
+    "pyro_get_environments", "pyro_clear_environments", "pyro_mlp_train", "pyro_learned_phase",
+    "pyro_set_learned_passes", "pyro_acceptance_learned", "pyro_time_phase", "pyro_sync",
 ]
+
+MLP_PARAMS = 4868
 
 
 class PyroError(RuntimeError):
@@ -100,6 +105,20 @@ def lib():
         L.pyro_record_fetch.argtypes = [vp, vp, C.POINTER(C.c_int)]
         L.pyro_time_phase.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]
+        L.pyro_set_passes.argtypes = [vp, C.c_int, C.c_int]
+        L.pyro_mlp_set_weights.argtypes = [vp, vp]
+        L.pyro_mlp_get_weights.argtypes = [vp, vp]
+        L.pyro_mlp_eval.argtypes = [vp, vp, C.c_int, vp]
+        L.pyro_vmf_draw.argtypes = [vp, vp, C.c_int, vp]
+        L.pyro_mlp_loss_grad.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_double), vp]
+        L.pyro_gather_environments.argtypes = [vp, C.c_int, C.c_uint32, C.c_int,
+                                               C.POINTER(C.c_int)]
+        L.pyro_get_environments.argtypes = [vp, vp, C.POINTER(C.c_int)]
+        L.pyro_clear_environments.argtypes = [vp]
+        L.pyro_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_float, vp]
+        L.pyro_learned_phase.argtypes = [vp, C.c_int, C.c_uint32, C.c_int]
+        L.pyro_set_learned_passes.argtypes = [vp, C.c_int]
+        L.pyro_acceptance_learned.argtypes = [vp, vp, vp]
         L.pyro_sync.argtypes = [vp]
         _lib = L
     return _lib
@@ -304,6 +323,74 @@ class Pyrochlore:
         out = np.empty(max(n.value, 1) * self.R, np.float64)
         _check(lib().pyro_record_fetch(self._ctx, _p(out), C.byref(n)))
         return out[: n.value * self.R].reshape(n.value, self.R)
+
+    def set_passes(self, n_metro: int, n_or: int) -> None:
+        _check(lib().pyro_set_passes(self._ctx, n_metro, n_or))
+        self.n_metro, self.n_or = n_metro, n_or
+
+    # -- learned proposal ---------------------------------------------------
+    def mlp_set_weights(self, w) -> None:
+        a = np.ascontiguousarray(w, np.float32)
+        if a.size != MLP_PARAMS:
+            raise PyroError(f"mlp_set_weights: expected {MLP_PARAMS} floats, got {a.size}")
+        _check(lib().pyro_mlp_set_weights(self._ctx, _p(a)))
+
+    def mlp_get_weights(self) -> np.ndarray:
+        w = np.empty(MLP_PARAMS, np.float32)
+        _check(lib().pyro_mlp_get_weights(self._ctx, _p(w)))
+        return w
+
+    def mlp_eval(self, rows9) -> np.ndarray:
+        a = np.ascontiguousarray(rows9, np.float32).reshape(-1, 9)
+        out = np.empty_like(a)
+        _check(lib().pyro_mlp_eval(self._ctx, _p(a), a.shape[0], _p(out)))
+        return out
+
+    def vmf_draw(self, rows4) -> np.ndarray:
+        a = np.ascontiguousarray(rows4, np.float32).reshape(-1, 4)
+        out = np.empty((a.shape[0], 3), np.float32)
+        _check(lib().pyro_vmf_draw(self._ctx, _p(a), a.shape[0], _p(out)))
+        return out
+
+    def mlp_loss_grad(self, env6):
+        a = np.ascontiguousarray(env6, np.float32).reshape(-1, 6)
+        g = np.empty(MLP_PARAMS, np.float32)
+        loss = C.c_double()
+        _check(lib().pyro_mlp_loss_grad(self._ctx, _p(a), a.shape[0], C.byref(loss), _p(g)))
+        return loss.value, g
+
+    def gather_environments(self, per_replica: int, draw: int, capacity: int) -> int:
+        n = C.c_int()
+        _check(lib().pyro_gather_environments(self._ctx, per_replica, draw, capacity, C.byref(n)))
+        return n.value
+
+    def environments(self) -> np.ndarray:
+        n = C.c_int()
+        _check(lib().pyro_get_environments(self._ctx, None, C.byref(n)))
+        out = np.empty((max(n.value, 1), 6), np.float32)
+        _check(lib().pyro_get_environments(self._ctx, _p(out), C.byref(n)))
+        return out[: n.value]
+
+    def clear_environments(self) -> None:
+        _check(lib().pyro_clear_environments(self._ctx))
+
+    def mlp_train(self, steps: int, batch: int = 4096, lr: float = 3e-3,
+                  lr_final: float = 3e-4) -> np.ndarray:
+        hist = np.zeros(max(steps, 1), np.float32)
+        _check(lib().pyro_mlp_train(self._ctx, steps, batch, lr, lr_final, _p(hist)))
+        return hist[:steps]
+
+    def learned_phase(self, colour: int, sweep: int, pass_: int = 0) -> None:
+        _check(lib().pyro_learned_phase(self._ctx, colour, sweep, pass_))
+
+    def set_learned_passes(self, n: int) -> None:
+        _check(lib().pyro_set_learned_passes(self._ctx, n))
+
+    def acceptance_learned(self):
+        a = np.empty(self.R, np.uint64)
+        p = np.empty(self.R, np.uint64)
+        _check(lib().pyro_acceptance_learned(self._ctx, _p(a), _p(p)))
+        return a, p
 
     def time_phase(self, kind: int, n_launches: int):
         """(mean ms per launch, first-to-last ms) of n back-to-back phase launches

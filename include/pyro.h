@@ -147,6 +147,41 @@ int pyro_adapt_cone(pyro_ctx *ctx, double target, int rounds, int sweeps_per_rou
 int pyro_record_start(pyro_ctx *ctx, int capacity);
 int pyro_record_fetch(pyro_ctx *ctx, double *rows, int *n_rows);
 
+/* Change the per-sweep schedule (Metropolis and over-relaxation passes; the
+ * reference's --McSteps/--OverRelaxation options, Pyrochlore.cu:1161-1216). */
+int pyro_set_passes(pyro_ctx *ctx, int n_metro, int n_or);
+
+/* ---- Learned local proposal (no reference counterpart) -------------------
+ * A 6 -> 64 -> 64 -> 4 ReLU network maps the environment (u = beta*h, s) of a
+ * spin to a von Mises-Fisher proposal vMF(mu, kappa); an exact
+ * Metropolis-Hastings test keeps the Boltzmann law stationary. 4868 FP32
+ * parameters packed W1[64][6] b1[64] W2[64][64] b2[64] W3[4][64] b3[4];
+ * inference on BF16 MFMA with FP32 accumulation, training in FP32. */
+int pyro_mlp_set_weights(pyro_ctx *ctx, const float *w);
+int pyro_mlp_get_weights(pyro_ctx *ctx, float *w);
+/* rows9: n x (u[3], s[3], sp[3]); out9: n x (o(u,s)[4], o(u,sp)[4],
+ * log MH ratio with h = u and beta = 1), through the phase kernel's code. */
+int pyro_mlp_eval(pyro_ctx *ctx, const float *rows9, int n, float *out9);
+/* Device vMF sampler: rows4 n x (mu[3], kappa) -> out3 n x 3. */
+int pyro_vmf_draw(pyro_ctx *ctx, const float *rows4, int n, float *out3);
+/* Mean loss (reverse KL to the Boltzmann conditional) and gradient over n
+ * environments env6 = (u[3], s[3]) rows, FP32 training kernels. */
+int pyro_mlp_loss_grad(pyro_ctx *ctx, const float *env6, int n, double *loss, float *grad);
+/* Environments of the current configuration: per_replica Philox-chosen sites
+ * of every replica appended to the device set (capacity rows kept). */
+int pyro_gather_environments(pyro_ctx *ctx, int per_replica, uint32_t draw, int capacity,
+                             int *n_total);
+int pyro_get_environments(pyro_ctx *ctx, float *env6, int *n);
+int pyro_clear_environments(pyro_ctx *ctx);
+/* Adam on minibatches drawn from the environment set; learning rate decays
+ * geometrically from lr to lr_final. loss_hist (optional) receives the mean
+ * batch loss of each step. */
+int pyro_mlp_train(pyro_ctx *ctx, int steps, int batch, float lr, float lr_final, float *loss_hist);
+/* One colour phase of learned proposals; learned passes per sweep. */
+int pyro_learned_phase(pyro_ctx *ctx, int colour, uint32_t sweep, int pass);
+int pyro_set_learned_passes(pyro_ctx *ctx, int n_learned);
+int pyro_acceptance_learned(pyro_ctx *ctx, uint64_t *accepted, uint64_t *proposed);
+
 /* Time n_launches back-to-back colour-phase launches of one kind (0 =
  * Metropolis, 1 = over-relaxation; colours cycle 0..3) with a HIP event pair
  * around each launch on the ctx stream. avg_ms: mean event-to-event time per

@@ -61,6 +61,17 @@ def lib():
                               C.POINTER(C.c_int)]
         L.orc_num_threads.restype = C.c_int
         L.orc_set_num_threads.argtypes = [C.c_int]
+        L.orc_mlp_nparams.restype = C.c_int
+        L.orc_mlp_features.argtypes = [_f64p, _f64p, _f64p]
+        L.orc_mlp_forward.argtypes = [_f64p, _f64p, _f64p]
+        L.orc_mlp_decode.argtypes = [_f64p, _f64p, C.POINTER(C.c_double)]
+        L.orc_mlp_loss_grad.argtypes = [_f64p, _f64p, C.c_int, C.c_void_p]
+        L.orc_mlp_loss_grad.restype = C.c_double
+        L.orc_vmf_log_norm.argtypes = [C.c_double]
+        L.orc_vmf_log_norm.restype = C.c_double
+        L.orc_vmf_log_density.argtypes = [_f64p, C.c_double, _f64p]
+        L.orc_vmf_log_density.restype = C.c_double
+        L.orc_vmf_sample.argtypes = [_f64p, C.c_double, C.c_double, C.c_double, _f64p]
         _lib = L
     return _lib
 
@@ -178,3 +189,63 @@ def num_threads() -> int:
 
 def set_num_threads(n: int) -> None:
     lib().orc_set_num_threads(n)
+
+
+# -- learned proposal -------------------------------------------------------
+def mlp_nparams() -> int:
+    return lib().orc_mlp_nparams()
+
+
+def mlp_features(u, s) -> np.ndarray:
+    x = np.zeros(6)
+    lib().orc_mlp_features(np.ascontiguousarray(u, np.float64), np.ascontiguousarray(s, np.float64), x)
+    return x
+
+
+def mlp_forward(w, x) -> np.ndarray:
+    o = np.zeros(4)
+    lib().orc_mlp_forward(np.ascontiguousarray(w, np.float64), np.ascontiguousarray(x, np.float64), o)
+    return o
+
+
+def mlp_decode(o):
+    mu = np.zeros(3)
+    k = C.c_double()
+    lib().orc_mlp_decode(np.ascontiguousarray(o, np.float64), mu, C.byref(k))
+    return mu, k.value
+
+
+def mlp_loss_grad(w, env, with_grad=True):
+    """Mean reverse KL over env rows (u, s) and its gradient (FP64)."""
+    env = np.ascontiguousarray(env, np.float64).reshape(-1, 6)
+    g = np.zeros(mlp_nparams()) if with_grad else None
+    loss = lib().orc_mlp_loss_grad(np.ascontiguousarray(w, np.float64), env, env.shape[0],
+                                   None if g is None else g.ctypes.data_as(C.c_void_p))
+    return loss, g
+
+
+def vmf_log_norm(k) -> float:
+    return lib().orc_vmf_log_norm(k)
+
+
+def vmf_log_density(mu, k, y) -> float:
+    return lib().orc_vmf_log_density(np.ascontiguousarray(mu, np.float64), k,
+                                     np.ascontiguousarray(y, np.float64))
+
+
+def vmf_sample(mu, k, u1, u2) -> np.ndarray:
+    out = np.zeros(3)
+    lib().orc_vmf_sample(np.ascontiguousarray(mu, np.float64), k, u1, u2, out)
+    return out
+
+
+def mlp_init(seed: int = 0) -> np.ndarray:
+    """He-style initial weights in the packed layout (W1 b1 W2 b2 W3 b3)."""
+    r = np.random.default_rng(seed)
+    w = np.concatenate([
+        r.normal(0, np.sqrt(2 / 6), 64 * 6), np.zeros(64),
+        r.normal(0, np.sqrt(2 / 64), 64 * 64), np.zeros(64),
+        r.normal(0, np.sqrt(1 / 64), 4 * 64), np.zeros(4),
+    ])
+    assert w.size == mlp_nparams()
+    return w

@@ -72,6 +72,17 @@ void orc_run(float *spins, int L, int R, int n_sweeps, uint32_t sweep0, int n_me
              int *replica_at, float jeff, uint64_t seed, uint64_t *accepted, double *dE,
              int *n_swaps_accepted);
 
+/* Learned proposal (pyro_oracle_mlp.c), FP64 throughout. env rows are
+ * (u = beta*h (3), s (3)); weights are the 4868 packed parameters. */
+int orc_mlp_nparams(void);
+void orc_mlp_features(const double u[3], const double s[3], double x[6]);
+void orc_mlp_forward(const double *w, const double x[6], double o[4]);
+void orc_mlp_decode(const double o[4], double mu[3], double *kappa);
+double orc_mlp_loss_grad(const double *w, const double *env, int n, double *grad);
+double orc_vmf_log_norm(double kappa);
+double orc_vmf_log_density(const double mu[3], double kappa, const double y[3]);
+void orc_vmf_sample(const double mu[3], double kappa, double u1, double u2, double out[3]);
+
 /* Threads used by the OpenMP loops (1 when built without OpenMP). */
 int orc_num_threads(void);
 void orc_set_num_threads(int n);
