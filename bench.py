@@ -69,7 +69,23 @@ def cpu_baseline(target_s: float = 15.0):
     }
 
 
+def calibrate_copy(mib: int = 512, reps: int = 5):
+    """A plain device-to-device copy of known size (torch's elementwise copy
+    kernel), run under rocprofv3 --pmc to scale FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_bytes_read": n * 4, "bytes_written": n * 4, "reps": reps}))
+
+
 def main():
+    if "--calibrate-copy" in sys.argv:
+        calibrate_copy()
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
