@@ -35,6 +35,23 @@ BYTES_PER_UPDATE = 60  # 12 N B read + 3 N B written per colour phase of N/4 upd
 HBM_PEAK_GBS = 8000.0
 
 
+OR_KERNEL = "k_phase_cells<1, colour, 16, false>"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    record (FETCH_SIZE and WRITE_SIZE passes scaled by a copy kernel of known
+    size), or None when no record for this kernel exists."""
+    try:
+        rec = json.load(open(TRAFFIC_FILE))
+    except (OSError, ValueError):
+        return None
+    if rec.get("kernel") != kernel:
+        return None
+    return rec.get("traffic_bytes_per_launch")
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -132,9 +149,13 @@ def main():
     log(f"phase timing: {args.phase_launches} over-relaxation launches")
     or_ms, _ = g.time_phase(1, args.phase_launches)
     met_ms, _ = g.time_phase(0, args.phase_launches // 4)
+    g.set_kernel_options(0, False)  # the site-per-thread kernel, for comparison
+    or_ms_v0, _ = g.time_phase(1, args.phase_launches // 4)
+    g.set_kernel_options(1, False)
     bytes_per_launch = BYTES_PER_UPDATE * n_total // 4
     achieved = bytes_per_launch / (or_ms * 1e-3) / 1e9
     g.close()
+    traffic = measured_traffic(OR_KERNEL)
 
     out = {
         "metric": "spin updates/s (plain sweep); tau_int-corrected independent samples/s",
@@ -166,15 +187,16 @@ def main():
         "acceptance": acc,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_phase<1> (over-relaxation colour phase)",
+            "kernel": OR_KERNEL + " (over-relaxation colour phase, one thread per cell)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "avg_launch_ms": or_ms,
             "bytes_per_launch": bytes_per_launch,
             "metropolis_avg_launch_ms": met_ms,
+            "site_per_thread_or_avg_launch_ms": or_ms_v0,
         },
     }
     if not args.no_cpu and rank == 0:

@@ -532,6 +532,18 @@ int pyro_set_learned_passes(pyro_ctx *ctx, int n_learned) {
   return 0;
 }
 
+int pyro_set_kernel_options(pyro_ctx *ctx, int phase_variant, int track_or_energy) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (phase_variant < 0 || phase_variant > 1)
+    return set_error(PYRO_EINVAL, "phase_variant %d not 0/1", phase_variant);
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  drop_graph(ctx);
+  ctx->phase_variant = phase_variant;
+  ctx->track_or = track_or_energy != 0;
+  return 0;
+}
+
 int pyro_set_passes(pyro_ctx *ctx, int n_metro, int n_or) {
   if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
   if (n_metro < 0 || n_or < 0 || n_metro > 255) return set_error(PYRO_EINVAL, "pass counts");

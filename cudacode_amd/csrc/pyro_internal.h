@@ -61,6 +61,8 @@ struct pyro_ctx {
   uint64_t seed = 0;
   float jeff = 1.0f;      // J * spin_len2
   int n_metro = 1, n_or = 0, pt_every = 0;
+  int phase_variant = 1;   // 0: wave per site (any L); 1: cell per thread (compile-time L)
+  bool track_or = false;   // FP64 energy bookkeeping also in over-relaxation phases
   hipStream_t stream = nullptr;
 
   // device state

@@ -122,6 +122,132 @@ __global__ __launch_bounds__(kPhaseBlock) void k_phase(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Cell-per-thread colour phase with the colour and the lattice size fixed at
+// compile time. Thread = one cell, all 4 sites of the colour: the 24 links
+// fold into constant offsets from three wrapped coordinates per axis, a
+// neighbour shared by two of the thread's targets is loaded once, and the 4
+// results are stored after all loads (the targets are mutually non-adjacent,
+// so deferring the stores changes nothing). Same Philox counters and float
+// expressions as k_phase, hence the same results bit for bit.
+//
+// Blocks are remapped so the hardware's round-robin dispatch over the 8 XCDs
+// (block b runs on XCD b % 8) gives each XCD a contiguous run of cells: the
+// z-1 / z+1 neighbour planes of a block then sit in the same XCD's L2.
+// grid = 1-D, n_blocks = R * ceil(ncell / 256), block = 256.
+// ---------------------------------------------------------------------------
+constexpr Link kLinks[16][6] = PYRO_LINKS16_INIT;
+
+template <int KIND, int COLOUR, int LT, bool TRACK>
+__global__ __launch_bounds__(256) void k_phase_cells(
+    float *__restrict__ spins, int blocks_per_replica, int pass, const float *__restrict__ beta,
+    const float *__restrict__ cosc, const uint32_t *__restrict__ ctl, uint32_t key0,
+    uint32_t key1, float jeff, unsigned long long *__restrict__ acc, double *__restrict__ dE) {
+  constexpr int L = LT, LL = LT * LT, NC = LT * LT * LT;
+  const int nb = gridDim.x;
+  const int b = blockIdx.x;
+  const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+  const int r = lb / blocks_per_replica;
+  const int cell = (lb - r * blocks_per_replica) * 256 + threadIdx.x;
+  const bool valid = cell < NC;
+  float *Sx = spins + (size_t)r * 48 * NC;
+  float *Sy = Sx + 16 * NC;
+  float *Sz = Sy + 16 * NC;
+
+  double de = 0.0;
+  unsigned nacc = 0;
+  if (valid) {
+    const int z = cell / LL;
+    const int y = (cell / L) % L;
+    const int x = cell % L;
+    const int X[3] = {x == 0 ? L - 1 : x - 1, x, x == L - 1 ? 0 : x + 1};
+    const int Y[3] = {(y == 0 ? L - 1 : y - 1) * L, y * L, (y == L - 1 ? 0 : y + 1) * L};
+    const int Z[3] = {(z == 0 ? L - 1 : z - 1) * LL, z * LL, (z == L - 1 ? 0 : z + 1) * LL};
+    f3 s[4], sp[4];
+    bool wr[4];
+    float fx[4], fy[4], fz[4];
+    double gx[4], gy[4], gz[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int site = COLOUR + 4 * k;
+      fx[k] = 0.0f;
+      fy[k] = 0.0f;
+      fz[k] = 0.0f;
+      gx[k] = gy[k] = gz[k] = 0.0;
+#pragma unroll
+      for (int n = 0; n < 6; ++n) {
+        const Link lk = kLinks[site][n];
+        const int j = lk.site * NC + Z[lk.dz + 1] + Y[lk.dy + 1] + X[lk.dx + 1];
+        const float a = Sx[j], bb = Sy[j], c = Sz[j];
+        fx[k] = fx[k] + a;
+        fy[k] = fy[k] + bb;
+        fz[k] = fz[k] + c;
+        if (KIND == 0 || TRACK) {
+          gx[k] += (double)a;
+          gy[k] += (double)bb;
+          gz[k] += (double)c;
+        }
+      }
+      const int i = site * NC + cell;
+      s[k] = {Sx[i], Sy[i], Sz[i]};
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int site = COLOUR + 4 * k;
+      if (KIND == 0) {
+        const u32x4 w = philox4x32_10(
+            rng_counter((uint32_t)cell, (uint32_t)site, (uint32_t)pass, kKindMetropolis,
+                        (uint32_t)r, ctl[kCtlSweep]),
+            key0, key1);
+        const f3 h = {-jeff * fx[k], -jeff * fy[k], -jeff * fz[k]};
+        sp[k] = cone_proposal(s[k], cosc[r], u01(w.x), u01(w.y));
+        wr[k] = metropolis_accept(h, s[k], sp[k], beta[r], u01(w.z));
+        nacc += wr[k] ? 1u : 0u;
+      } else {
+        const f3 f = {fx[k], fy[k], fz[k]};
+        sp[k] = overrelax(f, s[k]);
+        wr[k] = true;
+      }
+      if ((KIND == 0 || TRACK) && wr[k])
+        de += (double)jeff * (((double)sp[k].x - (double)s[k].x) * gx[k] +
+                              ((double)sp[k].y - (double)s[k].y) * gy[k] +
+                              ((double)sp[k].z - (double)s[k].z) * gz[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (wr[k]) {
+        const int i = (COLOUR + 4 * k) * NC + cell;
+        Sx[i] = sp[k].x;
+        Sy[i] = sp[k].y;
+        Sz[i] = sp[k].z;
+      }
+    }
+  }
+  if (KIND == 0 || TRACK) {
+    // A block covers cells of one replica only: one atomic per block.
+    __shared__ double s_de[4];
+    __shared__ unsigned s_acc[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    de = wave_sum(de);
+    unsigned wacc = nacc;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) wacc += __shfl_xor(wacc, off, 64);
+    if (lane == 0) {
+      s_de[wv] = de;
+      s_acc[wv] = wacc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double t = (s_de[0] + s_de[1]) + (s_de[2] + s_de[3]);
+      if (t != 0.0) atomicAdd(&dE[r], t);
+      if (KIND == 0) {
+        const unsigned a = s_acc[0] + s_acc[1] + s_acc[2] + s_acc[3];
+        if (a) atomicAdd(&acc[r], (unsigned long long)a);
+      }
+    }
+  }
+}
+
 // Uniform initial spins (Generate3DVectorList, Math.cu:132-143, written SoA).
 // grid = (ceil(ncell/256), R), block = 256, each thread one cell x 16 sites.
 __global__ __launch_bounds__(256) void k_init_spins(float *__restrict__ spins, int ncell,
@@ -325,9 +451,52 @@ static void launch_phase_t(pyro_ctx *ctx, int colour, int pass) {
       key_lo(ctx), key_hi(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE);
 }
 
+template <int KIND, int COLOUR, int LT, bool TRACK>
+static void launch_cells_t(pyro_ctx *ctx, int pass) {
+  const int bpr = (LT * LT * LT + 255) / 256;
+  k_phase_cells<KIND, COLOUR, LT, TRACK><<<bpr * ctx->R, 256, 0, ctx->stream>>>(
+      ctx->d_spins, bpr, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl, key_lo(ctx), key_hi(ctx),
+      ctx->jeff, ctx->d_acc, ctx->d_dE);
+}
+
+template <int KIND, int LT>
+static void launch_cells_l(pyro_ctx *ctx, int colour, int pass) {
+  const bool track = KIND == 1 && ctx->track_or;
+#define PYRO_CELLS_CASE(C)                                            \
+  case C:                                                             \
+    if (track) launch_cells_t<KIND, C, LT, true>(ctx, pass);          \
+    else launch_cells_t<KIND, C, LT, false>(ctx, pass);               \
+    break;
+  switch (colour) {
+    PYRO_CELLS_CASE(0)
+    PYRO_CELLS_CASE(1)
+    PYRO_CELLS_CASE(2)
+    PYRO_CELLS_CASE(3)
+  }
+#undef PYRO_CELLS_CASE
+}
+
+// Cell-per-thread kernels for the lattice sizes of the benchmark
+// configurations; returns false when L has no compile-time copy.
+template <int KIND>
+static bool launch_cells(pyro_ctx *ctx, int colour, int pass) {
+  switch (ctx->L) {
+    case 4: launch_cells_l<KIND, 4>(ctx, colour, pass); return true;
+    case 8: launch_cells_l<KIND, 8>(ctx, colour, pass); return true;
+    case 12: launch_cells_l<KIND, 12>(ctx, colour, pass); return true;
+    case 16: launch_cells_l<KIND, 16>(ctx, colour, pass); return true;
+    case 24: launch_cells_l<KIND, 24>(ctx, colour, pass); return true;
+    case 32: launch_cells_l<KIND, 32>(ctx, colour, pass); return true;
+    case 64: launch_cells_l<KIND, 64>(ctx, colour, pass); return true;
+    default: return false;
+  }
+}
+
 // Lattice sizes of the benchmark configurations get a compile-time copy.
 template <int KIND>
 static void launch_phase(pyro_ctx *ctx, int colour, int pass) {
+  if (ctx->phase_variant == 1 && launch_cells<KIND>(ctx, colour, pass)) return;
+  // The wave-per-site kernel always tracks the FP64 energy change.
   switch (ctx->L) {
     case 4: launch_phase_t<KIND, 4>(ctx, colour, pass); break;
     case 8: launch_phase_t<KIND, 8>(ctx, colour, pass); break;

@@ -33,7 +33,7 @@ EXPORTS = [
     "pyro_magnetization", "pyro_order_params", "pyro_sublattice_sums", "pyro_fold_order_params",
     "pyro_energy_incremental",
     "pyro_acceptance", "pyro_reset_acceptance", "pyro_adapt_cone", "pyro_record_start",
-    "pyro_record_fetch", "pyro_set_passes", "pyro_mlp_set_weights", "pyro_mlp_get_weights",
+    "pyro_record_fetch", "pyro_set_passes", "pyro_set_kernel_options", "pyro_mlp_set_weights", "pyro_mlp_get_weights",
     "pyro_mlp_eval", "pyro_vmf_draw", "pyro_mlp_loss_grad", "pyro_gather_environments",
     "pyro_get_environments", "pyro_clear_environments", "pyro_mlp_train", "pyro_learned_phase",
     "pyro_set_learned_passes", "pyro_acceptance_learned", "pyro_time_phase", "pyro_sync",
@@ -106,6 +106,7 @@ def lib():
         L.pyro_time_phase.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]
         L.pyro_set_passes.argtypes = [vp, C.c_int, C.c_int]
+        L.pyro_set_kernel_options.argtypes = [vp, C.c_int, C.c_int]
         L.pyro_mlp_set_weights.argtypes = [vp, vp]
         L.pyro_mlp_get_weights.argtypes = [vp, vp]
         L.pyro_mlp_eval.argtypes = [vp, vp, C.c_int, vp]
@@ -323,6 +324,9 @@ class Pyrochlore:
         out = np.empty(max(n.value, 1) * self.R, np.float64)
         _check(lib().pyro_record_fetch(self._ctx, _p(out), C.byref(n)))
         return out[: n.value * self.R].reshape(n.value, self.R)
+
+    def set_kernel_options(self, phase_variant: int = 1, track_or_energy: bool = False) -> None:
+        _check(lib().pyro_set_kernel_options(self._ctx, phase_variant, 1 if track_or_energy else 0))
 
     def set_passes(self, n_metro: int, n_or: int) -> None:
         _check(lib().pyro_set_passes(self._ctx, n_metro, n_or))

@@ -1,0 +1,54 @@
+"""Summarise rocprofv3 databases (rocpd SQLite) into the short text records
+kept under profiles/.
+
+  python -m cudacode_amd.rocprof_summary trace DB            per-kernel time table
+  python -m cudacode_amd.rocprof_summary pmc DB [DB ...]      mean counter value per kernel
+
+Counter passes are scaled by a copy kernel of known size run under the same
+counter (see bench.py --calibrate-copy): bytes = KiB * 1024 * scale.
+"""
+from __future__ import annotations
+
+import sqlite3
+import sys
+
+
+def trace(db: str, limit: int = 12) -> str:
+    c = sqlite3.connect(db)
+    rows = c.execute("select name, total_calls, total_duration, average, percentage "
+                     "from top_kernels limit ?", (limit,)).fetchall()
+    out = ["| kernel | calls | total us | avg us | % |", "|---|---|---|---|---|"]
+    for n, calls, tot, avg, pct in rows:
+        out.append(f"| `{short(n)}` | {calls} | {tot / 1e3:.1f} | {avg / 1e3:.2f} | {pct:.1f} |")
+    return "\n".join(out)
+
+
+def pmc(db: str) -> dict:
+    c = sqlite3.connect(db)
+    res = {}
+    for name, counter, n, avg in c.execute(
+            "select kernel_name, counter_name, count(*), avg(value) from counters_collection "
+            "group by kernel_name, counter_name"):
+        res[(short(name), counter)] = (n, avg)
+    return res
+
+
+def short(name: str) -> str:
+    return name.split("(")[0].replace("void ", "")
+
+
+def main(argv):
+    if len(argv) >= 2 and argv[0] == "trace":
+        print(trace(argv[1]))
+    elif len(argv) >= 2 and argv[0] == "pmc":
+        for db in argv[1:]:
+            for (k, cnt), (n, avg) in sorted(pmc(db).items(), key=lambda t: -t[1][1])[:8]:
+                print(f"| `{k}` | {cnt} | {n} | {avg:.1f} KiB |")
+    else:
+        print(__doc__)
+        return 2
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

@@ -151,6 +151,14 @@ int pyro_record_fetch(pyro_ctx *ctx, double *rows, int *n_rows);
  * reference's --McSteps/--OverRelaxation options, Pyrochlore.cu:1161-1216). */
 int pyro_set_passes(pyro_ctx *ctx, int n_metro, int n_or);
 
+/* Kernel options (no reference counterpart; the reference's block shape is the
+ * --BlockSpin option, Pyrochlore.cu:1152-1154). phase_variant 1 (default): one
+ * thread per cell and the colour's 4 sites, for L in {4,8,12,16,24,32,64};
+ * 0: one thread per site, any L. track_or_energy: also account the FP64
+ * energy change of over-relaxation updates (zero up to FP32 rounding) in the
+ * incremental energy; off by default. Results are identical either way. */
+int pyro_set_kernel_options(pyro_ctx *ctx, int phase_variant, int track_or_energy);
+
 /* ---- Learned local proposal (no reference counterpart) -------------------
  * A 6 -> 64 -> 64 -> 4 ReLU network maps the environment (u = beta*h, s) of a
  * spin to a von Mises-Fisher proposal vMF(mu, kappa); an exact
