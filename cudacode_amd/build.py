@@ -18,6 +18,7 @@ LIB = os.path.join(HERE, "libpyro.so")
 SOURCES = [
     "pyro_kernels.hip",
     "pyro_learned.hip",
+    "pyro_batch.hip",
     "pyro_capi.hip",
 ]
 
@@ -49,15 +50,18 @@ def build(verbose: bool = False) -> str:
     """Compile every source for gfx950 and link libpyro.so. Returns its path."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
-    for src in SOURCES:
+    objs, procs = [], []
+    for src in SOURCES:  # translation units compile in parallel
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         cmd = [hipcc(), *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c",
                os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        procs.append((src, subprocess.Popen(cmd)))
         objs.append(obj)
+    failed = [src for src, p in procs if p.wait() != 0]
+    if failed:
+        raise RuntimeError(f"hipcc failed on {', '.join(failed)}")
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:

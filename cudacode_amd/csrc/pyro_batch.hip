@@ -1,0 +1,346 @@
+// Batches of replicas with different lattice sizes, and the two sweep
+// executions that serve them (no reference counterpart: the reference runs
+// one lattice size per process, Pyrochlore.cu:1251-1257).
+//
+//  exec_mode 1 -- one persistent kernel per sweep. Replicas are independent
+//    between exchanges, so a block that owns whole replicas runs every colour
+//    phase of every pass of the sweep with block barriers only: no grid-wide
+//    synchronisation, one launch per sweep. Replicas are packed into bins of
+//    balanced work (longest-processing-time first); bins = blocks.
+//  exec_mode 2 -- per-phase kernels over a work list of 256-cell tiles of all
+//    replicas (perfect balance, one launch per colour phase), tiles ordered so
+//    each XCD gets a contiguous run of one replica's cells.
+//
+// Both call the same cell_update as the uniform kernels, with the replica's
+// global index in the Philox counters: every replica's trajectory is the one
+// a single-replica run of its size would produce, bit for bit.
+#include <algorithm>
+#include <numeric>
+
+#include "pyro_cell.h"
+#include "pyro_internal.h"
+
+namespace pyro {
+
+__constant__ Link c_links_batch[16][6] = PYRO_LINKS16_INIT;
+
+bool batch_size_supported(int L) {
+  return L == 4 || L == 8 || L == 12 || L == 16 || L == 24 || L == 32;
+}
+
+// --- persistent sweep ------------------------------------------------------
+template <int KIND, int LT, int BLOCK, bool TRACK>
+__device__ __forceinline__ void replica_pass(float *Sx, int r, int pass, float beta_r, float cosc_r,
+                                             uint32_t sweep, uint32_t key0, uint32_t key1, float jeff,
+                                             double &de, unsigned &nacc) {
+  constexpr int NC = LT * LT * LT;
+  // Item u = k * NC + cell updates site C + 4k of that cell: consecutive
+  // threads take consecutive cells of one site plane (coalesced), one site
+  // per iteration (the register budget of a 1024-thread block).
+#define PYRO_COLOUR(C)                                                                     \
+  _Pragma("unroll 1") for (int u = threadIdx.x; u < 4 * NC; u += BLOCK) {                  \
+    const int k = u / NC;                                                                  \
+    site_update<KIND, LT, TRACK>(Sx, c_links_batch, u - k * NC, C + 4 * k, r, pass, beta_r, \
+                                 cosc_r, sweep, key0, key1, jeff, de, nacc);               \
+  }                                                                                        \
+  __syncthreads();
+  PYRO_COLOUR(0)
+  PYRO_COLOUR(1)
+  PYRO_COLOUR(2)
+  PYRO_COLOUR(3)
+#undef PYRO_COLOUR
+}
+
+template <int LT, int BLOCK, bool TRACK>
+__device__ __attribute__((noinline)) void replica_sweep(float *Sx, int r, int n_metro, int n_or, float beta_r,
+                                              float cosc_r, uint32_t sweep, uint32_t key0,
+                                              uint32_t key1, float jeff, double &de, unsigned &nacc) {
+  for (int p = 0; p < n_metro; ++p)
+    replica_pass<0, LT, BLOCK, TRACK>(Sx, r, p, beta_r, cosc_r, sweep, key0, key1, jeff, de, nacc);
+  for (int o = 0; o < n_or; ++o)
+    replica_pass<1, LT, BLOCK, TRACK>(Sx, r, 0, beta_r, cosc_r, sweep, key0, key1, jeff, de, nacc);
+}
+
+// grid = n_bins, block = BLOCK. Bin b owns replicas order[bin_start[b] ..
+// bin_start[b+1]); each replica is swept start to finish by this block only,
+// so its counters are updated without atomics.
+template <int BLOCK, bool TRACK>
+__global__ __launch_bounds__(BLOCK) void k_batch_sweep(
+    float *__restrict__ spins, const long long *__restrict__ off, const int *__restrict__ Ls,
+    const int *__restrict__ order, const int *__restrict__ bin_start, int n_metro, int n_or,
+    const float *__restrict__ beta, const float *__restrict__ cosc,
+    const uint32_t *__restrict__ ctl, uint32_t key0, uint32_t key1, float jeff,
+    unsigned long long *__restrict__ acc, double *__restrict__ dE) {
+  const uint32_t sweep = ctl[kCtlSweep];
+  for (int q = bin_start[blockIdx.x]; q < bin_start[blockIdx.x + 1]; ++q) {
+    const int r = order[q];
+    float *Sx = spins + off[r];
+    const float b = beta[r], c = cosc[r];
+    double de = 0.0;
+    unsigned nacc = 0;
+    switch (Ls[r]) {
+      case 4: replica_sweep<4, BLOCK, TRACK>(Sx, r, n_metro, n_or, b, c, sweep, key0, key1, jeff, de, nacc); break;
+      case 8: replica_sweep<8, BLOCK, TRACK>(Sx, r, n_metro, n_or, b, c, sweep, key0, key1, jeff, de, nacc); break;
+      case 12: replica_sweep<12, BLOCK, TRACK>(Sx, r, n_metro, n_or, b, c, sweep, key0, key1, jeff, de, nacc); break;
+      case 16: replica_sweep<16, BLOCK, TRACK>(Sx, r, n_metro, n_or, b, c, sweep, key0, key1, jeff, de, nacc); break;
+      case 24: replica_sweep<24, BLOCK, TRACK>(Sx, r, n_metro, n_or, b, c, sweep, key0, key1, jeff, de, nacc); break;
+      case 32: replica_sweep<32, BLOCK, TRACK>(Sx, r, n_metro, n_or, b, c, sweep, key0, key1, jeff, de, nacc); break;
+      default: break;
+    }
+    block_sum<BLOCK>(de, nacc);
+    if (threadIdx.x == 0) {
+      dE[r] += de;
+      acc[r] += nacc;
+    }
+  }
+}
+
+// --- tile-list phases --------------------------------------------------------
+// grid = n_tiles (padded to a multiple of 8 with r = -1), block = 256.
+template <int KIND, int COLOUR, bool TRACK>
+__global__ __launch_bounds__(256) void k_batch_phase(
+    float *__restrict__ spins, const long long *__restrict__ off, const int *__restrict__ Ls,
+    const int *__restrict__ tiles, int pass, const float *__restrict__ beta,
+    const float *__restrict__ cosc, const uint32_t *__restrict__ ctl, uint32_t key0, uint32_t key1,
+    float jeff, unsigned long long *__restrict__ acc, double *__restrict__ dE) {
+  const int r = tiles[2 * blockIdx.x];
+  if (r < 0) return;  // padding tile: the whole block leaves together
+  const int cell = tiles[2 * blockIdx.x + 1] * 256 + threadIdx.x;
+  const int L = Ls[r];
+  float *Sx = spins + off[r];
+  const float b = KIND == 0 ? beta[r] : 0.0f, c = KIND == 0 ? cosc[r] : 0.0f;
+  const uint32_t sweep = KIND == 0 ? ctl[kCtlSweep] : 0u;
+  double de = 0.0;
+  unsigned nacc = 0;
+  if (cell < L * L * L) {
+    switch (L) {
+      case 4: cell_update<KIND, COLOUR, 4, TRACK>(Sx, cell, r, pass, b, c, sweep, key0, key1, jeff, de, nacc); break;
+      case 8: cell_update<KIND, COLOUR, 8, TRACK>(Sx, cell, r, pass, b, c, sweep, key0, key1, jeff, de, nacc); break;
+      case 12: cell_update<KIND, COLOUR, 12, TRACK>(Sx, cell, r, pass, b, c, sweep, key0, key1, jeff, de, nacc); break;
+      case 16: cell_update<KIND, COLOUR, 16, TRACK>(Sx, cell, r, pass, b, c, sweep, key0, key1, jeff, de, nacc); break;
+      case 24: cell_update<KIND, COLOUR, 24, TRACK>(Sx, cell, r, pass, b, c, sweep, key0, key1, jeff, de, nacc); break;
+      case 32: cell_update<KIND, COLOUR, 32, TRACK>(Sx, cell, r, pass, b, c, sweep, key0, key1, jeff, de, nacc); break;
+      default: break;
+    }
+  }
+  if (KIND == 0 || TRACK) {
+    block_sum<256>(de, nacc);
+    if (threadIdx.x == 0) {
+      if (de != 0.0) atomicAdd(&dE[r], de);
+      if (KIND == 0 && nacc) atomicAdd(&acc[r], (unsigned long long)nacc);
+    }
+  }
+}
+
+// --- initial spins and observables ------------------------------------------
+// grid = (ceil(max ncell / 256), R), block = 256; the same draws as k_init_spins.
+__global__ __launch_bounds__(256) void k_batch_init(float *__restrict__ spins,
+                                                    const long long *__restrict__ off,
+                                                    const int *__restrict__ Ls, uint32_t key0,
+                                                    uint32_t key1) {
+  const int r = blockIdx.y;
+  const int L = Ls[r];
+  const int nc = L * L * L;
+  const int cell = blockIdx.x * 256 + threadIdx.x;
+  if (cell >= nc) return;
+  float *Sx = spins + off[r];
+  const size_t plane = (size_t)16 * nc;
+  for (int site = 0; site < 16; ++site) {
+    const u32x4 w = philox4x32_10(
+        rng_counter((uint32_t)cell, (uint32_t)site, 0, kKindInit, (uint32_t)r, 0), key0, key1);
+    const f3 v = uniform_sphere(u01(w.x), u01(w.y));
+    const size_t i = (size_t)site * nc + cell;
+    Sx[i] = v.x;
+    Sx[plane + i] = v.y;
+    Sx[2 * plane + i] = v.z;
+  }
+}
+
+__device__ __forceinline__ int wrap_b(int v, int L) { return v < 0 ? v + L : (v >= L ? v - L : v); }
+
+// One block per replica: FP64 energy (each bond from both ends, halved) and
+// the 12 sublattice sums, in a fixed summation order. grid = R, block = 256.
+__global__ __launch_bounds__(256) void k_batch_observables(const float *__restrict__ spins,
+                                                           const long long *__restrict__ off,
+                                                           const int *__restrict__ Ls, float jeff,
+                                                           double *__restrict__ obs) {
+  const int r = blockIdx.x;
+  const int L = Ls[r], LL = L * L, nc = L * L * L;
+  const float *Sx = spins + off[r];
+  const float *Sy = Sx + 16 * nc;
+  const float *Sz = Sy + 16 * nc;
+  double v[kObs];
+#pragma unroll
+  for (int q = 0; q < kObs; ++q) v[q] = 0.0;
+  for (int cell = threadIdx.x; cell < nc; cell += 256) {
+    const int z = cell / LL, rem = cell - z * LL, y = rem / L, x = rem - y * L;
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int site = 4 * t + b;
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+        for (int n = 0; n < 6; ++n) {
+          const Link lk = kLinks[site][n];
+          const int j = lk.site * nc + (wrap_b(z + lk.dz, L) * L + wrap_b(y + lk.dy, L)) * L +
+                        wrap_b(x + lk.dx, L);
+          gx += (double)Sx[j];
+          gy += (double)Sy[j];
+          gz += (double)Sz[j];
+        }
+        const int i = site * nc + cell;
+        const double sx = Sx[i], sy = Sy[i], sz = Sz[i];
+        v[0] += sx * gx + sy * gy + sz * gz;
+        v[1 + 3 * b] += sx;
+        v[2 + 3 * b] += sy;
+        v[3 + 3 * b] += sz;
+      }
+    }
+  }
+  __shared__ double s_v[256];
+  for (int q = 0; q < kObs; ++q) {
+    s_v[threadIdx.x] = v[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) s_v[threadIdx.x] += s_v[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) obs[(size_t)r * kObs + q] = q == 0 ? 0.5 * (double)jeff * s_v[0] : s_v[0];
+    __syncthreads();
+  }
+}
+
+// --- host side ----------------------------------------------------------------
+static inline uint32_t bk0(const pyro_ctx *c) { return (uint32_t)c->seed; }
+static inline uint32_t bk1(const pyro_ctx *c) { return (uint32_t)(c->seed >> 32); }
+
+void batch_free(pyro_ctx *ctx) {
+  void *ptrs[] = {ctx->d_L, ctx->d_off, ctx->d_order, ctx->d_bin_start, ctx->d_tiles};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  ctx->d_L = nullptr;
+  ctx->d_off = nullptr;
+  ctx->d_order = ctx->d_bin_start = ctx->d_tiles = nullptr;
+}
+
+// Geometry tables, persistent-kernel bins (LPT packing of work ~ L^3 into
+// n_bins bins; n_bins <= 0 means one bin per replica) and the tile list.
+int batch_setup(pyro_ctx *ctx, int n_bins) {
+  const int R = ctx->R;
+  if (n_bins <= 0 || n_bins > R) n_bins = R;
+  std::vector<int> order(R);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ctx->h_L[a] > ctx->h_L[b]; });
+  std::vector<std::vector<int>> bins(n_bins);
+  std::vector<long long> load(n_bins, 0);
+  for (int r : order) {
+    const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    bins[b].push_back(r);
+    load[b] += (long long)ctx->h_L[r] * ctx->h_L[r] * ctx->h_L[r];
+  }
+  std::vector<int> flat, start(1, 0);
+  for (auto &bv : bins) {
+    flat.insert(flat.end(), bv.begin(), bv.end());
+    start.push_back((int)flat.size());
+  }
+  // Tiles in replica order, dealt to the 8 XCDs in contiguous chunks:
+  // hardware block b runs on XCD b % 8, so list slot b takes logical tile
+  // (b % 8) * chunk + b / 8.
+  std::vector<int> logical;
+  for (int r = 0; r < R; ++r) {
+    const int nc = ctx->h_L[r] * ctx->h_L[r] * ctx->h_L[r];
+    for (int t = 0; t < (nc + 255) / 256; ++t) {
+      logical.push_back(r);
+      logical.push_back(t);
+    }
+  }
+  const int nt = (int)logical.size() / 2;
+  const int chunk = (nt + 7) / 8;
+  const int padded = chunk * 8;
+  std::vector<int> tiles(2 * (size_t)padded, -1);
+  for (int b = 0; b < padded; ++b) {
+    const int lt = (b % 8) * chunk + b / 8;
+    if (lt < nt) {
+      tiles[2 * b] = logical[2 * lt];
+      tiles[2 * b + 1] = logical[2 * lt + 1];
+    }
+  }
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  batch_free(ctx);
+  PYRO_HIP(hipMalloc((void **)&ctx->d_L, sizeof(int) * R));
+  PYRO_HIP(hipMalloc((void **)&ctx->d_off, sizeof(long long) * R));
+  PYRO_HIP(hipMalloc((void **)&ctx->d_order, sizeof(int) * R));
+  PYRO_HIP(hipMalloc((void **)&ctx->d_bin_start, sizeof(int) * (n_bins + 1)));
+  PYRO_HIP(hipMalloc((void **)&ctx->d_tiles, sizeof(int) * tiles.size()));
+  PYRO_HIP(hipMemcpy(ctx->d_L, ctx->h_L.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+  PYRO_HIP(hipMemcpy(ctx->d_off, ctx->h_off.data(), sizeof(long long) * R, hipMemcpyHostToDevice));
+  PYRO_HIP(hipMemcpy(ctx->d_order, flat.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+  PYRO_HIP(hipMemcpy(ctx->d_bin_start, start.data(), sizeof(int) * (n_bins + 1), hipMemcpyHostToDevice));
+  PYRO_HIP(hipMemcpy(ctx->d_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice));
+  ctx->n_bins = n_bins;
+  ctx->n_tiles = padded;
+  return 0;
+}
+
+int launch_batch_init_spins(pyro_ctx *ctx) {
+  int maxL = *std::max_element(ctx->h_L.begin(), ctx->h_L.end());
+  dim3 grid((maxL * maxL * maxL + 255) / 256, ctx->R);
+  k_batch_init<<<grid, 256, 0, ctx->stream>>>(ctx->d_spins, ctx->d_off, ctx->d_L, bk0(ctx), bk1(ctx));
+  PYRO_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_batch_observables(pyro_ctx *ctx) {
+  k_batch_observables<<<ctx->R, 256, 0, ctx->stream>>>(ctx->d_spins, ctx->d_off, ctx->d_L, ctx->jeff,
+                                                       ctx->d_obs);
+  PYRO_HIP(hipGetLastError());
+  return 0;
+}
+
+template <int BLOCK, bool TRACK>
+static void launch_persistent_t(pyro_ctx *ctx) {
+  k_batch_sweep<BLOCK, TRACK><<<ctx->n_bins, BLOCK, 0, ctx->stream>>>(
+      ctx->d_spins, ctx->d_off, ctx->d_L, ctx->d_order, ctx->d_bin_start, ctx->n_metro, ctx->n_or,
+      ctx->d_beta, ctx->d_cosc, ctx->d_ctl, bk0(ctx), bk1(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE);
+}
+
+template <int KIND, int COLOUR, bool TRACK>
+static void launch_tiles_t(pyro_ctx *ctx, int pass) {
+  k_batch_phase<KIND, COLOUR, TRACK><<<ctx->n_tiles, 256, 0, ctx->stream>>>(
+      ctx->d_spins, ctx->d_off, ctx->d_L, ctx->d_tiles, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl,
+      bk0(ctx), bk1(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE);
+}
+
+template <int KIND, bool TRACK>
+static void launch_tiles_pass(pyro_ctx *ctx, int pass) {
+  launch_tiles_t<KIND, 0, TRACK>(ctx, pass);
+  launch_tiles_t<KIND, 1, TRACK>(ctx, pass);
+  launch_tiles_t<KIND, 2, TRACK>(ctx, pass);
+  launch_tiles_t<KIND, 3, TRACK>(ctx, pass);
+}
+
+int launch_batch_sweep_body(pyro_ctx *ctx) {
+  const bool tr = ctx->track_or;
+  if (ctx->exec_mode == 1) {
+    if (ctx->batch_block == 1024) {
+      if (tr) launch_persistent_t<1024, true>(ctx);
+      else launch_persistent_t<1024, false>(ctx);
+    } else if (ctx->batch_block == 512) {
+      if (tr) launch_persistent_t<512, true>(ctx);
+      else launch_persistent_t<512, false>(ctx);
+    } else {
+      if (tr) launch_persistent_t<256, true>(ctx);
+      else launch_persistent_t<256, false>(ctx);
+    }
+  } else {
+    for (int p = 0; p < ctx->n_metro; ++p) launch_tiles_pass<0, false>(ctx, p);
+    for (int o = 0; o < ctx->n_or; ++o) {
+      if (tr) launch_tiles_pass<1, true>(ctx, 0);
+      else launch_tiles_pass<1, false>(ctx, 0);
+    }
+  }
+  PYRO_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace pyro

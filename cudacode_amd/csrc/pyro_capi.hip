@@ -45,12 +45,16 @@ static int reanchor(pyro_ctx *ctx) {
 
 // One sweep as kernel launches (also the body that gets captured).
 static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
-  for (int p = 0; p < ctx->n_metro; ++p)
-    for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_metropolis_phase(ctx, c, p, nullptr));
-  for (int p = 0; p < ctx->n_learned; ++p)
-    for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_learned_phase(ctx, c, p));
-  for (int o = 0; o < ctx->n_or; ++o)
-    for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_overrelax_phase(ctx, c));
+  if (ctx->exec_mode != 0) {
+    PYRO_TRY(launch_batch_sweep_body(ctx));
+  } else {
+    for (int p = 0; p < ctx->n_metro; ++p)
+      for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_metropolis_phase(ctx, c, p, nullptr));
+    for (int p = 0; p < ctx->n_learned; ++p)
+      for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_learned_phase(ctx, c, p));
+    for (int o = 0; o < ctx->n_or; ++o)
+      for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_overrelax_phase(ctx, c));
+  }
   if (ctx->pt_every > 0 && (sweep_index + 1u) % (uint32_t)ctx->pt_every == 0u) {
     PYRO_TRY(launch_observables(ctx));
     PYRO_TRY(launch_pt_swap(ctx, 0, 0, 1));
@@ -59,11 +63,16 @@ static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
   return 0;
 }
 
+static uint64_t replica_cells(const pyro_ctx *ctx, int r) {
+  const uint64_t L = (uint64_t)ctx->h_L[r];
+  return L * L * L;
+}
+
 static void account_sweeps(pyro_ctx *ctx, uint32_t n) {
-  const uint64_t per = (uint64_t)ctx->n_metro * 16ull * (uint64_t)ctx->ncell;
-  for (auto &p : ctx->h_proposed) p += per * n;
-  const uint64_t per_l = (uint64_t)ctx->n_learned * 16ull * (uint64_t)ctx->ncell;
-  for (auto &p : ctx->h_proposed_learned) p += per_l * n;
+  for (int r = 0; r < ctx->R; ++r)
+    ctx->h_proposed[r] += (uint64_t)ctx->n_metro * 16ull * replica_cells(ctx, r) * n;
+  for (size_t r = 0; r < ctx->h_proposed_learned.size(); ++r)
+    ctx->h_proposed_learned[r] += (uint64_t)ctx->n_learned * 16ull * replica_cells(ctx, (int)r) * n;
   for (uint32_t i = 0; i < n; ++i) {
     if (ctx->pt_every > 0 && (ctx->h_sweep + 1u) % (uint32_t)ctx->pt_every == 0u) ++ctx->h_pt_round;
     ++ctx->h_sweep;
@@ -111,51 +120,66 @@ int pyro_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
   return 0;
 }
 
-int pyro_create(const pyro_config *cfg, pyro_ctx **out) {
-  if (!cfg || !out) return set_error(PYRO_EINVAL, "pyro_create: null argument");
-  *out = nullptr;
-  if (cfg->L < 2 || cfg->L > 256)
-    return set_error(PYRO_EINVAL, "pyro_create: L=%d outside [2, 256]", cfg->L);
-  if (cfg->n_replicas < 1 || cfg->n_replicas > 65535)
-    return set_error(PYRO_EINVAL, "pyro_create: n_replicas=%d outside [1, 65535]", cfg->n_replicas);
-  if (!cfg->temperatures) return set_error(PYRO_EINVAL, "pyro_create: null temperatures");
-  for (int r = 0; r < cfg->n_replicas; ++r)
-    if (!(cfg->temperatures[r] > 0.0) || !std::isfinite(cfg->temperatures[r]))
-      return set_error(PYRO_EINVAL, "pyro_create: temperature[%d]=%g not positive", r,
-                       cfg->temperatures[r]);
-  if (cfg->n_metro < 0 || cfg->n_or < 0 || cfg->pt_every < 0)
+// Shared construction of uniform and batch ctxs: Ls[r] is replica r's size.
+static int create_ctx(const std::vector<int> &Ls, const double *temps, uint64_t seed, double J,
+                      double spin_len2, int n_metro, int n_or, int pt_every, float cos_cone,
+                      int device, pyro_ctx **out) {
+  const int R = (int)Ls.size();
+  if (R < 1 || R > 65535) return set_error(PYRO_EINVAL, "pyro_create: n_replicas=%d outside [1, 65535]", R);
+  if (!temps) return set_error(PYRO_EINVAL, "pyro_create: null temperatures");
+  for (int r = 0; r < R; ++r)
+    if (!(temps[r] > 0.0) || !std::isfinite(temps[r]))
+      return set_error(PYRO_EINVAL, "pyro_create: temperature[%d]=%g not positive", r, temps[r]);
+  if (n_metro < 0 || n_or < 0 || pt_every < 0 || n_metro > 255)
     return set_error(PYRO_EINVAL, "pyro_create: negative pass counts");
-  if (!(cfg->cos_cone >= -1.0f && cfg->cos_cone < 1.0f))
-    return set_error(PYRO_EINVAL, "pyro_create: cos_cone=%g outside [-1, 1)", cfg->cos_cone);
-  const size_t ncell = (size_t)cfg->L * cfg->L * cfg->L;
-  const size_t nfloat = (size_t)cfg->n_replicas * 48 * ncell;
-  if (ncell * 16 > (size_t)INT32_MAX)
-    return set_error(PYRO_EINVAL, "pyro_create: L=%d too large for 32-bit site indices", cfg->L);
-
+  if (!(cos_cone >= -1.0f && cos_cone < 1.0f))
+    return set_error(PYRO_EINVAL, "pyro_create: cos_cone=%g outside [-1, 1)", cos_cone);
   int ndev = 0;
   PYRO_HIP(hipGetDeviceCount(&ndev));
-  if (cfg->device < 0 || cfg->device >= ndev)
-    return set_error(PYRO_EINVAL, "pyro_create: device %d of %d", cfg->device, ndev);
-  PYRO_HIP(hipSetDevice(cfg->device));
+  if (device < 0 || device >= ndev)
+    return set_error(PYRO_EINVAL, "pyro_create: device %d of %d", device, ndev);
+  PYRO_HIP(hipSetDevice(device));
 
   pyro_ctx *ctx = new pyro_ctx();
-  ctx->L = cfg->L;
-  ctx->R = cfg->n_replicas;
-  ctx->ncell = (int)ncell;
-  ctx->device = cfg->device;
-  ctx->seed = cfg->seed;
-  ctx->jeff = (float)(cfg->J * cfg->spin_len2);
-  ctx->n_metro = cfg->n_metro;
-  ctx->n_or = cfg->n_or;
-  ctx->pt_every = cfg->pt_every;
-  ctx->obs_blocks = (ctx->ncell + kObsBlock - 1) / kObsBlock;
-  ctx->h_proposed.assign(ctx->R, 0);
+  ctx->R = R;
+  ctx->h_L = Ls;
+  ctx->hetero = false;
+  for (int r = 1; r < R; ++r)
+    if (Ls[r] != Ls[0]) ctx->hetero = true;
+  ctx->L = ctx->hetero ? 0 : Ls[0];
+  ctx->ncell = ctx->hetero ? 0 : Ls[0] * Ls[0] * Ls[0];
+  ctx->h_off.resize(R);
+  size_t total = 0;
+  for (int r = 0; r < R; ++r) {
+    ctx->h_off[r] = (long long)total;
+    total += (size_t)48 * Ls[r] * Ls[r] * Ls[r];
+  }
+  ctx->total_floats = total;
+  // Exchange groups: one per lattice size, ascending.
+  std::vector<int> sizes(Ls);
+  std::sort(sizes.begin(), sizes.end());
+  sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+  int go = 0;
+  for (int L : sizes) {
+    const int n = (int)std::count(Ls.begin(), Ls.end(), L);
+    ctx->h_group_off.push_back(go);
+    ctx->h_group_len.push_back(n);
+    go += n;
+  }
+  ctx->exec_mode = ctx->hetero ? 2 : 0;
+  ctx->device = device;
+  ctx->seed = seed;
+  ctx->jeff = (float)(J * spin_len2);
+  ctx->n_metro = n_metro;
+  ctx->n_or = n_or;
+  ctx->pt_every = pt_every;
+  ctx->obs_blocks = ctx->hetero ? 1 : (ctx->ncell + kObsBlock - 1) / kObsBlock;
+  ctx->h_proposed.assign(R, 0);
 
   auto fail = [&](int rc) {
     pyro_destroy(ctx);
     return rc;
   };
-  const int R = ctx->R;
 #define ALLOC(ptr, count)                                                                  \
   do {                                                                                     \
     if (hipMalloc((void **)&(ptr), sizeof(*(ptr)) * (size_t)(count)) != hipSuccess)        \
@@ -164,7 +188,7 @@ int pyro_create(const pyro_config *cfg, pyro_ctx **out) {
   } while (0)
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(set_error(PYRO_EHIP, "pyro_create: stream creation failed"));
-  ALLOC(ctx->d_spins, nfloat);
+  ALLOC(ctx->d_spins, total);
   ALLOC(ctx->d_temps, R);
   ALLOC(ctx->d_beta, R);
   ALLOC(ctx->d_cosc, R);
@@ -178,7 +202,7 @@ int pyro_create(const pyro_config *cfg, pyro_ctx **out) {
 #undef ALLOC
 
   std::vector<uint32_t> ctl(kCtlWords, 0u);
-  std::vector<float> cosc(R, cfg->cos_cone);
+  std::vector<float> cosc(R, cos_cone);
   if (hipMemcpy(ctx->d_ctl, ctl.data(), sizeof(uint32_t) * kCtlWords, hipMemcpyHostToDevice) !=
           hipSuccess ||
       hipMemset(ctx->d_acc, 0, sizeof(unsigned long long) * R) != hipSuccess ||
@@ -186,12 +210,42 @@ int pyro_create(const pyro_config *cfg, pyro_ctx **out) {
       hipMemset(ctx->d_anchor, 0, sizeof(double) * R) != hipSuccess ||
       hipMemcpy(ctx->d_cosc, cosc.data(), sizeof(float) * R, hipMemcpyHostToDevice) != hipSuccess)
     return fail(set_error(PYRO_EHIP, "pyro_create: initial upload failed"));
-  int rc = pyro_set_temperatures(ctx, cfg->temperatures);
+  int rc = batch_setup(ctx, 0);
+  if (rc) return fail(rc);
+  rc = pyro_set_temperatures(ctx, temps);
   if (rc) return fail(rc);
   rc = pyro_init_random_spins(ctx);
   if (rc) return fail(rc);
   *out = ctx;
   return 0;
+}
+
+int pyro_create(const pyro_config *cfg, pyro_ctx **out) {
+  if (!cfg || !out) return set_error(PYRO_EINVAL, "pyro_create: null argument");
+  *out = nullptr;
+  if (cfg->L < 2 || cfg->L > 256)
+    return set_error(PYRO_EINVAL, "pyro_create: L=%d outside [2, 256]", cfg->L);
+  if ((size_t)cfg->L * cfg->L * cfg->L * 16 > (size_t)INT32_MAX)
+    return set_error(PYRO_EINVAL, "pyro_create: L=%d too large for 32-bit site indices", cfg->L);
+  if (cfg->n_replicas < 1 || cfg->n_replicas > 65535)
+    return set_error(PYRO_EINVAL, "pyro_create: n_replicas=%d outside [1, 65535]", cfg->n_replicas);
+  std::vector<int> Ls(cfg->n_replicas, cfg->L);
+  return create_ctx(Ls, cfg->temperatures, cfg->seed, cfg->J, cfg->spin_len2, cfg->n_metro,
+                    cfg->n_or, cfg->pt_every, cfg->cos_cone, cfg->device, out);
+}
+
+int pyro_create_batch(const pyro_batch_config *cfg, pyro_ctx **out) {
+  if (!cfg || !out || !cfg->L) return set_error(PYRO_EINVAL, "pyro_create_batch: null argument");
+  *out = nullptr;
+  if (cfg->n_replicas < 1 || cfg->n_replicas > 65535)
+    return set_error(PYRO_EINVAL, "pyro_create_batch: n_replicas=%d outside [1, 65535]",
+                     cfg->n_replicas);
+  std::vector<int> Ls(cfg->L, cfg->L + cfg->n_replicas);
+  for (int r = 0; r < cfg->n_replicas; ++r)
+    if (!batch_size_supported(Ls[r]))
+      return set_error(PYRO_EINVAL, "pyro_create_batch: L[%d]=%d not in {4,8,12,16,24,32}", r, Ls[r]);
+  return create_ctx(Ls, cfg->temperatures, cfg->seed, cfg->J, cfg->spin_len2, cfg->n_metro,
+                    cfg->n_or, cfg->pt_every, cfg->cos_cone, cfg->device, out);
 }
 
 int pyro_destroy(pyro_ctx *ctx) {
@@ -200,6 +254,7 @@ int pyro_destroy(pyro_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   drop_graph(ctx);
   mlp_free(ctx);
+  batch_free(ctx);
   void *ptrs[] = {ctx->d_spins, ctx->d_temps, ctx->d_beta,     ctx->d_cosc, ctx->d_replica_at,
                   ctx->d_acc,   ctx->d_dE,    ctx->d_anchor,   ctx->d_partials, ctx->d_obs,
                   ctx->d_ctl,   ctx->d_series};
@@ -222,7 +277,7 @@ int pyro_init_random_spins(pyro_ctx *ctx) {
 int pyro_set_spins(pyro_ctx *ctx, const float *soa) {
   if (!ctx || !soa) return set_error(PYRO_EINVAL, "pyro_set_spins: null argument");
   PYRO_TRY(device_guard(ctx));
-  const size_t n = (size_t)ctx->R * 48 * ctx->ncell;
+  const size_t n = ctx->total_floats;
   PYRO_HIP(hipMemcpyAsync(ctx->d_spins, soa, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
   PYRO_TRY(reanchor(ctx));
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
@@ -232,7 +287,7 @@ int pyro_set_spins(pyro_ctx *ctx, const float *soa) {
 int pyro_get_spins(pyro_ctx *ctx, float *soa) {
   if (!ctx || !soa) return set_error(PYRO_EINVAL, "pyro_get_spins: null argument");
   PYRO_TRY(device_guard(ctx));
-  const size_t n = (size_t)ctx->R * 48 * ctx->ncell;
+  const size_t n = ctx->total_floats;
   PYRO_HIP(hipMemcpyAsync(soa, ctx->d_spins, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   return 0;
@@ -241,7 +296,7 @@ int pyro_get_spins(pyro_ctx *ctx, float *soa) {
 int pyro_spins_device_ptr(pyro_ctx *ctx, float **dev_ptr, size_t *n_floats) {
   if (!ctx || !dev_ptr || !n_floats) return set_error(PYRO_EINVAL, "pyro_spins_device_ptr: null");
   *dev_ptr = ctx->d_spins;
-  *n_floats = (size_t)ctx->R * 48 * ctx->ncell;
+  *n_floats = ctx->total_floats;
   return 0;
 }
 
@@ -256,9 +311,13 @@ int pyro_set_temperatures(pyro_ctx *ctx, const double *T) {
       return set_error(PYRO_EINVAL, "pyro_set_temperatures: T[%d]=%g not positive", r, T[r]);
     beta[r] = (float)(1.0 / T[r]);
   }
+  // Rungs grouped by lattice size (ascending), each group by descending
+  // temperature, ties by replica index (gsl_sort_largest_index order).
   std::iota(order.begin(), order.end(), 0);
-  // Descending temperature, ties by replica index (gsl_sort_largest_index order).
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return T[a] > T[b]; });
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (ctx->h_L[a] != ctx->h_L[b]) return ctx->h_L[a] < ctx->h_L[b];
+    return T[a] > T[b];
+  });
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   PYRO_HIP(hipMemcpy(ctx->d_temps, T, sizeof(double) * R, hipMemcpyHostToDevice));
   PYRO_HIP(hipMemcpy(ctx->d_beta, beta.data(), sizeof(float) * R, hipMemcpyHostToDevice));
@@ -484,7 +543,10 @@ int pyro_order_params(pyro_ctx *ctx, double *op12) {
   if (!ctx || !op12) return set_error(PYRO_EINVAL, "pyro_order_params: null argument");
   std::vector<double> sub((size_t)ctx->R * 12);
   PYRO_TRY(pyro_sublattice_sums(ctx, sub.data()));
-  return pyro_fold_order_params(sub.data(), ctx->R, 16.0 * ctx->ncell, op12);
+  for (int r = 0; r < ctx->R; ++r)
+    PYRO_TRY(pyro_fold_order_params(sub.data() + 12 * (size_t)r, 1,
+                                    16.0 * (double)replica_cells(ctx, r), op12 + 12 * (size_t)r));
+  return 0;
 }
 
 int pyro_energy_incremental(pyro_ctx *ctx, double *E) {
@@ -525,10 +587,13 @@ int pyro_set_learned_passes(pyro_ctx *ctx, int n_learned) {
   if (n_learned < 0 || n_learned > 255) return set_error(PYRO_EINVAL, "n_learned %d", n_learned);
   if (n_learned > 0 && !ctx->mlp_ready)
     return set_error(PYRO_ESTATE, "pyro_set_learned_passes: set or train the network first");
+  if (n_learned > 0 && ctx->hetero)
+    return set_error(PYRO_ESTATE, "pyro_set_learned_passes: needs a uniform lattice size");
   PYRO_TRY(device_guard(ctx));
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   drop_graph(ctx);  // the captured sweep has the old schedule
   ctx->n_learned = n_learned;
+  if (n_learned > 0) ctx->exec_mode = 0;  // learned passes run as per-phase kernels
   return 0;
 }
 
@@ -542,6 +607,152 @@ int pyro_set_kernel_options(pyro_ctx *ctx, int phase_variant, int track_or_energ
   ctx->phase_variant = phase_variant;
   ctx->track_or = track_or_energy != 0;
   return 0;
+}
+
+int pyro_set_exec_mode(pyro_ctx *ctx, int mode, int block, int n_bins) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (mode < 0 || mode > 2) return set_error(PYRO_EINVAL, "exec mode %d not 0/1/2", mode);
+  if (mode == 0 && ctx->hetero)
+    return set_error(PYRO_EINVAL, "exec mode 0 needs a uniform lattice size");
+  if (mode != 0 && !batch_size_supported(ctx->hetero ? 4 : ctx->L))
+    return set_error(PYRO_EINVAL, "exec mode %d needs L in {4,8,12,16,24,32}", mode);
+  if (mode != 0 && ctx->n_learned > 0)
+    return set_error(PYRO_ESTATE, "learned passes run only in exec mode 0");
+  if (mode == 1 && block != 256 && block != 512 && block != 1024)
+    return set_error(PYRO_EINVAL, "persistent block size %d not 256/512/1024", block);
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  drop_graph(ctx);
+  if (mode == 1 && n_bins != ctx->n_bins) PYRO_TRY(batch_setup(ctx, n_bins));
+  ctx->exec_mode = mode;
+  if (mode == 1) ctx->batch_block = block;
+  return 0;
+}
+
+int pyro_get_exec_mode(pyro_ctx *ctx, int *mode, int *block, int *n_bins) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (mode) *mode = ctx->exec_mode;
+  if (block) *block = ctx->batch_block;
+  if (n_bins) *n_bins = ctx->n_bins;
+  return 0;
+}
+
+int pyro_get_layout(pyro_ctx *ctx, int *L, long long *offset) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  for (int r = 0; r < ctx->R; ++r) {
+    if (L) L[r] = ctx->h_L[r];
+    if (offset) offset[r] = ctx->h_off[r];
+  }
+  return 0;
+}
+
+// Time `sweeps` sweeps (eager launches, HIP events on the ctx stream) in each
+// candidate execution, restoring the full state after each trial so tuning
+// leaves the chain untouched; keep the fastest. Candidates: per-phase kernels
+// (uniform ctx: cell-per-thread and site-per-thread), the persistent sweep
+// kernel with 256/512/1024 threads and one replica per block, and the tile
+// list. results (optional) receives {mode, block, variant, ms per sweep} rows.
+int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (sweeps < 1 || sweeps > 1000) return set_error(PYRO_EINVAL, "pyro_autotune: sweeps %d", sweeps);
+  if (ctx->n_learned > 0) return set_error(PYRO_ESTATE, "pyro_autotune: learned passes are per-phase only");
+  PYRO_TRY(device_guard(ctx));
+  struct Cand { int mode, block, variant; };
+  std::vector<Cand> cands;
+  if (!ctx->hetero) {
+    cands.push_back({0, 0, 1});
+    cands.push_back({0, 0, 0});
+  }
+  if (batch_size_supported(ctx->hetero ? 4 : ctx->L)) {
+    cands.push_back({1, 256, 1});
+    cands.push_back({1, 512, 1});
+    cands.push_back({1, 1024, 1});
+    cands.push_back({2, 256, 1});
+  }
+  if (cands.empty()) return 0;
+  // Snapshot of everything a sweep changes.
+  const int R = ctx->R;
+  float *s_spins = nullptr;
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  PYRO_HIP(hipMalloc((void **)&s_spins, sizeof(float) * ctx->total_floats));
+  std::vector<double> temps(R), dE(R);
+  std::vector<float> beta(R), cosc(R);
+  std::vector<int> at(R);
+  std::vector<unsigned long long> acc(R);
+  std::vector<uint32_t> ctl(kCtlWords);
+  int rc = 0;
+  auto snap = [&](bool save) -> int {
+    const hipMemcpyKind k = save ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice;
+    if (hipMemcpy(save ? (void *)s_spins : (void *)ctx->d_spins,
+                  save ? (void *)ctx->d_spins : (void *)s_spins, sizeof(float) * ctx->total_floats,
+                  hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(save ? (void *)temps.data() : (void *)ctx->d_temps, save ? (void *)ctx->d_temps : (void *)temps.data(), sizeof(double) * R, k) != hipSuccess ||
+        hipMemcpy(save ? (void *)dE.data() : (void *)ctx->d_dE, save ? (void *)ctx->d_dE : (void *)dE.data(), sizeof(double) * R, k) != hipSuccess ||
+        hipMemcpy(save ? (void *)beta.data() : (void *)ctx->d_beta, save ? (void *)ctx->d_beta : (void *)beta.data(), sizeof(float) * R, k) != hipSuccess ||
+        hipMemcpy(save ? (void *)cosc.data() : (void *)ctx->d_cosc, save ? (void *)ctx->d_cosc : (void *)cosc.data(), sizeof(float) * R, k) != hipSuccess ||
+        hipMemcpy(save ? (void *)at.data() : (void *)ctx->d_replica_at, save ? (void *)ctx->d_replica_at : (void *)at.data(), sizeof(int) * R, k) != hipSuccess ||
+        hipMemcpy(save ? (void *)acc.data() : (void *)ctx->d_acc, save ? (void *)ctx->d_acc : (void *)acc.data(), sizeof(unsigned long long) * R, k) != hipSuccess ||
+        hipMemcpy(save ? (void *)ctl.data() : (void *)ctx->d_ctl, save ? (void *)ctx->d_ctl : (void *)ctl.data(), sizeof(uint32_t) * kCtlWords, k) != hipSuccess)
+      return set_error(PYRO_EHIP, "pyro_autotune: state snapshot failed");
+    return 0;
+  };
+  rc = snap(true);
+  const uint32_t h_sweep = ctx->h_sweep, h_round = ctx->h_pt_round;
+  const std::vector<uint64_t> h_prop = ctx->h_proposed;
+  const int mode0 = ctx->exec_mode, block0 = ctx->batch_block, variant0 = ctx->phase_variant;
+  double best = 1e30;
+  Cand best_c = {mode0, block0, variant0};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = set_error(PYRO_EHIP, "pyro_autotune: event creation failed");
+  int nres = 0;
+  for (size_t i = 0; i < cands.size() && !rc; ++i) {
+    const Cand c = cands[i];
+    drop_graph(ctx);
+    if (c.mode == 1 && ctx->n_bins != R) rc = batch_setup(ctx, R);
+    if (rc) break;
+    ctx->exec_mode = c.mode;
+    ctx->batch_block = c.block ? c.block : block0;
+    ctx->phase_variant = c.variant;
+    rc = enqueue_sweep(ctx, ctx->h_sweep);  // warm-up (code load, caches)
+    if (!rc && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = set_error(PYRO_EHIP, "event record");
+    for (int s = 0; s < sweeps && !rc; ++s) rc = enqueue_sweep(ctx, ctx->h_sweep + 1u + (uint32_t)s);
+    float ms = 0.0f;
+    if (!rc && (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = set_error(PYRO_EHIP, "pyro_autotune: timing failed");
+    if (rc) break;
+    const double per = ms / sweeps;
+    if (results) {
+      results[4 * nres + 0] = c.mode;
+      results[4 * nres + 1] = c.block;
+      results[4 * nres + 2] = c.variant;
+      results[4 * nres + 3] = per;
+    }
+    ++nres;
+    if (per < best) {
+      best = per;
+      best_c = c;
+    }
+    rc = snap(false);
+    ctx->h_sweep = h_sweep;
+    ctx->h_pt_round = h_round;
+    ctx->h_proposed = h_prop;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(s_spins);
+  drop_graph(ctx);
+  ctx->exec_mode = best_c.mode;
+  ctx->batch_block = best_c.block ? best_c.block : block0;
+  ctx->phase_variant = best_c.variant;
+  if (n_results) *n_results = nres;
+  if (rc) {
+    ctx->exec_mode = mode0;
+    ctx->batch_block = block0;
+    ctx->phase_variant = variant0;
+  }
+  return rc;
 }
 
 int pyro_set_passes(pyro_ctx *ctx, int n_metro, int n_or) {
@@ -626,15 +837,14 @@ int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, dou
       rc = set_error(PYRO_EHIP, "pyro_time_phase: event creation failed");
       break;
     }
+  // The events ride in the kernel's own dispatch (hipExtLaunchKernelGGL), so
+  // each pair brackets the kernel's execution and nothing else.
   for (int i = 0; i < n_launches && rc == 0; ++i) {
-    if (hipEventRecord(ev[2 * i], ctx->stream) != hipSuccess) {
-      rc = set_error(PYRO_EHIP, "pyro_time_phase: event record failed");
-      break;
-    }
+    ctx->time_ev0 = ev[2 * i];
+    ctx->time_ev1 = ev[2 * i + 1];
     rc = kind == 0 ? launch_metropolis_phase(ctx, i & 3, 0, nullptr)
                    : launch_overrelax_phase(ctx, i & 3);
-    if (rc == 0 && hipEventRecord(ev[2 * i + 1], ctx->stream) != hipSuccess)
-      rc = set_error(PYRO_EHIP, "pyro_time_phase: event record failed");
+    ctx->time_ev0 = ctx->time_ev1 = nullptr;
   }
   if (rc == 0 && hipStreamSynchronize(ctx->stream) != hipSuccess)
     rc = set_error(PYRO_EHIP, "pyro_time_phase: synchronise failed");

@@ -1,0 +1,198 @@
+// One cell's update for one colour phase, shared by every phase kernel (the
+// uniform-lattice kernels and the heterogeneous batch kernels), so all of them
+// produce the same spins bit for bit.
+//
+// A colour c is the independent set {c, c+4, c+8, c+12} of every cell
+// (Pyrochlore.cu:36-43). With the colour and the lattice size fixed at compile
+// time the 24 links of the cell's 4 target sites fold into constant offsets
+// from three wrapped coordinates per axis; a neighbour shared by two targets
+// is loaded once; the 4 results are stored after all loads (targets are
+// mutually non-adjacent, so deferring the stores changes nothing).
+#pragma once
+
+#include "pyro_internal.h"
+
+namespace pyro {
+
+constexpr Link kLinks[16][6] = PYRO_LINKS16_INIT;
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// KIND 0: Metropolis with the cone proposal (MCMultipleTemperaturesPerSystem,
+// Pyrochlore.cu:829-885). KIND 1: over-relaxation by pi (OverRelaxationByPiStep,
+// Pyrochlore.cu:975-1007). TRACK: FP64 energy change also for KIND 1.
+// Sx: this replica's spins ([x,y,z][site][cell]); r: the replica's global
+// index (keys the Philox counters). Adds the energy change to de and the
+// accepted proposals to nacc.
+template <int KIND, int COLOUR, int LT, bool TRACK>
+__device__ __forceinline__ void cell_update(float *__restrict__ Sx, int cell, int r, int pass,
+                                            float beta_r, float cosc_r, uint32_t sweep,
+                                            uint32_t key0, uint32_t key1, float jeff, double &de,
+                                            unsigned &nacc) {
+  constexpr int L = LT, LL = LT * LT, NC = LT * LT * LT;
+  float *Sy = Sx + 16 * NC;
+  float *Sz = Sy + 16 * NC;
+  const int z = cell / LL;
+  const int y = (cell / L) % L;
+  const int x = cell % L;
+  const int X[3] = {x == 0 ? L - 1 : x - 1, x, x == L - 1 ? 0 : x + 1};
+  const int Y[3] = {(y == 0 ? L - 1 : y - 1) * L, y * L, (y == L - 1 ? 0 : y + 1) * L};
+  const int Z[3] = {(z == 0 ? L - 1 : z - 1) * LL, z * LL, (z == L - 1 ? 0 : z + 1) * LL};
+  f3 s[4], sp[4];
+  bool wr[4];
+  float fx[4], fy[4], fz[4];
+  double gx[4], gy[4], gz[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int site = COLOUR + 4 * k;
+    fx[k] = 0.0f;
+    fy[k] = 0.0f;
+    fz[k] = 0.0f;
+    gx[k] = gy[k] = gz[k] = 0.0;
+#pragma unroll
+    for (int n = 0; n < 6; ++n) {
+      const Link lk = kLinks[site][n];
+      const int j = lk.site * NC + Z[lk.dz + 1] + Y[lk.dy + 1] + X[lk.dx + 1];
+      const float a = Sx[j], b = Sy[j], c = Sz[j];
+      fx[k] = fx[k] + a;
+      fy[k] = fy[k] + b;
+      fz[k] = fz[k] + c;
+      if (KIND == 0 || TRACK) {
+        gx[k] += (double)a;
+        gy[k] += (double)b;
+        gz[k] += (double)c;
+      }
+    }
+    const int i = site * NC + cell;
+    s[k] = {Sx[i], Sy[i], Sz[i]};
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int site = COLOUR + 4 * k;
+    if (KIND == 0) {
+      const u32x4 w = philox4x32_10(
+          rng_counter((uint32_t)cell, (uint32_t)site, (uint32_t)pass, kKindMetropolis, (uint32_t)r,
+                      sweep),
+          key0, key1);
+      const f3 h = {-jeff * fx[k], -jeff * fy[k], -jeff * fz[k]};
+      sp[k] = cone_proposal(s[k], cosc_r, u01(w.x), u01(w.y));
+      wr[k] = metropolis_accept(h, s[k], sp[k], beta_r, u01(w.z));
+      nacc += wr[k] ? 1u : 0u;
+    } else {
+      const f3 f = {fx[k], fy[k], fz[k]};
+      sp[k] = overrelax(f, s[k]);
+      wr[k] = true;
+    }
+    if ((KIND == 0 || TRACK) && wr[k])
+      de += (double)jeff * (((double)sp[k].x - (double)s[k].x) * gx[k] +
+                            ((double)sp[k].y - (double)s[k].y) * gy[k] +
+                            ((double)sp[k].z - (double)s[k].z) * gz[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (wr[k]) {
+      const int i = (COLOUR + 4 * k) * NC + cell;
+      Sx[i] = sp[k].x;
+      Sy[i] = sp[k].y;
+      Sz[i] = sp[k].z;
+    }
+  }
+}
+
+// One site's update (site = colour + 4k, runtime), for kernels that cannot
+// afford the registers of cell_update (its 4 concurrent targets): the same
+// float expressions in the same order, so the same results bit for bit.
+// links: the link table in constant memory of the calling translation unit.
+template <int KIND, int LT, bool TRACK>
+__device__ __forceinline__ void site_update(float *__restrict__ Sx, const Link (*links)[6], int cell,
+                                            int site, int r, int pass, float beta_r, float cosc_r,
+                                            uint32_t sweep, uint32_t key0, uint32_t key1,
+                                            float jeff, double &de, unsigned &nacc) {
+  constexpr int L = LT, LL = LT * LT, NC = LT * LT * LT;
+  float *Sy = Sx + 16 * NC;
+  float *Sz = Sy + 16 * NC;
+  const int z = cell / LL;
+  const int y = (cell / L) % L;
+  const int x = cell % L;
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    const Link lk = links[site][n];
+    const int x1 = x + lk.dx, y1 = y + lk.dy, z1 = z + lk.dz;
+    const int j = lk.site * NC + (z1 < 0 ? z1 + L : (z1 >= L ? z1 - L : z1)) * LL +
+                  (y1 < 0 ? y1 + L : (y1 >= L ? y1 - L : y1)) * L +
+                  (x1 < 0 ? x1 + L : (x1 >= L ? x1 - L : x1));
+    const float a = Sx[j], b = Sy[j], c = Sz[j];
+    fx = fx + a;
+    fy = fy + b;
+    fz = fz + c;
+    if (KIND == 0 || TRACK) {
+      gx += (double)a;
+      gy += (double)b;
+      gz += (double)c;
+    }
+  }
+  const int i = site * NC + cell;
+  const f3 s = {Sx[i], Sy[i], Sz[i]};
+  f3 sp;
+  bool wr;
+  if (KIND == 0) {
+    const u32x4 w = philox4x32_10(
+        rng_counter((uint32_t)cell, (uint32_t)site, (uint32_t)pass, kKindMetropolis, (uint32_t)r,
+                    sweep),
+        key0, key1);
+    const f3 h = {-jeff * fx, -jeff * fy, -jeff * fz};
+    sp = cone_proposal(s, cosc_r, u01(w.x), u01(w.y));
+    wr = metropolis_accept(h, s, sp, beta_r, u01(w.z));
+    nacc += wr ? 1u : 0u;
+  } else {
+    const f3 f = {fx, fy, fz};
+    sp = overrelax(f, s);
+    wr = true;
+  }
+  if (wr) {
+    if (KIND == 0 || TRACK)
+      de += (double)jeff * (((double)sp.x - (double)s.x) * gx + ((double)sp.y - (double)s.y) * gy +
+                            ((double)sp.z - (double)s.z) * gz);
+    Sx[i] = sp.x;
+    Sy[i] = sp.y;
+    Sz[i] = sp.z;
+  }
+}
+
+// Block-wide sum of (de, nacc); the result is valid in thread 0. Every thread
+// of the block must call it. Uses 2 * (blockDim / 64) words of shared memory.
+template <int BLOCK>
+__device__ __forceinline__ void block_sum(double &de, unsigned &nacc) {
+  constexpr int NW = BLOCK / 64;
+  __shared__ double s_de[NW];
+  __shared__ unsigned s_acc[NW];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  de = wave_sum_d(de);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) nacc += __shfl_xor(nacc, off, 64);
+  if (lane == 0) {
+    s_de[wv] = de;
+    s_acc[wv] = nacc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    unsigned a = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      t += s_de[w];
+      a += s_acc[w];
+    }
+    de = t;
+    nacc = a;
+  }
+  __syncthreads();
+}
+
+}  // namespace pyro

@@ -63,6 +63,9 @@ struct pyro_ctx {
   int n_metro = 1, n_or = 0, pt_every = 0;
   int phase_variant = 1;   // 0: wave per site (any L); 1: cell per thread (compile-time L)
   bool track_or = false;   // FP64 energy bookkeeping also in over-relaxation phases
+  // When set, the next phase launch records these events as part of its
+  // dispatch (hipExtLaunchKernelGGL), timing the kernel itself.
+  hipEvent_t time_ev0 = nullptr, time_ev1 = nullptr;
   hipStream_t stream = nullptr;
 
   // device state
@@ -81,8 +84,29 @@ struct pyro_ctx {
   int obs_blocks = 0;
   int record_cap = 0;
 
+  // Replica geometry. Uniform ctx: every replica has L; a batch ctx mixes
+  // sizes, replica r's spins start at float offset h_off[r].
+  bool hetero = false;
+  std::vector<int> h_L;
+  std::vector<long long> h_off;
+  size_t total_floats = 0;
+  int *d_L = nullptr;
+  long long *d_off = nullptr;
+  // Exchange groups (equal L): rungs h_group_off[g] .. +h_group_len[g] of d_replica_at.
+  std::vector<int> h_group_off, h_group_len;
+  // Execution mode of a sweep: 0 per-phase kernels (uniform only), 1 one
+  // persistent kernel per sweep (a block owns whole replicas), 2 per-phase
+  // kernels over a work list of 256-cell tiles of all replicas.
+  int exec_mode = 0;
+  int batch_block = 256;            // block size of the persistent kernel
+  int *d_order = nullptr;           // replicas in bin order
+  int *d_bin_start = nullptr;       // n_bins + 1 offsets into d_order
+  int n_bins = 0;
+  int *d_tiles = nullptr;           // 2 ints per tile: replica, tile index (-1: padding)
+  int n_tiles = 0;
+
   // learned proposal
-  int n_learned = 0;                     // learned passes per sweep (after Metropolis)
+  int n_learned = 0;                    // learned passes per sweep (after Metropolis)
   float *d_mlp_w = nullptr;              // packed FP32 master weights (kMlpParams)
   float *d_mlp_m = nullptr, *d_mlp_v = nullptr;  // Adam moments
   uint16_t *d_mlp_pack = nullptr;        // bf16 operand images for the MFMA forward
@@ -119,6 +143,14 @@ int launch_pt_swap(pyro_ctx *ctx, int parity, uint32_t round, int from_ctl);
 int launch_end_sweep(pyro_ctx *ctx);             // record row, advance sweep counter
 int launch_set_anchor(pyro_ctx *ctx);            // d_anchor = d_obs energy, d_dE = 0
 int launch_set_sweep(pyro_ctx *ctx, uint32_t sweep, uint32_t round);
+
+// Batches and alternative sweep executions (pyro_batch.hip).
+bool batch_size_supported(int L);
+int batch_setup(pyro_ctx *ctx, int n_bins);       // bins, tiles, device geometry arrays
+int launch_batch_init_spins(pyro_ctx *ctx);
+int launch_batch_observables(pyro_ctx *ctx);      // fills d_obs
+int launch_batch_sweep_body(pyro_ctx *ctx);       // Metropolis + OR passes of one sweep
+void batch_free(pyro_ctx *ctx);
 
 // Learned proposal (pyro_learned.hip).
 int launch_learned_phase(pyro_ctx *ctx, int colour, int pass);

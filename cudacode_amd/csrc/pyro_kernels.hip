@@ -10,6 +10,9 @@
 // Random numbers are drawn in registers (Philox4x32-10, one block function
 // per update), so the reference's pre-filled uniform buffer and its HBM
 // round trip (MonteCarlo.cu:3-13, 16 B per spin per sweep) are gone.
+#include <hip/hip_ext.h>
+
+#include "pyro_cell.h"
 #include "pyro_internal.h"
 
 namespace pyro {
@@ -123,127 +126,39 @@ __global__ __launch_bounds__(kPhaseBlock) void k_phase(
 }
 
 // ---------------------------------------------------------------------------
-// Cell-per-thread colour phase with the colour and the lattice size fixed at
-// compile time. Thread = one cell, all 4 sites of the colour: the 24 links
-// fold into constant offsets from three wrapped coordinates per axis, a
-// neighbour shared by two of the thread's targets is loaded once, and the 4
-// results are stored after all loads (the targets are mutually non-adjacent,
-// so deferring the stores changes nothing). Same Philox counters and float
-// expressions as k_phase, hence the same results bit for bit.
+// Cell-per-thread colour phase (pyro_cell.h) with the colour and the lattice
+// size fixed at compile time. Same Philox counters and float expressions as
+// k_phase, hence the same results bit for bit.
 //
 // Blocks are remapped so the hardware's round-robin dispatch over the 8 XCDs
 // (block b runs on XCD b % 8) gives each XCD a contiguous run of cells: the
 // z-1 / z+1 neighbour planes of a block then sit in the same XCD's L2.
-// grid = 1-D, n_blocks = R * ceil(ncell / 256), block = 256.
+// grid = 1-D, n_blocks = R * ceil(ncell / 256), block = 256; a block covers
+// cells of one replica only, so one atomic per block.
 // ---------------------------------------------------------------------------
-constexpr Link kLinks[16][6] = PYRO_LINKS16_INIT;
-
 template <int KIND, int COLOUR, int LT, bool TRACK>
 __global__ __launch_bounds__(256) void k_phase_cells(
     float *__restrict__ spins, int blocks_per_replica, int pass, const float *__restrict__ beta,
     const float *__restrict__ cosc, const uint32_t *__restrict__ ctl, uint32_t key0,
     uint32_t key1, float jeff, unsigned long long *__restrict__ acc, double *__restrict__ dE) {
-  constexpr int L = LT, LL = LT * LT, NC = LT * LT * LT;
+  constexpr int NC = LT * LT * LT;
   const int nb = gridDim.x;
   const int b = blockIdx.x;
   const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
   const int r = lb / blocks_per_replica;
   const int cell = (lb - r * blocks_per_replica) * 256 + threadIdx.x;
-  const bool valid = cell < NC;
-  float *Sx = spins + (size_t)r * 48 * NC;
-  float *Sy = Sx + 16 * NC;
-  float *Sz = Sy + 16 * NC;
-
   double de = 0.0;
   unsigned nacc = 0;
-  if (valid) {
-    const int z = cell / LL;
-    const int y = (cell / L) % L;
-    const int x = cell % L;
-    const int X[3] = {x == 0 ? L - 1 : x - 1, x, x == L - 1 ? 0 : x + 1};
-    const int Y[3] = {(y == 0 ? L - 1 : y - 1) * L, y * L, (y == L - 1 ? 0 : y + 1) * L};
-    const int Z[3] = {(z == 0 ? L - 1 : z - 1) * LL, z * LL, (z == L - 1 ? 0 : z + 1) * LL};
-    f3 s[4], sp[4];
-    bool wr[4];
-    float fx[4], fy[4], fz[4];
-    double gx[4], gy[4], gz[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int site = COLOUR + 4 * k;
-      fx[k] = 0.0f;
-      fy[k] = 0.0f;
-      fz[k] = 0.0f;
-      gx[k] = gy[k] = gz[k] = 0.0;
-#pragma unroll
-      for (int n = 0; n < 6; ++n) {
-        const Link lk = kLinks[site][n];
-        const int j = lk.site * NC + Z[lk.dz + 1] + Y[lk.dy + 1] + X[lk.dx + 1];
-        const float a = Sx[j], bb = Sy[j], c = Sz[j];
-        fx[k] = fx[k] + a;
-        fy[k] = fy[k] + bb;
-        fz[k] = fz[k] + c;
-        if (KIND == 0 || TRACK) {
-          gx[k] += (double)a;
-          gy[k] += (double)bb;
-          gz[k] += (double)c;
-        }
-      }
-      const int i = site * NC + cell;
-      s[k] = {Sx[i], Sy[i], Sz[i]};
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int site = COLOUR + 4 * k;
-      if (KIND == 0) {
-        const u32x4 w = philox4x32_10(
-            rng_counter((uint32_t)cell, (uint32_t)site, (uint32_t)pass, kKindMetropolis,
-                        (uint32_t)r, ctl[kCtlSweep]),
-            key0, key1);
-        const f3 h = {-jeff * fx[k], -jeff * fy[k], -jeff * fz[k]};
-        sp[k] = cone_proposal(s[k], cosc[r], u01(w.x), u01(w.y));
-        wr[k] = metropolis_accept(h, s[k], sp[k], beta[r], u01(w.z));
-        nacc += wr[k] ? 1u : 0u;
-      } else {
-        const f3 f = {fx[k], fy[k], fz[k]};
-        sp[k] = overrelax(f, s[k]);
-        wr[k] = true;
-      }
-      if ((KIND == 0 || TRACK) && wr[k])
-        de += (double)jeff * (((double)sp[k].x - (double)s[k].x) * gx[k] +
-                              ((double)sp[k].y - (double)s[k].y) * gy[k] +
-                              ((double)sp[k].z - (double)s[k].z) * gz[k]);
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (wr[k]) {
-        const int i = (COLOUR + 4 * k) * NC + cell;
-        Sx[i] = sp[k].x;
-        Sy[i] = sp[k].y;
-        Sz[i] = sp[k].z;
-      }
-    }
-  }
+  if (cell < NC)
+    cell_update<KIND, COLOUR, LT, TRACK>(spins + (size_t)r * 48 * NC, cell, r, pass,
+                                         KIND == 0 ? beta[r] : 0.0f, KIND == 0 ? cosc[r] : 0.0f,
+                                         KIND == 0 ? ctl[kCtlSweep] : 0u, key0, key1, jeff, de,
+                                         nacc);
   if (KIND == 0 || TRACK) {
-    // A block covers cells of one replica only: one atomic per block.
-    __shared__ double s_de[4];
-    __shared__ unsigned s_acc[4];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    de = wave_sum(de);
-    unsigned wacc = nacc;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) wacc += __shfl_xor(wacc, off, 64);
-    if (lane == 0) {
-      s_de[wv] = de;
-      s_acc[wv] = wacc;
-    }
-    __syncthreads();
+    block_sum<256>(de, nacc);
     if (threadIdx.x == 0) {
-      const double t = (s_de[0] + s_de[1]) + (s_de[2] + s_de[3]);
-      if (t != 0.0) atomicAdd(&dE[r], t);
-      if (KIND == 0) {
-        const unsigned a = s_acc[0] + s_acc[1] + s_acc[2] + s_acc[3];
-        if (a) atomicAdd(&acc[r], (unsigned long long)a);
-      }
+      if (de != 0.0) atomicAdd(&dE[r], de);
+      if (KIND == 0 && nacc) atomicAdd(&acc[r], (unsigned long long)nacc);
     }
   }
 }
@@ -355,11 +270,14 @@ __global__ __launch_bounds__(256) void k_observables_final(const double *__restr
 // on the energies in obs. Rungs are ordered by descending temperature;
 // pairs (c, c+1), c = parity, parity+2, ... One thread: R is small and the
 // pairs are visited in order, as in the reference.
+// A batch of mixed lattice sizes exchanges within groups of equal size: R is
+// the group's rung count, replica_at its rungs, `group` keys its draws, and
+// the exchange-round counter advances after the last group.
 __global__ void k_pt_swap(int R, const double *__restrict__ obs, double *__restrict__ temps,
                           float *__restrict__ beta, float *__restrict__ cosc,
                           int *__restrict__ replica_at, uint32_t *__restrict__ ctl,
                           uint32_t key0, uint32_t key1, int parity, uint32_t round,
-                          int from_ctl) {
+                          int from_ctl, int group, int last_group) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (from_ctl) {
     round = ctl[kCtlPtRound];
@@ -370,7 +288,8 @@ __global__ void k_pt_swap(int R, const double *__restrict__ obs, double *__restr
     const int t1 = replica_at[c], t2 = replica_at[c + 1];
     const double T1 = temps[t1], T2 = temps[t2];
     const double arg = (1.0 / T1 - 1.0 / T2) * (obs[(size_t)t1 * kObs] - obs[(size_t)t2 * kObs]);
-    const u32x4 w = philox4x32_10(rng_counter((uint32_t)c, 0, 0, kKindSwap, 0, round), key0, key1);
+    const u32x4 w = philox4x32_10(
+        rng_counter((uint32_t)c, 0, 0, kKindSwap, (uint32_t)group, round), key0, key1);
     const double u = u01_53(w.x, w.y);
     if (arg >= 0.0 || u < exp(arg)) {
       temps[t1] = T2;
@@ -387,7 +306,7 @@ __global__ void k_pt_swap(int R, const double *__restrict__ obs, double *__restr
     }
   }
   ctl[kCtlSwapsAcc] += (uint32_t)nacc;
-  if (from_ctl) ctl[kCtlPtRound] = round + 1u;
+  if (from_ctl && last_group) ctl[kCtlPtRound] = round + 1u;
 }
 
 // End of a sweep: append one row (incremental energy per rung) to the series
@@ -437,6 +356,7 @@ static inline uint32_t key_lo(const pyro_ctx *c) { return (uint32_t)c->seed; }
 static inline uint32_t key_hi(const pyro_ctx *c) { return (uint32_t)(c->seed >> 32); }
 
 int launch_init_spins(pyro_ctx *ctx) {
+  if (ctx->hetero) return launch_batch_init_spins(ctx);
   dim3 grid((ctx->ncell + 255) / 256, ctx->R);
   k_init_spins<<<grid, 256, 0, ctx->stream>>>(ctx->d_spins, ctx->ncell, key_lo(ctx), key_hi(ctx));
   PYRO_HIP(hipGetLastError());
@@ -446,6 +366,14 @@ int launch_init_spins(pyro_ctx *ctx) {
 template <int KIND, int LT>
 static void launch_phase_t(pyro_ctx *ctx, int colour, int pass) {
   dim3 grid((ctx->ncell + kPhaseCells - 1) / kPhaseCells, ctx->R);
+  if (ctx->time_ev0) {
+    hipExtLaunchKernelGGL(k_phase<KIND, LT>, grid, dim3(kPhaseBlock), 0, ctx->stream, ctx->time_ev0,
+                          ctx->time_ev1, 0, ctx->d_spins, ctx->L, ctx->ncell, colour, pass,
+                          (const float *)ctx->d_beta, (const float *)ctx->d_cosc,
+                          (const uint32_t *)ctx->d_ctl, key_lo(ctx), key_hi(ctx), ctx->jeff,
+                          ctx->d_acc, ctx->d_dE);
+    return;
+  }
   k_phase<KIND, LT><<<grid, kPhaseBlock, 0, ctx->stream>>>(
       ctx->d_spins, ctx->L, ctx->ncell, colour, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl,
       key_lo(ctx), key_hi(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE);
@@ -454,6 +382,14 @@ static void launch_phase_t(pyro_ctx *ctx, int colour, int pass) {
 template <int KIND, int COLOUR, int LT, bool TRACK>
 static void launch_cells_t(pyro_ctx *ctx, int pass) {
   const int bpr = (LT * LT * LT + 255) / 256;
+  if (ctx->time_ev0) {
+    hipExtLaunchKernelGGL(k_phase_cells<KIND, COLOUR, LT, TRACK>, dim3(bpr * ctx->R), dim3(256), 0,
+                          ctx->stream, ctx->time_ev0, ctx->time_ev1, 0, ctx->d_spins, bpr, pass,
+                          (const float *)ctx->d_beta, (const float *)ctx->d_cosc,
+                          (const uint32_t *)ctx->d_ctl, key_lo(ctx), key_hi(ctx), ctx->jeff,
+                          ctx->d_acc, ctx->d_dE);
+    return;
+  }
   k_phase_cells<KIND, COLOUR, LT, TRACK><<<bpr * ctx->R, 256, 0, ctx->stream>>>(
       ctx->d_spins, bpr, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl, key_lo(ctx), key_hi(ctx),
       ctx->jeff, ctx->d_acc, ctx->d_dE);
@@ -510,18 +446,21 @@ static void launch_phase(pyro_ctx *ctx, int colour, int pass) {
 }
 
 int launch_metropolis_phase(pyro_ctx *ctx, int colour, int pass, const uint32_t *) {
+  if (ctx->hetero) return set_error(PYRO_ESTATE, "single colour phases need a uniform lattice size");
   launch_phase<0>(ctx, colour, pass);
   PYRO_HIP(hipGetLastError());
   return 0;
 }
 
 int launch_overrelax_phase(pyro_ctx *ctx, int colour) {
+  if (ctx->hetero) return set_error(PYRO_ESTATE, "single colour phases need a uniform lattice size");
   launch_phase<1>(ctx, colour, 0);
   PYRO_HIP(hipGetLastError());
   return 0;
 }
 
 int launch_observables(pyro_ctx *ctx) {
+  if (ctx->hetero) return launch_batch_observables(ctx);
   dim3 grid(ctx->obs_blocks, ctx->R);
   k_observables<<<grid, kObsBlock, 0, ctx->stream>>>(ctx->d_spins, ctx->L, ctx->ncell,
                                                      ctx->d_partials);
@@ -533,10 +472,15 @@ int launch_observables(pyro_ctx *ctx) {
 }
 
 int launch_pt_swap(pyro_ctx *ctx, int parity, uint32_t round, int from_ctl) {
-  k_pt_swap<<<1, 64, 0, ctx->stream>>>(ctx->R, ctx->d_obs, ctx->d_temps, ctx->d_beta, ctx->d_cosc,
-                                       ctx->d_replica_at, ctx->d_ctl, key_lo(ctx), key_hi(ctx),
-                                       parity, round, from_ctl);
-  PYRO_HIP(hipGetLastError());
+  const int ng = (int)ctx->h_group_off.size();
+  for (int g = 0; g < ng; ++g) {
+    k_pt_swap<<<1, 64, 0, ctx->stream>>>(ctx->h_group_len[g], ctx->d_obs, ctx->d_temps,
+                                         ctx->d_beta, ctx->d_cosc,
+                                         ctx->d_replica_at + ctx->h_group_off[g], ctx->d_ctl,
+                                         key_lo(ctx), key_hi(ctx), parity, round, from_ctl, g,
+                                         g == ng - 1);
+    PYRO_HIP(hipGetLastError());
+  }
   return 0;
 }
 

@@ -33,7 +33,9 @@ EXPORTS = [
     "pyro_magnetization", "pyro_order_params", "pyro_sublattice_sums", "pyro_fold_order_params",
     "pyro_energy_incremental",
     "pyro_acceptance", "pyro_reset_acceptance", "pyro_adapt_cone", "pyro_record_start",
-    "pyro_record_fetch", "pyro_set_passes", "pyro_set_kernel_options", "pyro_mlp_set_weights", "pyro_mlp_get_weights",
+    "pyro_record_fetch", "pyro_set_passes", "pyro_set_kernel_options", "pyro_create_batch",
+    "pyro_get_layout", "pyro_set_exec_mode", "pyro_get_exec_mode", "pyro_autotune",
+    "pyro_mlp_set_weights", "pyro_mlp_get_weights",
     "pyro_mlp_eval", "pyro_vmf_draw", "pyro_mlp_loss_grad", "pyro_gather_environments",
     "pyro_get_environments", "pyro_clear_environments", "pyro_mlp_train", "pyro_learned_phase",
     "pyro_set_learned_passes", "pyro_acceptance_learned", "pyro_time_phase", "pyro_sync",
@@ -50,6 +52,22 @@ class Config(C.Structure):
     _fields_ = [
         ("L", C.c_int),
         ("n_replicas", C.c_int),
+        ("temperatures", C.POINTER(C.c_double)),
+        ("seed", C.c_uint64),
+        ("J", C.c_double),
+        ("spin_len2", C.c_double),
+        ("n_metro", C.c_int),
+        ("n_or", C.c_int),
+        ("pt_every", C.c_int),
+        ("cos_cone", C.c_float),
+        ("device", C.c_int),
+    ]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [
+        ("n_replicas", C.c_int),
+        ("L", C.POINTER(C.c_int)),
         ("temperatures", C.POINTER(C.c_double)),
         ("seed", C.c_uint64),
         ("J", C.c_double),
@@ -107,6 +125,12 @@ def lib():
                                       C.POINTER(C.c_double)]
         L.pyro_set_passes.argtypes = [vp, C.c_int, C.c_int]
         L.pyro_set_kernel_options.argtypes = [vp, C.c_int, C.c_int]
+        L.pyro_create_batch.argtypes = [C.POINTER(BatchConfig), C.POINTER(vp)]
+        L.pyro_get_layout.argtypes = [vp, vp, vp]
+        L.pyro_set_exec_mode.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.pyro_get_exec_mode.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int)]
+        L.pyro_autotune.argtypes = [vp, C.c_int, vp, C.POINTER(C.c_int)]
         L.pyro_mlp_set_weights.argtypes = [vp, vp]
         L.pyro_mlp_get_weights.argtypes = [vp, vp]
         L.pyro_mlp_eval.argtypes = [vp, vp, C.c_int, vp]
@@ -328,6 +352,29 @@ class Pyrochlore:
     def set_kernel_options(self, phase_variant: int = 1, track_or_energy: bool = False) -> None:
         _check(lib().pyro_set_kernel_options(self._ctx, phase_variant, 1 if track_or_energy else 0))
 
+    def set_exec_mode(self, mode: int, block: int = 1024, n_bins: int = 0) -> None:
+        """0: per-phase kernels; 1: one persistent kernel per sweep; 2: tile-list phases."""
+        _check(lib().pyro_set_exec_mode(self._ctx, mode, block, n_bins))
+
+    def exec_mode(self):
+        m, b, n = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().pyro_get_exec_mode(self._ctx, C.byref(m), C.byref(b), C.byref(n)))
+        return m.value, b.value, n.value
+
+    def autotune(self, sweeps: int = 5) -> np.ndarray:
+        """Try every execution, keep the fastest; rows (mode, block, variant, ms/sweep)."""
+        res = np.zeros(4 * 16, np.float64)
+        n = C.c_int()
+        _check(lib().pyro_autotune(self._ctx, sweeps, _p(res), C.byref(n)))
+        return res[: 4 * n.value].reshape(n.value, 4)
+
+    def layout(self):
+        """(L per replica, float offset of each replica's spins in get_spins())."""
+        L = np.empty(self.R, np.int32)
+        off = np.empty(self.R, np.int64)
+        _check(lib().pyro_get_layout(self._ctx, _p(L), _p(off)))
+        return L, off
+
     def set_passes(self, n_metro: int, n_or: int) -> None:
         _check(lib().pyro_set_passes(self._ctx, n_metro, n_or))
         self.n_metro, self.n_or = n_metro, n_or
@@ -405,3 +452,36 @@ class Pyrochlore:
 
     def sync(self) -> None:
         _check(lib().pyro_sync(self._ctx))
+
+
+class Batch(Pyrochlore):
+    """Replicas of different lattice sizes in one ctx (sizes in {4,8,12,16,24,32}).
+
+    Replica r's spins are the 48*L[r]^3 floats at layout()[1][r] of get_spins();
+    exchanges happen within groups of equal size. Sweeps run as one persistent
+    kernel per sweep (mode 1) or as tile-list phases (mode 2, the default);
+    autotune() picks the faster.
+    """
+
+    def __init__(self, L, temperatures, seed: int = 56081376, J: float = 1.0,
+                 spin_len2: float = 1.0, n_metro: int = 1, n_or: int = 0, pt_every: int = 0,
+                 cos_cone: float = -1.0, device: int = 0):
+        self._ctx = C.c_void_p()
+        self._L = np.ascontiguousarray(L, np.int32)
+        self._temps_init = np.ascontiguousarray(temperatures, np.float64)
+        if self._L.size != self._temps_init.size:
+            raise PyroError("Batch: one temperature per replica")
+        self.R = int(self._L.size)
+        self.L = None
+        self.N = 16 * self._L.astype(np.int64) ** 3
+        self.seed, self.J, self.spin_len2 = int(seed), float(J), float(spin_len2)
+        self.n_metro, self.n_or, self.pt_every = int(n_metro), int(n_or), int(pt_every)
+        cfg = BatchConfig(self.R, self._L.ctypes.data_as(C.POINTER(C.c_int)),
+                          self._temps_init.ctypes.data_as(C.POINTER(C.c_double)),
+                          self.seed & 0xFFFFFFFFFFFFFFFF, self.J, self.spin_len2, self.n_metro,
+                          self.n_or, self.pt_every, float(cos_cone), int(device))
+        _check(lib().pyro_create_batch(C.byref(cfg), C.byref(self._ctx)))
+
+    @property
+    def n_floats(self) -> int:
+        return int(3 * self.N.sum())

@@ -17,9 +17,10 @@ def trace(db: str, limit: int = 12) -> str:
     c = sqlite3.connect(db)
     rows = c.execute("select name, total_calls, total_duration, average, percentage "
                      "from top_kernels limit ?", (limit,)).fetchall()
+    # top_kernels reports durations in microseconds.
     out = ["| kernel | calls | total us | avg us | % |", "|---|---|---|---|---|"]
     for n, calls, tot, avg, pct in rows:
-        out.append(f"| `{short(n)}` | {calls} | {tot / 1e3:.1f} | {avg / 1e3:.2f} | {pct:.1f} |")
+        out.append(f"| `{short(n)}` | {calls} | {tot:.1f} | {avg:.3f} | {pct:.1f} |")
     return "\n".join(out)
 
 

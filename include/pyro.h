@@ -62,6 +62,46 @@ const char *pyro_last_error(void);
 int pyro_create(const pyro_config *cfg, pyro_ctx **out);
 int pyro_destroy(pyro_ctx *ctx);
 
+/* A batch of replicas of different lattice sizes in one ctx (no reference
+ * counterpart: the reference runs one size per process, Pyrochlore.cu:1251).
+ * L[r] in {4, 8, 12, 16, 24, 32}. Replica r's spins occupy 48*L[r]^3 floats
+ * at offset pyro_get_layout()[r] of the spin buffer, in the uniform layout.
+ * Parallel tempering exchanges within groups of equal L (rungs grouped by
+ * ascending L, each group by descending T). Every function of this header
+ * accepts a batch ctx except the single-phase, learned-proposal and
+ * phase-timing calls, which need one lattice size (PYRO_ESTATE). */
+typedef struct pyro_batch_config {
+  int n_replicas;
+  const int *L;               /* n_replicas lattice sizes */
+  const double *temperatures; /* n_replicas temperatures */
+  uint64_t seed;
+  double J;
+  double spin_len2;
+  int n_metro, n_or, pt_every;
+  float cos_cone;
+  int device;
+} pyro_batch_config;
+int pyro_create_batch(const pyro_batch_config *cfg, pyro_ctx **out);
+int pyro_get_layout(pyro_ctx *ctx, int *L, long long *offset);
+
+/* Sweep execution. mode 0: per-phase kernels (uniform ctx only; the
+ * reference's launch structure, Pyrochlore.cu:105-127); mode 1: one
+ * persistent kernel per sweep in which a block of `block` threads (256, 512,
+ * 1024) sweeps whole replicas through every phase with block barriers only,
+ * replicas packed into n_bins balanced bins (<= 0: one replica per block);
+ * mode 2: per-phase kernels over a list of 256-cell tiles of all replicas.
+ * Modes 1 and 2 need L in {4,8,12,16,24,32}. Results are identical in every
+ * mode. Default: 0 for a uniform ctx, 2 for a batch. */
+int pyro_set_exec_mode(pyro_ctx *ctx, int mode, int block, int n_bins);
+int pyro_get_exec_mode(pyro_ctx *ctx, int *mode, int *block, int *n_bins);
+
+/* Autotuner: time `sweeps` sweeps of each candidate execution (per-phase
+ * cell- and site-per-thread kernels, persistent kernel at 256/512/1024
+ * threads, tile list), restoring the state after each trial, and keep the
+ * fastest. results (optional, 4 doubles per candidate): mode, block,
+ * phase variant, ms per sweep. */
+int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results);
+
 /* Lattice tables: links16 (Pyrochlore.cu:152-167) and IndependentSubLattices
  * (Pyrochlore.cu:36-43), as (dx, dy, dz, site) and 24 sets of 4 sites. */
 int pyro_links16(int8_t out[16][6][4]);
@@ -191,9 +231,10 @@ int pyro_set_learned_passes(pyro_ctx *ctx, int n_learned);
 int pyro_acceptance_learned(pyro_ctx *ctx, uint64_t *accepted, uint64_t *proposed);
 
 /* Time n_launches back-to-back colour-phase launches of one kind (0 =
- * Metropolis, 1 = over-relaxation; colours cycle 0..3) with a HIP event pair
- * around each launch on the ctx stream. avg_ms: mean event-to-event time per
- * launch; span_ms: first to last event. The phases run for real (the state
+ * Metropolis, 1 = over-relaxation; colours cycle 0..3) on the ctx stream,
+ * each launch carrying its own start/stop HIP events in its dispatch
+ * (hipExtLaunchKernelGGL). avg_ms: mean kernel time per launch; span_ms:
+ * first start to last stop. The phases run for real (the state
  * advances). The reference's counterpart is the ENABLE_TIMING cudaEvent block
  * around Mc (MonteCarlo.cu:36-63). */
 int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, double *span_ms);

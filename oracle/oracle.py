@@ -61,6 +61,8 @@ def lib():
                               C.POINTER(C.c_int)]
         L.orc_num_threads.restype = C.c_int
         L.orc_set_num_threads.argtypes = [C.c_int]
+        L.orc_set_replica_base.argtypes = [C.c_int]
+        L.orc_set_swap_group.argtypes = [C.c_int]
         L.orc_mlp_nparams.restype = C.c_int
         L.orc_mlp_features.argtypes = [_f64p, _f64p, _f64p]
         L.orc_mlp_forward.argtypes = [_f64p, _f64p, _f64p]
@@ -181,6 +183,14 @@ class OracleRun:
 
     def observables(self):
         return observables(self.spins, self.L, self.R, self.jeff)
+
+
+def set_replica_base(base: int) -> None:
+    lib().orc_set_replica_base(base)
+
+
+def set_swap_group(group: int) -> None:
+    lib().orc_set_swap_group(group)
 
 
 def num_threads() -> int:

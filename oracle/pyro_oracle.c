@@ -73,8 +73,20 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
+/* Offset added to the replica index of every Metropolis / init counter, and
+ * the group index of the swap counters: lets one replica (or one
+ * equal-size group) of a heterogeneous batch be replayed on its own. */
+static int g_replica_base = 0;
+static int g_swap_group = 0;
+void orc_set_replica_base(int base) { g_replica_base = base; }
+void orc_set_swap_group(int group) { g_swap_group = group; }
+
 static void draw(uint32_t cell, uint32_t site, uint32_t pass, uint32_t kind, uint32_t replica,
                  uint32_t sweep, uint64_t seed, uint32_t out[4]) {
+  if (kind == KIND_SWAP)
+    replica = (uint32_t)g_swap_group;
+  else
+    replica += (uint32_t)g_replica_base;
   uint32_t ctr[4] = {cell, site | (pass << 8) | (kind << 24), replica, sweep};
   uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   orc_philox4x32_10(ctr, key, out);

@@ -83,6 +83,13 @@ double orc_vmf_log_norm(double kappa);
 double orc_vmf_log_density(const double mu[3], double kappa, const double y[3]);
 void orc_vmf_sample(const double mu[3], double kappa, double u1, double u2, double out[3]);
 
+/* Replay part of a heterogeneous batch: base is added to the replica index of
+ * every Metropolis/init counter; group replaces the replica word of the swap
+ * counters (the batch swaps within groups of equal lattice size). Both 0 by
+ * default. */
+void orc_set_replica_base(int base);
+void orc_set_swap_group(int group);
+
 /* Threads used by the OpenMP loops (1 when built without OpenMP). */
 int orc_num_threads(void);
 void orc_set_num_threads(int n);

@@ -1,0 +1,139 @@
+"""GPU tests of mixed-size batches and of the alternative sweep executions.
+
+Every replica of a batch must follow exactly the trajectory that the oracle
+produces for that replica alone (its global index keys its Philox counters),
+in every execution mode: the persistent one-kernel-per-sweep kernel (256, 512
+and 1024 threads per block) and the tile-list phases. Exchanges happen within
+groups of equal lattice size, each group keyed by its own draws.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0xBA7C4
+
+
+def say(msg):
+    print(f"[gpu-batch] {msg}", flush=True)
+
+
+def batch(Ls, temps, **kw):
+    from cudacode_amd import pyro
+    say(f"batch L={list(Ls)} {kw}")
+    return pyro.Batch(list(Ls), list(temps), seed=SEED, **kw)
+
+
+def replica_spins(g, r):
+    s = g.get_spins()
+    L, off = g.layout()
+    return s[off[r]: off[r] + 48 * L[r] ** 3]
+
+
+def oracle_replica(L, T, r, sweeps, **kw):
+    oracle.set_replica_base(r)
+    try:
+        o = oracle.OracleRun(L, [T], seed=SEED, **kw)
+        o.run(sweeps)
+    finally:
+        oracle.set_replica_base(0)
+    return o
+
+
+MODES = [(1, 256), (1, 512), (1, 1024), (2, 256)]
+
+
+@pytest.mark.parametrize("mode,block", MODES)
+def test_batch_replicas_match_oracle(mode, block):
+    Ls = [8, 4, 12, 8, 16, 4]
+    temps = [0.3, 0.05, 0.8, 1.5, 0.2, 0.6]
+    kw = dict(n_metro=1, n_or=3, cos_cone=0.4)
+    with batch(Ls, temps, **kw) as g:
+        g.set_exec_mode(mode, block)
+        g.Mc(7, use_graph=(mode == 1))
+        for r, (L, T) in enumerate(zip(Ls, temps)):
+            o = oracle_replica(L, T, r, 7, **kw)
+            assert np.array_equal(replica_spins(g, r), o.spins), (mode, block, r)
+        a, _ = g.acceptance()
+        assert a.sum() > 0
+
+
+def test_batch_parallel_tempering_within_groups():
+    Ls = [4, 4, 4, 4, 8, 8, 8]
+    temps = [0.2, 0.35, 0.5, 0.8, 0.25, 0.4, 0.9]
+    kw = dict(n_metro=1, n_or=2, pt_every=2, cos_cone=0.3)
+    with batch(Ls, temps, **kw) as g:
+        g.set_exec_mode(2, 256)
+        g.Mc(12)
+        T, at = g.temperatures()
+        s = g.get_spins()
+        L_, off = g.layout()
+    groups = [(0, [0, 1, 2, 3]), (1, [4, 5, 6])]
+    for gi, members in groups:
+        L = Ls[members[0]]
+        oracle.set_replica_base(members[0])
+        oracle.set_swap_group(gi)
+        try:
+            o = oracle.OracleRun(L, [temps[m] for m in members], seed=SEED, **kw)
+            o.run(12)
+        finally:
+            oracle.set_replica_base(0)
+            oracle.set_swap_group(0)
+        got = np.concatenate([s[off[m]: off[m] + 48 * L ** 3] for m in members])
+        assert np.array_equal(got, o.spins), gi
+        assert np.array_equal(T[members], o.temps)
+        # rungs of this group, as global replica indices
+        start = 0 if gi == 0 else 4
+        assert np.array_equal(at[start: start + len(members)], np.array(members)[o.replica_at])
+
+
+@pytest.mark.parametrize("mode,block", MODES)
+def test_uniform_ctx_modes_identical(mode, block):
+    from cudacode_amd import pyro
+    from cudacode_amd.pyro import geometric_ladder
+    temps = geometric_ladder(0.02, 1.0, 8)
+    kw = dict(n_metro=1, n_or=4, pt_every=3, cos_cone=0.5)
+    with pyro.Pyrochlore(16, temps, seed=SEED, **kw) as a, \
+            pyro.Pyrochlore(16, temps, seed=SEED, **kw) as b:
+        b.set_exec_mode(mode, block)
+        a.Mc(9)
+        b.Mc(9)
+        assert np.array_equal(a.get_spins(), b.get_spins())
+        assert np.array_equal(a.temperatures()[1], b.temperatures()[1])
+        assert np.array_equal(a.acceptance()[0], b.acceptance()[0])
+
+
+def test_batch_observables_and_order_params():
+    Ls = [4, 8, 12]
+    with batch(Ls, [0.5, 0.5, 0.5], n_metro=1, n_or=1, cos_cone=0.5) as g:
+        g.Mc(3)
+        E = g.TotalEnergyv2()
+        op = g.DoMeasurement()
+        s = g.get_spins()
+        _, off = g.layout()
+        for r, L in enumerate(Ls):
+            ref = oracle.observables(s[off[r]: off[r] + 48 * L ** 3].copy(), L, 1, 1.0)[0]
+            assert E[r] == pytest.approx(ref[0], rel=1e-12)
+            ferro = ref[1:].reshape(4, 3).sum(axis=0) / (16 * L ** 3)
+            assert np.allclose(op[r, 6:9], ferro, atol=1e-12)
+        assert np.allclose(g.energy_incremental(), E, rtol=1e-6)
+
+
+def test_autotune_keeps_the_state():
+    Ls = [8, 12, 16, 8, 24, 12, 16, 24]
+    temps = np.linspace(0.1, 1.0, 8)
+    with batch(Ls, temps, n_metro=1, n_or=4, pt_every=2, cos_cone=0.4) as g, \
+            batch(Ls, temps, n_metro=1, n_or=4, pt_every=2, cos_cone=0.4) as h:
+        g.Mc(4)
+        h.Mc(4)
+        before = g.get_spins()
+        res = g.autotune(3)
+        assert len(res) == 4 and np.all(res[:, 3] > 0)
+        say(f"autotune results (mode, block, variant, ms/sweep): {res.tolist()}")
+        assert np.array_equal(g.get_spins(), before)
+        assert g.sweep_counters() == h.sweep_counters()
+        g.Mc(6)
+        h.Mc(6)
+        assert np.array_equal(g.get_spins(), h.get_spins())
