@@ -99,6 +99,170 @@ def calibrate_copy(mib: int = 512, reps: int = 5):
     print(json.dumps({"calibration_copy_bytes_read": n * 4, "bytes_written": n * 4, "reps": reps}))
 
 
+def learned_leg(g, args, plain_series, plain_sweeps_per_s, plain_taus, plain_samples):
+    """The learned local proposal on the same ensemble: environments (beta h, s)
+    gathered from the plain sampler across the temperature ladder (16 snapshots
+    x 64 replicas x 1024 sites = 1,048,576), the network trained on the GPU,
+    then sweeps of 1 learned pass + 10 over-relaxation passes timed like the
+    plain ones. Reports sweeps/s, tau_int, independent samples/s and their
+    ratio to the plain sampler, and the agreement of the mean energy per rung."""
+    from cudacode_amd import pyro, stats
+    n_snap, per_rep = 16, 1024
+    log("learned: gathering environments from the plain sampler")
+    g.clear_environments()
+    for k in range(n_snap):
+        g.Mc(5)
+        g.gather_environments(per_rep, 1000 + k, n_snap * per_rep * R)
+    g.mlp_set_weights(pyro.mlp_init(SEED))
+    log(f"learned: training {args.train_steps} steps on the GPU")
+    t0 = time.perf_counter()
+    hist = g.mlp_train(args.train_steps, batch=4096, lr=3e-3, lr_final=3e-4)
+    g.sync()
+    train_s = time.perf_counter() - t0
+    g.set_passes(0, N_OR)
+    g.set_learned_passes(1)
+    g.Mc(args.warmup)
+    g.reset_acceptance()
+    g.record_start(args.steps)
+    g.sync()
+    log(f"learned: timed {args.steps} sweeps")
+    t0 = time.perf_counter()
+    g.Mc(args.steps)
+    g.sync()
+    dt = time.perf_counter() - t0
+    series = g.record_fetch()
+    a, p = g.acceptance_learned()
+    sweeps_per_s = args.steps / dt
+    taus = [stats.tau_int(series[:, k])[0] for k in range(R)]
+    samples = [stats.independent_samples_per_s(sweeps_per_s, t) for t in taus]
+    ratio = np.array(samples) / np.array(plain_samples)
+    # Mean energy per rung, learned vs plain, in units of the combined error.
+    n_spins = 16 * L ** 3
+    z = []
+    for k in range(R):
+        e1, e2 = plain_series[:, k] / n_spins, series[:, k] / n_spins
+        s1 = stats.binning_error(e1, 20)
+        s2 = stats.binning_error(e2, 20)
+        z.append(abs(e1.mean() - e2.mean()) / max(np.hypot(s1, s2), 1e-300))
+    g.set_learned_passes(0)
+    g.set_passes(N_METRO, N_OR)
+    return {
+        "sweep": "1 learned pass + 10 over-relaxation passes (plain: 1 Metropolis + 10)",
+        "environments": n_snap * per_rep * R,
+        "train_steps": args.train_steps,
+        "train_s": train_s,
+        "loss_first": float(hist[:10].mean()),
+        "loss_last": float(hist[-50:].mean()),
+        "acceptance": float(a.sum() / max(p.sum(), 1)),
+        "sweeps_per_s": sweeps_per_s,
+        "plain_sweeps_per_s": plain_sweeps_per_s,
+        "spin_updates_per_s": args.steps * (1 + N_OR) * n_spins * R / dt,
+        "tau_int_sweeps_median": float(np.median(taus)),
+        "tau_int_sweeps_lowest_T": float(taus[-1]),
+        "plain_tau_int_sweeps_median": float(np.median(plain_taus)),
+        "indep_samples_per_s_median": float(np.median(samples)),
+        "indep_samples_per_s_lowest_T": float(samples[-1]),
+        "samples_ratio_median": float(np.median(ratio)),
+        "samples_ratio_lowest_T": float(ratio[-1]),
+        "samples_ratio_min": float(ratio.min()),
+        "energy_agreement_max_z": float(max(z)),
+        "energy_agreement_rungs_within_2sigma": int(sum(x < 2 for x in z)),
+    }
+
+
+def line(metric_value, steps, warmup, dt, workload, extra):
+    out = {
+        "metric": "spin updates/s (plain sweep); tau_int-corrected independent samples/s",
+        "value": metric_value, "unit": "spin updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": 1e3 * dt / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": workload},
+    }
+    out.update(extra)
+    return out
+
+
+def workload_c1(args):
+    """L=4 (1,024 spins), one replica at T=0.5, 1 Metropolis (cone adapted to
+    50%) + 5 over-relaxation per sweep, 10^4 sweeps."""
+    from cudacode_amd import pyro, stats
+    g = pyro.Pyrochlore(4, [0.5], seed=SEED, n_metro=1, n_or=5, cos_cone=0.5)
+    g.CalculateSolidAngle(0.5, 20, 10)
+    g.Mc(args.warmup)
+    g.record_start(10000)
+    g.sync()
+    t0 = time.perf_counter()
+    g.Mc(10000)
+    g.sync()
+    dt = time.perf_counter() - t0
+    e = g.record_fetch()[:, 0] / 1024
+    tau = stats.tau_int(e)[0]
+    c, c_err = stats.specific_heat(e * 1024, 0.5, 1024)
+    g.close()
+    return line(10000 * 6 * 1024 / dt, 10000, args.warmup, dt,
+                "L=4 (1,024 spins), 1 replica, T=0.5, 1 Metropolis + 5 over-relaxation, 10^4 sweeps",
+                {"energy_per_spin": float(e.mean()), "energy_err": stats.binning_error(e),
+                 "specific_heat": c, "specific_heat_err": c_err, "tau_int_sweeps": tau,
+                 "indep_samples_per_s": stats.independent_samples_per_s(10000 / dt, tau)})
+
+
+def workload_c3(args):
+    """256 replicas, L drawn from the seed over {8,12,16,24}, 32 temperatures
+    from 0.05 to 1.0 (8 replicas each), 1 Metropolis + 10 over-relaxation,
+    swaps every 10 sweeps within equal-size groups; execution chosen by the
+    autotuner (persistent per-sweep kernel or tile-list phases)."""
+    from cudacode_amd import pyro
+    rs = np.random.default_rng(SEED)
+    Ls = rs.choice([8, 12, 16, 24], size=256)
+    ladder = pyro.geometric_ladder(0.05, 1.0, 32)
+    temps = np.repeat(ladder, 8)
+    g = pyro.Batch(Ls, temps, seed=SEED, n_metro=1, n_or=10, pt_every=10, cos_cone=0.5)
+    g.CalculateSolidAngle(0.5, 5, 10)
+    g.Mc(args.warmup)
+    tune = g.autotune(5)
+    mode = g.exec_mode()
+    g.sync()
+    steps = max(args.steps // 4, 10)
+    t0 = time.perf_counter()
+    g.Mc(steps)
+    g.sync()
+    dt = time.perf_counter() - t0
+    n_spins = int(g.N.sum())
+    g.close()
+    return line(steps * 11 * n_spins / dt, steps, args.warmup, dt,
+                "256 replicas, L from the seed over {8,12,16,24}, 32 temperatures 0.05..1.0, "
+                "1 Metropolis + 10 over-relaxation, swaps every 10 within equal-size groups",
+                {"spins": n_spins, "exec_mode": {"mode": mode[0], "block": mode[1], "bins": mode[2]},
+                 "autotune_ms_per_sweep": tune.tolist(),
+                 "L_counts": {int(k): int((Ls == k).sum()) for k in (8, 12, 16, 24)}})
+
+
+def workload_c5(args):
+    """L=64 (4,194,304 spins) x 32 replicas, T from 0.02 to 0.5, swaps every 10
+    sweeps, 1 Metropolis + 10 over-relaxation per sweep (1.61 GB resident)."""
+    from cudacode_amd import pyro
+    temps = pyro.geometric_ladder(0.02, 0.5, 32)
+    g = pyro.Pyrochlore(64, temps, seed=SEED, n_metro=1, n_or=10, pt_every=10, cos_cone=0.5)
+    g.Mc(max(args.warmup // 20, 2))
+    steps = max(args.steps // 40, 10)
+    g.sync()
+    t0 = time.perf_counter()
+    g.Mc(steps)
+    g.sync()
+    dt = time.perf_counter() - t0
+    or_ms, _ = g.time_phase(1, 40)
+    n_total = 16 * 64 ** 3 * 32
+    achieved = BYTES_PER_UPDATE * n_total / 4 / (or_ms * 1e-3) / 1e9
+    g.close()
+    return line(steps * 11 * n_total / dt, steps, 0, dt,
+                "L=64 (4,194,304 spins) x 32 replicas, T 0.02..0.5, 1 Metropolis + 10 "
+                "over-relaxation, swaps every 10 sweeps",
+                {"roofline": {"bound": "hbm", "kernel": "k_phase_cells<1, colour, 64, false>",
+                              "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                              "avg_launch_ms": or_ms}})
+
+
 def main():
     if "--calibrate-copy" in sys.argv:
         calibrate_copy()
@@ -109,12 +273,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--phase-launches", type=int, default=400)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5"],
+                    help="c2 (default): the L=16 x 64 parallel-tempering line; others print "
+                         "their own line")
+    ap.add_argument("--train-steps", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the replicas of "
               "this workload are all resident on one MI355X.", file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
+    if args.workload != "c2":
+        fn = {"c1": workload_c1, "c3": workload_c3, "c5": workload_c5}[args.workload]
+        print(json.dumps(fn(args)), flush=True)
+        return
 
     from cudacode_amd import pyro, stats
 
@@ -154,6 +327,9 @@ def main():
     g.set_kernel_options(1, False)
     bytes_per_launch = BYTES_PER_UPDATE * n_total // 4
     achieved = bytes_per_launch / (or_ms * 1e-3) / 1e9
+    learned = None
+    if not args.no_learned:
+        learned = learned_leg(g, args, series, sweeps_per_s, taus, samples)
     g.close()
     traffic = measured_traffic(OR_KERNEL)
 
@@ -199,6 +375,8 @@ def main():
             "site_per_thread_or_avg_launch_ms": or_ms_v0,
         },
     }
+    if learned is not None:
+        out["learned_vs_plain"] = learned
     if not args.no_cpu and rank == 0:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline()

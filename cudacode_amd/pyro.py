@@ -187,6 +187,20 @@ def fold_order_params(sub12, n_spins: float) -> np.ndarray:
     return out
 
 
+def mlp_init(seed: int = 0) -> np.ndarray:
+    """Initial weights of the learned proposal in the packed layout
+    W1[64][6] b1[64] W2[64][64] b2[64] W3[4][64] b3[4]: He-normal hidden layers,
+    a small output layer, zero biases except b3[3] = log(kappa) = 1."""
+    r = np.random.default_rng(seed & 0xFFFFFFFF)
+    w = np.concatenate([
+        r.normal(0, np.sqrt(2 / 6), 64 * 6), np.zeros(64),
+        r.normal(0, np.sqrt(2 / 64), 64 * 64), np.zeros(64),
+        r.normal(0, 0.1 / np.sqrt(64), 4 * 64), np.array([0.0, 0.0, 0.0, 1.0]),
+    ]).astype(np.float32)
+    assert w.size == MLP_PARAMS
+    return w
+
+
 def geometric_ladder(t_min: float, t_max: float, n: int) -> np.ndarray:
     """n temperatures T_k = t_min (t_max/t_min)^(k/(n-1))."""
     if n == 1:
