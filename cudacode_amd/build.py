@@ -19,6 +19,7 @@ SOURCES = [
     "pyro_kernels.hip",
     "pyro_learned.hip",
     "pyro_batch.hip",
+    "pyro_io.hip",
     "pyro_capi.hip",
 ]
 

@@ -35,6 +35,8 @@ EXPORTS = [
     "pyro_acceptance", "pyro_reset_acceptance", "pyro_adapt_cone", "pyro_record_start",
     "pyro_record_fetch", "pyro_set_passes", "pyro_set_kernel_options", "pyro_create_batch",
     "pyro_get_layout", "pyro_set_exec_mode", "pyro_get_exec_mode", "pyro_autotune",
+    "pyro_checkpoint_save", "pyro_checkpoint_load", "pyro_export_configurations",
+    "pyro_import_configurations",
     "pyro_mlp_set_weights", "pyro_mlp_get_weights",
     "pyro_mlp_eval", "pyro_vmf_draw", "pyro_mlp_loss_grad", "pyro_gather_environments",
     "pyro_get_environments", "pyro_clear_environments", "pyro_mlp_train", "pyro_learned_phase",
@@ -131,6 +133,10 @@ def lib():
         L.pyro_get_exec_mode.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                          C.POINTER(C.c_int)]
         L.pyro_autotune.argtypes = [vp, C.c_int, vp, C.POINTER(C.c_int)]
+        L.pyro_checkpoint_save.argtypes = [vp, C.c_char_p]
+        L.pyro_checkpoint_load.argtypes = [vp, C.c_char_p]
+        L.pyro_export_configurations.argtypes = [vp, C.c_char_p, C.c_int, C.c_int]
+        L.pyro_import_configurations.argtypes = [vp, C.c_char_p, C.c_int, C.c_int]
         L.pyro_mlp_set_weights.argtypes = [vp, vp]
         L.pyro_mlp_get_weights.argtypes = [vp, vp]
         L.pyro_mlp_eval.argtypes = [vp, vp, C.c_int, vp]
@@ -381,6 +387,23 @@ class Pyrochlore:
         n = C.c_int()
         _check(lib().pyro_autotune(self._ctx, sweeps, _p(res), C.byref(n)))
         return res[: 4 * n.value].reshape(n.value, 4)
+
+    # -- checkpoint / configuration files ----------------------------------
+    def save_checkpoint(self, path: str) -> None:
+        _check(lib().pyro_checkpoint_save(self._ctx, os.fsencode(path)))
+
+    def load_checkpoint(self, path: str) -> None:
+        _check(lib().pyro_checkpoint_load(self._ctx, os.fsencode(path)))
+
+    def SaveConfigurations(self, prefix: str, append: bool = True, bytes_per_value: int = 8):
+        """Reference files <prefix><k>.dat (k = temperature rank), AoS records (Math.cu:618)."""
+        _check(lib().pyro_export_configurations(self._ctx, os.fsencode(prefix), int(append),
+                                                bytes_per_value))
+
+    def ReadConfigurations(self, prefix: str, rep: int = 0, bytes_per_value: int = 8):
+        """Load record `rep` of the reference files (Math.cu:662)."""
+        _check(lib().pyro_import_configurations(self._ctx, os.fsencode(prefix), rep,
+                                                bytes_per_value))
 
     def layout(self):
         """(L per replica, float offset of each replica's spins in get_spins())."""

@@ -35,6 +35,7 @@ extern "C" {
 #define PYRO_EHIP (-2)     /* HIP runtime error */
 #define PYRO_ENOMEM (-3)   /* device or host allocation failed */
 #define PYRO_ESTATE (-4)   /* call not valid in the ctx's current state */
+#define PYRO_EIO (-5)      /* file missing, short, corrupt or not matching the ctx */
 
 typedef struct pyro_ctx pyro_ctx;
 
@@ -229,6 +230,27 @@ int pyro_mlp_train(pyro_ctx *ctx, int steps, int batch, float lr, float lr_final
 int pyro_learned_phase(pyro_ctx *ctx, int colour, uint32_t sweep, int pass);
 int pyro_set_learned_passes(pyro_ctx *ctx, int n_learned);
 int pyro_acceptance_learned(pyro_ctx *ctx, uint64_t *accepted, uint64_t *proposed);
+
+/* Checkpoint / resume (own binary format, FNV-1a checksummed): spins,
+ * temperatures, cones, rung assignment, sweep and exchange counters, seed,
+ * schedule, energy bookkeeping, acceptance counters and the learned
+ * proposal's weights. A resumed run continues bit for bit (the random
+ * numbers are counter-based). Load checks that the file matches the ctx's
+ * replica count and sizes; a short or corrupt file is PYRO_EIO and leaves
+ * the ctx unchanged. (The reference's resume path reads InitialConf files,
+ * Pyrochlore.cu:1528-1552.) */
+int pyro_checkpoint_save(pyro_ctx *ctx, const char *path);
+int pyro_checkpoint_load(pyro_ctx *ctx, const char *path);
+
+/* The reference's configuration files (SaveConfigurations /
+ * ReadConfigurations, Math.cu:618-701): "<prefix><k>.dat" per temperature
+ * rank k (descending T), each record 3N values in AoS order (s0x s0y s0z
+ * s1x ...), 8-byte (the reference's TYPE=double) or 4-byte values, one
+ * record appended per call when append != 0. Import reads record `rep`; a
+ * missing file or a short read is PYRO_EIO (the reference exits or returns 1,
+ * Math.cu:676-689). Uniform lattice size only. */
+int pyro_export_configurations(pyro_ctx *ctx, const char *prefix, int append, int bytes_per_value);
+int pyro_import_configurations(pyro_ctx *ctx, const char *prefix, int rep, int bytes_per_value);
 
 /* Time n_launches back-to-back colour-phase launches of one kind (0 =
  * Metropolis, 1 = over-relaxation; colours cycle 0..3) on the ctx stream,

@@ -1,0 +1,112 @@
+"""Checkpoint/resume and the reference's configuration files, through the C ABI.
+
+- a run resumed from a checkpoint continues bit for bit (counter-based random
+  numbers, exact FP32 spins, rung assignment and counters restored);
+- a truncated or corrupted checkpoint, or one for another geometry, is an
+  error and leaves the ctx untouched (the reference returns 1 on a short read,
+  Math.cu:683-689);
+- exported configuration files have the reference's layout: one file per
+  temperature rank (descending T), AoS records of 3N FP64 values, appended
+  one record per call (Math.cu:618-658); importing a record restores it.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0xC0FFEE
+
+
+def ctx(L=4, temps=(0.2, 0.5, 1.1, 2.0), **kw):
+    from cudacode_amd import pyro
+    print(f"[gpu-io] create L={L} R={len(temps)}", flush=True)
+    return pyro.Pyrochlore(L, list(temps), seed=SEED, **kw)
+
+
+KW = dict(n_metro=1, n_or=3, pt_every=2, cos_cone=0.3)
+
+
+def test_resume_is_bit_identical(tmp_path):
+    path = str(tmp_path / "run.ckpt")
+    with ctx(**KW) as a:
+        a.Mc(30)
+        ref = a.get_spins()
+        ref_T, ref_at = a.temperatures()
+        ref_acc = a.acceptance()[0]
+    with ctx(**KW) as b:
+        b.Mc(11)
+        b.save_checkpoint(path)
+    with ctx(**KW) as c:
+        c.Mc(3)  # a different state, overwritten by the load
+        c.load_checkpoint(path)
+        assert c.sweep_counters() == (11, 5)
+        c.Mc(19)
+        assert np.array_equal(c.get_spins(), ref)
+        T, at = c.temperatures()
+        assert np.array_equal(at, ref_at) and np.array_equal(T, ref_T)
+        assert np.array_equal(c.acceptance()[0], ref_acc)
+        assert np.allclose(c.energy_incremental(), c.TotalEnergyv2(), rtol=1e-6)
+
+
+def test_bad_checkpoints_are_rejected(tmp_path):
+    from cudacode_amd import pyro
+    path = str(tmp_path / "ok.ckpt")
+    with ctx(**KW) as a:
+        a.Mc(4)
+        a.save_checkpoint(path)
+        blob = open(path, "rb").read()
+        before = a.get_spins()
+        cases = {
+            "truncated": blob[: len(blob) // 2],
+            "flipped": blob[:200] + bytes([blob[200] ^ 1]) + blob[201:],
+            "garbage": b"not a checkpoint at all",
+        }
+        for name, data in cases.items():
+            p = str(tmp_path / f"{name}.ckpt")
+            open(p, "wb").write(data)
+            with pytest.raises(pyro.PyroError, match="checkpoint_load"):
+                a.load_checkpoint(p)
+            assert np.array_equal(a.get_spins(), before), name
+        with pytest.raises(pyro.PyroError, match="cannot open"):
+            a.load_checkpoint(str(tmp_path / "missing.ckpt"))
+    with ctx(L=8, **KW) as other:
+        with pytest.raises(pyro.PyroError, match="differ"):
+            other.load_checkpoint(path)
+
+
+def test_reference_configuration_files(tmp_path):
+    from cudacode_amd import pyro
+    prefix = str(tmp_path / "Conf")
+    L, temps = 4, [0.2, 1.1, 0.5, 2.0]
+    N = 16 * L ** 3
+    with ctx(L, temps, **KW) as a:
+        a.Mc(5)
+        s1 = a.get_spins().reshape(4, 3, N)
+        T, at = a.temperatures()
+        a.SaveConfigurations(prefix, append=False)
+        a.Mc(5)
+        s2 = a.get_spins().reshape(4, 3, N)
+        _, at2 = a.temperatures()
+        a.SaveConfigurations(prefix, append=True)
+    files = sorted(os.listdir(tmp_path))
+    assert files == [f"Conf{k}.dat" for k in range(4)]
+    for k in range(4):
+        raw = np.fromfile(f"{prefix}{k}.dat", dtype="<f8")
+        assert raw.size == 2 * 3 * N
+        rec = raw.reshape(2, N, 3)  # AoS: spin-major, component-minor
+        assert np.array_equal(rec[0], s1[at[k]].T.astype(np.float64))
+        assert np.array_equal(rec[1], s2[at2[k]].T.astype(np.float64))
+    # rank 0 is the hottest temperature
+    assert T[at[0]] == max(temps)
+    with ctx(L, temps, **KW) as b:
+        b.ReadConfigurations(prefix, rep=1)
+        got = b.get_spins().reshape(4, 3, N)
+        _, atb = b.temperatures()
+        for k in range(4):
+            assert np.array_equal(got[atb[k]], s2[at2[k]])
+        with pytest.raises(pyro.PyroError, match="end of file"):
+            b.ReadConfigurations(prefix, rep=2)
+        with pytest.raises(pyro.PyroError, match="cannot open"):
+            b.ReadConfigurations(str(tmp_path / "Nope"), rep=0)
