@@ -39,6 +39,13 @@ HIP_FLAGS = [
     "-Wno-unknown-warning-option",
 ]
 
+# Per-source additions. The learned-proposal kernels keep MFMA accumulators in
+# ordinary VGPRs (gfx950 MFMAs read and write either register file), which
+# saves the accumulator-to-VGPR copies before every activation.
+EXTRA_FLAGS = {
+    "pyro_learned.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+}
+
 
 def hipcc() -> str:
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
@@ -54,7 +61,7 @@ def build(verbose: bool = False) -> str:
     objs, procs = [], []
     for src in SOURCES:  # translation units compile in parallel
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
-        cmd = [hipcc(), *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c",
+        cmd = [hipcc(), *HIP_FLAGS, *EXTRA_FLAGS.get(src, []), "-I", os.path.join(ROOT, "include"), "-c",
                os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -18,56 +18,86 @@
 namespace pyro {
 
 typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
 
 enum : uint32_t { kKindTrain = 6u, kKindEnv = 7u };
 
 __constant__ Link c_links_learned[16][6] = PYRO_LINKS16_INIT;
 
+// Round to nearest even, as the hardware conversion (v_cvt_pk_bf16_f32) does.
 __host__ __device__ __forceinline__ uint32_t bf16_bits(float f) {
   uint32_t u = __builtin_bit_cast(uint32_t, f);
-  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  u += 0x7FFFu + ((u >> 16) & 1u);
   return u >> 16;
 }
 
-__device__ __forceinline__ uint32_t pack2(float lo, float hi) {
-  return bf16_bits(lo) | (bf16_bits(hi) << 16);
-}
-
-__device__ __forceinline__ s16x4 frag(uint32_t lo, uint32_t hi) {
-  u32x2 v = {lo, hi};
-  return __builtin_bit_cast(s16x4, v);
+__device__ __forceinline__ uint32_t cvt2(float lo, float hi) {
+  const f32x2 v = {lo, hi};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
 
 __device__ __forceinline__ float relu(float x) { return fmaxf(x, 0.0f); }
 
+// ReLU of 8 consecutive accumulator registers, converted to the 8 bf16 of
+// one B operand of a 32x32x16 MFMA. Rounding to bf16 keeps the sign, so the
+// ReLU is applied after it, as a signed 16-bit max with 0 on the bit patterns
+// (negative values and -0 have the sign bit set): two values per packed
+// integer instruction, and no NaN canonicalisation of the accumulators.
+__device__ __forceinline__ bf16x8 act8(const f32x16 &d, int base) {
+  u32x4v p;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) p[j] = cvt2(d[base + 2 * j], d[base + 2 * j + 1]);
+  const s16x8 b = __builtin_bit_cast(s16x8, p);
+  const s16x8 z = {};
+  return __builtin_bit_cast(bf16x8, __builtin_elementwise_max(b, z));
+}
+
+// Hidden unit that 32x32x16 k-step t, lane half h, element j stands for. The
+// accumulator of a 32x32 MFMA holds, in lane half h, rows 8b + 4h + r of its
+// 32-row tile (register 4b + r); reading 8 consecutive registers 8t'..8t'+7
+// as the next layer's B operand therefore feeds the units
+//   32 (t / 2) + 8 (2 (t % 2) + j / 4) + 4 h + j % 4,
+// and the A operand (weights) is packed with its columns in the same order,
+// which leaves the contraction unchanged.
+__host__ __device__ __forceinline__ int kstep_unit(int t, int h, int j) {
+  return 32 * (t >> 1) + 8 * (2 * (t & 1) + (j >> 2)) + 4 * h + (j & 3);
+}
+
 // Weight fragments held in registers by every wave.
 struct MlpW {
-  s16x4 a1[2];      // [row tile]
-  s16x4 a2[2][8];   // [row tile][k step]
-  s16x4 a3[8];      // [k step]
-  float b2[2][16];  // bias of the rows this lane's accumulators hold
+  s16x4 a1[2];     // layer 1 (32x32x8): [row tile]
+  bf16x8 a2[2][4]; // layer 2 (32x32x16): [row tile][k step]
+  bf16x8 a3[4];    // layer 3 (32x32x16, rows 4..31 zero): [k step]
+  f32x16 b2[2];    // bias of the rows this lane's accumulators hold: the
+                   // layer-2 accumulators start from it, so the MFMA adds it
   float b3[4];
 };
-
-__device__ __forceinline__ s16x4 load_frag(const uint16_t *__restrict__ p) {
-  const u32x2 v = *reinterpret_cast<const u32x2 *>(p);
-  return __builtin_bit_cast(s16x4, v);
-}
 
 __device__ __forceinline__ void mlp_load(const uint16_t *__restrict__ pack,
                                          const float *__restrict__ w, MlpW &W) {
   const int lane = threadIdx.x & 63, h = lane >> 5, n = lane & 31;
 #pragma unroll
-  for (int tj = 0; tj < 2; ++tj) W.a1[tj] = load_frag(pack + kPackP1 + (32 * tj + n) * 8 + 4 * h);
+  for (int tj = 0; tj < 2; ++tj)
+    W.a1[tj] = __builtin_bit_cast(
+        s16x4, *reinterpret_cast<const u32x2 *>(pack + kPackP1 + (32 * tj + n) * 8 + 4 * h));
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-    for (int t = 0; t < 8; ++t)
-      W.a2[ti][t] = load_frag(pack + kPackP2 + (32 * ti + n) * 64 + 8 * t + 4 * h);
+    for (int t = 0; t < 4; ++t)
+      W.a2[ti][t] = __builtin_bit_cast(
+          bf16x8, *reinterpret_cast<const u32x4v *>(pack + kPackP2 + ((ti * 4 + t) * 64 + lane) * 8));
 #pragma unroll
-  for (int t = 0; t < 8; ++t) W.a3[t] = load_frag(pack + kPackP3 + n * 64 + 8 * t + 4 * h);
+  for (int t = 0; t < 4; ++t)
+    W.a3[t] = __builtin_bit_cast(
+        bf16x8, *reinterpret_cast<const u32x4v *>(pack + kPackP3 + (t * 64 + lane) * 8));
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
@@ -78,52 +108,49 @@ __device__ __forceinline__ void mlp_load(const uint16_t *__restrict__ pack,
 
 // Forward pass of the 64 samples of a wave (one per lane). Every lane of the
 // wave must call it. x: this lane's 6 features; o: this lane's 4 outputs.
+// 4 MFMA 32x32x8 (layer 1, bias folded as a constant input) + 16 + 8 MFMA
+// 32x32x16 (layers 2 and 3).
 __device__ __forceinline__ void mlp_forward(const MlpW &W, const float x[6], float o[4]) {
   const int lane = threadIdx.x & 63, h = lane >> 5, n = lane & 31;
-  const uint32_t p0 = pack2(x[0], x[1]), p1 = pack2(x[2], x[3]), p2 = pack2(x[4], x[5]);
+  const uint32_t p0 = cvt2(x[0], x[1]), p1 = cvt2(x[2], x[3]), p2 = cvt2(x[4], x[5]);
   const uint32_t p3 = 0x3F80u;  // bf16(1.0) low, 0 high: the bias column
   s16x4 b1[2];
 #pragma unroll
   for (int tn = 0; tn < 2; ++tn) {
     const int src = 32 * tn + n;
     const uint32_t q0 = __shfl(p0, src), q1 = __shfl(p1, src), q2 = __shfl(p2, src);
-    b1[tn] = h ? frag(q2, p3) : frag(q0, q1);
+    const u32x2 v = h ? u32x2{q2, p3} : u32x2{q0, q1};
+    b1[tn] = __builtin_bit_cast(s16x4, v);
   }
   const f32x16 zero = {};
-  s16x4 b2[2][8];
+  bf16x8 b2[2][4];
 #pragma unroll
   for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
     for (int tn = 0; tn < 2; ++tn) {
       const f32x16 d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W.a1[tj], b1[tn], zero, 0, 0, 0);
-#pragma unroll
-      for (int b = 0; b < 4; ++b)
-        b2[tn][4 * tj + b] = frag(pack2(relu(d[4 * b]), relu(d[4 * b + 1])),
-                                  pack2(relu(d[4 * b + 2]), relu(d[4 * b + 3])));
+      b2[tn][2 * tj] = act8(d, 0);
+      b2[tn][2 * tj + 1] = act8(d, 8);
     }
-  s16x4 b3[2][8];
+  bf16x8 b3[2][4];
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
     for (int tn = 0; tn < 2; ++tn) {
-      f32x16 acc = zero;
+      f32x16 acc = W.b2[ti];
 #pragma unroll
-      for (int t = 0; t < 8; ++t)
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W.a2[ti][t], b2[tn][t], acc, 0, 0, 0);
-#pragma unroll
-      for (int b = 0; b < 4; ++b)
-        b3[tn][4 * ti + b] =
-            frag(pack2(relu(acc[4 * b] + W.b2[ti][4 * b]), relu(acc[4 * b + 1] + W.b2[ti][4 * b + 1])),
-                 pack2(relu(acc[4 * b + 2] + W.b2[ti][4 * b + 2]),
-                       relu(acc[4 * b + 3] + W.b2[ti][4 * b + 3])));
+      for (int t = 0; t < 4; ++t)
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.a2[ti][t], b2[tn][t], acc, 0, 0, 0);
+      b3[tn][2 * ti] = act8(acc, 0);
+      b3[tn][2 * ti + 1] = act8(acc, 8);
     }
   f32x16 d3[2];
 #pragma unroll
   for (int tn = 0; tn < 2; ++tn) {
     f32x16 acc = zero;
 #pragma unroll
-    for (int t = 0; t < 8; ++t)
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W.a3[t], b3[tn][t], acc, 0, 0, 0);
+    for (int t = 0; t < 4; ++t)
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.a3[t], b3[tn][t], acc, 0, 0, 0);
     d3[tn] = acc;
   }
   // Rows 0..3 of column n sit in lane n (upper half zero-padded rows elsewhere).
@@ -203,8 +230,9 @@ __device__ __forceinline__ int wrap_l(int v, int L) { return v < 0 ? v + L : (v 
 
 // One colour phase of learned proposals on every replica; same geometry as
 // the Metropolis phase (block = 4 waves = the 4 sites of the colour, lane =
-// cell). grid = (ceil(ncell/64), R).
-template <int LT>
+// cell), with G groups of 64 cells per block so the weight fragments loaded
+// into registers serve G forward-pass pairs. grid = (ceil(ncell/(64 G)), R).
+template <int LT, int G>
 __global__ __launch_bounds__(kPhaseBlock) void k_learned_phase(
     float *__restrict__ spins, int L_rt, int ncell_rt, int colour, int pass,
     const float *__restrict__ beta, const uint32_t *__restrict__ ctl, uint32_t key0,
@@ -215,73 +243,76 @@ __global__ __launch_bounds__(kPhaseBlock) void k_learned_phase(
   const int r = blockIdx.y;
   const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int cell = blockIdx.x * kPhaseCells + lane;
-  const bool valid = cell < ncell;
   const int site = colour + 4 * k;
   const size_t plane = (size_t)16 * ncell;
   float *Sx = spins + (size_t)r * 3 * plane;
   float *Sy = Sx + plane;
   float *Sz = Sy + plane;
+  const float bt = beta[r];
+  const uint32_t sweep = ctl[kCtlSweep];
 
   MlpW W;
   mlp_load(pack, w, W);
 
-  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-  double gx = 0.0, gy = 0.0, gz = 0.0;
-  f3 s = {0.0f, 0.0f, 1.0f};
-  int i = 0;
-  if (valid) {
-    const int LL = L * L;
-    const int z = cell / LL;
-    const int rem = cell - z * LL;
-    const int y = rem / L;
-    const int x = rem - y * L;
-#pragma unroll
-    for (int nb = 0; nb < 6; ++nb) {
-      const Link lk = c_links_learned[site][nb];
-      const int j = lk.site * ncell + (wrap_l(z + lk.dz, L) * L + wrap_l(y + lk.dy, L)) * L +
-                    wrap_l(x + lk.dx, L);
-      const float a = Sx[j], b = Sy[j], c = Sz[j];
-      fx = fx + a;
-      fy = fy + b;
-      fz = fz + c;
-      gx += (double)a;
-      gy += (double)b;
-      gz += (double)c;
-    }
-    i = site * ncell + cell;
-    s = {Sx[i], Sy[i], Sz[i]};
-  }
-  const float bt = beta[r];
-  const f3 hf = {-jeff * fx, -jeff * fy, -jeff * fz};
-  const f3 u = {bt * hf.x, bt * hf.y, bt * hf.z};
-  float x[6], o[4];
-  features(u, s, x);
-  mlp_forward(W, x, o);
-  const Vmf fwd = decode(o);
-  const u32x4 rnd = philox4x32_10(
-      rng_counter((uint32_t)cell, (uint32_t)site, (uint32_t)pass, kKindLearned, (uint32_t)r,
-                  ctl[kCtlSweep]),
-      key0, key1);
-  const f3 sp = vmf_sample(fwd, u01(rnd.x), u01(rnd.y));
-  features(u, sp, x);
-  mlp_forward(W, x, o);
-  const Vmf rev = decode(o);
-  const float la = mh_log_ratio(hf, bt, s, sp, fwd, rev);
-  const bool accept = valid && (u01(rnd.z) < expf(fminf(la, 0.0f)));
   double de = 0.0;
-  if (accept) {
-    de = (double)jeff * (((double)sp.x - (double)s.x) * gx + ((double)sp.y - (double)s.y) * gy +
-                         ((double)sp.z - (double)s.z) * gz);
-    Sx[i] = sp.x;
-    Sy[i] = sp.y;
-    Sz[i] = sp.z;
+  unsigned nacc = 0;
+  for (int g = 0; g < G; ++g) {
+    const int cell = (blockIdx.x * G + g) * kPhaseCells + lane;
+    const bool valid = cell < ncell;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    f3 s = {0.0f, 0.0f, 1.0f};
+    int i = 0;
+    if (valid) {
+      const int LL = L * L;
+      const int z = cell / LL;
+      const int rem = cell - z * LL;
+      const int y = rem / L;
+      const int x = rem - y * L;
+#pragma unroll
+      for (int nb = 0; nb < 6; ++nb) {
+        const Link lk = c_links_learned[site][nb];
+        const int j = lk.site * ncell + (wrap_l(z + lk.dz, L) * L + wrap_l(y + lk.dy, L)) * L +
+                      wrap_l(x + lk.dx, L);
+        const float a = Sx[j], b = Sy[j], c = Sz[j];
+        fx = fx + a;
+        fy = fy + b;
+        fz = fz + c;
+        gx += (double)a;
+        gy += (double)b;
+        gz += (double)c;
+      }
+      i = site * ncell + cell;
+      s = {Sx[i], Sy[i], Sz[i]};
+    }
+    const f3 hf = {-jeff * fx, -jeff * fy, -jeff * fz};
+    const f3 u = {bt * hf.x, bt * hf.y, bt * hf.z};
+    float x[6], o[4];
+    features(u, s, x);
+    mlp_forward(W, x, o);
+    const Vmf fwd = decode(o);
+    const u32x4 rnd = philox4x32_10(
+        rng_counter((uint32_t)cell, (uint32_t)site, (uint32_t)pass, kKindLearned, (uint32_t)r, sweep),
+        key0, key1);
+    const f3 sp = vmf_sample(fwd, u01(rnd.x), u01(rnd.y));
+    features(u, sp, x);
+    mlp_forward(W, x, o);
+    const Vmf rev = decode(o);
+    const float la = mh_log_ratio(hf, bt, s, sp, fwd, rev);
+    const bool accept = valid && (u01(rnd.z) < expf(fminf(la, 0.0f)));
+    if (accept) {
+      de += (double)jeff * (((double)sp.x - (double)s.x) * gx + ((double)sp.y - (double)s.y) * gy +
+                            ((double)sp.z - (double)s.z) * gz);
+      Sx[i] = sp.x;
+      Sy[i] = sp.y;
+      Sz[i] = sp.z;
+    }
+    nacc += (unsigned)__popcll(__ballot(accept));
   }
   __shared__ double s_de[4];
   __shared__ unsigned s_acc[4];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) de += __shfl_xor(de, off, 64);
-  const unsigned nacc = (unsigned)__popcll(__ballot(accept));
   if (lane == 0) {
     s_de[k] = de;
     s_acc[k] = nacc;
@@ -551,14 +582,16 @@ __global__ void k_mlp_pack(const float *__restrict__ w, uint16_t *__restrict__ p
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= kPackSize) return;
   float val = 0.0f;
-  if (p < kPackP2) {
+  if (p < kPackP2) {  // [64][8]: W1 columns, bias, 0
     const int row = p / 8, col = p % 8;
     val = col < 6 ? w[kOffW1 + row * 6 + col] : (col == 6 ? w[kOffB1 + row] : 0.0f);
-  } else if (p < kPackP3) {
-    val = w[kOffW2 + (p - kPackP2)];
-  } else {
-    const int q = p - kPackP3, row = q / 64, col = q % 64;
-    val = row < 4 ? w[kOffW3 + row * 64 + col] : 0.0f;
+  } else if (p < kPackP3) {  // [ti][t][lane][j]
+    const int q = p - kPackP2, j = q & 7, lane = (q >> 3) & 63, t = (q >> 9) & 3, ti = q >> 11;
+    val = w[kOffW2 + (32 * ti + (lane & 31)) * 64 + kstep_unit(t, lane >> 5, j)];
+  } else {  // [t][lane][j], rows 4..31 zero
+    const int q = p - kPackP3, j = q & 7, lane = (q >> 3) & 63, t = q >> 9;
+    const int row = lane & 31;
+    val = row < 4 ? w[kOffW3 + row * 64 + kstep_unit(t, lane >> 5, j)] : 0.0f;
   }
   pack[p] = (uint16_t)bf16_bits(val);
 }
@@ -604,10 +637,13 @@ int mlp_pack(pyro_ctx *ctx) {
   return 0;
 }
 
+constexpr int kLearnedGroups = 2;  // 64-cell groups per block
+
 template <int LT>
 static void launch_learned_t(pyro_ctx *ctx, int colour, int pass) {
-  dim3 grid((ctx->ncell + kPhaseCells - 1) / kPhaseCells, ctx->R);
-  k_learned_phase<LT><<<grid, kPhaseBlock, 0, ctx->stream>>>(
+  constexpr int G = kLearnedGroups;
+  dim3 grid((ctx->ncell + kPhaseCells * G - 1) / (kPhaseCells * G), ctx->R);
+  k_learned_phase<LT, G><<<grid, kPhaseBlock, 0, ctx->stream>>>(
       ctx->d_spins, ctx->L, ctx->ncell, colour, pass, ctx->d_beta, ctx->d_ctl, klo(ctx), khi(ctx),
       ctx->jeff, ctx->d_mlp_pack, ctx->d_mlp_w, ctx->d_acc_learned, ctx->d_dE);
 }
