@@ -136,16 +136,31 @@ def learned_leg(g, args, plain_series, plain_sweeps_per_s, plain_taus, plain_sam
     taus = [stats.tau_int(series[:, k])[0] for k in range(R)]
     samples = [stats.independent_samples_per_s(sweeps_per_s, t) for t in taus]
     ratio = np.array(samples) / np.array(plain_samples)
-    # Mean energy per rung, learned vs plain, in units of the combined error.
-    n_spins = 16 * L ** 3
-    z = []
-    for k in range(R):
-        e1, e2 = plain_series[:, k] / n_spins, series[:, k] / n_spins
-        s1 = stats.binning_error(e1, 20)
-        s2 = stats.binning_error(e2, 20)
-        z.append(abs(e1.mean() - e2.mean()) / max(np.hypot(s1, s2), 1e-300))
     g.set_learned_passes(0)
     g.set_passes(N_METRO, N_OR)
+    # Control: the plain sampler again, continuing from the learned chain's
+    # state. The spread of (plain, plain) z-scores shows how far the slowest
+    # rungs still drift over these run lengths; the learned chain is judged
+    # against it, not against the normal law alone.
+    log(f"learned: plain control {args.steps} sweeps")
+    g.record_start(args.steps)
+    g.Mc(args.steps)
+    control = g.record_fetch()
+    n_spins = 16 * L ** 3
+
+    def zscores(a, b):
+        z = []
+        for k in range(R):
+            e1, e2 = a[:, k] / n_spins, b[:, k] / n_spins
+            s1 = stats.binning_error(e1, 20)
+            s2 = stats.binning_error(e2, 20)
+            z.append(abs(e1.mean() - e2.mean()) / max(np.hypot(s1, s2), 1e-300))
+        return np.array(z)
+
+    rung_T = np.sort(pyro.geometric_ladder(T_MIN, T_MAX, R))[::-1]  # columns: descending T
+    z = zscores(plain_series, series)
+    z_ctl = zscores(plain_series, control)
+    z_lc = zscores(series, control)
     return {
         "sweep": "1 learned pass + 10 over-relaxation passes (plain: 1 Metropolis + 10)",
         "environments": n_snap * per_rep * R,
@@ -165,8 +180,13 @@ def learned_leg(g, args, plain_series, plain_sweeps_per_s, plain_taus, plain_sam
         "samples_ratio_median": float(np.median(ratio)),
         "samples_ratio_lowest_T": float(ratio[-1]),
         "samples_ratio_min": float(ratio.min()),
-        "energy_agreement_max_z": float(max(z)),
-        "energy_agreement_rungs_within_2sigma": int(sum(x < 2 for x in z)),
+        "energy_agreement_max_z": float(z.max()),
+        "energy_agreement_rungs_within_2sigma": int((z < 2).sum()),
+        "control_plain_vs_plain_max_z": float(z_ctl.max()),
+        "control_plain_vs_plain_within_2sigma": int((z_ctl < 2).sum()),
+        "learned_vs_control_max_z": float(z_lc.max()),
+        "learned_vs_control_within_2sigma": int((z_lc < 2).sum()),
+        "worst_rungs_T": [float(rung_T[k]) for k in np.argsort(-z)[:3]],
     }
 
 

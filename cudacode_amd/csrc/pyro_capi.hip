@@ -57,6 +57,10 @@ static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
   }
   if (ctx->pt_every > 0 && (sweep_index + 1u) % (uint32_t)ctx->pt_every == 0u) {
     PYRO_TRY(launch_observables(ctx));
+    // The full energies computed for the swap also re-anchor the incremental
+    // energy, so FP32 rounding in the untracked over-relaxation passes cannot
+    // accumulate beyond one exchange interval.
+    PYRO_TRY(launch_set_anchor(ctx));
     PYRO_TRY(launch_pt_swap(ctx, 0, 0, 1));
   }
   PYRO_TRY(launch_end_sweep(ctx));

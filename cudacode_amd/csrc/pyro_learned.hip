@@ -161,9 +161,19 @@ __device__ __forceinline__ void mlp_forward(const MlpW &W, const float x[6], flo
   }
 }
 
+// Hardware reciprocal, reciprocal square root, 2^x and log2 (about 1 ulp).
+// The proposal's densities need not round like the host code: the network
+// output is whatever it is, and the sampler and its density only have to
+// describe the same law to float accuracy.
+__device__ __forceinline__ float rcp_f(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsq_f(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float exp_f(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504089f); }
+__device__ __forceinline__ float log_f(float x) { return __builtin_amdgcn_logf(x) * 0.693147180560f; }
+
 __device__ __forceinline__ void features(f3 u, f3 s, float x[6]) {
-  const float r = sqrtf(u.x * u.x + u.y * u.y + u.z * u.z);
-  const float g = r > 0.0f ? log1pf(r) / r : 1.0f;
+  const float r2 = u.x * u.x + u.y * u.y + u.z * u.z;
+  const float ir = rsq_f(r2);
+  const float g = r2 > 0.0f ? log_f(1.0f + r2 * ir) * ir : 1.0f;
   x[0] = u.x * g;
   x[1] = u.y * g;
   x[2] = u.z * g;
@@ -172,21 +182,24 @@ __device__ __forceinline__ void features(f3 u, f3 s, float x[6]) {
   x[5] = s.z;
 }
 
-struct Vmf { f3 mu; float kappa, log_norm; };
+struct Vmf { f3 mu; float kappa, em2k, log_norm; };  // em2k = expm1(-2 kappa)
 
 __device__ __forceinline__ Vmf decode(const float o[4]) {
   Vmf v;
-  const float nn = sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
-  if (nn > 0.0f) {
-    v.mu = {o[0] / nn, o[1] / nn, o[2] / nn};
+  const float n2 = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+  if (n2 > 0.0f) {
+    const float in = rsq_f(n2);
+    v.mu = {o[0] * in, o[1] * in, o[2] * in};
   } else {
     v.mu = {0.0f, 0.0f, 1.0f};
   }
   const float lk = fminf(fmaxf(o[3], kLogKappaMin), kLogKappaMax);
-  v.kappa = expf(lk);
+  v.kappa = exp_f(lk);
   // log of kappa / (2 pi (1 - exp(-2 kappa))); the exp(-kappa) factor of the
   // normaliser is carried by the -kappa |mu - y|^2 / 2 term of the density.
-  v.log_norm = lk - 1.83787706641f - logf(-expm1f(-2.0f * v.kappa));
+  // expm1 keeps its full precision: for small kappa it is the normaliser.
+  v.em2k = expm1f(-2.0f * v.kappa);
+  v.log_norm = lk - 1.83787706641f - log_f(-v.em2k);
   return v;
 }
 
@@ -203,7 +216,7 @@ __device__ __forceinline__ float vmf_log_density(const Vmf &v, f3 y) {
 __device__ __forceinline__ f3 vmf_sample(const Vmf &v, float u1, float u2) {
   const float k = v.kappa;
   const float p = 1.0f - u1;  // (0, 1]
-  float w = 1.0f + log1pf((1.0f - p) * expm1f(-2.0f * k)) / k;
+  float w = 1.0f + log1pf((1.0f - p) * v.em2k) * rcp_f(k);
   w = fminf(1.0f, fmaxf(-1.0f, w));
   const float st = sqrtf(fmaxf(0.0f, 1.0f - w * w));
   float ps, pc;
@@ -213,8 +226,12 @@ __device__ __forceinline__ f3 vmf_sample(const Vmf &v, float u1, float u2) {
   const f3 r = {st * pc, st * ps, up ? w : -w};
   const f3 N = {m.x, m.y, up ? m.z + 1.0f : m.z - 1.0f};
   const float d = up ? 1.0f + m.z : 1.0f - m.z;
-  const float scal = (N.x * r.x + N.y * r.y + N.z * r.z) / d;
+  const float scal = (N.x * r.x + N.y * r.y + N.z * r.z) * rcp_f(d);
   f3 o = {N.x * scal - r.x, N.y * scal - r.y, N.z * scal - r.z};
+  // The stored spin is normalised with correctly rounded sqrt and divide, as
+  // every other update does: a hardware rsq would leave |s| off 1 by a biased
+  // ulp, and the over-relaxation passes that renormalise the spin afterwards
+  // would turn that bias into an untracked energy drift.
   return normalise(o);
 }
 
@@ -227,6 +244,13 @@ __device__ __forceinline__ float mh_log_ratio(f3 h, float beta, f3 s, f3 sp, con
 }
 
 __device__ __forceinline__ int wrap_l(int v, int L) { return v < 0 ? v + L : (v >= L ? v - L : v); }
+
+// Periodic wrap of v in [-1, L]; a mask when L is a compile-time power of two.
+template <int LT>
+__device__ __forceinline__ int wrap_t(int v, int L) {
+  if constexpr (LT > 0 && (LT & (LT - 1)) == 0) return v & (LT - 1);
+  return wrap_l(v, L);
+}
 
 // One colour phase of learned proposals on every replica; same geometry as
 // the Metropolis phase (block = 4 waves = the 4 sites of the colour, lane =
@@ -272,8 +296,8 @@ __global__ __launch_bounds__(kPhaseBlock) void k_learned_phase(
 #pragma unroll
       for (int nb = 0; nb < 6; ++nb) {
         const Link lk = c_links_learned[site][nb];
-        const int j = lk.site * ncell + (wrap_l(z + lk.dz, L) * L + wrap_l(y + lk.dy, L)) * L +
-                      wrap_l(x + lk.dx, L);
+        const int j = lk.site * ncell + (wrap_t<LT>(z + lk.dz, L) * L + wrap_t<LT>(y + lk.dy, L)) * L +
+                      wrap_t<LT>(x + lk.dx, L);
         const float a = Sx[j], b = Sy[j], c = Sz[j];
         fx = fx + a;
         fy = fy + b;
@@ -299,7 +323,7 @@ __global__ __launch_bounds__(kPhaseBlock) void k_learned_phase(
     mlp_forward(W, x, o);
     const Vmf rev = decode(o);
     const float la = mh_log_ratio(hf, bt, s, sp, fwd, rev);
-    const bool accept = valid && (u01(rnd.z) < expf(fminf(la, 0.0f)));
+    const bool accept = valid && (u01(rnd.z) < exp_f(fminf(la, 0.0f)));
     if (accept) {
       de += (double)jeff * (((double)sp.x - (double)s.x) * gx + ((double)sp.y - (double)s.y) * gy +
                             ((double)sp.z - (double)s.z) * gz);
@@ -371,6 +395,7 @@ __global__ void k_vmf_draw(const float *__restrict__ rows, int n, uint32_t key0,
   Vmf v;
   v.mu = {q[0], q[1], q[2]};
   v.kappa = q[3];
+  v.em2k = expm1f(-2.0f * v.kappa);
   v.log_norm = 0.0f;
   const u32x4 rnd = philox4x32_10(rng_counter((uint32_t)idx, 0, 0, kKindLearned, 0, 0), key0, key1);
   const f3 y = vmf_sample(v, u01(rnd.x), u01(rnd.y));

@@ -57,7 +57,7 @@ def lib():
         L.orc_pt_swap.restype = C.c_int
         L.orc_run.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int,
                               C.c_int, C.POINTER(C.c_uint32), _f64p, _f32p, _f32p, _i32p,
-                              C.c_float, C.c_uint64, C.c_void_p, C.c_void_p,
+                              C.c_float, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
                               C.POINTER(C.c_int)]
         L.orc_num_threads.restype = C.c_int
         L.orc_set_num_threads.argtypes = [C.c_int]
@@ -178,7 +178,7 @@ class OracleRun:
         lib().orc_run(self.spins, self.L, self.R, n, self.sweep, self.n_metro, self.n_or,
                       self.pt_every, C.byref(self.pt_round), self.temps, self.beta, self.cosc,
                       self.replica_at, self.jeff, self.seed, _ptr(self.accepted), _ptr(self.dE),
-                      C.byref(self.swaps))
+                      _ptr(self.anchor), C.byref(self.swaps))
         self.sweep += n
 
     def observables(self):

@@ -413,7 +413,7 @@ int orc_pt_swap(int R, int parity, uint32_t round, uint64_t seed, const double *
 void orc_run(float *spins, int L, int R, int n_sweeps, uint32_t sweep0, int n_metro, int n_or,
              int pt_every, uint32_t *pt_round, double *temps, float *beta, float *cos_cone,
              int *replica_at, float jeff, uint64_t seed, uint64_t *accepted, double *dE,
-             int *n_swaps_accepted) {
+             double *anchor, int *n_swaps_accepted) {
   double *obs = NULL;
   double *en = NULL;
   if (pt_every > 0) {
@@ -430,6 +430,11 @@ void orc_run(float *spins, int L, int R, int n_sweeps, uint32_t sweep0, int n_me
     if (pt_every > 0 && (sweep + 1) % (uint32_t)pt_every == 0) {
       orc_observables(spins, L, R, jeff, obs);
       for (int r = 0; r < R; r++) en[r] = obs[13 * r];
+      if (anchor && dE)
+        for (int r = 0; r < R; r++) {
+          anchor[r] = en[r];
+          dE[r] = 0.0;
+        }
       int a = orc_pt_swap(R, (int)(*pt_round & 1u), *pt_round, seed, en, temps, beta, cos_cone,
                           replica_at);
       if (n_swaps_accepted) *n_swaps_accepted += a;

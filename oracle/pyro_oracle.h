@@ -65,12 +65,13 @@ int orc_pt_swap(int R, int parity, uint32_t round, uint64_t seed, const double *
 /* Full sweeps: n_metro Metropolis passes + n_or over-relaxation passes, each
  * over colours 0..3; after every pt_every sweeps (if > 0) a swap with parity
  * alternating from the stored round counter. sweep0 is the first sweep index;
- * *pt_round is advanced. energy series optional (NULL): if given, after every
- * sweep the full FP64 energy of each replica in rung order is appended. */
+ * *pt_round is advanced. anchor optional (NULL): if given, the full energies
+ * computed for each swap become the new anchor of the incremental energy
+ * (anchor[r] = E_r, dE[r] = 0), as the device library does. */
 void orc_run(float *spins, int L, int R, int n_sweeps, uint32_t sweep0, int n_metro, int n_or,
              int pt_every, uint32_t *pt_round, double *temps, float *beta, float *cos_cone,
              int *replica_at, float jeff, uint64_t seed, uint64_t *accepted, double *dE,
-             int *n_swaps_accepted);
+             double *anchor, int *n_swaps_accepted);
 
 /* Learned proposal (pyro_oracle_mlp.c), FP64 throughout. env rows are
  * (u = beta*h (3), s (3)); weights are the 4868 packed parameters. */

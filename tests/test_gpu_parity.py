@@ -295,3 +295,15 @@ def test_untracked_overrelaxation_drift_is_rounding_only():
         E = g.TotalEnergyv2()
         inc = g.energy_incremental()
         assert np.allclose(inc, E, rtol=1e-6, atol=1e-3)
+
+
+def test_exchange_reanchors_incremental_energy():
+    """The energies computed for each swap re-anchor the incremental energy:
+    right after a swap it equals the recomputed energy exactly, so untracked
+    over-relaxation rounding cannot build up past one exchange interval."""
+    temps = [0.02, 0.1, 0.4, 1.0]
+    with gpu(16, temps, track=False, n_metro=1, n_or=10, pt_every=10, cos_cone=0.5) as g:
+        g.Mc(20)
+        assert np.array_equal(g.energy_incremental(), g.TotalEnergyv2())
+        g.Mc(3)
+        assert np.allclose(g.energy_incremental(), g.TotalEnergyv2(), rtol=1e-6, atol=1e-3)

@@ -90,6 +90,23 @@ def test_incremental_energy_matches_recompute():
     assert run.accepted.sum() > 0
 
 
+def test_exchange_reanchors_incremental_energy():
+    """The full energies computed for a swap become the anchor (dE = 0), and
+    between swaps the books follow the moves: after 3 sweeps past the swap at
+    sweep 20, anchor + dE equals the recomputed energy again."""
+    L = 3
+    run = oracle.OracleRun(L, [0.3, 1.0, 4.0], seed=5, n_metro=1, n_or=3, pt_every=10,
+                           cos_cone=0.5)
+    start = run.anchor.copy()
+    run.run(20)
+    assert np.array_equal(run.dE, np.zeros(3))
+    assert not np.array_equal(run.anchor, start)
+    assert np.allclose(run.anchor, run.observables()[:, 0], rtol=0, atol=0)
+    run.run(3)
+    assert np.any(run.dE != 0)
+    assert np.allclose(run.anchor + run.dE, run.observables()[:, 0], rtol=1e-11, atol=1e-9)
+
+
 def test_overrelaxation_conserves_energy():
     L, R = 4, 2
     s = fresh(L, R)
