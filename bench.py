@@ -33,6 +33,8 @@ N_METRO, N_OR, PT_EVERY = 1, 10, 10
 SEED = 56081376
 BYTES_PER_UPDATE = 60  # 12 N B read + 3 N B written per colour phase of N/4 updates
 HBM_PEAK_GBS = 8000.0
+FLOP_PER_FORWARD = 2 * (6 * 64 + 64 * 64 + 64 * 4)  # MLP 6 -> 64 -> 64 -> 4
+BF16_PEAK_TFLOPS = 2500.0  # dense bf16 MFMA
 
 
 OR_KERNEL = "k_phase_cells<1, colour, 16, false>"
@@ -133,6 +135,8 @@ def learned_leg(g, args, plain_series, plain_sweeps_per_s, plain_taus, plain_sam
     series = g.record_fetch()
     a, p = g.acceptance_learned()
     sweeps_per_s = args.steps / dt
+    log("learned: phase timing, 200 learned launches")
+    learned_ms, _ = g.time_phase(2, 200)
     taus = [stats.tau_int(series[:, k])[0] for k in range(R)]
     samples = [stats.independent_samples_per_s(sweeps_per_s, t) for t in taus]
     ratio = np.array(samples) / np.array(plain_samples)
@@ -147,6 +151,17 @@ def learned_leg(g, args, plain_series, plain_sweeps_per_s, plain_taus, plain_sam
     g.Mc(args.steps)
     control = g.record_fetch()
     n_spins = 16 * L ** 3
+    # Dense work of one learned colour phase: two forward passes per proposal
+    # of 2 (6*64 + 64*64 + 64*4) = 9,472 FLOP each, N/4 proposals per replica.
+    proposals = n_spins // 4 * R
+    flop = proposals * 2 * FLOP_PER_FORWARD
+    learned_phase = {
+        "kernel": "k_learned_phase<16, 2>", "avg_launch_ms": learned_ms,
+        "proposals_per_launch": proposals, "flop_per_launch": flop,
+        "achieved_tflops": flop / (learned_ms * 1e-3) / 1e12, "peak_tflops": BF16_PEAK_TFLOPS,
+        "frac_mfma": flop / (learned_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
+        "achieved_gbs": proposals * BYTES_PER_UPDATE / (learned_ms * 1e-3) / 1e9,
+    }
 
     def zscores(a, b):
         z = []
@@ -169,6 +184,7 @@ def learned_leg(g, args, plain_series, plain_sweeps_per_s, plain_taus, plain_sam
         "loss_first": float(hist[:10].mean()),
         "loss_last": float(hist[-50:].mean()),
         "acceptance": float(a.sum() / max(p.sum(), 1)),
+        "phase": learned_phase,
         "sweeps_per_s": sweeps_per_s,
         "plain_sweeps_per_s": plain_sweeps_per_s,
         "spin_updates_per_s": args.steps * (1 + N_OR) * n_spins * R / dt,

@@ -200,7 +200,7 @@ static int create_ctx(const std::vector<int> &Ls, const double *temps, uint64_t 
   ALLOC(ctx->d_acc, R);
   ALLOC(ctx->d_dE, R);
   ALLOC(ctx->d_anchor, R);
-  ALLOC(ctx->d_partials, (size_t)R * ctx->obs_blocks * kObs);
+  ALLOC(ctx->d_partials, (size_t)R * ctx->obs_blocks * kObsPartials);
   ALLOC(ctx->d_obs, (size_t)R * kObs);
   ALLOC(ctx->d_ctl, kCtlWords);
 #undef ALLOC
@@ -831,7 +831,7 @@ int pyro_record_fetch(pyro_ctx *ctx, double *rows, int *n_rows) {
 
 int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, double *span_ms) {
   if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
-  if ((kind != 0 && kind != 1) || n_launches < 1 || n_launches > 100000)
+  if (kind < 0 || kind > 2 || n_launches < 1 || n_launches > 100000)
     return set_error(PYRO_EINVAL, "pyro_time_phase: kind %d / n_launches %d", kind, n_launches);
   PYRO_TRY(device_guard(ctx));
   std::vector<hipEvent_t> ev(2 * (size_t)n_launches, nullptr);
@@ -846,8 +846,9 @@ int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, dou
   for (int i = 0; i < n_launches && rc == 0; ++i) {
     ctx->time_ev0 = ev[2 * i];
     ctx->time_ev1 = ev[2 * i + 1];
-    rc = kind == 0 ? launch_metropolis_phase(ctx, i & 3, 0, nullptr)
-                   : launch_overrelax_phase(ctx, i & 3);
+    rc = kind == 0   ? launch_metropolis_phase(ctx, i & 3, 0, nullptr)
+         : kind == 1 ? launch_overrelax_phase(ctx, i & 3)
+                     : launch_learned_phase(ctx, i & 3, 0);
     ctx->time_ev0 = ctx->time_ev1 = nullptr;
   }
   if (rc == 0 && hipStreamSynchronize(ctx->stream) != hipSuccess)
@@ -867,6 +868,8 @@ int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, dou
   if (rc) return rc;
   if (kind == 0)
     for (auto &p : ctx->h_proposed) p += 4ull * (uint64_t)ctx->ncell * (uint64_t)n_launches;
+  if (kind == 2)
+    for (auto &p : ctx->h_proposed_learned) p += 4ull * (uint64_t)ctx->ncell * (uint64_t)n_launches;
   if (avg_ms) *avg_ms = sum / n_launches;
   if (span_ms) *span_ms = span;
   return 0;

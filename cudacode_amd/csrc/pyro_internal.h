@@ -26,6 +26,7 @@ enum CtlWord : int {
 };
 
 constexpr int kObs = 13;  // energy + 4 sublattices x 3 components
+constexpr int kObsPartials = 16;  // per observables block: 4 bases x (energy, mx, my, mz)
 constexpr int kPhaseBlock = 256;   // 4 waves: one per site of the colour
 constexpr int kPhaseCells = 64;    // cells per block (one per lane)
 constexpr int kObsBlock = 256;     // cells per block in the observable pass
@@ -47,7 +48,6 @@ constexpr int kPackP2 = kPackP1 + 64 * 8;
 constexpr int kPackP3 = kPackP2 + 64 * 64;
 constexpr int kPackSize = kPackP3 + 32 * 64;
 // Per-sample training buffers, [unit][batch]: x(6) a1(64) a2(64) d3(4) d2(64) d1(64) loss(1)
-constexpr int kTrainRows = 6 + 64 + 64 + 4 + 64 + 64 + 1;
 constexpr float kLogKappaMin = -4.60517019f;  // ln 0.01
 constexpr float kLogKappaMax = 8.31776617f;   // ln 4096
 
@@ -77,7 +77,7 @@ struct pyro_ctx {
   unsigned long long *d_acc = nullptr;  // R, accepted Metropolis proposals
   double *d_dE = nullptr;          // R, accumulated energy change since the anchor
   double *d_anchor = nullptr;      // R, full energy at the anchor
-  double *d_partials = nullptr;    // R * obs_blocks * kObs
+  double *d_partials = nullptr;    // R * obs_blocks * kObsPartials
   double *d_obs = nullptr;         // R * kObs
   uint32_t *d_ctl = nullptr;       // kCtlWords
   double *d_series = nullptr;      // record_cap * R

@@ -183,70 +183,84 @@ __global__ __launch_bounds__(256) void k_init_spins(float *__restrict__ spins, i
   }
 }
 
-// Per-block FP64 partial sums of the energy and the 12 sublattice sums
+// Per-block FP64 partial sums of the energy and the sublattice-b magnetisation
 // (TotalEnergyGPU, Pyrochlore.cu:344-389; MagnetizationGPU, :277-342;
-// Measurement, :526-578). grid = (obs_blocks, R), block = 256.
+// Measurement, :526-578). grid = (obs_blocks, 4, R): blockIdx.y is the basis
+// index b, and a thread sums the 4 sites 4t + b of its cell with all 72
+// neighbour loads independent, so a launch has 4x the waves of a cell-per-thread
+// layout and no dependent load chains. partials[((r * 4 + b) * nblk + blk) * 4 + q],
+// q = (energy, mx, my, mz).
+template <int B>
+__device__ __forceinline__ void obs_cell(const float *__restrict__ Sx, const float *__restrict__ Sy,
+                                         const float *__restrict__ Sz, int L, int ncell, int cell,
+                                         double v[4]) {
+  const int LL = L * L;
+  const int z = cell / LL;
+  const int rem = cell - z * LL;
+  const int y = rem / L;
+  const int x = rem - y * L;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int site = 4 * t + B;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+    for (int n = 0; n < 6; ++n) {
+      const Link lk = kLinks[site][n];
+      const int x1 = wrap(x + lk.dx, L);
+      const int y1 = wrap(y + lk.dy, L);
+      const int z1 = wrap(z + lk.dz, L);
+      const size_t j = (size_t)lk.site * ncell + (size_t)((z1 * L + y1) * L + x1);
+      gx += (double)Sx[j];
+      gy += (double)Sy[j];
+      gz += (double)Sz[j];
+    }
+    const size_t i = (size_t)site * ncell + cell;
+    const double sx = Sx[i], sy = Sy[i], sz = Sz[i];
+    v[0] += sx * gx + sy * gy + sz * gz;
+    v[1] += sx;
+    v[2] += sy;
+    v[3] += sz;
+  }
+}
+
 __global__ __launch_bounds__(kObsBlock) void k_observables(const float *__restrict__ spins,
                                                            int L, int ncell,
                                                            double *__restrict__ partials) {
-  const int r = blockIdx.y;
+  const int b = blockIdx.y;
+  const int r = blockIdx.z;
   const int cell = blockIdx.x * kObsBlock + threadIdx.x;
   const size_t plane = (size_t)16 * ncell;
   const float *Sx = spins + (size_t)r * 3 * plane;
   const float *Sy = Sx + plane;
   const float *Sz = Sy + plane;
-  double v[kObs];
-#pragma unroll
-  for (int q = 0; q < kObs; ++q) v[q] = 0.0;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (cell < ncell) {
-    const int LL = L * L;
-    const int z = cell / LL;
-    const int rem = cell - z * LL;
-    const int y = rem / L;
-    const int x = rem - y * L;
-    for (int t = 0; t < 4; ++t) {
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {  // b = site % 4 stays a compile-time index
-        const int site = 4 * t + b;
-        double gx = 0.0, gy = 0.0, gz = 0.0;
-#pragma unroll
-        for (int n = 0; n < 6; ++n) {
-          const Link lk = c_links[site][n];
-          const int x1 = wrap(x + lk.dx, L);
-          const int y1 = wrap(y + lk.dy, L);
-          const int z1 = wrap(z + lk.dz, L);
-          const size_t j = (size_t)lk.site * ncell + (size_t)((z1 * L + y1) * L + x1);
-          gx += (double)Sx[j];
-          gy += (double)Sy[j];
-          gz += (double)Sz[j];
-        }
-        const size_t i = (size_t)site * ncell + cell;
-        const double sx = Sx[i], sy = Sy[i], sz = Sz[i];
-        v[0] += sx * gx + sy * gy + sz * gz;
-        v[1 + 3 * b] += sx;
-        v[2 + 3 * b] += sy;
-        v[3 + 3 * b] += sz;
-      }
+    switch (b) {
+      case 0: obs_cell<0>(Sx, Sy, Sz, L, ncell, cell, v); break;
+      case 1: obs_cell<1>(Sx, Sy, Sz, L, ncell, cell, v); break;
+      case 2: obs_cell<2>(Sx, Sy, Sz, L, ncell, cell, v); break;
+      default: obs_cell<3>(Sx, Sy, Sz, L, ncell, cell, v); break;
     }
   }
-  __shared__ double s_v[4][kObs];
+  __shared__ double s_v[4][4];
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
 #pragma unroll
-  for (int q = 0; q < kObs; ++q) {
+  for (int q = 0; q < 4; ++q) {
     const double t = wave_sum(v[q]);
     if (lane == 0) s_v[wv][q] = t;
   }
   __syncthreads();
-  if (threadIdx.x < kObs) {
+  if (threadIdx.x < 4) {
     const int q = threadIdx.x;
     const double t = (s_v[0][q] + s_v[1][q]) + (s_v[2][q] + s_v[3][q]);
-    partials[((size_t)r * gridDim.x + blockIdx.x) * kObs + q] = t;
+    partials[(((size_t)r * 4 + b) * gridDim.x + blockIdx.x) * 4 + q] = t;
   }
 }
 
-// Fixed-order second stage: obs[r][q] = sum over blocks, then energy scaled by
-// jeff/2 (each bond was visited from both ends). grid = R, block = 256.
+// Fixed-order second stage: obs[r][0] = jeff/2 * sum over b and blocks of the
+// energy partials (each bond was visited from both ends); obs[r][1 + 3b + c] =
+// sum over blocks of component c of sublattice b. grid = R, block = 256.
 __global__ __launch_bounds__(256) void k_observables_final(const double *__restrict__ partials,
                                                            int nblk, float jeff,
                                                            double *__restrict__ obs) {
@@ -254,7 +268,14 @@ __global__ __launch_bounds__(256) void k_observables_final(const double *__restr
   __shared__ double s_v[256];
   for (int q = 0; q < kObs; ++q) {
     double t = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += 256) t += partials[((size_t)r * nblk + b) * kObs + q];
+    if (q == 0) {
+      for (int k = threadIdx.x; k < 4 * nblk; k += 256)
+        t += partials[((size_t)r * 4 * nblk + k) * 4];
+    } else {
+      const int b = (q - 1) / 3, c = (q - 1) % 3;
+      for (int k = threadIdx.x; k < nblk; k += 256)
+        t += partials[(((size_t)r * 4 + b) * nblk + k) * 4 + 1 + c];
+    }
     s_v[threadIdx.x] = t;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -461,7 +482,7 @@ int launch_overrelax_phase(pyro_ctx *ctx, int colour) {
 
 int launch_observables(pyro_ctx *ctx) {
   if (ctx->hetero) return launch_batch_observables(ctx);
-  dim3 grid(ctx->obs_blocks, ctx->R);
+  dim3 grid(ctx->obs_blocks, 4, ctx->R);
   k_observables<<<grid, kObsBlock, 0, ctx->stream>>>(ctx->d_spins, ctx->L, ctx->ncell,
                                                      ctx->d_partials);
   PYRO_HIP(hipGetLastError());

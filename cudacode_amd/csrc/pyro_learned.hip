@@ -12,6 +12,7 @@
 // Training (forward, backward, Adam) runs in FP32 on the vector units.
 #include <cmath>
 #include <cstring>
+#include <hip/hip_ext.h>
 
 #include "pyro_internal.h"
 
@@ -460,140 +461,168 @@ __device__ __forceinline__ double log_norm_d(double k) {
   return log(k) - log(2.0 * M_PI) - k - log(-expm1(-2.0 * k));
 }
 
-// Per-sample forward, loss and backward. Sample n of the batch reads env row
-// `rowsel` (explicit list) or a Philox-chosen row. Writes the training rows
-// [unit][batch] consumed by k_train_reduce. grid = ceil(B/64), block = 64.
+// Forward, loss and backward of 64 samples per block (one per thread), then
+// the block's partial gradient. Sample n of the batch reads env row
+// rowsel[n] (explicit list) or a Philox-chosen row. Activations and deltas
+// stay in LDS, [unit][sample] with a padded row stride so that the
+// gradient loop (threads = consecutive parameters, all at the same sample)
+// reads without bank conflicts; a2 is overwritten by d2 and a1 by d1 once
+// the gradients that need them are taken. part[block][p] (p < kMlpParams:
+// gradient sums over the block's samples in sample order; p = kMlpParams:
+// loss sum). grid = ceil(B/64), block = 64.
+constexpr int kGradLen = kMlpParams + 1;
+constexpr int kLdsRow = 65;
+
+__device__ __forceinline__ void grad_range(int p0, int p1, const float *__restrict__ sd,
+                                           const float *__restrict__ sa, int a_rows,
+                                           float *__restrict__ part, int nvalid) {
+  // Parameters [p0, p1): weights d_row x a_row (a_rows columns per row), then
+  // biases (p >= p0 + rows * a_rows) which sum d alone.
+  for (int p = p0 + (int)threadIdx.x; p < p1; p += 64) {
+    const int q = p - p0;
+    const int rows_w = (p1 - p0) / (a_rows + 1);  // weight rows = bias count
+    float t = 0.0f;
+    if (q < rows_w * a_rows) {
+      const float *d = sd + (q / a_rows) * kLdsRow;
+      const float *a = sa + (q % a_rows) * kLdsRow;
+      for (int m = 0; m < nvalid; ++m) t += d[m] * a[m];
+    } else {
+      const float *d = sd + (q - rows_w * a_rows) * kLdsRow;
+      for (int m = 0; m < nvalid; ++m) t += d[m];
+    }
+    part[p] = t;
+  }
+}
+
 __global__ __launch_bounds__(64) void k_train_samples(const float *__restrict__ env, int n_env,
                                                       const int *__restrict__ rowsel, int B,
                                                       uint32_t step, uint32_t key0, uint32_t key1,
                                                       const float *__restrict__ w,
-                                                      float *__restrict__ T) {
-  const int n = blockIdx.x * 64 + threadIdx.x;
-  if (n >= B) return;
-  int row;
-  if (rowsel) {
-    row = rowsel[n];
-  } else {
-    const u32x4 rnd = philox4x32_10(rng_counter((uint32_t)n, 0, 0, kKindTrain, 0, step), key0, key1);
-    row = (int)(rnd.x % (uint32_t)n_env);
+                                                      float *__restrict__ part_all) {
+  __shared__ float sx[6 * kLdsRow], sa1[64 * kLdsRow], sa2[64 * kLdsRow], sd3[4 * kLdsRow],
+      sl[64];
+  const int li = threadIdx.x;
+  const int n = blockIdx.x * 64 + li;
+  const int nvalid = min(64, B - (int)blockIdx.x * 64);
+  float *part = part_all + (size_t)blockIdx.x * kGradLen;
+  float go[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (n < B) {
+    int row;
+    if (rowsel) {
+      row = rowsel[n];
+    } else {
+      const u32x4 rnd = philox4x32_10(rng_counter((uint32_t)n, 0, 0, kKindTrain, 0, step), key0, key1);
+      row = (int)(rnd.x % (uint32_t)n_env);
+    }
+    const float *e = env + (size_t)row * 6;
+    const f3 u = {e[0], e[1], e[2]};
+    const f3 s = {e[3], e[4], e[5]};
+    float x[6];
+    features(u, s, x);
+    for (int k = 0; k < 6; ++k) sx[k * kLdsRow + li] = x[k];
+    for (int j = 0; j < 64; ++j) {
+      float t = w[kOffB1 + j];
+      for (int k = 0; k < 6; ++k) t += w[kOffW1 + j * 6 + k] * x[k];
+      sa1[j * kLdsRow + li] = relu(t);
+    }
+    for (int j = 0; j < 64; ++j) {
+      float t = w[kOffB2 + j];
+      for (int k = 0; k < 64; ++k) t += w[kOffW2 + j * 64 + k] * sa1[k * kLdsRow + li];
+      sa2[j * kLdsRow + li] = relu(t);
+    }
+    float o[4];
+    for (int q = 0; q < 4; ++q) {
+      float t = w[kOffB3 + q];
+      for (int k = 0; k < 64; ++k) t += w[kOffW3 + q * 64 + k] * sa2[k * kLdsRow + li];
+      o[q] = t;
+    }
+    // Reverse KL to vMF(u/|u|, |u|) and its gradient wrt o, in FP64.
+    const double on = sqrt((double)o[0] * o[0] + (double)o[1] * o[1] + (double)o[2] * o[2]);
+    double mu[3] = {0.0, 0.0, 1.0};
+    if (on > 0.0) {
+      mu[0] = o[0] / on;
+      mu[1] = o[1] / on;
+      mu[2] = o[2] / on;
+    }
+    double lk = o[3];
+    const bool clamped = lk < (double)kLogKappaMin || lk > (double)kLogKappaMax;
+    lk = fmin(fmax(lk, (double)kLogKappaMin), (double)kLogKappaMax);
+    const double kap = exp(lk);
+    const double ks = sqrt((double)u.x * u.x + (double)u.y * u.y + (double)u.z * u.z);
+    double hat[3] = {0.0, 0.0, 0.0};
+    if (ks > 0.0) {
+      hat[0] = u.x / ks;
+      hat[1] = u.y / ks;
+      hat[2] = u.z / ks;
+    }
+    const double c = mu[0] * hat[0] + mu[1] * hat[1] + mu[2] * hat[2];
+    const double A = langevin(kap);
+    const double kl = log_norm_d(kap) - log_norm_d(ks) + A * (kap - ks * c);
+    go[3] = clamped ? 0.0f : (float)(langevin_d(kap) * (kap - ks * c) * kap);
+    if (on > 0.0) {
+      const double gm[3] = {-A * ks * hat[0], -A * ks * hat[1], -A * ks * hat[2]};
+      const double proj = gm[0] * mu[0] + gm[1] * mu[1] + gm[2] * mu[2];
+      for (int q = 0; q < 3; ++q) go[q] = (float)((gm[q] - proj * mu[q]) / on);
+    }
+    sl[li] = (float)kl;
+    for (int q = 0; q < 4; ++q) sd3[q * kLdsRow + li] = go[q];
   }
-  const float *e = env + (size_t)row * 6;
-  const f3 u = {e[0], e[1], e[2]};
-  const f3 s = {e[3], e[4], e[5]};
-  float x[6];
-  features(u, s, x);
-  float *Tx = T, *Ta1 = T + 6 * (size_t)B, *Ta2 = Ta1 + 64 * (size_t)B, *Td3 = Ta2 + 64 * (size_t)B,
-        *Td2 = Td3 + 4 * (size_t)B, *Td1 = Td2 + 64 * (size_t)B, *Tl = Td1 + 64 * (size_t)B;
-  for (int k = 0; k < 6; ++k) Tx[(size_t)k * B + n] = x[k];
-  float a1[64], a2[64];
-  for (int j = 0; j < 64; ++j) {
-    float t = w[kOffB1 + j];
-    for (int k = 0; k < 6; ++k) t += w[kOffW1 + j * 6 + k] * x[k];
-    a1[j] = relu(t);
-    Ta1[(size_t)j * B + n] = a1[j];
-  }
-  for (int j = 0; j < 64; ++j) {
-    float t = w[kOffB2 + j];
-    for (int k = 0; k < 64; ++k) t += w[kOffW2 + j * 64 + k] * a1[k];
-    a2[j] = relu(t);
-    Ta2[(size_t)j * B + n] = a2[j];
-  }
-  float o[4];
-  for (int q = 0; q < 4; ++q) {
-    float t = w[kOffB3 + q];
-    for (int k = 0; k < 64; ++k) t += w[kOffW3 + q * 64 + k] * a2[k];
-    o[q] = t;
-  }
-  // Reverse KL to vMF(u/|u|, |u|) and its gradient wrt o, in FP64.
-  const double on = sqrt((double)o[0] * o[0] + (double)o[1] * o[1] + (double)o[2] * o[2]);
-  double mu[3] = {0.0, 0.0, 1.0};
-  if (on > 0.0) {
-    mu[0] = o[0] / on;
-    mu[1] = o[1] / on;
-    mu[2] = o[2] / on;
-  }
-  double lk = o[3];
-  const bool clamped = lk < (double)kLogKappaMin || lk > (double)kLogKappaMax;
-  lk = fmin(fmax(lk, (double)kLogKappaMin), (double)kLogKappaMax);
-  const double kap = exp(lk);
-  const double ks = sqrt((double)u.x * u.x + (double)u.y * u.y + (double)u.z * u.z);
-  double hat[3] = {0.0, 0.0, 0.0};
-  if (ks > 0.0) {
-    hat[0] = u.x / ks;
-    hat[1] = u.y / ks;
-    hat[2] = u.z / ks;
-  }
-  const double c = mu[0] * hat[0] + mu[1] * hat[1] + mu[2] * hat[2];
-  const double A = langevin(kap);
-  const double kl = log_norm_d(kap) - log_norm_d(ks) + A * (kap - ks * c);
-  float go[4];
-  go[3] = clamped ? 0.0f : (float)(langevin_d(kap) * (kap - ks * c) * kap);
-  if (on > 0.0) {
-    const double gm[3] = {-A * ks * hat[0], -A * ks * hat[1], -A * ks * hat[2]};
-    const double proj = gm[0] * mu[0] + gm[1] * mu[1] + gm[2] * mu[2];
-    for (int q = 0; q < 3; ++q) go[q] = (float)((gm[q] - proj * mu[q]) / on);
-  } else {
-    go[0] = go[1] = go[2] = 0.0f;
-  }
-  Tl[n] = (float)kl;
-  for (int q = 0; q < 4; ++q) Td3[(size_t)q * B + n] = go[q];
-  // d2 in place of a2, then d1 in place of a1 (a1 is already stored).
-  for (int k = 0; k < 64; ++k) {
+  __syncthreads();
+  // Layer 3 (d3 x a2) and the loss.
+  grad_range(kOffW3, kMlpParams, sd3, sa2, 64, part, nvalid);
+  if (li == 0) {
     float t = 0.0f;
-    for (int q = 0; q < 4; ++q) t += w[kOffW3 + q * 64 + k] * go[q];
-    a2[k] = a2[k] > 0.0f ? t : 0.0f;
-    Td2[(size_t)k * B + n] = a2[k];
+    for (int m = 0; m < nvalid; ++m) t += sl[m];
+    part[kMlpParams] = t;
   }
-  for (int k = 0; k < 64; ++k) {
-    float t = 0.0f;
-    for (int j = 0; j < 64; ++j) t += w[kOffW2 + j * 64 + k] * a2[j];
-    a1[k] = a1[k] > 0.0f ? t : 0.0f;
-    Td1[(size_t)k * B + n] = a1[k];
+  __syncthreads();
+  if (n < B)  // d2 replaces a2 (this thread's column only)
+    for (int k = 0; k < 64; ++k) {
+      float t = 0.0f;
+      for (int q = 0; q < 4; ++q) t += w[kOffW3 + q * 64 + k] * go[q];
+      float &a = sa2[k * kLdsRow + li];
+      a = a > 0.0f ? t : 0.0f;
+    }
+  __syncthreads();
+  grad_range(kOffW2, kOffW3, sa2, sa1, 64, part, nvalid);  // d2 x a1
+  __syncthreads();
+  if (n < B) {  // d1 replaces a1
+    float d1[64];
+#pragma unroll
+    for (int k = 0; k < 64; ++k) d1[k] = 0.0f;
+    for (int j = 0; j < 64; ++j) {
+      const float d2 = sa2[j * kLdsRow + li];
+#pragma unroll
+      for (int k = 0; k < 64; ++k) d1[k] += w[kOffW2 + j * 64 + k] * d2;
+    }
+#pragma unroll
+    for (int k = 0; k < 64; ++k) {
+      float &a = sa1[k * kLdsRow + li];
+      a = a > 0.0f ? d1[k] : 0.0f;
+    }
   }
+  __syncthreads();
+  grad_range(kOffW1, kOffW2, sa1, sx, 6, part, nvalid);  // d1 x x
 }
 
-// Batch-mean gradient, one parameter per thread, fixed summation order
-// (FP64 accumulation). grad[kMlpParams] receives the mean loss.
-__global__ void k_train_reduce(const float *__restrict__ T, int B, float *__restrict__ grad) {
+// Batch-mean gradient and loss: grad[p] = sum over blocks of part[blk][p] in
+// block order (FP64 accumulation) / B. One parameter per thread.
+__global__ void k_train_reduce(const float *__restrict__ part, int nblk, int B,
+                               float *__restrict__ grad) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p > kMlpParams) return;
-  const float *Tx = T, *Ta1 = T + 6 * (size_t)B, *Ta2 = Ta1 + 64 * (size_t)B,
-              *Td3 = Ta2 + 64 * (size_t)B, *Td2 = Td3 + 4 * (size_t)B, *Td1 = Td2 + 64 * (size_t)B,
-              *Tl = Td1 + 64 * (size_t)B;
-  const float *d = nullptr, *a = nullptr;
-  if (p < kOffB1) {
-    d = Td1 + (size_t)(p / 6) * B;
-    a = Tx + (size_t)(p % 6) * B;
-  } else if (p < kOffW2) {
-    d = Td1 + (size_t)(p - kOffB1) * B;
-  } else if (p < kOffB2) {
-    const int q = p - kOffW2;
-    d = Td2 + (size_t)(q / 64) * B;
-    a = Ta1 + (size_t)(q % 64) * B;
-  } else if (p < kOffW3) {
-    d = Td2 + (size_t)(p - kOffB2) * B;
-  } else if (p < kOffB3) {
-    const int q = p - kOffW3;
-    d = Td3 + (size_t)(q / 64) * B;
-    a = Ta2 + (size_t)(q % 64) * B;
-  } else if (p < kMlpParams) {
-    d = Td3 + (size_t)(p - kOffB3) * B;
-  } else {
-    d = Tl;
-  }
+  if (p >= kGradLen) return;
   double t = 0.0;
-  if (a) {
-    for (int n = 0; n < B; ++n) t += (double)d[n] * (double)a[n];
-  } else {
-    for (int n = 0; n < B; ++n) t += (double)d[n];
-  }
+  for (int b = 0; b < nblk; ++b) t += (double)part[(size_t)b * kGradLen + p];
   grad[p] = (float)(t / B);
 }
 
+// Adam step on every parameter; thread 0 also files the batch loss
+// (g[kMlpParams]) into loss_out when given.
 __global__ void k_adam(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
                        const float *__restrict__ g, float lr, float b1, float b2, float eps,
-                       float c1, float c2) {
+                       float c1, float c2, float *__restrict__ loss_out) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0 && loss_out) *loss_out = g[kMlpParams];
   if (p >= kMlpParams) return;
   const float gp = g[p];
   const float mp = b1 * m[p] + (1.0f - b1) * gp;
@@ -668,6 +697,14 @@ template <int LT>
 static void launch_learned_t(pyro_ctx *ctx, int colour, int pass) {
   constexpr int G = kLearnedGroups;
   dim3 grid((ctx->ncell + kPhaseCells * G - 1) / (kPhaseCells * G), ctx->R);
+  if (ctx->time_ev0) {
+    hipExtLaunchKernelGGL(k_learned_phase<LT, G>, grid, dim3(kPhaseBlock), 0, ctx->stream,
+                          ctx->time_ev0, ctx->time_ev1, 0, ctx->d_spins, ctx->L, ctx->ncell, colour,
+                          pass, (const float *)ctx->d_beta, (const uint32_t *)ctx->d_ctl, klo(ctx),
+                          khi(ctx), ctx->jeff, (const uint16_t *)ctx->d_mlp_pack,
+                          (const float *)ctx->d_mlp_w, ctx->d_acc_learned, ctx->d_dE);
+    return;
+  }
   k_learned_phase<LT, G><<<grid, kPhaseBlock, 0, ctx->stream>>>(
       ctx->d_spins, ctx->L, ctx->ncell, colour, pass, ctx->d_beta, ctx->d_ctl, klo(ctx), khi(ctx),
       ctx->jeff, ctx->d_mlp_pack, ctx->d_mlp_w, ctx->d_acc_learned, ctx->d_dE);
@@ -770,12 +807,14 @@ int pyro_vmf_draw(pyro_ctx *ctx, const float *rows4, int n, float *out3) {
   return rc;
 }
 
+// Partial-gradient buffer for batches of up to B samples: one kGradLen row
+// per 64-sample block.
 static int ensure_train(pyro_ctx *ctx, int B) {
   if (ctx->train_cap >= B) return 0;
   if (ctx->d_train) PYRO_HIP(hipFree(ctx->d_train));
   ctx->d_train = nullptr;
   ctx->train_cap = 0;
-  PYRO_HIP(hipMalloc((void **)&ctx->d_train, sizeof(float) * kTrainRows * (size_t)B));
+  PYRO_HIP(hipMalloc((void **)&ctx->d_train, sizeof(float) * kGradLen * (size_t)((B + 63) / 64)));
   ctx->train_cap = B;
   return 0;
 }
@@ -803,7 +842,8 @@ int pyro_mlp_loss_grad(pyro_ctx *ctx, const float *env6, int n, double *loss, fl
   if (!rc) {
     k_train_samples<<<(n + 63) / 64, 64, 0, ctx->stream>>>(d_env, n, d_sel, n, 0, klo(ctx), khi(ctx),
                                                           ctx->d_mlp_w, ctx->d_train);
-    k_train_reduce<<<(kMlpParams + 1 + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_train, n, ctx->d_grad);
+    k_train_reduce<<<(kGradLen + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_train, (n + 63) / 64, n,
+                                                                     ctx->d_grad);
     if (hipGetLastError() != hipSuccess) rc = set_error(PYRO_EHIP, "pyro_mlp_loss_grad: launch failed");
   }
   if (!rc && (hipStreamSynchronize(ctx->stream) != hipSuccess ||
@@ -869,26 +909,34 @@ int pyro_mlp_train(pyro_ctx *ctx, int steps, int batch, float lr, float lr_final
   if (ctx->env_n < 1) return set_error(PYRO_ESTATE, "pyro_mlp_train: no environments gathered");
   PYRO_HIP(hipSetDevice(ctx->device));
   PYRO_TRY(ensure_train(ctx, batch));
+  float *d_loss = nullptr;
+  if (loss_hist && steps > 0 && hipMalloc((void **)&d_loss, sizeof(float) * (size_t)steps) != hipSuccess)
+    return set_error(PYRO_ENOMEM, "pyro_mlp_train: loss history allocation failed");
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  for (int s = 0; s < steps; ++s) {
+  int rc = 0;
+  for (int s = 0; s < steps && rc == 0; ++s) {
     const int t = ++ctx->adam_t;
     const double frac = steps > 1 ? (double)s / (steps - 1) : 0.0;
     const float lr_s = (float)(lr * std::pow((double)lr_final / lr, frac));  // geometric decay
     k_train_samples<<<(batch + 63) / 64, 64, 0, ctx->stream>>>(ctx->d_env, ctx->env_n, nullptr, batch,
                                                               (uint32_t)t, klo(ctx), khi(ctx),
                                                               ctx->d_mlp_w, ctx->d_train);
-    k_train_reduce<<<(kMlpParams + 1 + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_train, batch, ctx->d_grad);
+    k_train_reduce<<<(kGradLen + 255) / 256, 256, 0, ctx->stream>>>(
+        ctx->d_train, (batch + 63) / 64, batch, ctx->d_grad);
     k_adam<<<(kMlpParams + 255) / 256, 256, 0, ctx->stream>>>(
         ctx->d_mlp_w, ctx->d_mlp_m, ctx->d_mlp_v, ctx->d_grad, lr_s, b1, b2, eps,
-        (float)(1.0 - std::pow((double)b1, t)), (float)(1.0 - std::pow((double)b2, t)));
-    PYRO_HIP(hipGetLastError());
-    if (loss_hist)
-      PYRO_HIP(hipMemcpyAsync(loss_hist + s, ctx->d_grad + kMlpParams, sizeof(float),
-                              hipMemcpyDeviceToHost, ctx->stream));
+        (float)(1.0 - std::pow((double)b1, t)), (float)(1.0 - std::pow((double)b2, t)),
+        d_loss ? d_loss + s : nullptr);
+    if (hipGetLastError() != hipSuccess) rc = set_error(PYRO_EHIP, "pyro_mlp_train: launch failed");
   }
-  PYRO_TRY(mlp_pack(ctx));
-  PYRO_HIP(hipStreamSynchronize(ctx->stream));
-  return 0;
+  if (rc == 0) rc = mlp_pack(ctx);
+  if (rc == 0 && hipStreamSynchronize(ctx->stream) != hipSuccess)
+    rc = set_error(PYRO_EHIP, "pyro_mlp_train: synchronise failed");
+  if (rc == 0 && d_loss &&
+      hipMemcpy(loss_hist, d_loss, sizeof(float) * (size_t)steps, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = set_error(PYRO_EHIP, "pyro_mlp_train: loss history download failed");
+  if (d_loss) (void)hipFree(d_loss);
+  return rc;
 }
 
 int pyro_learned_phase(pyro_ctx *ctx, int colour, uint32_t sweep, int pass) {

@@ -253,7 +253,8 @@ int pyro_export_configurations(pyro_ctx *ctx, const char *prefix, int append, in
 int pyro_import_configurations(pyro_ctx *ctx, const char *prefix, int rep, int bytes_per_value);
 
 /* Time n_launches back-to-back colour-phase launches of one kind (0 =
- * Metropolis, 1 = over-relaxation; colours cycle 0..3) on the ctx stream,
+ * Metropolis, 1 = over-relaxation, 2 = learned proposal, which needs weights
+ * set; colours cycle 0..3) on the ctx stream,
  * each launch carrying its own start/stop HIP events in its dispatch
  * (hipExtLaunchKernelGGL). avg_ms: mean kernel time per launch; span_ms:
  * first start to last stop. The phases run for real (the state
