@@ -55,16 +55,30 @@ static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
     for (int o = 0; o < ctx->n_or; ++o)
       for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_overrelax_phase(ctx, c));
   }
-  if (ctx->pt_every > 0 && (sweep_index + 1u) % (uint32_t)ctx->pt_every == 0u) {
+  const bool exchange = ctx->pt_every > 0 && (sweep_index + 1u) % (uint32_t)ctx->pt_every == 0u;
+  const bool measure =
+      ctx->measure_every > 0 && (sweep_index + 1u) % (uint32_t)ctx->measure_every == 0u;
+  if (exchange || measure) {
     PYRO_TRY(launch_observables(ctx));
-    // The full energies computed for the swap also re-anchor the incremental
-    // energy, so FP32 rounding in the untracked over-relaxation passes cannot
-    // accumulate beyond one exchange interval.
+    // The full energies also re-anchor the incremental energy, so FP32
+    // rounding in the untracked over-relaxation passes cannot accumulate
+    // beyond one exchange (or measurement) interval.
     PYRO_TRY(launch_set_anchor(ctx));
-    PYRO_TRY(launch_pt_swap(ctx, 0, 0, 1));
   }
+  // The measurement sees the configuration of this sweep at the rungs it was
+  // simulated at, before the exchange moves temperatures.
+  if (measure) PYRO_TRY(launch_measure_row(ctx));
+  if (exchange) PYRO_TRY(launch_pt_swap(ctx, 0, 0, 1));
   PYRO_TRY(launch_end_sweep(ctx));
   return 0;
+}
+
+// Sweeps per captured graph: a whole number of exchange and measurement
+// intervals, so that every replay starts aligned with both cadences.
+static uint32_t graph_block(const pyro_ctx *ctx) {
+  const uint32_t a = ctx->pt_every > 0 ? (uint32_t)ctx->pt_every : 1u;
+  const uint32_t b = ctx->measure_every > 0 ? (uint32_t)ctx->measure_every : 1u;
+  return (uint32_t)std::lcm(a, b);
 }
 
 static uint64_t replica_cells(const pyro_ctx *ctx, int r) {
@@ -261,7 +275,7 @@ int pyro_destroy(pyro_ctx *ctx) {
   batch_free(ctx);
   void *ptrs[] = {ctx->d_spins, ctx->d_temps, ctx->d_beta,     ctx->d_cosc, ctx->d_replica_at,
                   ctx->d_acc,   ctx->d_dE,    ctx->d_anchor,   ctx->d_partials, ctx->d_obs,
-                  ctx->d_ctl,   ctx->d_series};
+                  ctx->d_ctl,   ctx->d_series, ctx->d_meas};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -386,7 +400,7 @@ int pyro_sweep(pyro_ctx *ctx, int n_sweeps, int use_graph) {
   if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
   if (n_sweeps < 0) return set_error(PYRO_EINVAL, "n_sweeps %d < 0", n_sweeps);
   PYRO_TRY(device_guard(ctx));
-  const uint32_t block = ctx->pt_every > 0 ? (uint32_t)ctx->pt_every : 1u;
+  const uint32_t block = graph_block(ctx);
   uint32_t left = (uint32_t)n_sweeps;
   if (!use_graph) {
     while (left > 0) {
@@ -395,8 +409,8 @@ int pyro_sweep(pyro_ctx *ctx, int n_sweeps, int use_graph) {
       --left;
     }
   }
-  // A captured block always starts on a multiple of the exchange cadence, so
-  // the swap lands after the same sweeps as in eager mode.
+  // A captured block always starts on a multiple of the exchange and
+  // measurement cadences, so both land after the same sweeps as in eager mode.
   while (left > 0 && ctx->h_sweep % block != 0u) {
     PYRO_TRY(enqueue_sweep(ctx, ctx->h_sweep));
     account_sweeps(ctx, 1);
@@ -518,28 +532,7 @@ int pyro_magnetization(pyro_ctx *ctx, double *M3) {
 int pyro_fold_order_params(const double *sub12, int R, double n_spins, double *op12) {
   if (!sub12 || !op12 || R < 0 || !(n_spins > 0.0))
     return set_error(PYRO_EINVAL, "pyro_fold_order_params: bad argument");
-  const double a = 0.5773502691896258, b = 0.816496580927726, c = 0.4082482904638631,
-               d = 0.7071067811865475;
-  for (int r = 0; r < R; ++r) {
-    const double *s = sub12 + (size_t)r * 12;
-    const double *s0 = s, *s1 = s + 3, *s2 = s + 6, *s3 = s + 9;
-    double m[12];
-    m[0] = a * (s0[0] + s0[1] + s0[2]) + a * (-s1[0] - s1[1] + s1[2]) +
-           a * (s2[0] - s2[1] - s2[2]) + a * (-s3[0] + s3[1] - s3[2]);
-    m[1] = (-b * s0[0] + c * s0[1] + c * s0[2]) + (b * s1[0] - c * s1[1] + c * s1[2]) +
-           (-b * s2[0] - c * s2[1] - c * s2[2]) + (b * s3[0] + c * s3[1] - c * s3[2]);
-    m[2] = d * (-s0[1] + s0[2] + s1[1] + s1[2] + s2[1] - s2[2] - s3[1] - s3[2]);
-    m[3] = d * (-s0[1] + s0[2] - s1[1] - s1[2] + s2[1] - s2[2] + s3[1] + s3[2]);
-    m[4] = d * (s0[0] - s0[2] + s1[0] + s1[2] - s2[0] - s2[2] - s3[0] + s3[2]);
-    m[5] = d * (-s0[0] + s0[1] + s1[0] - s1[1] + s2[0] + s2[1] - s3[0] - s3[1]);
-    m[6] = s0[0] + s1[0] + s2[0] + s3[0];
-    m[7] = s0[1] + s1[1] + s2[1] + s3[1];
-    m[8] = s0[2] + s1[2] + s2[2] + s3[2];
-    m[9] = d * (-s0[1] - s0[2] - s1[1] + s1[2] + s2[1] + s2[2] + s3[1] - s3[2]);
-    m[10] = d * (-s0[0] - s0[2] - s1[0] + s1[2] + s2[0] - s2[2] + s3[0] + s3[2]);
-    m[11] = d * (-s0[0] - s0[1] + s1[0] + s1[1] + s2[0] - s2[1] - s3[0] + s3[1]);
-    for (int q = 0; q < 12; ++q) op12[(size_t)r * 12 + q] = m[q] / n_spins;
-  }
+  for (int r = 0; r < R; ++r) fold_order_params(sub12 + (size_t)r * 12, n_spins, op12 + (size_t)r * 12);
   return 0;
 }
 
@@ -825,6 +818,85 @@ int pyro_record_fetch(pyro_ctx *ctx, double *rows, int *n_rows) {
   *n_rows = (int)row;
   if (rows && row > 0)
     PYRO_HIP(hipMemcpy(rows, ctx->d_series, sizeof(double) * (size_t)row * ctx->R,
+                       hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pyro_anneal(pyro_ctx *ctx, double T_start, int steps, int sweeps_per_step) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (!(T_start > 0.0) || !std::isfinite(T_start) || steps < 1 || sweeps_per_step < 1)
+    return set_error(PYRO_EINVAL, "pyro_anneal: T_start %g, steps %d, sweeps_per_step %d", T_start,
+                     steps, sweeps_per_step);
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  // Annealing sweeps have no exchanges and no measurements (the reference
+  // runs Mc alone at each step); both cadences come back afterwards.
+  const int pt = ctx->pt_every, me = ctx->measure_every;
+  ctx->pt_every = 0;
+  ctx->measure_every = 0;
+  drop_graph(ctx);
+  int rc = 0;
+  for (int m = 0; m < steps && rc == 0; ++m) {
+    rc = launch_anneal_beta(ctx, T_start, m, steps, 0);
+    if (rc == 0) rc = pyro_sweep(ctx, sweeps_per_step, 1);
+  }
+  const int rc2 = launch_anneal_beta(ctx, T_start, steps, steps, 1);
+  drop_graph(ctx);
+  ctx->pt_every = pt;
+  ctx->measure_every = me;
+  if (rc) return rc;
+  if (rc2) return rc2;
+  PYRO_TRY(reanchor(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int pyro_measure_start(pyro_ctx *ctx, int every, int capacity) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (every < 1 || capacity < 1)
+    return set_error(PYRO_EINVAL, "pyro_measure_start: every %d / capacity %d must be >= 1", every,
+                     capacity);
+  const uint32_t a = ctx->pt_every > 0 ? (uint32_t)ctx->pt_every : 1u;
+  if (std::lcm(a, (uint32_t)every) > 100000u)
+    return set_error(PYRO_EINVAL,
+                     "pyro_measure_start: every %d and pt_every %d give a capture block above 1e5 sweeps",
+                     every, ctx->pt_every);
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  drop_graph(ctx);  // the captured sweeps have the old cadence and pointers
+  if (capacity > ctx->meas_cap) {
+    if (ctx->d_meas) PYRO_HIP(hipFree(ctx->d_meas));
+    ctx->d_meas = nullptr;
+    ctx->meas_cap = 0;
+    if (hipMalloc((void **)&ctx->d_meas, sizeof(double) * kMeasQ * (size_t)capacity * ctx->R) !=
+        hipSuccess)
+      return set_error(PYRO_ENOMEM, "pyro_measure_start: %d rows do not fit", capacity);
+    ctx->meas_cap = capacity;
+  }
+  ctx->measure_every = every;
+  const uint32_t w[2] = {0u, (uint32_t)capacity};
+  PYRO_HIP(hipMemcpy(ctx->d_ctl + kCtlMeasRow, w, sizeof w, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int pyro_measure_stop(pyro_ctx *ctx) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->measure_every != 0) drop_graph(ctx);
+  ctx->measure_every = 0;
+  return 0;
+}
+
+int pyro_measure_fetch(pyro_ctx *ctx, double *rows, int *n_rows) {
+  if (!ctx || !n_rows) return set_error(PYRO_EINVAL, "pyro_measure_fetch: null argument");
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  uint32_t row = 0;
+  PYRO_HIP(hipMemcpy(&row, ctx->d_ctl + kCtlMeasRow, sizeof row, hipMemcpyDeviceToHost));
+  *n_rows = (int)row;
+  if (rows && row > 0)
+    PYRO_HIP(hipMemcpy(rows, ctx->d_meas, sizeof(double) * kMeasQ * (size_t)row * ctx->R,
                        hipMemcpyDeviceToHost));
   return 0;
 }

@@ -22,11 +22,15 @@ enum CtlWord : int {
   kCtlRecordRow = 3,  // rows written to the energy series
   kCtlRecordOn = 4,   // 1 while recording
   kCtlRecordCap = 5,  // series capacity in rows
+  kCtlMeasRow = 6,    // rows written to the measurement series
+  kCtlMeasCap = 7,    // measurement series capacity in rows (0 = off)
   kCtlWords = 8,
 };
 
 constexpr int kObs = 13;  // energy + 4 sublattices x 3 components
-constexpr int kObsPartials = 16;  // per observables block: 4 bases x (energy, mx, my, mz)
+constexpr int kObsPartials = 16;
+// Measurement row, per rung: E, |m_A2|, |m_E|, |m_T1|, |m_F| (= |M| / N), |m_T2|.
+constexpr int kMeasQ = 6;  // per observables block: 4 bases x (energy, mx, my, mz)
 constexpr int kPhaseBlock = 256;   // 4 waves: one per site of the colour
 constexpr int kPhaseCells = 64;    // cells per block (one per lane)
 constexpr int kObsBlock = 256;     // cells per block in the observable pass
@@ -113,12 +117,18 @@ struct pyro_ctx {
   float *d_grad = nullptr;               // kMlpParams gradient + 1 loss slot
   float *d_env = nullptr;                // environments (u = beta h, s), 6 floats each
   int env_cap = 0, env_n = 0;
-  float *d_train = nullptr;              // per-sample activations / deltas of a batch
+  float *d_train = nullptr;              // per-block partial gradients of a batch
   int train_cap = 0;
   int adam_t = 0;
   bool mlp_ready = false;
   unsigned long long *d_acc_learned = nullptr;  // R, accepted learned proposals
   std::vector<uint64_t> h_proposed_learned;
+
+  // measurement series: every measure_every sweeps one row of kMeasQ values
+  // per rung (descending T) is appended on the device
+  int measure_every = 0;
+  double *d_meas = nullptr;
+  int meas_cap = 0;
 
   // host mirrors
   uint32_t h_sweep = 0, h_pt_round = 0;
@@ -141,6 +151,8 @@ int launch_overrelax_phase(pyro_ctx *ctx, int colour);
 int launch_observables(pyro_ctx *ctx);           // fills d_obs
 int launch_pt_swap(pyro_ctx *ctx, int parity, uint32_t round, int from_ctl);
 int launch_end_sweep(pyro_ctx *ctx);             // record row, advance sweep counter
+int launch_measure_row(pyro_ctx *ctx);           // append a measurement row from d_obs
+int launch_anneal_beta(pyro_ctx *ctx, double T0, int m, int steps, int restore);
 int launch_set_anchor(pyro_ctx *ctx);            // d_anchor = d_obs energy, d_dE = 0
 int launch_set_sweep(pyro_ctx *ctx, uint32_t sweep, uint32_t round);
 

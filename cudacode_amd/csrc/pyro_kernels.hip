@@ -354,6 +354,47 @@ __global__ __launch_bounds__(256) void k_end_sweep(int R, uint32_t *__restrict__
   }
 }
 
+// Measurement row (the per-measurement quantities of Pyrochlore.cu:1641-1672):
+// for each rung k (descending T) the full energy and the moduli of the 5
+// order-parameter groups of the replica at that rung, appended at row
+// ctl[kCtlMeasRow] while it is below ctl[kCtlMeasCap]. dL gives per-replica
+// sizes in a mixed batch (else L). grid = 1, block = 256.
+__global__ __launch_bounds__(256) void k_measure_row(int R, const double *__restrict__ obs,
+                                                     const int *__restrict__ replica_at,
+                                                     const int *__restrict__ dL, int L,
+                                                     uint32_t *__restrict__ ctl,
+                                                     double *__restrict__ rows) {
+  const uint32_t row = ctl[kCtlMeasRow];
+  if (row >= ctl[kCtlMeasCap]) return;  // uniform over the block
+  for (int k = threadIdx.x; k < R; k += blockDim.x) {
+    const int rep = replica_at[k];
+    const double Lr = dL ? (double)dL[rep] : (double)L;
+    const double *o = obs + (size_t)rep * kObs;
+    double m[12];
+    fold_order_params(o + 1, 16.0 * Lr * Lr * Lr, m);
+    double *out = rows + ((size_t)row * R + k) * kMeasQ;
+    out[0] = o[0];
+    out[1] = fabs(m[0]);
+    out[2] = sqrt(m[1] * m[1] + m[2] * m[2]);
+    out[3] = sqrt(m[3] * m[3] + m[4] * m[4] + m[5] * m[5]);
+    out[4] = sqrt(m[6] * m[6] + m[7] * m[7] + m[8] * m[8]);
+    out[5] = sqrt(m[9] * m[9] + m[10] * m[10] + m[11] * m[11]);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) ctl[kCtlMeasRow] = row + 1u;
+}
+
+// Annealing step m of `steps` (SimulatedAnnealingThread, MonteCarlo.cu:88-92):
+// replica r runs at T0 - (T0 - T_r) * m * dT with dT = 1 / steps, evaluated in
+// that order in FP64; m = steps restores beta = 1 / T_r exactly.
+__global__ void k_anneal_beta(int R, const double *__restrict__ temps, double T0, int m, double dT,
+                              int restore, float *__restrict__ beta) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const double T = restore ? temps[r] : T0 - (T0 - temps[r]) * m * dT;
+  beta[r] = (float)(1.0 / T);
+}
+
 __global__ void k_set_anchor(int R, const double *__restrict__ obs, double *__restrict__ anchor,
                              double *__restrict__ dE) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -508,6 +549,22 @@ int launch_pt_swap(pyro_ctx *ctx, int parity, uint32_t round, int from_ctl) {
 int launch_end_sweep(pyro_ctx *ctx) {
   k_end_sweep<<<1, 256, 0, ctx->stream>>>(ctx->R, ctx->d_ctl, ctx->d_anchor, ctx->d_dE,
                                           ctx->d_replica_at, ctx->d_series);
+  PYRO_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_measure_row(pyro_ctx *ctx) {
+  k_measure_row<<<1, 256, 0, ctx->stream>>>(ctx->R, ctx->d_obs, ctx->d_replica_at,
+                                            ctx->hetero ? ctx->d_L : nullptr, ctx->L, ctx->d_ctl,
+                                            ctx->d_meas);
+  PYRO_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_anneal_beta(pyro_ctx *ctx, double T0, int m, int steps, int restore) {
+  k_anneal_beta<<<(ctx->R + 255) / 256, 256, 0, ctx->stream>>>(ctx->R, ctx->d_temps, T0, m,
+                                                               1.0 / (double)steps, restore,
+                                                               ctx->d_beta);
   PYRO_HIP(hipGetLastError());
   return 0;
 }

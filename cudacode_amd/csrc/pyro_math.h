@@ -274,4 +274,30 @@ PYRO_HD f3 overrelax(f3 h, f3 s) {
   return normalise(v);
 }
 
+// The 12 order-parameter components of Measurement (Pyrochlore.cu:580-624)
+// from the sublattice sums s[3 b + c] (b = basis site, c = component),
+// divided by n_spins: m[0] A2 (all-in-all-out), m[1..2] E, m[3..5] T1,
+// m[6..8] ferromagnetic (the magnetisation per spin), m[9..11] T2. With this
+// normalisation the ideal all-in-all-out state has |m_A2| = 1.
+PYRO_HD void fold_order_params(const double *s, double n_spins, double *m) {
+  const double a = 0.5773502691896258, b = 0.816496580927726, c = 0.4082482904638631,
+               d = 0.7071067811865475;
+  const double *s0 = s, *s1 = s + 3, *s2 = s + 6, *s3 = s + 9;
+  m[0] = a * (s0[0] + s0[1] + s0[2]) + a * (-s1[0] - s1[1] + s1[2]) +
+         a * (s2[0] - s2[1] - s2[2]) + a * (-s3[0] + s3[1] - s3[2]);
+  m[1] = (-b * s0[0] + c * s0[1] + c * s0[2]) + (b * s1[0] - c * s1[1] + c * s1[2]) +
+         (-b * s2[0] - c * s2[1] - c * s2[2]) + (b * s3[0] + c * s3[1] - c * s3[2]);
+  m[2] = d * (-s0[1] + s0[2] + s1[1] + s1[2] + s2[1] - s2[2] - s3[1] - s3[2]);
+  m[3] = d * (-s0[1] + s0[2] - s1[1] - s1[2] + s2[1] - s2[2] + s3[1] + s3[2]);
+  m[4] = d * (s0[0] - s0[2] + s1[0] + s1[2] - s2[0] - s2[2] - s3[0] + s3[2]);
+  m[5] = d * (-s0[0] + s0[1] + s1[0] - s1[1] + s2[0] + s2[1] - s3[0] - s3[1]);
+  m[6] = s0[0] + s1[0] + s2[0] + s3[0];
+  m[7] = s0[1] + s1[1] + s2[1] + s3[1];
+  m[8] = s0[2] + s1[2] + s2[2] + s3[2];
+  m[9] = d * (-s0[1] - s0[2] - s1[1] + s1[2] + s2[1] + s2[2] + s3[1] - s3[2]);
+  m[10] = d * (-s0[0] - s0[2] - s1[0] + s1[2] + s2[0] - s2[2] + s3[0] + s3[2]);
+  m[11] = d * (-s0[0] - s0[1] + s1[0] + s1[1] + s2[0] - s2[1] - s3[0] + s3[1]);
+  for (int q = 0; q < 12; ++q) m[q] = m[q] / n_spins;
+}
+
 }  // namespace pyro

@@ -33,7 +33,8 @@ EXPORTS = [
     "pyro_magnetization", "pyro_order_params", "pyro_sublattice_sums", "pyro_fold_order_params",
     "pyro_energy_incremental",
     "pyro_acceptance", "pyro_reset_acceptance", "pyro_adapt_cone", "pyro_record_start",
-    "pyro_record_fetch", "pyro_set_passes", "pyro_set_kernel_options", "pyro_create_batch",
+    "pyro_record_fetch", "pyro_anneal", "pyro_measure_start", "pyro_measure_stop", "pyro_measure_fetch",
+    "pyro_set_passes", "pyro_set_kernel_options", "pyro_create_batch",
     "pyro_get_layout", "pyro_set_exec_mode", "pyro_get_exec_mode", "pyro_autotune",
     "pyro_checkpoint_save", "pyro_checkpoint_load", "pyro_export_configurations",
     "pyro_import_configurations",
@@ -44,6 +45,7 @@ EXPORTS = [
 ]
 
 MLP_PARAMS = 4868
+MEAS_Q = 6  # per rung and measurement: E, |m_A2|, |m_E|, |m_T1|, |m_F|, |m_T2|
 
 
 class PyroError(RuntimeError):
@@ -123,6 +125,10 @@ def lib():
         L.pyro_adapt_cone.argtypes = [vp, C.c_double, C.c_int, C.c_int]
         L.pyro_record_start.argtypes = [vp, C.c_int]
         L.pyro_record_fetch.argtypes = [vp, vp, C.POINTER(C.c_int)]
+        L.pyro_anneal.argtypes = [vp, C.c_double, C.c_int, C.c_int]
+        L.pyro_measure_start.argtypes = [vp, C.c_int, C.c_int]
+        L.pyro_measure_stop.argtypes = [vp]
+        L.pyro_measure_fetch.argtypes = [vp, vp, C.POINTER(C.c_int)]
         L.pyro_time_phase.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]
         L.pyro_set_passes.argtypes = [vp, C.c_int, C.c_int]
@@ -368,6 +374,27 @@ class Pyrochlore:
         out = np.empty(max(n.value, 1) * self.R, np.float64)
         _check(lib().pyro_record_fetch(self._ctx, _p(out), C.byref(n)))
         return out[: n.value * self.R].reshape(n.value, self.R)
+
+    def SimulatedAnnealing(self, steps: int, sweeps_per_step: int,
+                           T_start: float = 20.0) -> None:
+        """Linear schedule from T_start toward each replica's temperature
+        (MonteCarlo.cu:68-104,295-344), no exchanges meanwhile."""
+        _check(lib().pyro_anneal(self._ctx, float(T_start), int(steps), int(sweeps_per_step)))
+
+    def measure_start(self, every: int, capacity: int) -> None:
+        """Append a measurement row per rung every `every` sweeps, on the device."""
+        _check(lib().pyro_measure_start(self._ctx, int(every), int(capacity)))
+
+    def measure_stop(self) -> None:
+        _check(lib().pyro_measure_stop(self._ctx))
+
+    def measure_fetch(self) -> np.ndarray:
+        """Rows so far, shape (n, R, MEAS_Q), rungs by descending T."""
+        n = C.c_int()
+        _check(lib().pyro_measure_fetch(self._ctx, None, C.byref(n)))
+        out = np.empty(max(n.value, 1) * self.R * MEAS_Q, np.float64)
+        _check(lib().pyro_measure_fetch(self._ctx, _p(out), C.byref(n)))
+        return out[: n.value * self.R * MEAS_Q].reshape(n.value, self.R, MEAS_Q)
 
     def set_kernel_options(self, phase_variant: int = 1, track_or_energy: bool = False) -> None:
         _check(lib().pyro_set_kernel_options(self._ctx, phase_variant, 1 if track_or_energy else 0))

@@ -188,6 +188,33 @@ int pyro_adapt_cone(pyro_ctx *ctx, double target, int rounds, int sweeps_per_rou
 int pyro_record_start(pyro_ctx *ctx, int capacity);
 int pyro_record_fetch(pyro_ctx *ctx, double *rows, int *n_rows);
 
+/* Simulated annealing (SimulatedAnnealing / SimulatedAnnealingThread,
+ * MonteCarlo.cu:68-104,295-344): `steps` steps; at step m = 0..steps-1 every
+ * replica runs `sweeps_per_step` sweeps at T0 - (T0 - T_r) m / steps, where
+ * T_r is its current temperature (the reference starts at T0 = 20 and, like
+ * here, stops one step above the targets). No exchanges or measurements
+ * during the schedule; afterwards beta = 1/T_r again, the cadences are back
+ * and the incremental energy is re-anchored. */
+int pyro_anneal(pyro_ctx *ctx, double T_start, int steps, int sweeps_per_step);
+
+/* Measurement series (the per-measurement accumulation of the reference's
+ * measurement loop, Pyrochlore.cu:1604-1672, and DoMeasurement :1081-1092):
+ * after every `every` sweeps (counted from sweep 0; the first row comes after
+ * the first sweep whose index + 1 is a multiple of `every`) the device
+ * computes the full FP64 energy and the sublattice sums of every replica and
+ * appends one row of PYRO_MEAS_Q doubles per rung, in rung order (descending
+ * T), before that sweep's exchange:
+ *   E, |m_A2|, |m_E|, |m_T1|, |m_F| (= |M| / N), |m_T2|
+ * (order parameters per spin, pyro_fold_order_params). No host round trip:
+ * the rows are written inside the captured sweeps. capacity rows are kept;
+ * measuring stops when full. rows: n_rows x R x PYRO_MEAS_Q (fetch with
+ * rows = NULL to get n_rows). The energies also re-anchor the incremental
+ * energy. pyro_measure_stop turns the cadence off and keeps the rows. */
+#define PYRO_MEAS_Q 6
+int pyro_measure_start(pyro_ctx *ctx, int every, int capacity);
+int pyro_measure_stop(pyro_ctx *ctx);
+int pyro_measure_fetch(pyro_ctx *ctx, double *rows, int *n_rows);
+
 /* Change the per-sweep schedule (Metropolis and over-relaxation passes; the
  * reference's --McSteps/--OverRelaxation options, Pyrochlore.cu:1161-1216). */
 int pyro_set_passes(pyro_ctx *ctx, int n_metro, int n_or);
