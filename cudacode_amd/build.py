@@ -20,6 +20,7 @@ SOURCES = [
     "pyro_learned.hip",
     "pyro_batch.hip",
     "pyro_io.hip",
+    "pyro_aniso.hip",
     "pyro_capi.hip",
 ]
 

@@ -43,6 +43,17 @@ static int reanchor(pyro_ctx *ctx) {
   return 0;
 }
 
+// After a change of the energy model: the captured sweeps hold the old
+// kernels, and the incremental energy restarts from a full recomputation.
+int reset_model_state(pyro_ctx *ctx) {
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  drop_graph(ctx);
+  PYRO_TRY(reanchor(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
 // One sweep as kernel launches (also the body that gets captured).
 static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
   if (ctx->exec_mode != 0) {
@@ -188,6 +199,7 @@ static int create_ctx(const std::vector<int> &Ls, const double *temps, uint64_t 
   ctx->device = device;
   ctx->seed = seed;
   ctx->jeff = (float)(J * spin_len2);
+  ctx->spin_len2 = spin_len2;
   ctx->n_metro = n_metro;
   ctx->n_or = n_or;
   ctx->pt_every = pt_every;
@@ -275,7 +287,7 @@ int pyro_destroy(pyro_ctx *ctx) {
   batch_free(ctx);
   void *ptrs[] = {ctx->d_spins, ctx->d_temps, ctx->d_beta,     ctx->d_cosc, ctx->d_replica_at,
                   ctx->d_acc,   ctx->d_dE,    ctx->d_anchor,   ctx->d_partials, ctx->d_obs,
-                  ctx->d_ctl,   ctx->d_series, ctx->d_meas};
+                  ctx->d_ctl,   ctx->d_series, ctx->d_meas, ctx->d_coup};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -586,6 +598,8 @@ int pyro_set_learned_passes(pyro_ctx *ctx, int n_learned) {
     return set_error(PYRO_ESTATE, "pyro_set_learned_passes: set or train the network first");
   if (n_learned > 0 && ctx->hetero)
     return set_error(PYRO_ESTATE, "pyro_set_learned_passes: needs a uniform lattice size");
+  if (n_learned > 0 && ctx->mat)
+    return set_error(PYRO_ESTATE, "pyro_set_learned_passes: the learned proposal is for the scalar model");
   PYRO_TRY(device_guard(ctx));
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   drop_graph(ctx);  // the captured sweep has the old schedule
@@ -611,6 +625,8 @@ int pyro_set_exec_mode(pyro_ctx *ctx, int mode, int block, int n_bins) {
   if (mode < 0 || mode > 2) return set_error(PYRO_EINVAL, "exec mode %d not 0/1/2", mode);
   if (mode == 0 && ctx->hetero)
     return set_error(PYRO_EINVAL, "exec mode 0 needs a uniform lattice size");
+  if (mode != 0 && ctx->mat)
+    return set_error(PYRO_ESTATE, "exec mode %d: exchange matrices run in execution mode 0 only", mode);
   if (mode != 0 && !batch_size_supported(ctx->hetero ? 4 : ctx->L))
     return set_error(PYRO_EINVAL, "exec mode %d needs L in {4,8,12,16,24,32}", mode);
   if (mode != 0 && ctx->n_learned > 0)
@@ -653,6 +669,7 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
   if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
   if (sweeps < 1 || sweeps > 1000) return set_error(PYRO_EINVAL, "pyro_autotune: sweeps %d", sweeps);
   if (ctx->n_learned > 0) return set_error(PYRO_ESTATE, "pyro_autotune: learned passes are per-phase only");
+  if (ctx->mat) return set_error(PYRO_ESTATE, "pyro_autotune: exchange matrices run in execution mode 0 only");
   PYRO_TRY(device_guard(ctx));
   struct Cand { int mode, block, variant; };
   std::vector<Cand> cands;

@@ -16,6 +16,75 @@ namespace pyro {
 
 constexpr Link kLinks[16][6] = PYRO_LINKS16_INIT;
 
+// Local-field models. Scalar (the Heisenberg J of pyro_create): a site's
+// field is -jeff times the plain FP32 neighbour sum f. Matrix (anisotropic
+// exchange and Zeeman term, pyro_set_couplings; the reference's
+// CalculateLocalExchangeField, Pyrochlore.cu:210-275): f = B_b +
+// sum_n C_{b, b_n} s_n is the field itself. coup holds kCoupFloats floats:
+// C as 9 row-major floats per ordered basis pair (b, b_n) at 9 (4 b + b_n),
+// scaled by spin_len2, then B as 3 floats per basis site at 144 + 3 b, scaled
+// by 2 spin_len2. The FP64 sums g feed the energy bookkeeping: g = sum_n s_n
+// (scalar) or sum_n C s_n (matrix, without B).
+template <bool MAT>
+__device__ __forceinline__ void field_start(const float *__restrict__ coup, int b, float &fx,
+                                            float &fy, float &fz) {
+  if constexpr (MAT) {
+    fx = coup[144 + 3 * b];
+    fy = coup[145 + 3 * b];
+    fz = coup[146 + 3 * b];
+  } else {
+    fx = 0.0f;
+    fy = 0.0f;
+    fz = 0.0f;
+  }
+}
+
+template <bool MAT, bool G>
+__device__ __forceinline__ void field_add(const float *__restrict__ coup, int b, int bn, float a,
+                                          float bb, float c, float &fx, float &fy, float &fz,
+                                          double &gx, double &gy, double &gz) {
+  if constexpr (MAT) {
+    const float *m = coup + 9 * (4 * b + bn);
+    fx = fx + ((m[0] * a + m[1] * bb) + m[2] * c);
+    fy = fy + ((m[3] * a + m[4] * bb) + m[5] * c);
+    fz = fz + ((m[6] * a + m[7] * bb) + m[8] * c);
+    if (G) {
+      gx += ((double)m[0] * a + (double)m[1] * bb) + (double)m[2] * c;
+      gy += ((double)m[3] * a + (double)m[4] * bb) + (double)m[5] * c;
+      gz += ((double)m[6] * a + (double)m[7] * bb) + (double)m[8] * c;
+    }
+  } else {
+    fx = fx + a;
+    fy = fy + bb;
+    fz = fz + c;
+    if (G) {
+      gx += (double)a;
+      gy += (double)bb;
+      gz += (double)c;
+    }
+  }
+}
+
+// The field of the Metropolis test.
+template <bool MAT>
+__device__ __forceinline__ f3 field_h(float jeff, float fx, float fy, float fz) {
+  if constexpr (MAT) return {fx, fy, fz};
+  else return {-jeff * fx, -jeff * fy, -jeff * fz};
+}
+
+// FP64 energy change of the move s -> sp: -(sp - s).h with h in FP64.
+template <bool MAT>
+__device__ __forceinline__ double field_de(float jeff, const float *__restrict__ coup, int b, f3 s,
+                                           f3 sp, double gx, double gy, double gz) {
+  if constexpr (MAT)
+    return -(((double)sp.x - (double)s.x) * (gx + (double)coup[144 + 3 * b]) +
+             ((double)sp.y - (double)s.y) * (gy + (double)coup[145 + 3 * b]) +
+             ((double)sp.z - (double)s.z) * (gz + (double)coup[146 + 3 * b]));
+  else
+    return (double)jeff * (((double)sp.x - (double)s.x) * gx + ((double)sp.y - (double)s.y) * gy +
+                           ((double)sp.z - (double)s.z) * gz);
+}
+
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -28,11 +97,12 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 // Sx: this replica's spins ([x,y,z][site][cell]); r: the replica's global
 // index (keys the Philox counters). Adds the energy change to de and the
 // accepted proposals to nacc.
-template <int KIND, int COLOUR, int LT, bool TRACK>
+template <int KIND, int COLOUR, int LT, bool TRACK, bool MAT = false>
 __device__ __forceinline__ void cell_update(float *__restrict__ Sx, int cell, int r, int pass,
                                             float beta_r, float cosc_r, uint32_t sweep,
                                             uint32_t key0, uint32_t key1, float jeff, double &de,
-                                            unsigned &nacc) {
+                                            unsigned &nacc,
+                                            const float *__restrict__ coup = nullptr) {
   constexpr int L = LT, LL = LT * LT, NC = LT * LT * LT;
   float *Sy = Sx + 16 * NC;
   float *Sz = Sy + 16 * NC;
@@ -49,23 +119,15 @@ __device__ __forceinline__ void cell_update(float *__restrict__ Sx, int cell, in
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int site = COLOUR + 4 * k;
-    fx[k] = 0.0f;
-    fy[k] = 0.0f;
-    fz[k] = 0.0f;
+    field_start<MAT>(coup, COLOUR, fx[k], fy[k], fz[k]);
     gx[k] = gy[k] = gz[k] = 0.0;
 #pragma unroll
     for (int n = 0; n < 6; ++n) {
       const Link lk = kLinks[site][n];
       const int j = lk.site * NC + Z[lk.dz + 1] + Y[lk.dy + 1] + X[lk.dx + 1];
       const float a = Sx[j], b = Sy[j], c = Sz[j];
-      fx[k] = fx[k] + a;
-      fy[k] = fy[k] + b;
-      fz[k] = fz[k] + c;
-      if (KIND == 0 || TRACK) {
-        gx[k] += (double)a;
-        gy[k] += (double)b;
-        gz[k] += (double)c;
-      }
+      field_add<MAT, KIND == 0 || TRACK>(coup, COLOUR, lk.site & 3, a, b, c, fx[k], fy[k], fz[k],
+                                         gx[k], gy[k], gz[k]);
     }
     const int i = site * NC + cell;
     s[k] = {Sx[i], Sy[i], Sz[i]};
@@ -78,7 +140,7 @@ __device__ __forceinline__ void cell_update(float *__restrict__ Sx, int cell, in
           rng_counter((uint32_t)cell, (uint32_t)site, (uint32_t)pass, kKindMetropolis, (uint32_t)r,
                       sweep),
           key0, key1);
-      const f3 h = {-jeff * fx[k], -jeff * fy[k], -jeff * fz[k]};
+      const f3 h = field_h<MAT>(jeff, fx[k], fy[k], fz[k]);
       sp[k] = cone_proposal(s[k], cosc_r, u01(w.x), u01(w.y));
       wr[k] = metropolis_accept(h, s[k], sp[k], beta_r, u01(w.z));
       nacc += wr[k] ? 1u : 0u;
@@ -88,9 +150,7 @@ __device__ __forceinline__ void cell_update(float *__restrict__ Sx, int cell, in
       wr[k] = true;
     }
     if ((KIND == 0 || TRACK) && wr[k])
-      de += (double)jeff * (((double)sp[k].x - (double)s[k].x) * gx[k] +
-                            ((double)sp[k].y - (double)s[k].y) * gy[k] +
-                            ((double)sp[k].z - (double)s[k].z) * gz[k]);
+      de += field_de<MAT>(jeff, coup, COLOUR, s[k], sp[k], gx[k], gy[k], gz[k]);
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -107,18 +167,21 @@ __device__ __forceinline__ void cell_update(float *__restrict__ Sx, int cell, in
 // afford the registers of cell_update (its 4 concurrent targets): the same
 // float expressions in the same order, so the same results bit for bit.
 // links: the link table in constant memory of the calling translation unit.
-template <int KIND, int LT, bool TRACK>
+template <int KIND, int LT, bool TRACK, bool MAT = false>
 __device__ __forceinline__ void site_update(float *__restrict__ Sx, const Link (*links)[6], int cell,
                                             int site, int r, int pass, float beta_r, float cosc_r,
                                             uint32_t sweep, uint32_t key0, uint32_t key1,
-                                            float jeff, double &de, unsigned &nacc) {
+                                            float jeff, double &de, unsigned &nacc,
+                                            const float *__restrict__ coup = nullptr) {
   constexpr int L = LT, LL = LT * LT, NC = LT * LT * LT;
   float *Sy = Sx + 16 * NC;
   float *Sz = Sy + 16 * NC;
   const int z = cell / LL;
   const int y = (cell / L) % L;
   const int x = cell % L;
-  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  const int bs = site & 3;
+  float fx, fy, fz;
+  field_start<MAT>(coup, bs, fx, fy, fz);
   double gx = 0.0, gy = 0.0, gz = 0.0;
 #pragma unroll
   for (int n = 0; n < 6; ++n) {
@@ -128,14 +191,7 @@ __device__ __forceinline__ void site_update(float *__restrict__ Sx, const Link (
                   (y1 < 0 ? y1 + L : (y1 >= L ? y1 - L : y1)) * L +
                   (x1 < 0 ? x1 + L : (x1 >= L ? x1 - L : x1));
     const float a = Sx[j], b = Sy[j], c = Sz[j];
-    fx = fx + a;
-    fy = fy + b;
-    fz = fz + c;
-    if (KIND == 0 || TRACK) {
-      gx += (double)a;
-      gy += (double)b;
-      gz += (double)c;
-    }
+    field_add<MAT, KIND == 0 || TRACK>(coup, bs, lk.site & 3, a, b, c, fx, fy, fz, gx, gy, gz);
   }
   const int i = site * NC + cell;
   const f3 s = {Sx[i], Sy[i], Sz[i]};
@@ -146,7 +202,7 @@ __device__ __forceinline__ void site_update(float *__restrict__ Sx, const Link (
         rng_counter((uint32_t)cell, (uint32_t)site, (uint32_t)pass, kKindMetropolis, (uint32_t)r,
                     sweep),
         key0, key1);
-    const f3 h = {-jeff * fx, -jeff * fy, -jeff * fz};
+    const f3 h = field_h<MAT>(jeff, fx, fy, fz);
     sp = cone_proposal(s, cosc_r, u01(w.x), u01(w.y));
     wr = metropolis_accept(h, s, sp, beta_r, u01(w.z));
     nacc += wr ? 1u : 0u;
@@ -156,9 +212,7 @@ __device__ __forceinline__ void site_update(float *__restrict__ Sx, const Link (
     wr = true;
   }
   if (wr) {
-    if (KIND == 0 || TRACK)
-      de += (double)jeff * (((double)sp.x - (double)s.x) * gx + ((double)sp.y - (double)s.y) * gy +
-                            ((double)sp.z - (double)s.z) * gz);
+    if (KIND == 0 || TRACK) de += field_de<MAT>(jeff, coup, bs, s, sp, gx, gy, gz);
     Sx[i] = sp.x;
     Sy[i] = sp.y;
     Sz[i] = sp.z;

@@ -29,6 +29,7 @@ enum CtlWord : int {
 
 constexpr int kObs = 13;  // energy + 4 sublattices x 3 components
 constexpr int kObsPartials = 16;
+constexpr int kCoupFloats = 156;  // matrix field model: 16 x 9 exchange + 4 x 3 Zeeman
 // Measurement row, per rung: E, |m_A2|, |m_E|, |m_T1|, |m_F| (= |M| / N), |m_T2|.
 constexpr int kMeasQ = 6;  // per observables block: 4 bases x (energy, mx, my, mz)
 constexpr int kPhaseBlock = 256;   // 4 waves: one per site of the colour
@@ -124,6 +125,14 @@ struct pyro_ctx {
   unsigned long long *d_acc_learned = nullptr;  // R, accepted learned proposals
   std::vector<uint64_t> h_proposed_learned;
 
+  // local-field model: scalar jeff, or (mat) exchange matrices and Zeeman
+  // fields per basis site, kCoupFloats floats scaled by spin_len2 (d_coup;
+  // h_coup the same values on the host)
+  double spin_len2 = 1.0;
+  bool mat = false;
+  float *d_coup = nullptr;
+  float h_coup[pyro::kCoupFloats] = {};
+
   // measurement series: every measure_every sweeps one row of kMeasQ values
   // per rung (descending T) is appended on the device
   int measure_every = 0;
@@ -152,6 +161,9 @@ int launch_observables(pyro_ctx *ctx);           // fills d_obs
 int launch_pt_swap(pyro_ctx *ctx, int parity, uint32_t round, int from_ctl);
 int launch_end_sweep(pyro_ctx *ctx);             // record row, advance sweep counter
 int launch_measure_row(pyro_ctx *ctx);           // append a measurement row from d_obs
+int launch_phase_mat(pyro_ctx *ctx, int kind, int colour, int pass);  // matrix field model
+int reset_model_state(pyro_ctx *ctx);  // drop captured sweeps, re-anchor the energy
+int launch_observables_partials_mat(pyro_ctx *ctx);
 int launch_anneal_beta(pyro_ctx *ctx, double T0, int m, int steps, int restore);
 int launch_set_anchor(pyro_ctx *ctx);            // d_anchor = d_obs energy, d_dE = 0
 int launch_set_sweep(pyro_ctx *ctx, uint32_t sweep, uint32_t round);

@@ -713,6 +713,7 @@ static void launch_learned_t(pyro_ctx *ctx, int colour, int pass) {
 int launch_learned_phase(pyro_ctx *ctx, int colour, int pass) {
   if (!ctx->mlp_ready) return set_error(PYRO_ESTATE, "learned proposal: no weights set");
   if (ctx->hetero) return set_error(PYRO_ESTATE, "learned proposal: needs a uniform lattice size");
+  if (ctx->mat) return set_error(PYRO_ESTATE, "learned proposal: it is for the scalar model");
   switch (ctx->L) {
     case 8: launch_learned_t<8>(ctx, colour, pass); break;
     case 16: launch_learned_t<16>(ctx, colour, pass); break;
@@ -861,6 +862,7 @@ int pyro_mlp_loss_grad(pyro_ctx *ctx, const float *env6, int n, double *loss, fl
 int pyro_gather_environments(pyro_ctx *ctx, int per_replica, uint32_t draw, int capacity, int *n_total) {
   if (!ctx || per_replica < 1 || capacity < 1) return set_error(PYRO_EINVAL, "pyro_gather_environments: bad argument");
   if (ctx->hetero) return set_error(PYRO_ESTATE, "pyro_gather_environments: needs a uniform lattice size");
+  if (ctx->mat) return set_error(PYRO_ESTATE, "pyro_gather_environments: the learned proposal is for the scalar model");
   PYRO_HIP(hipSetDevice(ctx->device));
   if (capacity > ctx->env_cap) {
     PYRO_HIP(hipStreamSynchronize(ctx->stream));

@@ -33,7 +33,8 @@ EXPORTS = [
     "pyro_magnetization", "pyro_order_params", "pyro_sublattice_sums", "pyro_fold_order_params",
     "pyro_energy_incremental",
     "pyro_acceptance", "pyro_reset_acceptance", "pyro_adapt_cone", "pyro_record_start",
-    "pyro_record_fetch", "pyro_anneal", "pyro_measure_start", "pyro_measure_stop", "pyro_measure_fetch",
+    "pyro_record_fetch", "pyro_set_couplings", "pyro_get_couplings", "pyro_exchange_matrices",
+    "pyro_zeeman_fields", "pyro_anneal", "pyro_measure_start", "pyro_measure_stop", "pyro_measure_fetch",
     "pyro_set_passes", "pyro_set_kernel_options", "pyro_create_batch",
     "pyro_get_layout", "pyro_set_exec_mode", "pyro_get_exec_mode", "pyro_autotune",
     "pyro_checkpoint_save", "pyro_checkpoint_load", "pyro_export_configurations",
@@ -126,6 +127,10 @@ def lib():
         L.pyro_record_start.argtypes = [vp, C.c_int]
         L.pyro_record_fetch.argtypes = [vp, vp, C.POINTER(C.c_int)]
         L.pyro_anneal.argtypes = [vp, C.c_double, C.c_int, C.c_int]
+        L.pyro_set_couplings.argtypes = [vp, vp, vp]
+        L.pyro_get_couplings.argtypes = [vp, C.POINTER(C.c_int), vp]
+        L.pyro_exchange_matrices.argtypes = [C.c_double] * 4 + [vp]
+        L.pyro_zeeman_fields.argtypes = [vp, C.c_double, C.c_double, vp]
         L.pyro_measure_start.argtypes = [vp, C.c_int, C.c_int]
         L.pyro_measure_stop.argtypes = [vp]
         L.pyro_measure_fetch.argtypes = [vp, vp, C.POINTER(C.c_int)]
@@ -197,6 +202,25 @@ def fold_order_params(sub12, n_spins: float) -> np.ndarray:
     out = np.empty_like(s)
     _check(lib().pyro_fold_order_params(_p(s), s.shape[0], float(n_spins), _p(out)))
     return out
+
+
+# Exchange constants of the reference's defaults (Pyrochlore.cu:139-142).
+YB2TI2O7_J = (-0.09 / 0.29, -0.22 / 0.29, -1.0, 0.0)
+
+
+def exchange_matrices(J1: float, J2: float, J3: float, J4: float) -> np.ndarray:
+    """Exchange matrices C[b, b'] (4, 4, 3, 3) of the J1..J4 parameterisation."""
+    out = np.zeros(144, np.float64)
+    _check(lib().pyro_exchange_matrices(float(J1), float(J2), float(J3), float(J4), _p(out)))
+    return out.reshape(4, 4, 3, 3)
+
+
+def zeeman_fields(B, gxy: float = 1.0, gz: float = 1.0) -> np.ndarray:
+    """Zeeman field per basis site (4, 3) in meV for a field B (3,) in tesla."""
+    b = np.ascontiguousarray(B, np.float64).reshape(3)
+    out = np.zeros(12, np.float64)
+    _check(lib().pyro_zeeman_fields(_p(b), float(gxy), float(gz), _p(out)))
+    return out.reshape(4, 3)
 
 
 def mlp_init(seed: int = 0) -> np.ndarray:
@@ -374,6 +398,23 @@ class Pyrochlore:
         out = np.empty(max(n.value, 1) * self.R, np.float64)
         _check(lib().pyro_record_fetch(self._ctx, _p(out), C.byref(n)))
         return out[: n.value * self.R].reshape(n.value, self.R)
+
+    def set_couplings(self, C=None, B_site=None) -> None:
+        """Matrix field model: C (4, 4, 3, 3) exchange matrices, B_site (4, 3)
+        Zeeman fields (or None); C=None returns to the scalar J model."""
+        if C is None:
+            _check(lib().pyro_set_couplings(self._ctx, None, None))
+            return
+        c = np.ascontiguousarray(C, np.float64).reshape(144)
+        b = None if B_site is None else np.ascontiguousarray(B_site, np.float64).reshape(12)
+        _check(lib().pyro_set_couplings(self._ctx, _p(c), None if b is None else _p(b)))
+
+    def couplings(self):
+        """(is_matrix, the 156 FP32 values the kernels use)."""
+        m = C.c_int()
+        out = np.zeros(156, np.float32)
+        _check(lib().pyro_get_couplings(self._ctx, C.byref(m), _p(out)))
+        return bool(m.value), out
 
     def SimulatedAnnealing(self, steps: int, sweeps_per_step: int,
                            T_start: float = 20.0) -> None:

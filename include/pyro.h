@@ -188,6 +188,32 @@ int pyro_adapt_cone(pyro_ctx *ctx, double target, int rounds, int sweeps_per_rou
 int pyro_record_start(pyro_ctx *ctx, int capacity);
 int pyro_record_fetch(pyro_ctx *ctx, double *rows, int *n_rows);
 
+/* Anisotropic exchange and Zeeman field: the matrix local-field model
+ * (InitializeCouplings3D, Pyrochlore.cu:643-711; couplings_const :169-184;
+ * the field of CalculateLocalExchangeField :210-275; the energy of the CPU
+ * TotalEnergy :468-523). C144[9 (4 b + b') + 3 i + j] is the 3x3 exchange
+ * matrix C_{b b'} between basis sites b != b' (C_{b' b} must be C_{b b'}^T;
+ * C_{b b} is unused); B12[3 b + i] the Zeeman field on basis site b (NULL:
+ * none). Energy H = -spin_len2 (sum_<ij> s_i^T C s_j + 2 sum_i B_b . s_i) and
+ * local field h_i = spin_len2 (sum_j C s_j + 2 B_b), so C = -J I, B = 0 is
+ * the scalar model of pyro_create. The colour phases and observables use it
+ * from this call on (FP32 field, FP64 energies); the incremental energy is
+ * re-anchored. C144 = NULL returns to the scalar model. Uniform lattices in
+ * execution mode 0 only; not with the learned proposal (PYRO_ESTATE). */
+int pyro_set_couplings(pyro_ctx *ctx, const double *C144, const double *B12);
+/* is_matrix: 1 when the matrix model is on; coup156: the FP32 values the
+ * kernels use (spin_len2 C, then 2 spin_len2 B). Either may be NULL. */
+int pyro_get_couplings(pyro_ctx *ctx, int *is_matrix, float *coup156);
+/* The exchange matrices of the J1..J4 parameterisation (couplings_const,
+ * Pyrochlore.cu:169-184; the reference's defaults J1 = -0.09/0.29,
+ * J2 = -0.22/0.29, J3 = -1, J4 = 0 at :139-142). J1 = J2 = 1, J3 = J4 = 0
+ * gives C = -I, the Heisenberg antiferromagnet. */
+int pyro_exchange_matrices(double J1, double J2, double J3, double J4, double *C144);
+/* Zeeman fields per basis site for a field B3 in tesla and a g-tensor with
+ * components g_xy, g_z about each site's local [111] axis, in meV
+ * (InitializeMagneticField, Pyrochlore.cu:414-450). */
+int pyro_zeeman_fields(const double *B3, double gxy, double gz, double *B12);
+
 /* Simulated annealing (SimulatedAnnealing / SimulatedAnnealingThread,
  * MonteCarlo.cu:68-104,295-344): `steps` steps; at step m = 0..steps-1 every
  * replica runs `sweeps_per_step` sweeps at T0 - (T0 - T_r) m / steps, where

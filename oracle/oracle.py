@@ -59,6 +59,7 @@ def lib():
                               C.c_int, C.POINTER(C.c_uint32), _f64p, _f32p, _f32p, _i32p,
                               C.c_float, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
                               C.POINTER(C.c_int)]
+        L.orc_set_couplings.argtypes = [C.c_void_p]
         L.orc_num_threads.restype = C.c_int
         L.orc_set_num_threads.argtypes = [C.c_int]
         L.orc_set_replica_base.argtypes = [C.c_int]
@@ -183,6 +184,22 @@ class OracleRun:
 
     def observables(self):
         return observables(self.spins, self.L, self.R, self.jeff)
+
+
+_coup_keep = None
+
+
+def set_couplings(coup156) -> None:
+    """Matrix field model (156 float32: spin_len2 C, then 2 spin_len2 B) for
+    the following oracle calls, or None for the scalar model."""
+    global _coup_keep
+    if coup156 is None:
+        _coup_keep = None
+        lib().orc_set_couplings(None)
+        return
+    _coup_keep = np.ascontiguousarray(coup156, np.float32).copy()
+    assert _coup_keep.size == 156
+    lib().orc_set_couplings(_coup_keep.ctypes.data_as(C.c_void_p))
 
 
 def set_replica_base(base: int) -> None:

@@ -227,24 +227,64 @@ static long nbr_index(int L, int x, int y, int z, const int8_t *lk) {
   return ((long)(z1 * L + y1) * L + x1) + (long)lk[3] * nc;
 }
 
-/* Neighbour sum of site `site` at (x,y,z): FP32 in link order (the field the
- * update uses) and FP64 (for the energy bookkeeping). Pyrochlore.cu:210-275
- * with C = -I (Heisenberg AFM, J1=J2=1, J3=J4=0), so h = -jeff * sum. */
+/* Local-field model. NULL: the scalar Heisenberg model, C = -I (J1=J2=1,
+ * J3=J4=0), h = -jeff * (neighbour sum). Otherwise the 156 floats of the
+ * matrix model (CalculateLocalExchangeField, Pyrochlore.cu:210-275, with the
+ * Zeeman term of InitializeMagneticField :414-450): coup[9 (4 b + b') + 3 i + j]
+ * = spin_len2 C_{b b'}[i][j], coup[144 + 3 b + i] = 2 spin_len2 B_b[i];
+ * h = B_b + sum_n C_{b b_n} s_n, energy -sum_i s_i.(B_b + (1/2) sum_n C s_n). */
+static const float *g_coup = NULL;
+
+void orc_set_couplings(const float *coup156) { g_coup = coup156; }
+
+/* Field of site `site` at (x,y,z): FP32 in link order (the field the update
+ * uses: the plain neighbour sum for the scalar model, B + sum C s for the
+ * matrix model) and FP64 (for the energy bookkeeping: sum s, or sum C s
+ * without B). */
 static void nbr_sum(const float *S, int L, int x, int y, int z, int site, float f[3],
                     double d[3]) {
   long nc = (long)L * L * L;
   const float *Sx = S, *Sy = S + 16 * nc, *Sz = S + 32 * nc;
-  f[0] = f[1] = f[2] = 0.0f;
+  int b = site & 3;
+  if (g_coup) {
+    f[0] = g_coup[144 + 3 * b];
+    f[1] = g_coup[145 + 3 * b];
+    f[2] = g_coup[146 + 3 * b];
+  } else {
+    f[0] = f[1] = f[2] = 0.0f;
+  }
   d[0] = d[1] = d[2] = 0.0;
   for (int k = 0; k < 6; k++) {
     long j = nbr_index(L, x, y, z, LINKS[site][k]);
-    f[0] = f[0] + Sx[j];
-    f[1] = f[1] + Sy[j];
-    f[2] = f[2] + Sz[j];
-    d[0] += (double)Sx[j];
-    d[1] += (double)Sy[j];
-    d[2] += (double)Sz[j];
+    float a = Sx[j], bb = Sy[j], c = Sz[j];
+    if (g_coup) {
+      const float *m = g_coup + 9 * (4 * b + (LINKS[site][k][3] & 3));
+      f[0] = f[0] + ((m[0] * a + m[1] * bb) + m[2] * c);
+      f[1] = f[1] + ((m[3] * a + m[4] * bb) + m[5] * c);
+      f[2] = f[2] + ((m[6] * a + m[7] * bb) + m[8] * c);
+      d[0] += ((double)m[0] * a + (double)m[1] * bb) + (double)m[2] * c;
+      d[1] += ((double)m[3] * a + (double)m[4] * bb) + (double)m[5] * c;
+      d[2] += ((double)m[6] * a + (double)m[7] * bb) + (double)m[8] * c;
+    } else {
+      f[0] = f[0] + a;
+      f[1] = f[1] + bb;
+      f[2] = f[2] + c;
+      d[0] += (double)a;
+      d[1] += (double)bb;
+      d[2] += (double)c;
+    }
   }
+}
+
+/* FP64 energy change of s -> sp at site basis b from the FP64 sums d. */
+static double move_de(float jeff, int b, const float s[3], const float sp[3], const double d[3]) {
+  if (g_coup)
+    return -(((double)sp[0] - (double)s[0]) * (d[0] + (double)g_coup[144 + 3 * b]) +
+             ((double)sp[1] - (double)s[1]) * (d[1] + (double)g_coup[145 + 3 * b]) +
+             ((double)sp[2] - (double)s[2]) * (d[2] + (double)g_coup[146 + 3 * b]));
+  return (double)jeff * (((double)sp[0] - (double)s[0]) * d[0] +
+                         ((double)sp[1] - (double)s[1]) * d[1] +
+                         ((double)sp[2] - (double)s[2]) * d[2]);
 }
 
 void orc_init_spins(float *spins, int L, int R, uint64_t seed) {
@@ -296,6 +336,11 @@ void orc_metropolis_phase(float *spins, int L, int R, int colour, uint32_t sweep
         double d[3];
         nbr_sum(S, L, x, y, z, site, f, d);
         float h[3] = {-jeff * f[0], -jeff * f[1], -jeff * f[2]};
+        if (g_coup) {
+          h[0] = f[0];
+          h[1] = f[1];
+          h[2] = f[2];
+        }
         uint32_t w[4];
         draw((uint32_t)cell, (uint32_t)site, (uint32_t)pass, KIND_METROPOLIS, (uint32_t)r, sweep,
              seed, w);
@@ -303,9 +348,7 @@ void orc_metropolis_phase(float *spins, int L, int R, int colour, uint32_t sweep
         orc_cone_proposal(s, cos_cone[r], u01(w[0]), u01(w[1]), sp);
         if (orc_metropolis_accept(h, s, sp, beta[r], u01(w[2]))) {
           acc++;
-          de += (double)jeff * (((double)sp[0] - (double)s[0]) * d[0] +
-                                ((double)sp[1] - (double)s[1]) * d[1] +
-                                ((double)sp[2] - (double)s[2]) * d[2]);
+          de += move_de(jeff, site & 3, s, sp, d);
           Sx[i] = sp[0];
           Sy[i] = sp[1];
           Sz[i] = sp[2];
@@ -336,9 +379,7 @@ void orc_overrelax_phase(float *spins, int L, int R, int colour, float jeff, dou
         nbr_sum(S, L, x, y, z, site, f, d);
         float sp[3];
         orc_overrelax(f, s, sp);
-        de += (double)jeff * (((double)sp[0] - (double)s[0]) * d[0] +
-                              ((double)sp[1] - (double)s[1]) * d[1] +
-                              ((double)sp[2] - (double)s[2]) * d[2]);
+        de += move_de(jeff, site & 3, s, sp, d);
         Sx[i] = sp[0];
         Sy[i] = sp[1];
         Sz[i] = sp[2];
@@ -367,14 +408,19 @@ void orc_observables(const float *spins, int L, int R, float jeff, double *out) 
         float f[3];
         double d[3];
         nbr_sum(S, L, x, y, z, site, f, d);
-        e += (double)Sx[i] * d[0] + (double)Sy[i] * d[1] + (double)Sz[i] * d[2];
         int b = site & 3;
+        if (g_coup)
+          e += (double)Sx[i] * ((double)g_coup[144 + 3 * b] + 0.5 * d[0]) +
+               (double)Sy[i] * ((double)g_coup[145 + 3 * b] + 0.5 * d[1]) +
+               (double)Sz[i] * ((double)g_coup[146 + 3 * b] + 0.5 * d[2]);
+        else
+          e += (double)Sx[i] * d[0] + (double)Sy[i] * d[1] + (double)Sz[i] * d[2];
         sub[3 * b + 0] += Sx[i];
         sub[3 * b + 1] += Sy[i];
         sub[3 * b + 2] += Sz[i];
       }
     }
-    out[r * 13 + 0] = 0.5 * (double)jeff * e;
+    out[r * 13 + 0] = g_coup ? -e : 0.5 * (double)jeff * e;
     for (int i = 0; i < 12; i++) out[r * 13 + 1 + i] = sub[i];
   }
 }

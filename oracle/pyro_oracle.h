@@ -62,6 +62,12 @@ void orc_observables(const float *spins, int L, int R, float jeff, double *out);
 int orc_pt_swap(int R, int parity, uint32_t round, uint64_t seed, const double *energy,
                 double *temps, float *beta, float *cos_cone, int *replica_at);
 
+/* Local-field model for every following call: NULL = scalar Heisenberg
+ * (-jeff * neighbour sum); else 156 floats, the exchange matrices (times
+ * spin_len2, 9 per ordered basis pair) and Zeeman fields (times 2 spin_len2,
+ * 3 per basis site). The array must outlive its use. */
+void orc_set_couplings(const float *coup156);
+
 /* Full sweeps: n_metro Metropolis passes + n_or over-relaxation passes, each
  * over colours 0..3; after every pt_every sweeps (if > 0) a swap with parity
  * alternating from the stored round counter. sweep0 is the first sweep index;

@@ -1,0 +1,100 @@
+"""Exchange matrices and Zeeman fields of the matrix field model (CPU, host
+arithmetic of the C ABI; no device calls).
+
+Pinned by the lattice's symmetry rather than by retyping the table: the
+basis sites sit at r_0 = (0,0,0), r_1 = (1,1,0)/4, r_2 = (0,1,1)/4,
+r_3 = (1,0,1)/4 of the reference's unit cell (Pyrochlore.cu:1339-1353), the
+vertices of a tetrahedron about (1,1,1)/8. Every operation R of its point
+group T_d permutes the vertices (R v_b = v_pi(b)) and must map the couplings
+onto themselves: C[pi(b), pi(b')] = R C[b, b'] R^T and g_pi(b) = R g_b R^T.
+A wrong sign or a misplaced entry breaks that for some R.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from cudacode_amd import pyro
+
+VERTS = np.array([[0, 0, 0], [2, 2, 0], [0, 2, 2], [2, 0, 2]], float) - 1.0  # 8 (r_b - centre)
+
+
+def td_operations():
+    """Signed permutation matrices (O_h) that map the vertex set to itself,
+    with the induced permutation of basis sites."""
+    ops = []
+    for perm in itertools.permutations(range(3)):
+        for signs in itertools.product((1, -1), repeat=3):
+            R = np.zeros((3, 3))
+            for i, p in enumerate(perm):
+                R[i, p] = signs[i]
+            img = VERTS @ R.T
+            pi = []
+            for v in img:
+                hit = np.where(np.all(np.isclose(VERTS, v), axis=1))[0]
+                if len(hit) != 1:
+                    break
+                pi.append(int(hit[0]))
+            else:
+                ops.append((R, pi))
+    return ops
+
+
+def test_td_has_24_operations():
+    ops = td_operations()
+    assert len(ops) == 24
+    assert len({tuple(p) for _, p in ops}) == 24  # all permutations of 4 vertices
+
+
+@pytest.mark.parametrize("J", [(1.0, 0.0, 0.0, 0.0), (0.0, 1.0, 0.0, 0.0), (0.0, 0.0, 1.0, 0.0),
+                               (0.0, 0.0, 0.0, 1.0), pyro.YB2TI2O7_J])
+def test_exchange_matrices_are_td_invariant(J):
+    C = pyro.exchange_matrices(*J)
+    for b in range(4):
+        assert np.all(C[b, b] == 0)
+        for bp in range(4):
+            assert np.array_equal(C[bp, b], C[b, bp].T)
+    for R, pi in td_operations():
+        for b in range(4):
+            for bp in range(4):
+                if b != bp:
+                    assert np.allclose(C[pi[b], pi[bp]], R @ C[b, bp] @ R.T, atol=1e-15), (R, b, bp)
+
+
+def test_four_independent_exchange_constants():
+    """J1..J4 span a 4-dimensional space of couplings (the number of
+    symmetry-allowed nearest-neighbour exchange constants of the pyrochlore
+    lattice)."""
+    basis = np.stack([pyro.exchange_matrices(*np.eye(4)[k]).ravel() for k in range(4)])
+    assert np.linalg.matrix_rank(basis) == 4
+
+
+def test_heisenberg_limit_is_minus_identity():
+    C = pyro.exchange_matrices(1.0, 1.0, 0.0, 0.0)
+    for b in range(4):
+        for bp in range(4):
+            assert np.array_equal(C[b, bp], np.zeros((3, 3)) if b == bp else -np.eye(3))
+
+
+def test_zeeman_fields_follow_the_local_axes():
+    mu = 0.0578838261305
+    B = np.array([0.3, -1.2, 0.7])
+    # isotropic g: the same field on every site
+    F = pyro.zeeman_fields(B, 2.0, 2.0)
+    assert np.allclose(F, np.tile(2.0 * mu * B, (4, 1)), rtol=1e-14)
+    # anisotropic g: g_b = gxy I + (gz - gxy) n n^T with n the site's local axis
+    gxy, gz = 4.18, 1.77
+    F = pyro.zeeman_fields(B, gxy, gz)
+    for b in range(4):
+        n = VERTS[b] / np.linalg.norm(VERTS[b])
+        g = gxy * np.eye(3) + (gz - gxy) * np.outer(n, n)
+        assert np.allclose(F[b], mu * g @ B, rtol=1e-13)
+    for R, pi in td_operations():
+        for b in range(4):
+            n = VERTS[b] / np.linalg.norm(VERTS[b])
+            m = VERTS[pi[b]] / np.linalg.norm(VERTS[pi[b]])
+            assert np.allclose(R @ n, m)
+    # a field along a site's own axis is scaled by g_z on that site
+    n0 = VERTS[0] / np.linalg.norm(VERTS[0])
+    F = pyro.zeeman_fields(n0, gxy, gz)
+    assert np.allclose(F[0], mu * gz * n0, rtol=1e-13)
