@@ -225,6 +225,8 @@ def workload_c1(args):
     g = pyro.Pyrochlore(4, [0.5], seed=SEED, n_metro=1, n_or=5, cos_cone=0.5)
     g.CalculateSolidAngle(0.5, 20, 10)
     g.Mc(args.warmup)
+    tune = g.autotune(5)
+    mode = g.exec_mode()
     g.record_start(10000)
     g.sync()
     t0 = time.perf_counter()
@@ -237,7 +239,9 @@ def workload_c1(args):
     g.close()
     return line(10000 * 6 * 1024 / dt, 10000, args.warmup, dt,
                 "L=4 (1,024 spins), 1 replica, T=0.5, 1 Metropolis + 5 over-relaxation, 10^4 sweeps",
-                {"energy_per_spin": float(e.mean()), "energy_err": stats.binning_error(e),
+                {"exec_mode": {"mode": mode[0], "block": mode[1], "bins": mode[2]},
+                 "autotune_ms_per_sweep": tune.tolist(),
+                 "energy_per_spin": float(e.mean()), "energy_err": stats.binning_error(e),
                  "specific_heat": c, "specific_heat_err": c_err, "tau_int_sweeps": tau,
                  "indep_samples_per_s": stats.independent_samples_per_s(10000 / dt, tau)})
 

@@ -85,11 +85,15 @@ static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
 }
 
 // Sweeps per captured graph: a whole number of exchange and measurement
-// intervals, so that every replay starts aligned with both cadences.
+// intervals, so that every replay starts aligned with both cadences, and at
+// least kMinGraphSweeps so that small lattices (a few microseconds of work per
+// sweep) do not pay a graph launch per sweep.
+constexpr uint32_t kMinGraphSweeps = 16;
 static uint32_t graph_block(const pyro_ctx *ctx) {
   const uint32_t a = ctx->pt_every > 0 ? (uint32_t)ctx->pt_every : 1u;
   const uint32_t b = ctx->measure_every > 0 ? (uint32_t)ctx->measure_every : 1u;
-  return (uint32_t)std::lcm(a, b);
+  const uint32_t l = (uint32_t)std::lcm(a, b);
+  return l * ((kMinGraphSweeps + l - 1) / l);
 }
 
 static uint64_t replica_cells(const pyro_ctx *ctx, int r) {
@@ -728,13 +732,28 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
     ctx->exec_mode = c.mode;
     ctx->batch_block = c.block ? c.block : block0;
     ctx->phase_variant = c.variant;
-    rc = enqueue_sweep(ctx, ctx->h_sweep);  // warm-up (code load, caches)
-    if (!rc && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = set_error(PYRO_EHIP, "event record");
-    for (int s = 0; s < sweeps && !rc; ++s) rc = enqueue_sweep(ctx, ctx->h_sweep + 1u + (uint32_t)s);
+    // Time the candidate as pyro_sweep runs it: `sweeps` sweeps captured in
+    // one graph, replayed once to warm up (code load, caches) and once timed.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      rc = set_error(PYRO_EHIP, "pyro_autotune: capture failed");
+      break;
+    }
+    for (int s = 0; s < sweeps && !rc; ++s) rc = enqueue_sweep(ctx, ctx->h_sweep + (uint32_t)s);
+    const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+    if (!rc && ce != hipSuccess) rc = set_error(PYRO_EHIP, "pyro_autotune: capture failed");
+    if (!rc && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+      rc = set_error(PYRO_EHIP, "pyro_autotune: graph instantiation failed");
     float ms = 0.0f;
-    if (!rc && (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+    if (!rc && (hipGraphLaunch(exec, ctx->stream) != hipSuccess ||
+                hipEventRecord(e0, ctx->stream) != hipSuccess ||
+                hipGraphLaunch(exec, ctx->stream) != hipSuccess ||
+                hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
       rc = set_error(PYRO_EHIP, "pyro_autotune: timing failed");
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (rc) break;
     const double per = ms / sweeps;
     if (results) {

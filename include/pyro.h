@@ -98,8 +98,8 @@ int pyro_get_exec_mode(pyro_ctx *ctx, int *mode, int *block, int *n_bins);
 
 /* Autotuner: time `sweeps` sweeps of each candidate execution (per-phase
  * cell- and site-per-thread kernels, persistent kernel at 256/512/1024
- * threads, tile list), restoring the state after each trial, and keep the
- * fastest. results (optional, 4 doubles per candidate): mode, block,
+ * threads, tile list), captured in one graph as pyro_sweep replays them,
+ * restoring the state after each trial, and keep the fastest. results (optional, 4 doubles per candidate): mode, block,
  * phase variant, ms per sweep. */
 int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results);
 
