@@ -631,6 +631,8 @@ int pyro_set_exec_mode(pyro_ctx *ctx, int mode, int block, int n_bins) {
     return set_error(PYRO_EINVAL, "exec mode 0 needs a uniform lattice size");
   if (mode != 0 && ctx->mat)
     return set_error(PYRO_ESTATE, "exec mode %d: exchange matrices run in execution mode 0 only", mode);
+  if (mode != 0 && ctx->post_rotate)
+    return set_error(PYRO_ESTATE, "exec mode %d: the post-Metropolis rotation runs in execution mode 0 only", mode);
   if (mode != 0 && !batch_size_supported(ctx->hetero ? 4 : ctx->L))
     return set_error(PYRO_EINVAL, "exec mode %d needs L in {4,8,12,16,24,32}", mode);
   if (mode != 0 && ctx->n_learned > 0)
@@ -674,6 +676,8 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
   if (sweeps < 1 || sweeps > 1000) return set_error(PYRO_EINVAL, "pyro_autotune: sweeps %d", sweeps);
   if (ctx->n_learned > 0) return set_error(PYRO_ESTATE, "pyro_autotune: learned passes are per-phase only");
   if (ctx->mat) return set_error(PYRO_ESTATE, "pyro_autotune: exchange matrices run in execution mode 0 only");
+  if (ctx->post_rotate)
+    return set_error(PYRO_ESTATE, "pyro_autotune: the post-Metropolis rotation runs in execution mode 0 only");
   PYRO_TRY(device_guard(ctx));
   struct Cand { int mode, block, variant; };
   std::vector<Cand> cands;
@@ -855,6 +859,17 @@ int pyro_record_fetch(pyro_ctx *ctx, double *rows, int *n_rows) {
   if (rows && row > 0)
     PYRO_HIP(hipMemcpy(rows, ctx->d_series, sizeof(double) * (size_t)row * ctx->R,
                        hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pyro_set_post_rotate(pyro_ctx *ctx, int on) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (on && (ctx->hetero || ctx->exec_mode != 0))
+    return set_error(PYRO_ESTATE, "pyro_set_post_rotate: execution mode 0 on a uniform lattice only");
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  drop_graph(ctx);  // the captured sweeps carry the old flag
+  ctx->post_rotate = on != 0;
   return 0;
 }
 

@@ -102,7 +102,8 @@ __device__ __forceinline__ void cell_update(float *__restrict__ Sx, int cell, in
                                             float beta_r, float cosc_r, uint32_t sweep,
                                             uint32_t key0, uint32_t key1, float jeff, double &de,
                                             unsigned &nacc,
-                                            const float *__restrict__ coup = nullptr) {
+                                            const float *__restrict__ coup = nullptr,
+                                            bool post_rotate = false) {
   constexpr int L = LT, LL = LT * LT, NC = LT * LT * LT;
   float *Sy = Sx + 16 * NC;
   float *Sz = Sy + 16 * NC;
@@ -144,6 +145,10 @@ __device__ __forceinline__ void cell_update(float *__restrict__ Sx, int cell, in
       sp[k] = cone_proposal(s[k], cosc_r, u01(w.x), u01(w.y));
       wr[k] = metropolis_accept(h, s[k], sp[k], beta_r, u01(w.z));
       nacc += wr[k] ? 1u : 0u;
+      if (post_rotate) {  // uniform over the block: a kernel argument
+        sp[k] = rotate_about(h, wr[k] ? sp[k] : s[k], u01(w.w));
+        wr[k] = true;
+      }
     } else {
       const f3 f = {fx[k], fy[k], fz[k]};
       sp[k] = overrelax(f, s[k]);
@@ -172,7 +177,8 @@ __device__ __forceinline__ void site_update(float *__restrict__ Sx, const Link (
                                             int site, int r, int pass, float beta_r, float cosc_r,
                                             uint32_t sweep, uint32_t key0, uint32_t key1,
                                             float jeff, double &de, unsigned &nacc,
-                                            const float *__restrict__ coup = nullptr) {
+                                            const float *__restrict__ coup = nullptr,
+                                            bool post_rotate = false) {
   constexpr int L = LT, LL = LT * LT, NC = LT * LT * LT;
   float *Sy = Sx + 16 * NC;
   float *Sz = Sy + 16 * NC;
@@ -206,6 +212,10 @@ __device__ __forceinline__ void site_update(float *__restrict__ Sx, const Link (
     sp = cone_proposal(s, cosc_r, u01(w.x), u01(w.y));
     wr = metropolis_accept(h, s, sp, beta_r, u01(w.z));
     nacc += wr ? 1u : 0u;
+    if (post_rotate) {
+      sp = rotate_about(h, wr ? sp : s, u01(w.w));
+      wr = true;
+    }
   } else {
     const f3 f = {fx, fy, fz};
     sp = overrelax(f, s);

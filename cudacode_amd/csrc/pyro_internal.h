@@ -130,6 +130,7 @@ struct pyro_ctx {
   // h_coup the same values on the host)
   double spin_len2 = 1.0;
   bool mat = false;
+  bool post_rotate = false;  // rotate about h after each Metropolis test
   float *d_coup = nullptr;
   float h_coup[pyro::kCoupFloats] = {};
 

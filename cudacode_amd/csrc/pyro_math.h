@@ -274,6 +274,29 @@ PYRO_HD f3 overrelax(f3 h, f3 s) {
   return normalise(v);
 }
 
+// Rotation of s about the local field h by the angle 2 pi u (Rodrigues'
+// formula), renormalised. The optional move after each Metropolis test
+// (_CudaRotate3DVector, Math.cu:168-197, applied at Pyrochlore.cu:872): it
+// keeps h.s, so the local energy, and any rotation angle preserves the
+// uniform measure on the circle about h. The reference's angle argument
+// works out to 2 pi^2 u (its half angle is passed in units of pi); here it
+// is uniform on [0, 2 pi). Unchanged when h = 0.
+PYRO_HD f3 rotate_about(f3 h, f3 s, float u) {
+  float hh = h.x * h.x + h.y * h.y + h.z * h.z;
+  if (!(hh > 0.0f)) return s;
+  float inv = 1.0f / sqrtf(hh);
+  f3 n = {h.x * inv, h.y * inv, h.z * inv};
+  float st, ct;
+  sincospi(2.0f * u, &st, &ct);
+  float ns = n.x * s.x + n.y * s.y + n.z * s.z;
+  float k = ns * (1.0f - ct);
+  f3 v;
+  v.x = (s.x * ct + (n.y * s.z - n.z * s.y) * st) + n.x * k;
+  v.y = (s.y * ct + (n.z * s.x - n.x * s.z) * st) + n.y * k;
+  v.z = (s.z * ct + (n.x * s.y - n.y * s.x) * st) + n.z * k;
+  return normalise(v);
+}
+
 // The 12 order-parameter components of Measurement (Pyrochlore.cu:580-624)
 // from the sublattice sums s[3 b + c] (b = basis site, c = component),
 // divided by n_spins: m[0] A2 (all-in-all-out), m[1..2] E, m[3..5] T1,

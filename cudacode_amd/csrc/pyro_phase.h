@@ -49,7 +49,8 @@ __global__ __launch_bounds__(kPhaseBlock) void k_phase(
     float *__restrict__ spins, int L_rt, int ncell_rt, int colour, int pass,
     const float *__restrict__ beta, const float *__restrict__ cosc,
     const uint32_t *__restrict__ ctl, uint32_t key0, uint32_t key1, float jeff,
-    unsigned long long *__restrict__ acc, double *__restrict__ dE, const float *__restrict__ coup) {
+    unsigned long long *__restrict__ acc, double *__restrict__ dE, const float *__restrict__ coup,
+    int post_rotate) {
   const int L = LT > 0 ? LT : L_rt;
   const int ncell = LT > 0 ? LT * LT * LT : ncell_rt;
   const int r = blockIdx.y;
@@ -96,6 +97,10 @@ __global__ __launch_bounds__(kPhaseBlock) void k_phase(
       sp = cone_proposal(s, cosc[r], u01(w.x), u01(w.y));
       write = metropolis_accept(h, s, sp, beta[r], u01(w.z));
       accepted = write ? 1 : 0;
+      if (post_rotate) {
+        sp = rotate_about(h, write ? sp : s, u01(w.w));
+        write = true;
+      }
     } else {
       const f3 f = {fx, fy, fz};
       sp = overrelax(f, s);
@@ -145,7 +150,7 @@ __global__ __launch_bounds__(256) void k_phase_cells(
     float *__restrict__ spins, int blocks_per_replica, int pass, const float *__restrict__ beta,
     const float *__restrict__ cosc, const uint32_t *__restrict__ ctl, uint32_t key0,
     uint32_t key1, float jeff, unsigned long long *__restrict__ acc, double *__restrict__ dE,
-    const float *__restrict__ coup) {
+    const float *__restrict__ coup, int post_rotate) {
   constexpr int NC = LT * LT * LT;
   const int nb = gridDim.x;
   const int b = blockIdx.x;
@@ -159,7 +164,7 @@ __global__ __launch_bounds__(256) void k_phase_cells(
                                               KIND == 0 ? beta[r] : 0.0f,
                                               KIND == 0 ? cosc[r] : 0.0f,
                                               KIND == 0 ? ctl[kCtlSweep] : 0u, key0, key1, jeff,
-                                              de, nacc, coup);
+                                              de, nacc, coup, KIND == 0 && post_rotate != 0);
   if (KIND == 0 || TRACK) {
     block_sum<256>(de, nacc);
     if (threadIdx.x == 0) {
@@ -261,12 +266,13 @@ static void launch_phase_t(pyro_ctx *ctx, int colour, int pass) {
                           ctx->time_ev0, ctx->time_ev1, 0, ctx->d_spins, ctx->L, ctx->ncell, colour,
                           pass, (const float *)ctx->d_beta, (const float *)ctx->d_cosc,
                           (const uint32_t *)ctx->d_ctl, key_lo(ctx), key_hi(ctx), ctx->jeff,
-                          ctx->d_acc, ctx->d_dE, (const float *)ctx->d_coup);
+                          ctx->d_acc, ctx->d_dE, (const float *)ctx->d_coup, (int)ctx->post_rotate);
     return;
   }
   k_phase<KIND, LT, MAT><<<grid, kPhaseBlock, 0, ctx->stream>>>(
       ctx->d_spins, ctx->L, ctx->ncell, colour, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl,
-      key_lo(ctx), key_hi(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE, ctx->d_coup);
+      key_lo(ctx), key_hi(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE, ctx->d_coup,
+      (int)ctx->post_rotate);
 }
 
 template <int KIND, int COLOUR, int LT, bool TRACK, bool MAT>
@@ -277,12 +283,12 @@ static void launch_cells_t(pyro_ctx *ctx, int pass) {
                           dim3(256), 0, ctx->stream, ctx->time_ev0, ctx->time_ev1, 0, ctx->d_spins,
                           bpr, pass, (const float *)ctx->d_beta, (const float *)ctx->d_cosc,
                           (const uint32_t *)ctx->d_ctl, key_lo(ctx), key_hi(ctx), ctx->jeff,
-                          ctx->d_acc, ctx->d_dE, (const float *)ctx->d_coup);
+                          ctx->d_acc, ctx->d_dE, (const float *)ctx->d_coup, (int)ctx->post_rotate);
     return;
   }
   k_phase_cells<KIND, COLOUR, LT, TRACK, MAT><<<bpr * ctx->R, 256, 0, ctx->stream>>>(
       ctx->d_spins, bpr, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl, key_lo(ctx), key_hi(ctx),
-      ctx->jeff, ctx->d_acc, ctx->d_dE, ctx->d_coup);
+      ctx->jeff, ctx->d_acc, ctx->d_dE, ctx->d_coup, (int)ctx->post_rotate);
 }
 
 template <int KIND, int LT, bool MAT>

@@ -245,6 +245,14 @@ int pyro_measure_fetch(pyro_ctx *ctx, double *rows, int *n_rows);
  * reference's --McSteps/--OverRelaxation options, Pyrochlore.cu:1161-1216). */
 int pyro_set_passes(pyro_ctx *ctx, int n_metro, int n_or);
 
+/* Optional move after every Metropolis test, accepted or not: rotate the spin
+ * about its local field by a Philox-drawn angle, uniform in [0, 2 pi)
+ * (_CudaRotate3DVector, Math.cu:168-197, applied at Pyrochlore.cu:872; the
+ * reference's angle works out to 2 pi^2 u). It keeps the local energy, so the
+ * Boltzmann law is unchanged. Off by default; execution mode 0 on a uniform
+ * lattice only. Acceptance counts are those of the Metropolis test. */
+int pyro_set_post_rotate(pyro_ctx *ctx, int on);
+
 /* Kernel options (no reference counterpart; the reference's block shape is the
  * --BlockSpin option, Pyrochlore.cu:1152-1154). phase_variant 1 (default): one
  * thread per cell and the colour's 4 sites, for L in {4,8,12,16,24,32,64};

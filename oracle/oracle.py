@@ -59,6 +59,8 @@ def lib():
                               C.c_int, C.POINTER(C.c_uint32), _f64p, _f32p, _f32p, _i32p,
                               C.c_float, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
                               C.POINTER(C.c_int)]
+        L.orc_set_post_rotate.argtypes = [C.c_int]
+        L.orc_rotate_about.argtypes = [_f32p, _f32p, C.c_float, _f32p]
         L.orc_set_couplings.argtypes = [C.c_void_p]
         L.orc_num_threads.restype = C.c_int
         L.orc_set_num_threads.argtypes = [C.c_int]
@@ -184,6 +186,17 @@ class OracleRun:
 
     def observables(self):
         return observables(self.spins, self.L, self.R, self.jeff)
+
+
+def set_post_rotate(on: bool) -> None:
+    lib().orc_set_post_rotate(1 if on else 0)
+
+
+def rotate_about(h, s, u) -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    lib().orc_rotate_about(np.ascontiguousarray(h, np.float32), np.ascontiguousarray(s, np.float32),
+                           float(u), out)
+    return out
 
 
 _coup_keep = None

@@ -276,6 +276,31 @@ static void nbr_sum(const float *S, int L, int x, int y, int z, int site, float 
   }
 }
 
+/* Optional rotation about h after each Metropolis test (Math.cu:168-197,
+ * Pyrochlore.cu:872), angle 2 pi u (Rodrigues), renormalised; s unchanged
+ * when h = 0. */
+static int g_post_rotate = 0;
+
+void orc_set_post_rotate(int on) { g_post_rotate = on != 0; }
+
+void orc_rotate_about(const float h[3], const float s[3], float u, float out[3]) {
+  float hh = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
+  if (!(hh > 0.0f)) {
+    out[0] = s[0]; out[1] = s[1]; out[2] = s[2];
+    return;
+  }
+  float inv = 1.0f / sqrtf(hh);
+  float n[3] = {h[0] * inv, h[1] * inv, h[2] * inv};
+  float st, ct;
+  orc_sincospi(2.0f * u, &st, &ct);
+  float ns = n[0] * s[0] + n[1] * s[1] + n[2] * s[2];
+  float k = ns * (1.0f - ct);
+  out[0] = (s[0] * ct + (n[1] * s[2] - n[2] * s[1]) * st) + n[0] * k;
+  out[1] = (s[1] * ct + (n[2] * s[0] - n[0] * s[2]) * st) + n[1] * k;
+  out[2] = (s[2] * ct + (n[0] * s[1] - n[1] * s[0]) * st) + n[2] * k;
+  renorm(out);
+}
+
 /* FP64 energy change of s -> sp at site basis b from the FP64 sums d. */
 static double move_de(float jeff, int b, const float s[3], const float sp[3], const double d[3]) {
   if (g_coup)
@@ -346,8 +371,15 @@ void orc_metropolis_phase(float *spins, int L, int R, int colour, uint32_t sweep
              seed, w);
         float sp[3];
         orc_cone_proposal(s, cos_cone[r], u01(w[0]), u01(w[1]), sp);
-        if (orc_metropolis_accept(h, s, sp, beta[r], u01(w[2]))) {
-          acc++;
+        int take = orc_metropolis_accept(h, s, sp, beta[r], u01(w[2]));
+        if (take) acc++;
+        if (g_post_rotate) {
+          float rot[3];
+          orc_rotate_about(h, take ? sp : s, u01(w[3]), rot);
+          sp[0] = rot[0]; sp[1] = rot[1]; sp[2] = rot[2];
+          take = 1;
+        }
+        if (take) {
           de += move_de(jeff, site & 3, s, sp, d);
           Sx[i] = sp[0];
           Sy[i] = sp[1];

@@ -68,6 +68,11 @@ int orc_pt_swap(int R, int parity, uint32_t round, uint64_t seed, const double *
  * 3 per basis site). The array must outlive its use. */
 void orc_set_couplings(const float *coup156);
 
+/* Rotation of s about h by 2 pi u after each Metropolis test (Math.cu:168-197):
+ * the move itself, and the switch for every following Metropolis phase. */
+void orc_rotate_about(const float h[3], const float s[3], float u, float out[3]);
+void orc_set_post_rotate(int on);
+
 /* Full sweeps: n_metro Metropolis passes + n_or over-relaxation passes, each
  * over colours 0..3; after every pt_every sweeps (if > 0) a swap with parity
  * alternating from the stored round counter. sweep0 is the first sweep index;

@@ -307,3 +307,24 @@ def test_exchange_reanchors_incremental_energy():
         assert np.array_equal(g.energy_incremental(), g.TotalEnergyv2())
         g.Mc(3)
         assert np.allclose(g.energy_incremental(), g.TotalEnergyv2(), rtol=1e-6, atol=1e-3)
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_post_rotation_matches_oracle(variant):
+    """The optional rotation about h after each Metropolis test: same spins
+    as the oracle bit for bit, unchanged acceptance counting, exact books."""
+    L, temps = 4, [0.1, 0.5, 2.0]
+    kw = dict(n_metro=1, n_or=1, pt_every=4, cos_cone=0.4)
+    with gpu(L, temps, variant=variant, **kw) as g:
+        g.set_post_rotate(True)
+        g.Mc(8)
+        spins, acc = g.get_spins(), g.acceptance()[0]
+        assert np.allclose(g.energy_incremental(), g.TotalEnergyv2(), rtol=1e-11, atol=1e-9)
+    oracle.set_post_rotate(True)
+    try:
+        o = orc(L, temps, **kw)
+        o.run(8)
+    finally:
+        oracle.set_post_rotate(False)
+    assert np.array_equal(spins, o.spins)
+    assert np.array_equal(acc, o.accepted)

@@ -201,3 +201,42 @@ def test_low_temperature_specific_heat_three_quarters():
     e, err = equilibrium_energy(L, T, 2000, 500, aiao, 0.995, n_or=5)
     c_est = (e + 1.0) / T
     assert abs(c_est - 0.75) < 4 * err / T + 0.04, (c_est, err / T)
+
+
+def test_rotation_about_field():
+    """rotate_about: unit norm, the component along h kept, the perpendicular
+    part turned by 2 pi u; u = 1/2 is the over-relaxation reflection."""
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        h = rng.normal(size=3).astype(np.float32)
+        s = rng.normal(size=3)
+        s = (s / np.linalg.norm(s)).astype(np.float32)
+        u = np.float32(rng.random())
+        out = oracle.rotate_about(h, s, u).astype(np.float64)
+        n = h / np.linalg.norm(h)
+        assert abs(np.linalg.norm(out) - 1) < 1e-6
+        assert abs(n @ out - n @ s) < 2e-6
+        p0, p1 = s - (n @ s) * n, out - (n @ out) * n
+        if np.linalg.norm(p0) > 1e-3:
+            ang = np.arctan2(n @ np.cross(p0, p1), p0 @ p1) % (2 * np.pi)
+            d = (ang - 2 * np.pi * float(u)) % (2 * np.pi)
+            assert min(d, 2 * np.pi - d) < 1e-4
+        half = oracle.rotate_about(h, s, np.float32(0.5)).astype(np.float64)
+        assert np.allclose(half, oracle.overrelax(h, s), atol=2e-6)
+    s = np.array([0.6, 0.0, 0.8], np.float32)
+    assert np.array_equal(oracle.rotate_about(np.zeros(3, np.float32), s, 0.3), s)
+
+
+def test_post_rotation_keeps_the_boltzmann_law():
+    """Metropolis sweeps with and without the rotation about h sample the same
+    energy distribution (the rotation keeps every local energy)."""
+    L, T = 2, 0.5
+    res = []
+    for on in (False, True):
+        oracle.set_post_rotate(on)
+        try:
+            res.append(equilibrium_energy(L, T, 4000, 300, None, 0.3, n_or=0, seed=5 + on))
+        finally:
+            oracle.set_post_rotate(False)
+    (e0, s0), (e1, s1) = res
+    assert abs(e0 - e1) < 4 * np.hypot(s0, s1), res
