@@ -37,7 +37,8 @@ FLOP_PER_FORWARD = 2 * (6 * 64 + 64 * 64 + 64 * 4)  # MLP 6 -> 64 -> 64 -> 4
 BF16_PEAK_TFLOPS = 2500.0  # dense bf16 MFMA
 
 
-OR_KERNEL = "k_phase_cells<1, colour, 16, false>"
+OR_KERNEL = "k_phase_cells<1, colour, 16, false, false>"
+OR_KERNEL_C5 = "k_phase_cells<1, colour, 64, false, false>"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic.json")
 
 
@@ -46,12 +47,13 @@ def measured_traffic(kernel: str):
     record (FETCH_SIZE and WRITE_SIZE passes scaled by a copy kernel of known
     size), or None when no record for this kernel exists."""
     try:
-        rec = json.load(open(TRAFFIC_FILE))
-    except (OSError, ValueError):
+        records = json.load(open(TRAFFIC_FILE))["records"]
+    except (OSError, ValueError, KeyError, TypeError):
         return None
-    if rec.get("kernel") != kernel:
-        return None
-    return rec.get("traffic_bytes_per_launch")
+    for rec in records:
+        if rec.get("kernel") == kernel:
+            return rec.get("traffic_bytes_per_launch")
+    return None
 
 
 def log(msg):
@@ -72,7 +74,7 @@ def cpu_baseline(target_s: float = 15.0):
     t0 = time.perf_counter()
     run.run(1)
     one = time.perf_counter() - t0
-    n = max(1, min(50, int(target_s / max(one, 1e-3)) - 1))
+    n = max(1, min(100, int(target_s / max(one, 1e-3)) - 1))
     log(f"cpu oracle: 1 sweep {one:.2f} s on {used} threads; timing {n} more")
     t0 = time.perf_counter()
     run.run(n)
@@ -90,7 +92,9 @@ def cpu_baseline(target_s: float = 15.0):
 
 def calibrate_copy(mib: int = 512, reps: int = 5):
     """A plain device-to-device copy of known size (torch's elementwise copy
-    kernel), run under rocprofv3 --pmc to scale FETCH_SIZE / WRITE_SIZE."""
+    kernel), run under rocprofv3 --pmc to scale FETCH_SIZE / WRITE_SIZE; also
+    timed (torch events on torch's stream, which runs the copy) as a practical
+    streaming-bandwidth reference beside the 8 TB/s peak."""
     import torch
     n = mib * (1 << 20) // 4
     a = torch.ones(n, device="cuda", dtype=torch.float32)
@@ -98,7 +102,16 @@ def calibrate_copy(mib: int = 512, reps: int = 5):
     for _ in range(reps):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"calibration_copy_bytes_read": n * 4, "bytes_written": n * 4, "reps": reps}))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    timed = 20
+    e0.record()
+    for _ in range(timed):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / timed
+    print(json.dumps({"calibration_copy_bytes_read": n * 4, "bytes_written": n * 4, "reps": reps,
+                      "copy_ms": ms, "copy_GBps_read_plus_write": 2 * n * 4 / (ms * 1e-3) / 1e9}))
 
 
 def learned_leg(g, args, plain_series, plain_sweeps_per_s, plain_taus, plain_samples):
@@ -297,10 +310,12 @@ def workload_c5(args):
     return line(steps * 11 * n_total / dt, steps, 0, dt,
                 "L=64 (4,194,304 spins) x 32 replicas, T 0.02..0.5, 1 Metropolis + 10 "
                 "over-relaxation, swaps every 10 sweeps",
-                {"roofline": {"bound": "hbm", "kernel": "k_phase_cells<1, colour, 64, false>",
+                {"roofline": {"bound": "hbm", "kernel": OR_KERNEL_C5,
                               "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                              "avg_launch_ms": or_ms}})
+                              "frac": achieved / HBM_PEAK_GBS,
+                              "traffic": measured_traffic(OR_KERNEL_C5),
+                              "avg_launch_ms": or_ms,
+                              "bytes_per_launch": BYTES_PER_UPDATE * n_total // 4}})
 
 
 def main():
@@ -367,6 +382,24 @@ def main():
     g.set_kernel_options(1, False)
     bytes_per_launch = BYTES_PER_UPDATE * n_total // 4
     achieved = bytes_per_launch / (or_ms * 1e-3) / 1e9
+    # A caller that hands host buffers over every sweep pays the PCIe copies
+    # too (pyro_set_spins re-anchors the energy as well): the host-inclusive
+    # rate, reported beside the HBM-resident value.
+    log("host transfer timing")
+    host = g.get_spins()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        host = g.get_spins()
+    get_ms = 1e3 * (time.perf_counter() - t0) / 5
+    t0 = time.perf_counter()
+    for _ in range(5):
+        g.set_spins(host)
+    set_ms = 1e3 * (time.perf_counter() - t0) / 5
+    host_transfer = {
+        "bytes": int(host.nbytes), "get_spins_ms": get_ms, "set_spins_ms": set_ms,
+        "host_inclusive_updates_per_s": (N_METRO + N_OR) * n_total / (
+            (ms_per_step + get_ms + set_ms) * 1e-3),
+    }
     learned = None
     if not args.no_learned:
         learned = learned_leg(g, args, series, sweeps_per_s, taus, samples)
@@ -401,6 +434,7 @@ def main():
             "series_sweeps": int(series.shape[0]),
         },
         "acceptance": acc,
+        "host_transfer": host_transfer,
         "roofline": {
             "bound": "hbm",
             "kernel": OR_KERNEL + " (over-relaxation colour phase, one thread per cell)",

@@ -7,12 +7,15 @@
 // counter-based, so no generator state exists: a run resumed from a
 // checkpoint continues bit for bit. Little-endian binary:
 //
-//   "PYROCKP1"  u32 version=1  u32 R  u64 seed  f64 jeff
+//   "PYROCKP1"  u32 version=2  u32 R  u64 seed  f64 jeff
 //   i32 n_metro n_learned n_or pt_every   u32 sweep pt_round
 //   i32 L[R]  f64 T[R]  f32 cos_cone[R]  i32 replica_at[R]
 //   f64 anchor[R]  f64 dE[R]  u64 accepted[R]  u64 proposed[R]
 //   u32 has_mlp  [f32 w[4868] if has_mlp]
+//   f64 spin_len2  u32 model (bit 0: exchange matrices, bit 1: rotation
+//   after the Metropolis test)  [f32 coup[156] if bit 0]
 //   u64 n_floats  f32 spins[n_floats]  u64 FNV-1a of every preceding byte
+// Version 1 files (no model block: scalar model, no rotation) still load.
 //
 // Reference files (SaveConfigurations / ReadConfigurations, Math.cu:618-701):
 // one file "<prefix><k>.dat" per temperature rank k (descending T), records
@@ -106,7 +109,7 @@ int pyro_checkpoint_save(pyro_ctx *ctx, const char *path) {
   if (!f) return set_error(PYRO_EIO, "pyro_checkpoint_save: cannot open %s", tmp.c_str());
   Writer wr{f};
   wr.put(kMagic, 8);
-  wr.put((uint32_t)1);
+  wr.put((uint32_t)2);
   wr.put((uint32_t)R);
   wr.put((uint64_t)ctx->seed);
   wr.put((double)ctx->jeff);
@@ -126,6 +129,9 @@ int pyro_checkpoint_save(pyro_ctx *ctx, const char *path) {
   wr.put(ctx->h_proposed.data(), sizeof(uint64_t) * R);
   wr.put((uint32_t)(w.empty() ? 0 : 1));
   if (!w.empty()) wr.put(w.data(), sizeof(float) * kMlpParams);
+  wr.put((double)ctx->spin_len2);
+  wr.put((uint32_t)((ctx->mat ? 1u : 0u) | (ctx->post_rotate ? 2u : 0u)));
+  if (ctx->mat) wr.put(ctx->h_coup, sizeof ctx->h_coup);
   wr.put((uint64_t)ctx->total_floats);
   wr.put(spins.data(), sizeof(float) * ctx->total_floats);
   const uint64_t h = wr.sum.h;
@@ -154,7 +160,7 @@ int pyro_checkpoint_load(pyro_ctx *ctx, const char *path) {
     return set_error(PYRO_EIO, "pyro_checkpoint_load: %s (%s)", what, path);
   };
   if (!rd.ok || std::memcmp(magic, kMagic, 8) != 0) return bad("not a checkpoint");
-  if (version != 1) return bad("unknown version");
+  if (version != 1 && version != 2) return bad("unknown version");
   if ((int)R != ctx->R) return bad("replica count differs from the ctx");
   const uint64_t seed = rd.get<uint64_t>();
   const double jeff = rd.get<double>();
@@ -183,6 +189,14 @@ int pyro_checkpoint_load(pyro_ctx *ctx, const char *path) {
     w.resize(kMlpParams);
     rd.get(w.data(), sizeof(float) * kMlpParams);
   }
+  double spin_len2 = ctx->spin_len2;
+  uint32_t model = 0;
+  float coup[kCoupFloats] = {};
+  if (version >= 2) {
+    spin_len2 = rd.get<double>();
+    model = rd.get<uint32_t>();
+    if (model & 1u) rd.get(coup, sizeof coup);
+  }
   const uint64_t n_floats = rd.get<uint64_t>();
   if (!rd.ok) return bad("truncated");
   if (n_floats != ctx->total_floats) return bad("spin count differs from the ctx");
@@ -206,10 +220,19 @@ int pyro_checkpoint_load(pyro_ctx *ctx, const char *path) {
     return set_error(PYRO_EIO, "pyro_checkpoint_load: bad schedule");
   if (n_learned > 0 && w.empty())
     return set_error(PYRO_EIO, "pyro_checkpoint_load: learned passes without weights");
+  if (model > 3u || !(spin_len2 > 0.0)) return set_error(PYRO_EIO, "pyro_checkpoint_load: bad model block");
+  for (float c : coup)
+    if (!std::isfinite(c)) return set_error(PYRO_EIO, "pyro_checkpoint_load: coupling not finite");
+  if (model != 0u && (ctx->hetero || ctx->exec_mode != 0))
+    return set_error(PYRO_ESTATE,
+                     "pyro_checkpoint_load: the saved model runs in execution mode 0 on a uniform lattice");
 
   PYRO_HIP(hipSetDevice(ctx->device));
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   if (!w.empty()) PYRO_TRY(pyro_mlp_set_weights(ctx, w.data()));
+  if ((model & 1u) && !ctx->d_coup)
+    PYRO_HIP(hipMalloc((void **)&ctx->d_coup, sizeof(float) * kCoupFloats));
+  if (model & 1u) PYRO_HIP(hipMemcpy(ctx->d_coup, coup, sizeof coup, hipMemcpyHostToDevice));
   std::vector<float> beta(R);
   for (uint32_t r = 0; r < R; ++r) beta[r] = (float)(1.0 / T[r]);
   PYRO_HIP(hipMemcpy(ctx->d_spins, spins.data(), sizeof(float) * n_floats, hipMemcpyHostToDevice));
@@ -236,6 +259,10 @@ int pyro_checkpoint_load(pyro_ctx *ctx, const char *path) {
   ctx->h_sweep = sweep;
   ctx->h_pt_round = pt_round;
   ctx->h_proposed = prop;
+  ctx->spin_len2 = spin_len2;
+  ctx->mat = (model & 1u) != 0;
+  ctx->post_rotate = (model & 2u) != 0;
+  std::memcpy(ctx->h_coup, coup, sizeof coup);
   return 0;
 }
 

@@ -1,7 +1,7 @@
 """Summarise rocprofv3 databases (rocpd SQLite) into the short text records
 kept under profiles/.
 
-  python -m cudacode_amd.rocprof_summary trace DB            per-kernel time table
+  python -m cudacode_amd.rocprof_summary trace DB [N]        per-kernel time table (top N)
   python -m cudacode_amd.rocprof_summary pmc DB [DB ...]      mean counter value per kernel
 
 Counter passes are scaled by a copy kernel of known size run under the same
@@ -40,7 +40,7 @@ def short(name: str) -> str:
 
 def main(argv):
     if len(argv) >= 2 and argv[0] == "trace":
-        print(trace(argv[1]))
+        print(trace(argv[1], int(argv[2]) if len(argv) > 2 else 12))
     elif len(argv) >= 2 and argv[0] == "pmc":
         for db in argv[1:]:
             for (k, cnt), (n, avg) in sorted(pmc(db).items(), key=lambda t: -t[1][1])[:8]:
