@@ -169,7 +169,7 @@ def run_protocol(g, out_dir: str, rep: int = 0, *, anneal_steps: int = 100,
     g.Mc(pt_rounds * max(g.pt_every, 1))
     g.Mc(thermalisation)
     # Repetition rep appends its record (the reference passes rep as the
-    # append flag, Pyrochlore.cu:1522-1527), so record rep of InitialConf<k>.dat
+    # append flag, Pyrochlore.cu:1517-1522), so record rep of InitialConf<k>.dat
     # is this repetition's starting point (ReadConfigurations(..., rep)).
     g.SaveConfigurations(os.path.join(out_dir, "InitialConf"), append=rep > 0)
     T = rung_temperatures(g)
