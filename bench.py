@@ -90,12 +90,31 @@ def cpu_baseline(target_s: float = 15.0):
     }
 
 
-def calibrate_copy(mib: int = 512, reps: int = 5):
+def calibrate_copy(mib: int = 512, reps: int = 5, sweep=()):
     """A plain device-to-device copy of known size (torch's elementwise copy
     kernel), run under rocprofv3 --pmc to scale FETCH_SIZE / WRITE_SIZE; also
     timed (torch events on torch's stream, which runs the copy) as a practical
-    streaming-bandwidth reference beside the 8 TB/s peak."""
+    streaming-bandwidth reference beside the 8 TB/s peak. `sweep`: further
+    sizes in MiB, timed the same way (sizes whose read + write fit the 256 MB
+    Infinity Cache show its rate)."""
     import torch
+    rates = []
+    for m in sweep:
+        k = m * (1 << 20) // 4
+        x = torch.ones(k, device="cuda", dtype=torch.float32)
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        f0.record()
+        for _ in range(50):
+            y.copy_(x)
+        f1.record()
+        torch.cuda.synchronize()
+        t = f0.elapsed_time(f1) / 50
+        rates.append({"MiB": m, "copy_ms": t, "copy_GBps_read_plus_write": 2 * k * 4 / (t * 1e-3) / 1e9})
+        del x, y
     n = mib * (1 << 20) // 4
     a = torch.ones(n, device="cuda", dtype=torch.float32)
     b = torch.empty_like(a)
@@ -111,7 +130,8 @@ def calibrate_copy(mib: int = 512, reps: int = 5):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / timed
     print(json.dumps({"calibration_copy_bytes_read": n * 4, "bytes_written": n * 4, "reps": reps,
-                      "copy_ms": ms, "copy_GBps_read_plus_write": 2 * n * 4 / (ms * 1e-3) / 1e9}))
+                      "copy_ms": ms, "copy_GBps_read_plus_write": 2 * n * 4 / (ms * 1e-3) / 1e9,
+                      "size_sweep": rates}))
 
 
 def learned_leg(g, args, plain_series, plain_sweeps_per_s, plain_taus, plain_samples):
@@ -169,7 +189,7 @@ def learned_leg(g, args, plain_series, plain_sweeps_per_s, plain_taus, plain_sam
     proposals = n_spins // 4 * R
     flop = proposals * 2 * FLOP_PER_FORWARD
     learned_phase = {
-        "kernel": "k_learned_phase<16, 2>", "avg_launch_ms": learned_ms,
+        "kernel": "k_learned_phase<16, 1>", "avg_launch_ms": learned_ms,
         "proposals_per_launch": proposals, "flop_per_launch": flop,
         "achieved_tflops": flop / (learned_ms * 1e-3) / 1e12, "peak_tflops": BF16_PEAK_TFLOPS,
         "frac_mfma": flop / (learned_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
@@ -320,7 +340,8 @@ def workload_c5(args):
 
 def main():
     if "--calibrate-copy" in sys.argv:
-        calibrate_copy()
+        sizes = [int(a) for a in sys.argv[sys.argv.index("--calibrate-copy") + 1:] if a.isdigit()]
+        calibrate_copy(sweep=sizes)
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)

@@ -258,7 +258,7 @@ __device__ __forceinline__ int wrap_t(int v, int L) {
 // cell), with G groups of 64 cells per block so the weight fragments loaded
 // into registers serve G forward-pass pairs. grid = (ceil(ncell/(64 G)), R).
 template <int LT, int G>
-__global__ __launch_bounds__(kPhaseBlock) void k_learned_phase(
+__global__ __launch_bounds__(kPhaseBlock, 3) void k_learned_phase(
     float *__restrict__ spins, int L_rt, int ncell_rt, int colour, int pass,
     const float *__restrict__ beta, const uint32_t *__restrict__ ctl, uint32_t key0,
     uint32_t key1, float jeff, const uint16_t *__restrict__ pack, const float *__restrict__ w,
@@ -691,7 +691,10 @@ int mlp_pack(pyro_ctx *ctx) {
   return 0;
 }
 
-constexpr int kLearnedGroups = 2;  // 64-cell groups per block
+// 64-cell groups per block. One group and at most 168 VGPRs (3 waves per
+// SIMD, a few spilled values) measured fastest: 48 us per L=16 x 64 launch,
+// against 55 us for two groups at 226 VGPRs (2 waves per SIMD).
+constexpr int kLearnedGroups = 1;
 
 template <int LT>
 static void launch_learned_t(pyro_ctx *ctx, int colour, int pass) {
