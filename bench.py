@@ -134,109 +134,50 @@ def calibrate_copy(mib: int = 512, reps: int = 5, sweep=()):
                       "size_sweep": rates}))
 
 
-def learned_leg(g, args, plain_series, plain_sweeps_per_s, plain_taus, plain_samples):
-    """The learned local proposal on the same ensemble: environments (beta h, s)
-    gathered from the plain sampler across the temperature ladder (16 snapshots
-    x 64 replicas x 1024 sites = 1,048,576), the network trained on the GPU,
-    then sweeps of 1 learned pass + 10 over-relaxation passes timed like the
-    plain ones. Reports sweeps/s, tau_int, independent samples/s and their
-    ratio to the plain sampler, and the agreement of the mean energy per rung."""
-    from cudacode_amd import pyro, stats
-    n_snap, per_rep = 16, 1024
-    log("learned: gathering environments from the plain sampler")
-    g.clear_environments()
-    for k in range(n_snap):
-        g.Mc(5)
-        g.gather_environments(per_rep, 1000 + k, n_snap * per_rep * R)
-    g.mlp_set_weights(pyro.mlp_init(SEED))
-    log(f"learned: training {args.train_steps} steps on the GPU")
-    t0 = time.perf_counter()
-    hist = g.mlp_train(args.train_steps, batch=4096, lr=3e-3, lr_final=3e-4)
-    g.sync()
-    train_s = time.perf_counter() - t0
-    g.set_passes(0, N_OR)
-    g.set_learned_passes(1)
-    g.Mc(args.warmup)
-    g.reset_acceptance()
-    g.record_start(args.steps)
-    g.sync()
+def learned_leg(g, args, plain):
+    """The learned local proposal on the same ensemble (cudacode_amd.learned):
+    environments (beta h, s) gathered from the plain sampler across the
+    temperature ladder (16 snapshots x 64 replicas x 1024 sites = 1,048,576),
+    the network trained on the GPU, then sweeps of 1 learned pass + 10
+    over-relaxation passes timed like the plain ones, then a plain control leg
+    of the same length. Reports sweeps/s, tau_int, independent samples/s and
+    their ratio to the plain sampler, the agreement of the mean energy per
+    rung, the learned phase's MFMA roofline, and which sampler the
+    fallback rule keeps."""
+    from cudacode_amd import learned, pyro
+    log("learned: gathering environments and training on the GPU")
+    info = learned.train(g, steps=args.train_steps, init_seed=SEED)
+    prev = learned.use_learned(g, 1)
     log(f"learned: timed {args.steps} sweeps")
-    t0 = time.perf_counter()
-    g.Mc(args.steps)
-    g.sync()
-    dt = time.perf_counter() - t0
-    series = g.record_fetch()
-    a, p = g.acceptance_learned()
-    sweeps_per_s = args.steps / dt
+    leg = learned.timed_leg(g, args.steps, warmup=args.warmup, learned=True)
     log("learned: phase timing, 200 learned launches")
     learned_ms, _ = g.time_phase(2, 200)
-    taus = [stats.tau_int(series[:, k])[0] for k in range(R)]
-    samples = [stats.independent_samples_per_s(sweeps_per_s, t) for t in taus]
-    ratio = np.array(samples) / np.array(plain_samples)
-    g.set_learned_passes(0)
-    g.set_passes(N_METRO, N_OR)
-    # Control: the plain sampler again, continuing from the learned chain's
-    # state. The spread of (plain, plain) z-scores shows how far the slowest
-    # rungs still drift over these run lengths; the learned chain is judged
-    # against it, not against the normal law alone.
+    learned.use_plain(g, prev)
     log(f"learned: plain control {args.steps} sweeps")
-    g.record_start(args.steps)
-    g.Mc(args.steps)
-    control = g.record_fetch()
+    control = learned.timed_leg(g, args.steps)
     n_spins = 16 * L ** 3
+    verdict = learned.judge(plain, leg, control, n_spins)
+    choice = learned.choose(g, verdict, prev)
+    learned.use_plain(g, prev)
     # Dense work of one learned colour phase: two forward passes per proposal
     # of 2 (6*64 + 64*64 + 64*4) = 9,472 FLOP each, N/4 proposals per replica.
     proposals = n_spins // 4 * R
     flop = proposals * 2 * FLOP_PER_FORWARD
-    learned_phase = {
+    rung_T = np.sort(pyro.geometric_ladder(T_MIN, T_MAX, R))[::-1]  # rungs: descending T
+    out = {"sweep": "1 learned pass + 10 over-relaxation passes (plain: 1 Metropolis + 10)"}
+    out.update(info)
+    out["phase"] = {
         "kernel": "k_learned_phase<16, 1>", "avg_launch_ms": learned_ms,
         "proposals_per_launch": proposals, "flop_per_launch": flop,
         "achieved_tflops": flop / (learned_ms * 1e-3) / 1e12, "peak_tflops": BF16_PEAK_TFLOPS,
         "frac_mfma": flop / (learned_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
         "achieved_gbs": proposals * BYTES_PER_UPDATE / (learned_ms * 1e-3) / 1e9,
     }
-
-    def zscores(a, b):
-        z = []
-        for k in range(R):
-            e1, e2 = a[:, k] / n_spins, b[:, k] / n_spins
-            s1 = stats.binning_error(e1, 20)
-            s2 = stats.binning_error(e2, 20)
-            z.append(abs(e1.mean() - e2.mean()) / max(np.hypot(s1, s2), 1e-300))
-        return np.array(z)
-
-    rung_T = np.sort(pyro.geometric_ladder(T_MIN, T_MAX, R))[::-1]  # columns: descending T
-    z = zscores(plain_series, series)
-    z_ctl = zscores(plain_series, control)
-    z_lc = zscores(series, control)
-    return {
-        "sweep": "1 learned pass + 10 over-relaxation passes (plain: 1 Metropolis + 10)",
-        "environments": n_snap * per_rep * R,
-        "train_steps": args.train_steps,
-        "train_s": train_s,
-        "loss_first": float(hist[:10].mean()),
-        "loss_last": float(hist[-50:].mean()),
-        "acceptance": float(a.sum() / max(p.sum(), 1)),
-        "phase": learned_phase,
-        "sweeps_per_s": sweeps_per_s,
-        "plain_sweeps_per_s": plain_sweeps_per_s,
-        "spin_updates_per_s": args.steps * (1 + N_OR) * n_spins * R / dt,
-        "tau_int_sweeps_median": float(np.median(taus)),
-        "tau_int_sweeps_lowest_T": float(taus[-1]),
-        "plain_tau_int_sweeps_median": float(np.median(plain_taus)),
-        "indep_samples_per_s_median": float(np.median(samples)),
-        "indep_samples_per_s_lowest_T": float(samples[-1]),
-        "samples_ratio_median": float(np.median(ratio)),
-        "samples_ratio_lowest_T": float(ratio[-1]),
-        "samples_ratio_min": float(ratio.min()),
-        "energy_agreement_max_z": float(z.max()),
-        "energy_agreement_rungs_within_2sigma": int((z < 2).sum()),
-        "control_plain_vs_plain_max_z": float(z_ctl.max()),
-        "control_plain_vs_plain_within_2sigma": int((z_ctl < 2).sum()),
-        "learned_vs_control_max_z": float(z_lc.max()),
-        "learned_vs_control_within_2sigma": int((z_lc < 2).sum()),
-        "worst_rungs_T": [float(rung_T[k]) for k in np.argsort(-z)[:3]],
-    }
+    out["spin_updates_per_s"] = leg.sweeps_per_s * (1 + N_OR) * n_spins * R
+    out.update(verdict)
+    out["worst_rungs_T"] = [float(rung_T[k]) for k in verdict.pop("worst_rungs")]
+    out["fallback_choice"] = choice
+    return out
 
 
 def line(metric_value, steps, warmup, dt, workload, extra):
@@ -254,8 +195,8 @@ def line(metric_value, steps, warmup, dt, workload, extra):
 def workload_c1(args):
     """L=4 (1,024 spins), one replica at T=0.5, 1 Metropolis (cone adapted to
     50%) + 5 over-relaxation per sweep, 10^4 sweeps."""
-    from cudacode_amd import pyro, stats
-    g = pyro.Pyrochlore(4, [0.5], seed=SEED, n_metro=1, n_or=5, cos_cone=0.5)
+    from cudacode_amd import presets, stats
+    g = presets.preset("c1").build()
     g.CalculateSolidAngle(0.5, 20, 10)
     g.Mc(args.warmup)
     tune = g.autotune(5)
@@ -284,12 +225,10 @@ def workload_c3(args):
     from 0.05 to 1.0 (8 replicas each), 1 Metropolis + 10 over-relaxation,
     swaps every 10 sweeps within equal-size groups; execution chosen by the
     autotuner (persistent per-sweep kernel or tile-list phases)."""
-    from cudacode_amd import pyro
-    rs = np.random.default_rng(SEED)
-    Ls = rs.choice([8, 12, 16, 24], size=256)
-    ladder = pyro.geometric_ladder(0.05, 1.0, 32)
-    temps = np.repeat(ladder, 8)
-    g = pyro.Batch(Ls, temps, seed=SEED, n_metro=1, n_or=10, pt_every=10, cos_cone=0.5)
+    from cudacode_amd import presets
+    cfg = presets.preset("c3")
+    Ls = cfg.sizes()
+    g = cfg.build()
     g.CalculateSolidAngle(0.5, 5, 10)
     g.Mc(args.warmup)
     tune = g.autotune(5)
@@ -312,10 +251,13 @@ def workload_c3(args):
 
 def workload_c5(args):
     """L=64 (4,194,304 spins) x 32 replicas, T from 0.02 to 0.5, swaps every 10
-    sweeps, 1 Metropolis + 10 over-relaxation per sweep (1.61 GB resident)."""
-    from cudacode_amd import pyro
-    temps = pyro.geometric_ladder(0.02, 0.5, 32)
-    g = pyro.Pyrochlore(64, temps, seed=SEED, n_metro=1, n_or=10, pt_every=10, cos_cone=0.5)
+    sweeps (1.61 GB resident). Plain sweeps of 1 Metropolis + 10
+    over-relaxation passes give the line's value; then the learned proposal,
+    trained on 1,048,576 environments of this ensemble, is interleaved
+    (1 Metropolis + 1 learned + 10 over-relaxation per sweep)."""
+    from cudacode_amd import learned, presets
+    cfg = presets.preset("c5")
+    g = cfg.build()
     g.Mc(max(args.warmup // 20, 2))
     steps = max(args.steps // 40, 10)
     g.sync()
@@ -326,16 +268,39 @@ def workload_c5(args):
     or_ms, _ = g.time_phase(1, 40)
     n_total = 16 * 64 ** 3 * 32
     achieved = BYTES_PER_UPDATE * n_total / 4 / (or_ms * 1e-3) / 1e9
+    extra = {"roofline": {"bound": "hbm", "kernel": OR_KERNEL_C5,
+                          "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": achieved / HBM_PEAK_GBS,
+                          "traffic": measured_traffic(OR_KERNEL_C5),
+                          "avg_launch_ms": or_ms,
+                          "bytes_per_launch": BYTES_PER_UPDATE * n_total // 4}}
+    if not args.no_learned:
+        log("c5: training the learned proposal on this ensemble")
+        info = learned.train(g, snapshots=16, per_replica=2048, steps=args.train_steps,
+                             init_seed=SEED)
+        g.set_learned_passes(cfg.n_learned)  # interleaved: Metropolis, learned, over-relaxation
+        g.Mc(2)
+        g.reset_acceptance()
+        g.sync()
+        t0 = time.perf_counter()
+        g.Mc(steps)
+        g.sync()
+        dl = time.perf_counter() - t0
+        a, p = g.acceptance_learned()
+        learned_ms, _ = g.time_phase(2, 8)
+        info.update({
+            "sweep": "1 Metropolis + 1 learned + 10 over-relaxation passes",
+            "ms_per_step": 1e3 * dl / steps,
+            "spin_updates_per_s": steps * 12 * n_total / dl,
+            "acceptance": float(a.sum() / max(p.sum(), 1)),
+            "learned_phase_avg_launch_ms": learned_ms,
+        })
+        extra["learned_interleaved"] = info
     g.close()
     return line(steps * 11 * n_total / dt, steps, 0, dt,
                 "L=64 (4,194,304 spins) x 32 replicas, T 0.02..0.5, 1 Metropolis + 10 "
-                "over-relaxation, swaps every 10 sweeps",
-                {"roofline": {"bound": "hbm", "kernel": OR_KERNEL_C5,
-                              "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": achieved / HBM_PEAK_GBS,
-                              "traffic": measured_traffic(OR_KERNEL_C5),
-                              "avg_launch_ms": or_ms,
-                              "bytes_per_launch": BYTES_PER_UPDATE * n_total // 4}})
+                "over-relaxation, swaps every 10 sweeps; then with 1 learned pass interleaved",
+                extra)
 
 
 def main():
@@ -390,10 +355,11 @@ def main():
     acc = float(np.mean(g.CalculateRatio()))
 
     series = g.record_fetch()  # rows: sweeps, columns: rungs (descending T)
-    _, at = g.temperatures()
-    taus = [stats.tau_int(series[:, k])[0] for k in range(R)]
+    taus = np.array([stats.tau_int(series[:, k])[0] for k in range(R)])
     sweeps_per_s = args.steps / dt
-    samples = [stats.independent_samples_per_s(sweeps_per_s, t) for t in taus]
+    samples = np.array([stats.independent_samples_per_s(sweeps_per_s, t) for t in taus])
+    from cudacode_amd.learned import Leg
+    plain = Leg(series, sweeps_per_s, taus, samples, acc)
 
     log(f"phase timing: {args.phase_launches} over-relaxation launches")
     or_ms, _ = g.time_phase(1, args.phase_launches)
@@ -423,7 +389,7 @@ def main():
     }
     learned = None
     if not args.no_learned:
-        learned = learned_leg(g, args, series, sweeps_per_s, taus, samples)
+        learned = learned_leg(g, args, plain)
     g.close()
     traffic = measured_traffic(OR_KERNEL)
 

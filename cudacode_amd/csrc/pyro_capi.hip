@@ -862,6 +862,28 @@ int pyro_record_fetch(pyro_ctx *ctx, double *rows, int *n_rows) {
   return 0;
 }
 
+int pyro_check_spins(pyro_ctx *ctx, double tol, long long *n_bad, long long *first_bad) {
+  if (!ctx || !n_bad) return set_error(PYRO_EINVAL, "pyro_check_spins: null argument");
+  if (!(tol >= 0.0)) return set_error(PYRO_EINVAL, "pyro_check_spins: tol %g < 0", tol);
+  PYRO_TRY(device_guard(ctx));
+  unsigned long long *d = nullptr;
+  const unsigned long long init[2] = {0ull, ~0ull};
+  PYRO_HIP(hipMalloc((void **)&d, sizeof init));
+  unsigned long long out[2] = {0ull, ~0ull};
+  int rc = 0;
+  if (hipMemcpyAsync(d, init, sizeof init, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    rc = set_error(PYRO_EHIP, "pyro_check_spins: upload failed");
+  if (!rc) rc = check_spins(ctx, tol, d);
+  if (!rc && (hipMemcpyAsync(out, d, sizeof out, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+              hipStreamSynchronize(ctx->stream) != hipSuccess))
+    rc = set_error(PYRO_EHIP, "pyro_check_spins: download failed");
+  (void)hipFree(d);
+  if (rc) return rc;
+  *n_bad = (long long)out[0];
+  if (first_bad) *first_bad = out[0] ? (long long)out[1] : -1;
+  return 0;
+}
+
 int pyro_set_post_rotate(pyro_ctx *ctx, int on) {
   if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
   if (on && (ctx->hetero || ctx->exec_mode != 0))

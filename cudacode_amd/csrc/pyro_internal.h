@@ -164,6 +164,7 @@ int launch_end_sweep(pyro_ctx *ctx);             // record row, advance sweep co
 int launch_measure_row(pyro_ctx *ctx);           // append a measurement row from d_obs
 int launch_phase_mat(pyro_ctx *ctx, int kind, int colour, int pass);  // matrix field model
 int reset_model_state(pyro_ctx *ctx);  // drop captured sweeps, re-anchor the energy
+int check_spins(pyro_ctx *ctx, double tol, unsigned long long *d_out);  // d_out: {count, first}
 int launch_observables_partials_mat(pyro_ctx *ctx);
 int launch_anneal_beta(pyro_ctx *ctx, double T0, int m, int steps, int restore);
 int launch_set_anchor(pyro_ctx *ctx);            // d_anchor = d_obs energy, d_dE = 0

@@ -164,6 +164,34 @@ __global__ void k_anneal_beta(int R, const double *__restrict__ temps, double T0
   beta[r] = (float)(1.0 / T);
 }
 
+// Spin guard: counts spins that are not finite or whose squared norm is off 1
+// by more than tol, and the smallest such global spin index (replica r's
+// spins are numbered from off[r] / 3). grid = (blocks, R), block = 256.
+__global__ __launch_bounds__(256) void k_check_spins(const float *__restrict__ spins,
+                                                     const long long *__restrict__ off,
+                                                     const int *__restrict__ dL, double tol,
+                                                     unsigned long long *__restrict__ out) {
+  const int r = blockIdx.y;
+  const long long L = dL[r];
+  const long long n = 16 * L * L * L;
+  const float *S = spins + off[r];
+  unsigned long long bad = 0, first = ~0ull;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float x = S[i], y = S[n + i], z = S[2 * n + i];
+    const double q = (double)x * x + (double)y * y + (double)z * z;
+    if (!(fabs(q - 1.0) <= tol)) {  // also catches NaN and infinities
+      ++bad;
+      const unsigned long long g = (unsigned long long)(off[r] / 3 + i);
+      first = first < g ? first : g;
+    }
+  }
+  if (bad) {
+    atomicAdd(&out[0], bad);
+    atomicMin(&out[1], first);
+  }
+}
+
 __global__ void k_set_anchor(int R, const double *__restrict__ obs, double *__restrict__ anchor,
                              double *__restrict__ dE) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -255,6 +283,17 @@ int launch_anneal_beta(pyro_ctx *ctx, double T0, int m, int steps, int restore) 
   k_anneal_beta<<<(ctx->R + 255) / 256, 256, 0, ctx->stream>>>(ctx->R, ctx->d_temps, T0, m,
                                                                1.0 / (double)steps, restore,
                                                                ctx->d_beta);
+  PYRO_HIP(hipGetLastError());
+  return 0;
+}
+
+int check_spins(pyro_ctx *ctx, double tol, unsigned long long *d_out) {
+  int Lmax = 0;
+  for (int L : ctx->h_L) Lmax = L > Lmax ? L : Lmax;
+  const long long n = 16ll * Lmax * Lmax * Lmax;
+  const int blocks = (int)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
+  k_check_spins<<<dim3(blocks, ctx->R), 256, 0, ctx->stream>>>(ctx->d_spins, ctx->d_off, ctx->d_L,
+                                                                 tol, d_out);
   PYRO_HIP(hipGetLastError());
   return 0;
 }

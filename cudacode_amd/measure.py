@@ -20,7 +20,9 @@ periodic file updates and optional Conf<rep>.dat snapshots.
 """
 from __future__ import annotations
 
+import json
 import os
+import time
 from dataclasses import dataclass
 
 import numpy as np
@@ -139,6 +141,20 @@ def write_all(out_dir: str, rep: int, s: Summary) -> dict:
     return paths
 
 
+def log_line(path: str, s: Summary, **extra) -> None:
+    """Append one JSON line to the run log: per-rung E/N, C/N, <|m_A2|^2>,
+    tau_int(E), plus any extra fields (sweeps, updates/s, acceptance)."""
+    rec = {
+        "time": time.time(), "n_measurements": s.n, "T": s.T.tolist(),
+        "E_per_spin": s.e[:, 0].tolist(), "E_err": s.e[:, 1].tolist(),
+        "C_per_spin": s.c[:, 0].tolist(), "C_err": s.c[:, 1].tolist(),
+        "m_A2_sq": s.m2[:, 0, 0].tolist(), "tau_int_E": s.tau_e.tolist(),
+    }
+    rec.update(extra)
+    with open(path, "a") as f:
+        f.write(json.dumps(rec, allow_nan=True) + "\n")
+
+
 def rung_temperatures(g) -> np.ndarray:
     """Rung temperatures in rung order (descending T; grouped by L for a batch)."""
     T, at = g.temperatures()
@@ -163,7 +179,9 @@ def run_protocol(g, out_dir: str, rep: int = 0, *, anneal_steps: int = 100,
     thermalisation sweeps, InitialConf<rep>.dat, then n_measures
     measurements sweeps_per_measure sweeps apart, writing the three result
     files every save_every measurements and at the end, and up to
-    store_confs Conf<rep>.dat records every confs_every measurements."""
+    store_confs Conf<rep>.dat records every confs_every measurements. Each
+    file update first runs the spin guard (an error if any spin is not
+    finite or off the unit sphere) and appends a line to run<rep>.jsonl."""
     os.makedirs(out_dir, exist_ok=True)
     g.SimulatedAnnealing(anneal_steps, anneal_sweeps, T_start)
     g.Mc(pt_rounds * max(g.pt_every, 1))
@@ -183,6 +201,10 @@ def run_protocol(g, out_dir: str, rep: int = 0, *, anneal_steps: int = 100,
     done, confs = 0, 0
     summary = None
     conf_prefix = os.path.join(out_dir, "Conf")
+    run_log = os.path.join(out_dir, f"run{rep}.jsonl")
+    spins = float(np.sum(N))
+    passes = g.n_metro + g.n_or
+    t0 = time.perf_counter()
     while done < n_measures:
         # Advance to the next file update or configuration snapshot.
         step = min(save_every - done % save_every, n_measures - done)
@@ -194,7 +216,16 @@ def run_protocol(g, out_dir: str, rep: int = 0, *, anneal_steps: int = 100,
             g.SaveConfigurations(conf_prefix, append=True)
             confs += 1
         if done % save_every == 0 or done == n_measures:
+            n_bad, first = g.check_spins(1e-4)
+            if n_bad:
+                raise RuntimeError(f"{n_bad} spins not finite or not unit (first index {first})")
             summary = summarise(g.measure_fetch(), T, N, n_bins)
             write_all(out_dir, rep, summary)
+            g.sync()
+            dt = time.perf_counter() - t0
+            acc, prop = g.acceptance()
+            log_line(run_log, summary, sweeps=done * sweeps_per_measure,
+                     spin_updates_per_s=done * sweeps_per_measure * passes * spins / dt,
+                     acceptance=(acc / np.maximum(prop, 1)).tolist())
     g.measure_stop()
     return summary

@@ -34,7 +34,7 @@ EXPORTS = [
     "pyro_energy_incremental",
     "pyro_acceptance", "pyro_reset_acceptance", "pyro_adapt_cone", "pyro_record_start",
     "pyro_record_fetch", "pyro_set_couplings", "pyro_get_couplings", "pyro_exchange_matrices",
-    "pyro_zeeman_fields", "pyro_set_post_rotate", "pyro_anneal", "pyro_measure_start", "pyro_measure_stop", "pyro_measure_fetch",
+    "pyro_zeeman_fields", "pyro_check_spins", "pyro_set_post_rotate", "pyro_anneal", "pyro_measure_start", "pyro_measure_stop", "pyro_measure_fetch",
     "pyro_set_passes", "pyro_set_kernel_options", "pyro_create_batch",
     "pyro_get_layout", "pyro_set_exec_mode", "pyro_get_exec_mode", "pyro_autotune",
     "pyro_checkpoint_save", "pyro_checkpoint_load", "pyro_export_configurations",
@@ -128,6 +128,8 @@ def lib():
         L.pyro_record_fetch.argtypes = [vp, vp, C.POINTER(C.c_int)]
         L.pyro_anneal.argtypes = [vp, C.c_double, C.c_int, C.c_int]
         L.pyro_set_post_rotate.argtypes = [vp, C.c_int]
+        L.pyro_check_spins.argtypes = [vp, C.c_double, C.POINTER(C.c_longlong),
+                                       C.POINTER(C.c_longlong)]
         L.pyro_set_couplings.argtypes = [vp, vp, vp]
         L.pyro_get_couplings.argtypes = [vp, C.POINTER(C.c_int), vp]
         L.pyro_exchange_matrices.argtypes = [C.c_double] * 4 + [vp]
@@ -409,6 +411,12 @@ class Pyrochlore:
         c = np.ascontiguousarray(C, np.float64).reshape(144)
         b = None if B_site is None else np.ascontiguousarray(B_site, np.float64).reshape(12)
         _check(lib().pyro_set_couplings(self._ctx, _p(c), None if b is None else _p(b)))
+
+    def check_spins(self, tol: float = 1e-5):
+        """(number of non-finite or non-unit spins, first such index or -1)."""
+        n, first = C.c_longlong(), C.c_longlong()
+        _check(lib().pyro_check_spins(self._ctx, float(tol), C.byref(n), C.byref(first)))
+        return n.value, first.value
 
     def set_post_rotate(self, on: bool) -> None:
         """Rotate each spin about its field after the Metropolis test (Math.cu:168-197)."""

@@ -245,6 +245,14 @@ int pyro_measure_fetch(pyro_ctx *ctx, double *rows, int *n_rows);
  * reference's --McSteps/--OverRelaxation options, Pyrochlore.cu:1161-1216). */
 int pyro_set_passes(pyro_ctx *ctx, int n_metro, int n_or);
 
+/* Spin guard (no reference counterpart; the reference checks nothing): counts
+ * the spins that are not finite or whose squared norm differs from 1 by more
+ * than tol (FP32 updates keep it within a few 1e-7), and returns the
+ * smallest such spin index (-1 if none). Spin i of replica r has index
+ * offset_r / 3 + i, offset_r being the replica's float offset in get_spins
+ * (pyro_get_layout; r * 48 L^3 for a uniform lattice). */
+int pyro_check_spins(pyro_ctx *ctx, double tol, long long *n_bad, long long *first_bad);
+
 /* Optional move after every Metropolis test, accepted or not: rotate the spin
  * about its local field by a Philox-drawn angle, uniform in [0, 2 pi)
  * (_CudaRotate3DVector, Math.cu:168-197, applied at Pyrochlore.cu:872; the

@@ -59,6 +59,7 @@ def lib():
                               C.c_int, C.POINTER(C.c_uint32), _f64p, _f32p, _f32p, _i32p,
                               C.c_float, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
                               C.POINTER(C.c_int)]
+        L.orc_set_update_order.argtypes = [C.c_void_p]
         L.orc_set_post_rotate.argtypes = [C.c_int]
         L.orc_rotate_about.argtypes = [_f32p, _f32p, C.c_float, _f32p]
         L.orc_set_couplings.argtypes = [C.c_void_p]
@@ -186,6 +187,21 @@ class OracleRun:
 
     def observables(self):
         return observables(self.spins, self.L, self.R, self.jeff)
+
+
+_order_keep = None
+
+
+def set_update_order(perm) -> None:
+    """Visit the updates of every colour phase in this order (cell * 4 + k), or
+    naturally for None."""
+    global _order_keep
+    if perm is None:
+        _order_keep = None
+        lib().orc_set_update_order(None)
+        return
+    _order_keep = np.ascontiguousarray(perm, np.int64).copy()
+    lib().orc_set_update_order(_order_keep.ctypes.data_as(C.c_void_p))
 
 
 def set_post_rotate(on: bool) -> None:

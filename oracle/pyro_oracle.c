@@ -301,6 +301,14 @@ void orc_rotate_about(const float h[3], const float s[3], float u, float out[3])
   renorm(out);
 }
 
+/* Order in which a colour phase visits its N/4 updates (cell * 4 + k, target
+ * site colour + 4 k): NULL = cell by cell. The updates of a phase are
+ * independent, so any order must give the same spins; tests use a shuffled
+ * order as the race check of the sublattice-parallel update. */
+static const int64_t *g_order = NULL;
+
+void orc_set_update_order(const int64_t *perm) { g_order = perm; }
+
 /* FP64 energy change of s -> sp at site basis b from the FP64 sums d. */
 static double move_de(float jeff, int b, const float s[3], const float sp[3], const double d[3]) {
   if (g_coup)
@@ -351,9 +359,12 @@ void orc_metropolis_phase(float *spins, int L, int R, int colour, uint32_t sweep
     uint64_t acc = 0;
     double de = 0.0;
 #pragma omp parallel for schedule(static) reduction(+ : acc, de)
-    for (long cell = 0; cell < nc; cell++) {
+    for (long t = 0; t < 4 * nc; t++) {
+      long upd = g_order ? (long)g_order[t] : t;
+      long cell = upd >> 2;
       int x = (int)(cell % L), y = (int)((cell / L) % L), z = (int)(cell / ((long)L * L));
-      for (int k = 0; k < 4; k++) {
+      {
+        int k = (int)(upd & 3);
         int site = colour + 4 * k;
         long i = (long)site * nc + cell;
         float s[3] = {Sx[i], Sy[i], Sz[i]};
@@ -400,9 +411,12 @@ void orc_overrelax_phase(float *spins, int L, int R, int colour, float jeff, dou
     float *Sx = S, *Sy = S + 16 * nc, *Sz = S + 32 * nc;
     double de = 0.0;
 #pragma omp parallel for schedule(static) reduction(+ : de)
-    for (long cell = 0; cell < nc; cell++) {
+    for (long t = 0; t < 4 * nc; t++) {
+      long upd = g_order ? (long)g_order[t] : t;
+      long cell = upd >> 2;
       int x = (int)(cell % L), y = (int)((cell / L) % L), z = (int)(cell / ((long)L * L));
-      for (int k = 0; k < 4; k++) {
+      {
+        int k = (int)(upd & 3);
         int site = colour + 4 * k;
         long i = (long)site * nc + cell;
         float s[3] = {Sx[i], Sy[i], Sz[i]};

@@ -73,6 +73,10 @@ void orc_set_couplings(const float *coup156);
 void orc_rotate_about(const float h[3], const float s[3], float u, float out[3]);
 void orc_set_post_rotate(int on);
 
+/* Visiting order of a colour phase's updates (cell * 4 + k; NULL = natural),
+ * for the order-independence check. The array must outlive its use. */
+void orc_set_update_order(const int64_t *perm);
+
 /* Full sweeps: n_metro Metropolis passes + n_or over-relaxation passes, each
  * over colours 0..3; after every pt_every sweeps (if > 0) a swap with parity
  * alternating from the stored round counter. sweep0 is the first sweep index;

@@ -141,3 +141,25 @@ def test_resume_keeps_the_energy_model(tmp_path):
         c.Mc(11)
         assert np.array_equal(c.get_spins(), ref)
         assert np.array_equal(c.TotalEnergyv2(), ref_E)
+
+
+def test_spin_guard_finds_bad_spins():
+    """The guard reports non-finite and non-unit spins, with the first index;
+    a state the kernels produced is clean."""
+    with ctx(L=4, temps=(0.3, 1.0), **KW) as g:
+        g.Mc(5)
+        assert g.check_spins(1e-5) == (0, -1)
+        s = g.get_spins()
+        N = 16 * 4 ** 3
+        bad = s.copy()
+        bad[1 * 3 * N + 17] = np.nan                       # replica 1, spin 17, x
+        for c in range(3):                                  # replica 0, spin 5: |s| = 1.01
+            bad[c * N + 5] *= np.float32(1.01)
+        g.set_spins(bad)
+        n, first = g.check_spins(1e-5)
+        assert (n, first) == (2, 5)
+        for c in range(3):
+            bad[c * N + 5] = s[c * N + 5]
+        g.set_spins(bad)
+        assert g.check_spins(1e-5) == (1, N + 17)
+        assert g.check_spins(np.inf)[0] == 1  # NaN fails any tolerance

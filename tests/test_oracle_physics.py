@@ -240,3 +240,24 @@ def test_post_rotation_keeps_the_boltzmann_law():
             oracle.set_post_rotate(False)
     (e0, s0), (e1, s1) = res
     assert abs(e0 - e1) < 4 * np.hypot(s0, s1), res
+
+
+def test_phase_order_independence():
+    """Race check of the sublattice-parallel update: the updates of a colour
+    phase visited in a shuffled order (and by several threads) give the same
+    spins and acceptance counts bit for bit, so no update reads a value
+    another update of the same phase writes."""
+    L, temps = 4, [0.2, 0.9]
+    kw = dict(n_metro=1, n_or=2, cos_cone=0.4)
+    a = oracle.OracleRun(L, temps, seed=9, **kw)
+    a.run(4)
+    rng = np.random.default_rng(2)
+    oracle.set_update_order(rng.permutation(4 * L ** 3))
+    try:
+        b = oracle.OracleRun(L, temps, seed=9, **kw)
+        b.run(4)
+    finally:
+        oracle.set_update_order(None)
+    assert np.array_equal(a.spins, b.spins)
+    assert np.array_equal(a.accepted, b.accepted)
+    assert np.allclose(a.dE, b.dE, rtol=1e-12, atol=1e-9)
