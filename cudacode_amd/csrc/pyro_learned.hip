@@ -171,6 +171,22 @@ __device__ __forceinline__ float rsq_f(float x) { return __builtin_amdgcn_rsqf(x
 __device__ __forceinline__ float exp_f(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504089f); }
 __device__ __forceinline__ float log_f(float x) { return __builtin_amdgcn_logf(x) * 0.693147180560f; }
 
+// expm1(-2 kappa) for kappa >= 0: Taylor to the 7th power below kappa = 1/4
+// (truncation < 3e-7 relative), exp(-2 kappa) - 1 above (no cancellation:
+// the result is below -0.39).
+__device__ __forceinline__ float expm1_neg2k(float kappa) {
+  const float x = -2.0f * kappa;
+  if (kappa > 0.25f) return exp_f(x) - 1.0f;
+  float p = 1.0f / 5040.0f;
+  p = p * x + 1.0f / 720.0f;
+  p = p * x + 1.0f / 120.0f;
+  p = p * x + 1.0f / 24.0f;
+  p = p * x + 1.0f / 6.0f;
+  p = p * x + 0.5f;
+  p = p * x + 1.0f;
+  return p * x;
+}
+
 __device__ __forceinline__ void features(f3 u, f3 s, float x[6]) {
   const float r2 = u.x * u.x + u.y * u.y + u.z * u.z;
   const float ir = rsq_f(r2);
@@ -198,8 +214,8 @@ __device__ __forceinline__ Vmf decode(const float o[4]) {
   v.kappa = exp_f(lk);
   // log of kappa / (2 pi (1 - exp(-2 kappa))); the exp(-kappa) factor of the
   // normaliser is carried by the -kappa |mu - y|^2 / 2 term of the density.
-  // expm1 keeps its full precision: for small kappa it is the normaliser.
-  v.em2k = expm1f(-2.0f * v.kappa);
+  // expm1 without cancellation: for small kappa it is the normaliser.
+  v.em2k = expm1_neg2k(v.kappa);
   v.log_norm = lk - 1.83787706641f - log_f(-v.em2k);
   return v;
 }
@@ -221,7 +237,8 @@ __device__ __forceinline__ f3 vmf_sample(const Vmf &v, float u1, float u2) {
   w = fminf(1.0f, fmaxf(-1.0f, w));
   const float st = sqrtf(fmaxf(0.0f, 1.0f - w * w));
   float ps, pc;
-  sincospi(2.0f * u2, &ps, &pc);
+  ps = __builtin_amdgcn_sinf(u2);  // hardware sin/cos take revolutions: sin(2 pi u2)
+  pc = __builtin_amdgcn_cosf(u2);
   const f3 m = v.mu;
   const bool up = m.z >= 0.0f;
   const f3 r = {st * pc, st * ps, up ? w : -w};
@@ -396,7 +413,7 @@ __global__ void k_vmf_draw(const float *__restrict__ rows, int n, uint32_t key0,
   Vmf v;
   v.mu = {q[0], q[1], q[2]};
   v.kappa = q[3];
-  v.em2k = expm1f(-2.0f * v.kappa);
+  v.em2k = expm1_neg2k(v.kappa);
   v.log_norm = 0.0f;
   const u32x4 rnd = philox4x32_10(rng_counter((uint32_t)idx, 0, 0, kKindLearned, 0, 0), key0, key1);
   const f3 y = vmf_sample(v, u01(rnd.x), u01(rnd.y));

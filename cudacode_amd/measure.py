@@ -179,17 +179,22 @@ def run_protocol(g, out_dir: str, rep: int = 0, *, anneal_steps: int = 100,
     thermalisation sweeps, InitialConf<rep>.dat, then n_measures
     measurements sweeps_per_measure sweeps apart, writing the three result
     files every save_every measurements and at the end, and up to
-    store_confs Conf<rep>.dat records every confs_every measurements. Each
+    store_confs Conf<rep>.dat records every confs_every measurements (both
+    files only for a single lattice size, as in the reference). Each
     file update first runs the spin guard (an error if any spin is not
     finite or off the unit sphere) and appends a line to run<rep>.jsonl."""
     os.makedirs(out_dir, exist_ok=True)
+    # The reference's configuration files hold one lattice size; a batch of
+    # mixed sizes keeps its configurations in checkpoints instead.
+    files = len(np.unique(np.atleast_1d(g.N))) == 1
     g.SimulatedAnnealing(anneal_steps, anneal_sweeps, T_start)
     g.Mc(pt_rounds * max(g.pt_every, 1))
     g.Mc(thermalisation)
     # Repetition rep appends its record (the reference passes rep as the
     # append flag, Pyrochlore.cu:1517-1522), so record rep of InitialConf<k>.dat
     # is this repetition's starting point (ReadConfigurations(..., rep)).
-    g.SaveConfigurations(os.path.join(out_dir, "InitialConf"), append=rep > 0)
+    if files:
+        g.SaveConfigurations(os.path.join(out_dir, "InitialConf"), append=rep > 0)
     T = rung_temperatures(g)
     N = rung_spins(g)
     # Measurements fall on sweeps whose index + 1 is a multiple of the
@@ -212,7 +217,7 @@ def run_protocol(g, out_dir: str, rep: int = 0, *, anneal_steps: int = 100,
             step = min(step, confs_every - done % confs_every)
         g.Mc(step * sweeps_per_measure)
         done += step
-        if store_confs and confs < store_confs and done % confs_every == 0:
+        if files and store_confs and confs < store_confs and done % confs_every == 0:
             g.SaveConfigurations(conf_prefix, append=True)
             confs += 1
         if done % save_every == 0 or done == n_measures:

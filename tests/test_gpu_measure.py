@@ -131,3 +131,34 @@ def test_protocol_writes_reference_files(tmp_path):
     assert -1.0 < e[-1, 1] < -0.6
     op = measure.read_table(str(tmp_path / "OrderParameters0.dat"))
     assert op.shape == (4, 21)
+
+
+def test_batch_rows_and_protocol(tmp_path):
+    """Mixed lattice sizes: measurement rows follow each rung's replica (its
+    own N for the order parameters), and the protocol runs on a batch
+    (configuration files are skipped: they hold one lattice size)."""
+    from cudacode_amd import measure, pyro
+    Ls = [4, 8, 4, 8]
+    temps = [0.3, 0.3, 1.2, 1.2]
+    with pyro.Batch(Ls, temps, seed=SEED, n_metro=1, n_or=1, pt_every=3, cos_cone=0.5) as g:
+        g.measure_start(3, 8)
+        g.Mc(6)
+        rows = g.measure_fetch()
+        assert rows.shape == (2, 4, pyro.MEAS_Q)
+        g.measure_stop()
+        g.set_passes(1, 1)
+        _, at = g.temperatures()
+        g.measure_start(1, 4)
+        g.Mc(1)  # sweep 7: no exchange after it, so rows see the current rungs
+        last = g.measure_fetch()[-1]
+        E = g.TotalEnergyv2()[at]
+        assert np.array_equal(last[:, 0], E)
+        M = g.Magnetization()[at] / np.asarray(g.N)[at][:, None]
+        assert np.allclose(last[:, 4], np.linalg.norm(M, axis=1), rtol=1e-12)
+        g.measure_stop()
+        s = measure.run_protocol(g, str(tmp_path), anneal_steps=2, anneal_sweeps=2, pt_rounds=1,
+                                 thermalisation=3, n_measures=8, sweeps_per_measure=3,
+                                 save_every=8, store_confs=1, confs_every=4, n_bins=4)
+    names = sorted(p.name for p in tmp_path.iterdir())
+    assert names == ["Energies0.dat", "Magnetization0.dat", "OrderParameters0.dat", "run0.jsonl"]
+    assert s.n == 8 and np.all(np.isfinite(s.e[:, 0]))
