@@ -152,11 +152,15 @@ def learned_leg(g, args, plain):
     leg = learned.timed_leg(g, args.steps, warmup=args.warmup, learned=True)
     log("learned: phase timing, 200 learned launches")
     learned_ms, _ = g.time_phase(2, 200)
+    log(f"learned: observables, {args.obs_sweeps} measured sweeps")
+    learned_obs = learned.observables_leg(g, args.obs_sweeps)
     learned.use_plain(g, prev)
     log(f"learned: plain control {args.steps} sweeps")
     control = learned.timed_leg(g, args.steps)
     n_spins = 16 * L ** 3
-    verdict = learned.judge(plain, leg, control, n_spins)
+    log(f"learned: plain observables, {args.obs_sweeps} measured sweeps")
+    plain_obs = learned.observables_leg(g, args.obs_sweeps)
+    verdict = learned.judge(plain, leg, control, n_spins, plain_obs, learned_obs)
     choice = learned.choose(g, verdict, prev)
     learned.use_plain(g, prev)
     # Dense work of one learned colour phase: two forward passes per proposal
@@ -210,14 +214,29 @@ def workload_c1(args):
     e = g.record_fetch()[:, 0] / 1024
     tau = stats.tau_int(e)[0]
     c, c_err = stats.specific_heat(e * 1024, 0.5, 1024)
+    cone = float(g.cos_cone()[0])
     g.close()
+    extra = {"exec_mode": {"mode": mode[0], "block": mode[1], "bins": mode[2]},
+             "autotune_ms_per_sweep": tune.tolist(),
+             "energy_per_spin": float(e.mean()), "energy_err": stats.binning_error(e),
+             "specific_heat": c, "specific_heat_err": c_err, "tau_int_sweeps": tau,
+             "indep_samples_per_s": stats.independent_samples_per_s(10000 / dt, tau)}
+    if not args.no_cpu:
+        # The whole workload on one host core (the phases of a 1,024-spin
+        # lattice are too small for threads to pay), with the adapted cone.
+        from oracle import oracle
+        oracle.set_num_threads(1)
+        run = oracle.OracleRun(4, [0.5], seed=SEED, n_metro=1, n_or=5, cos_cone=cone)
+        t0 = time.perf_counter()
+        run.run(10000)
+        cpu_dt = time.perf_counter() - t0
+        extra["cpu_baseline"] = {
+            "value": 10000 * 6 * 1024 / cpu_dt, "unit": "spin updates/s", "cores": 1,
+            "kind": "port", "sample": f"the full workload: 10^4 sweeps, 6.1e7 spin updates, "
+                                      f"{cpu_dt:.1f} s, C oracle on one thread"}
     return line(10000 * 6 * 1024 / dt, 10000, args.warmup, dt,
                 "L=4 (1,024 spins), 1 replica, T=0.5, 1 Metropolis + 5 over-relaxation, 10^4 sweeps",
-                {"exec_mode": {"mode": mode[0], "block": mode[1], "bins": mode[2]},
-                 "autotune_ms_per_sweep": tune.tolist(),
-                 "energy_per_spin": float(e.mean()), "energy_err": stats.binning_error(e),
-                 "specific_heat": c, "specific_heat_err": c_err, "tau_int_sweeps": tau,
-                 "indep_samples_per_s": stats.independent_samples_per_s(10000 / dt, tau)})
+                extra)
 
 
 def workload_c3(args):
@@ -319,6 +338,8 @@ def main():
                     help="c2 (default): the L=16 x 64 parallel-tempering line; others print "
                          "their own line")
     ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--obs-sweeps", type=int, default=2000,
+                    help="measured sweeps per observables leg (C/N, |m|^2 agreement and tau_int)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the replicas of "

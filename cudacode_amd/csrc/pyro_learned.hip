@@ -766,6 +766,25 @@ int pyro_mlp_set_weights(pyro_ctx *ctx, const float *w) {
   return 0;
 }
 
+int pyro_mlp_adam_step(pyro_ctx *ctx, const float *grad, float lr) {
+  if (!ctx || !grad || !(lr > 0.0f)) return set_error(PYRO_EINVAL, "pyro_mlp_adam_step: bad argument");
+  if (!ctx->d_mlp_w) return set_error(PYRO_ESTATE, "pyro_mlp_adam_step: no weights set");
+  for (int p = 0; p < kMlpParams; ++p)
+    if (!std::isfinite(grad[p])) return set_error(PYRO_EINVAL, "pyro_mlp_adam_step: grad[%d] not finite", p);
+  PYRO_HIP(hipSetDevice(ctx->device));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  PYRO_HIP(hipMemcpy(ctx->d_grad, grad, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const int t = ++ctx->adam_t;
+  k_adam<<<(kMlpParams + 255) / 256, 256, 0, ctx->stream>>>(
+      ctx->d_mlp_w, ctx->d_mlp_m, ctx->d_mlp_v, ctx->d_grad, lr, b1, b2, eps,
+      (float)(1.0 - std::pow((double)b1, t)), (float)(1.0 - std::pow((double)b2, t)), nullptr);
+  PYRO_HIP(hipGetLastError());
+  PYRO_TRY(mlp_pack(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
 int pyro_mlp_get_weights(pyro_ctx *ctx, float *w) {
   if (!ctx || !w) return set_error(PYRO_EINVAL, "pyro_mlp_get_weights: null argument");
   if (!ctx->d_mlp_w) return set_error(PYRO_ESTATE, "pyro_mlp_get_weights: no weights");

@@ -25,7 +25,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import pyro, stats
+from . import measure, pyro, stats
 
 
 def train(g, snapshots: int = 16, per_replica: int = 1024, sweeps_between: int = 5,
@@ -76,6 +76,22 @@ def timed_leg(g, sweeps: int, warmup: int = 0, learned: bool = False) -> Leg:
     return Leg(series, sps, taus, samples, float(a.sum() / max(p.sum(), 1)))
 
 
+def observables_leg(g, sweeps: int, n_bins: int = 20) -> measure.Summary:
+    """`sweeps` sweeps measured after every sweep (device rows): per-rung E/N,
+    C/N, order-parameter moduli with errors, and tau_int of E and |m_A2|^2 in
+    sweeps."""
+    g.measure_start(1, sweeps)
+    g.Mc(sweeps)
+    rows = g.measure_fetch()
+    g.measure_stop()
+    return measure.summarise(rows, measure.rung_temperatures(g), measure.rung_spins(g), n_bins)
+
+
+def zdiff(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """|a0 - b0| / hypot(a1, b1) for (value, error) rows."""
+    return np.abs(a[..., 0] - b[..., 0]) / np.maximum(np.hypot(a[..., 1], b[..., 1]), 1e-300)
+
+
 def zscores(a: np.ndarray, b: np.ndarray, n_spins, n_bins: int = 20) -> np.ndarray:
     """|mean difference| / combined binning error of the per-spin energy, per rung."""
     N = np.broadcast_to(np.asarray(n_spins, np.float64), (a.shape[1],))
@@ -101,8 +117,11 @@ def use_plain(g, prev: tuple) -> None:
     g.set_passes(*prev)
 
 
-def judge(plain: Leg, learned: Leg, control: Leg, n_spins) -> dict:
-    """Speed and agreement of a learned leg against the plain sampler."""
+def judge(plain: Leg, learned: Leg, control: Leg, n_spins, plain_obs=None,
+          learned_obs=None) -> dict:
+    """Speed and agreement of a learned leg against the plain sampler; with
+    observables legs, also the agreement of C/N and <|m_A2|^2> per rung and
+    tau_int per rung for E and |m_A2|^2."""
     ratio = learned.samples / plain.samples
     z = zscores(plain.series, learned.series, n_spins)
     z_ctl = zscores(plain.series, control.series, n_spins)
@@ -125,6 +144,23 @@ def judge(plain: Leg, learned: Leg, control: Leg, n_spins) -> dict:
         "learned_vs_control_max_z": float(z_lc.max()),
         "learned_vs_control_within_2sigma": int((z_lc < 2).sum()),
         "worst_rungs": [int(k) for k in np.argsort(-z)[:3]],
+        "tau_int_E_per_rung": [float(t) for t in learned.taus],
+        "plain_tau_int_E_per_rung": [float(t) for t in plain.taus],
+    } | (_obs_verdict(plain_obs, learned_obs) if plain_obs is not None else {})
+
+
+def _obs_verdict(p: measure.Summary, q: measure.Summary) -> dict:
+    zc = zdiff(p.c, q.c)
+    zm = zdiff(p.m2[:, 0], q.m2[:, 0])
+    return {
+        "specific_heat_max_z": float(np.nanmax(zc)),
+        "specific_heat_rungs_within_2sigma": int((zc < 2).sum()),
+        "m_A2_sq_max_z": float(np.nanmax(zm)),
+        "m_A2_sq_rungs_within_2sigma": int((zm < 2).sum()),
+        "tau_int_m_A2_sq_per_rung": [float(t) for t in q.tau_m2],
+        "plain_tau_int_m_A2_sq_per_rung": [float(t) for t in p.tau_m2],
+        "tau_int_m_A2_sq_median": float(np.nanmedian(q.tau_m2)),
+        "plain_tau_int_m_A2_sq_median": float(np.nanmedian(p.tau_m2)),
     }
 
 

@@ -42,7 +42,7 @@ EXPORTS = [
     "pyro_mlp_set_weights", "pyro_mlp_get_weights",
     "pyro_mlp_eval", "pyro_vmf_draw", "pyro_mlp_loss_grad", "pyro_gather_environments",
     "pyro_get_environments", "pyro_clear_environments", "pyro_mlp_train", "pyro_learned_phase",
-    "pyro_set_learned_passes", "pyro_acceptance_learned", "pyro_time_phase", "pyro_sync",
+    "pyro_set_learned_passes", "pyro_acceptance_learned", "pyro_mlp_adam_step", "pyro_time_phase", "pyro_sync",
 ]
 
 MLP_PARAMS = 4868
@@ -161,6 +161,7 @@ def lib():
         L.pyro_get_environments.argtypes = [vp, vp, C.POINTER(C.c_int)]
         L.pyro_clear_environments.argtypes = [vp]
         L.pyro_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_float, vp]
+        L.pyro_mlp_adam_step.argtypes = [vp, vp, C.c_float]
         L.pyro_learned_phase.argtypes = [vp, C.c_int, C.c_uint32, C.c_int]
         L.pyro_set_learned_passes.argtypes = [vp, C.c_int]
         L.pyro_acceptance_learned.argtypes = [vp, vp, vp]
@@ -542,6 +543,10 @@ class Pyrochlore:
 
     def clear_environments(self) -> None:
         _check(lib().pyro_clear_environments(self._ctx))
+
+    def mlp_adam_step(self, grad, lr: float) -> None:
+        g = np.ascontiguousarray(grad, np.float32)
+        _check(lib().pyro_mlp_adam_step(self._ctx, _p(g), float(lr)))
 
     def mlp_train(self, steps: int, batch: int = 4096, lr: float = 3e-3,
                   lr_final: float = 3e-4) -> np.ndarray:

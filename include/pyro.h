@@ -291,6 +291,10 @@ int pyro_gather_environments(pyro_ctx *ctx, int per_replica, uint32_t draw, int 
                              int *n_total);
 int pyro_get_environments(pyro_ctx *ctx, float *env6, int *n);
 int pyro_clear_environments(pyro_ctx *ctx);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected; moments
+ * reset by pyro_mlp_set_weights) with a caller's gradient, e.g. from
+ * pyro_mlp_loss_grad; the MFMA weight images are repacked. */
+int pyro_mlp_adam_step(pyro_ctx *ctx, const float *grad, float lr);
 /* Adam on minibatches drawn from the environment set; learning rate decays
  * geometrically from lr to lr_final. loss_hist (optional) receives the mean
  * batch loss of each step. */

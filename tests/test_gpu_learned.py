@@ -234,3 +234,27 @@ def test_trained_network_gains_over_plain_metropolis():
     say(f"overlap plain {q_plain} learned {q_learned} acceptance {a.sum() / p.sum():.3f}")
     assert a.sum() > 0.8 * p.sum()
     assert np.all(q_learned < q_plain - 0.02)
+
+
+def test_adam_step_closed_form_and_descent():
+    """With a constant gradient g every bias-corrected Adam step moves each
+    weight by -lr g / (|g| + eps); with the loss gradient of a batch, steps
+    lower that batch's loss."""
+    w0 = random_weights(7).astype(np.float32)
+    g = rng.normal(size=w0.size).astype(np.float32)
+    lr = 1e-3
+    with ctx() as c:
+        c.mlp_set_weights(w0)
+        for _ in range(3):
+            c.mlp_adam_step(g, lr)
+        w3 = c.mlp_get_weights()
+        expect = w0.astype(np.float64) - 3 * lr * g / (np.abs(g) + 1e-8)
+        assert np.allclose(w3, expect, rtol=1e-5, atol=1e-6)
+        c.mlp_set_weights(w0)
+        e = env_rows(1024)
+        loss0, grad = c.mlp_loss_grad(e)
+        for _ in range(20):
+            _, grad = c.mlp_loss_grad(e)
+            c.mlp_adam_step(grad, 3e-3)
+        loss1, _ = c.mlp_loss_grad(e)
+        assert loss1 < 0.8 * loss0
