@@ -30,6 +30,8 @@ enum CtlWord : int {
 constexpr int kObs = 13;  // energy + 4 sublattices x 3 components
 constexpr int kObsPartials = 16;
 constexpr int kCoupFloats = 156;  // matrix field model: 16 x 9 exchange + 4 x 3 Zeeman
+// Workgroups each replica half must span before a sweep's phases go to two streams.
+constexpr long long kSplitMinBlocks = 256;
 // Measurement row, per rung: E, |m_A2|, |m_E|, |m_T1|, |m_F| (= |M| / N), |m_T2|.
 constexpr int kMeasQ = 6;  // per observables block: 4 bases x (energy, mx, my, mz)
 constexpr int kPhaseBlock = 256;   // 4 waves: one per site of the colour
@@ -131,6 +133,15 @@ struct pyro_ctx {
   double spin_len2 = 1.0;
   bool mat = false;
   bool post_rotate = false;  // rotate about h after each Metropolis test
+
+  // Sweeps split into two replica halves on two streams (overlaps one
+  // kernel's tail with the next one's start); the launch range below is set
+  // by the sweep driver around each half and is empty otherwise.
+  int split = 0;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t launch_stream = nullptr;
+  int launch_r0 = 0, launch_nrep = 0;
   float *d_coup = nullptr;
   float h_coup[pyro::kCoupFloats] = {};
 

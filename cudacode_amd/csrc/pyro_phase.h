@@ -23,6 +23,14 @@
 
 namespace pyro {
 
+// The replicas and stream of the next phase launch: all replicas on the ctx
+// stream, unless a sweep split into replica halves on two streams set a range.
+static inline hipStream_t launch_stream(const pyro_ctx *c) {
+  return c->launch_stream ? c->launch_stream : c->stream;
+}
+static inline int launch_r0(const pyro_ctx *c) { return c->launch_nrep ? c->launch_r0 : 0; }
+static inline int launch_nrep(const pyro_ctx *c) { return c->launch_nrep ? c->launch_nrep : c->R; }
+
 static inline uint32_t key_lo(const pyro_ctx *c) { return (uint32_t)c->seed; }
 static inline uint32_t key_hi(const pyro_ctx *c) { return (uint32_t)(c->seed >> 32); }
 
@@ -50,10 +58,10 @@ __global__ __launch_bounds__(kPhaseBlock) void k_phase(
     const float *__restrict__ beta, const float *__restrict__ cosc,
     const uint32_t *__restrict__ ctl, uint32_t key0, uint32_t key1, float jeff,
     unsigned long long *__restrict__ acc, double *__restrict__ dE, const float *__restrict__ coup,
-    int post_rotate) {
+    int post_rotate, int r0) {
   const int L = LT > 0 ? LT : L_rt;
   const int ncell = LT > 0 ? LT * LT * LT : ncell_rt;
-  const int r = blockIdx.y;
+  const int r = r0 + (int)blockIdx.y;
   const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int cell = blockIdx.x * kPhaseCells + lane;
@@ -150,13 +158,14 @@ __global__ __launch_bounds__(256) void k_phase_cells(
     float *__restrict__ spins, int blocks_per_replica, int pass, const float *__restrict__ beta,
     const float *__restrict__ cosc, const uint32_t *__restrict__ ctl, uint32_t key0,
     uint32_t key1, float jeff, unsigned long long *__restrict__ acc, double *__restrict__ dE,
-    const float *__restrict__ coup, int post_rotate) {
+    const float *__restrict__ coup, int post_rotate, int r0) {
   constexpr int NC = LT * LT * LT;
   const int nb = gridDim.x;
   const int b = blockIdx.x;
   const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
-  const int r = lb / blocks_per_replica;
-  const int cell = (lb - r * blocks_per_replica) * 256 + threadIdx.x;
+  const int q = lb / blocks_per_replica;  // replica within the launch range
+  const int r = r0 + q;
+  const int cell = (lb - q * blocks_per_replica) * 256 + threadIdx.x;
   double de = 0.0;
   unsigned nacc = 0;
   if (cell < NC)
@@ -260,35 +269,38 @@ __global__ __launch_bounds__(kObsBlock) void k_observables(const float *__restri
 
 template <int KIND, int LT, bool MAT>
 static void launch_phase_t(pyro_ctx *ctx, int colour, int pass) {
-  dim3 grid((ctx->ncell + kPhaseCells - 1) / kPhaseCells, ctx->R);
+  dim3 grid((ctx->ncell + kPhaseCells - 1) / kPhaseCells, launch_nrep(ctx));
   if (ctx->time_ev0) {
-    hipExtLaunchKernelGGL(k_phase<KIND, LT, MAT>, grid, dim3(kPhaseBlock), 0, ctx->stream,
+    hipExtLaunchKernelGGL(k_phase<KIND, LT, MAT>, grid, dim3(kPhaseBlock), 0, launch_stream(ctx),
                           ctx->time_ev0, ctx->time_ev1, 0, ctx->d_spins, ctx->L, ctx->ncell, colour,
                           pass, (const float *)ctx->d_beta, (const float *)ctx->d_cosc,
                           (const uint32_t *)ctx->d_ctl, key_lo(ctx), key_hi(ctx), ctx->jeff,
-                          ctx->d_acc, ctx->d_dE, (const float *)ctx->d_coup, (int)ctx->post_rotate);
+                          ctx->d_acc, ctx->d_dE, (const float *)ctx->d_coup, (int)ctx->post_rotate,
+                          launch_r0(ctx));
     return;
   }
-  k_phase<KIND, LT, MAT><<<grid, kPhaseBlock, 0, ctx->stream>>>(
+  k_phase<KIND, LT, MAT><<<grid, kPhaseBlock, 0, launch_stream(ctx)>>>(
       ctx->d_spins, ctx->L, ctx->ncell, colour, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl,
       key_lo(ctx), key_hi(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE, ctx->d_coup,
-      (int)ctx->post_rotate);
+      (int)ctx->post_rotate, launch_r0(ctx));
 }
 
 template <int KIND, int COLOUR, int LT, bool TRACK, bool MAT>
 static void launch_cells_t(pyro_ctx *ctx, int pass) {
   const int bpr = (LT * LT * LT + 255) / 256;
+  const int nrep = launch_nrep(ctx);
   if (ctx->time_ev0) {
-    hipExtLaunchKernelGGL(k_phase_cells<KIND, COLOUR, LT, TRACK, MAT>, dim3(bpr * ctx->R),
-                          dim3(256), 0, ctx->stream, ctx->time_ev0, ctx->time_ev1, 0, ctx->d_spins,
-                          bpr, pass, (const float *)ctx->d_beta, (const float *)ctx->d_cosc,
-                          (const uint32_t *)ctx->d_ctl, key_lo(ctx), key_hi(ctx), ctx->jeff,
-                          ctx->d_acc, ctx->d_dE, (const float *)ctx->d_coup, (int)ctx->post_rotate);
+    hipExtLaunchKernelGGL(k_phase_cells<KIND, COLOUR, LT, TRACK, MAT>, dim3(bpr * nrep),
+                          dim3(256), 0, launch_stream(ctx), ctx->time_ev0, ctx->time_ev1, 0,
+                          ctx->d_spins, bpr, pass, (const float *)ctx->d_beta,
+                          (const float *)ctx->d_cosc, (const uint32_t *)ctx->d_ctl, key_lo(ctx),
+                          key_hi(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE,
+                          (const float *)ctx->d_coup, (int)ctx->post_rotate, launch_r0(ctx));
     return;
   }
-  k_phase_cells<KIND, COLOUR, LT, TRACK, MAT><<<bpr * ctx->R, 256, 0, ctx->stream>>>(
+  k_phase_cells<KIND, COLOUR, LT, TRACK, MAT><<<bpr * nrep, 256, 0, launch_stream(ctx)>>>(
       ctx->d_spins, bpr, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl, key_lo(ctx), key_hi(ctx),
-      ctx->jeff, ctx->d_acc, ctx->d_dE, ctx->d_coup, (int)ctx->post_rotate);
+      ctx->jeff, ctx->d_acc, ctx->d_dE, ctx->d_coup, (int)ctx->post_rotate, launch_r0(ctx));
 }
 
 template <int KIND, int LT, bool MAT>

@@ -34,7 +34,7 @@ EXPORTS = [
     "pyro_energy_incremental",
     "pyro_acceptance", "pyro_reset_acceptance", "pyro_adapt_cone", "pyro_record_start",
     "pyro_record_fetch", "pyro_set_couplings", "pyro_get_couplings", "pyro_exchange_matrices",
-    "pyro_zeeman_fields", "pyro_check_spins", "pyro_set_post_rotate", "pyro_anneal", "pyro_measure_start", "pyro_measure_stop", "pyro_measure_fetch",
+    "pyro_zeeman_fields", "pyro_set_streams", "pyro_get_streams", "pyro_check_spins", "pyro_set_post_rotate", "pyro_anneal", "pyro_measure_start", "pyro_measure_stop", "pyro_measure_fetch",
     "pyro_set_passes", "pyro_set_kernel_options", "pyro_create_batch",
     "pyro_get_layout", "pyro_set_exec_mode", "pyro_get_exec_mode", "pyro_autotune",
     "pyro_checkpoint_save", "pyro_checkpoint_load", "pyro_export_configurations",
@@ -128,6 +128,8 @@ def lib():
         L.pyro_record_fetch.argtypes = [vp, vp, C.POINTER(C.c_int)]
         L.pyro_anneal.argtypes = [vp, C.c_double, C.c_int, C.c_int]
         L.pyro_set_post_rotate.argtypes = [vp, C.c_int]
+        L.pyro_set_streams.argtypes = [vp, C.c_int]
+        L.pyro_get_streams.argtypes = [vp, C.POINTER(C.c_int)]
         L.pyro_check_spins.argtypes = [vp, C.c_double, C.POINTER(C.c_longlong),
                                        C.POINTER(C.c_longlong)]
         L.pyro_set_couplings.argtypes = [vp, vp, vp]
@@ -413,6 +415,15 @@ class Pyrochlore:
         b = None if B_site is None else np.ascontiguousarray(B_site, np.float64).reshape(12)
         _check(lib().pyro_set_couplings(self._ctx, _p(c), None if b is None else _p(b)))
 
+    def set_streams(self, n: int) -> None:
+        """1 or 2 streams for the phases of a sweep (identical results)."""
+        _check(lib().pyro_set_streams(self._ctx, int(n)))
+
+    def streams(self) -> int:
+        n = C.c_int()
+        _check(lib().pyro_get_streams(self._ctx, C.byref(n)))
+        return n.value
+
     def check_spins(self, tol: float = 1e-5):
         """(number of non-finite or non-unit spins, first such index or -1)."""
         n, first = C.c_longlong(), C.c_longlong()
@@ -464,11 +475,12 @@ class Pyrochlore:
         return m.value, b.value, n.value
 
     def autotune(self, sweeps: int = 5) -> np.ndarray:
-        """Try every execution, keep the fastest; rows (mode, block, variant, ms/sweep)."""
-        res = np.zeros(4 * 16, np.float64)
+        """Try every execution, keep the fastest; rows (mode, block, variant,
+        streams, ms/sweep)."""
+        res = np.zeros(5 * 16, np.float64)
         n = C.c_int()
         _check(lib().pyro_autotune(self._ctx, sweeps, _p(res), C.byref(n)))
-        return res[: 4 * n.value].reshape(n.value, 4)
+        return res[: 5 * n.value].reshape(n.value, 5)
 
     # -- checkpoint / configuration files ----------------------------------
     def save_checkpoint(self, path: str) -> None:

@@ -2,6 +2,7 @@
 kept under profiles/.
 
   python -m cudacode_amd.rocprof_summary trace DB [N]        per-kernel time table (top N)
+  python -m cudacode_amd.rocprof_summary grid DB [N]         per (kernel, grid size) time table
   python -m cudacode_amd.rocprof_summary pmc DB [DB ...]      mean counter value per kernel
 
 Counter passes are scaled by a copy kernel of known size run under the same
@@ -24,6 +25,22 @@ def trace(db: str, limit: int = 12) -> str:
     return "\n".join(out)
 
 
+def grid(db: str, limit: int = 16) -> str:
+    """Per (kernel, grid) dispatch statistics: the same kernel launched over
+    different replica ranges (e.g. a sweep's two replica halves and a timed
+    full-range launch) gets one row per grid size. Durations in us."""
+    c = sqlite3.connect(db)
+    rows = c.execute("select name, grid_x, grid_y, grid_z, workgroup_x, count(*), sum(duration), "
+                     "avg(duration), min(duration) from kernels group by name, grid_x, grid_y, "
+                     "grid_z order by sum(duration) desc limit ?", (limit,)).fetchall()
+    out = ["| kernel | grid (threads) | block | calls | total us | avg us | min us |",
+           "|---|---|---|---|---|---|---|"]
+    for n, gx, gy, gz, wx, calls, tot, avg, mn in rows:
+        out.append(f"| `{short(n)}` | {gx}x{gy}x{gz} | {wx} | {calls} | {tot / 1e3:.1f} | "
+                   f"{avg / 1e3:.3f} | {mn / 1e3:.3f} |")
+    return "\n".join(out)
+
+
 def pmc(db: str) -> dict:
     c = sqlite3.connect(db)
     res = {}
@@ -41,6 +58,8 @@ def short(name: str) -> str:
 def main(argv):
     if len(argv) >= 2 and argv[0] == "trace":
         print(trace(argv[1], int(argv[2]) if len(argv) > 2 else 12))
+    elif len(argv) >= 2 and argv[0] == "grid":
+        print(grid(argv[1], int(argv[2]) if len(argv) > 2 else 16))
     elif len(argv) >= 2 and argv[0] == "pmc":
         for db in argv[1:]:
             for (k, cnt), (n, avg) in sorted(pmc(db).items(), key=lambda t: -t[1][1])[:8]:

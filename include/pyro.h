@@ -97,10 +97,11 @@ int pyro_set_exec_mode(pyro_ctx *ctx, int mode, int block, int n_bins);
 int pyro_get_exec_mode(pyro_ctx *ctx, int *mode, int *block, int *n_bins);
 
 /* Autotuner: time `sweeps` sweeps of each candidate execution (per-phase
- * cell- and site-per-thread kernels, persistent kernel at 256/512/1024
- * threads, tile list), captured in one graph as pyro_sweep replays them,
- * restoring the state after each trial, and keep the fastest. results (optional, 4 doubles per candidate): mode, block,
- * phase variant, ms per sweep. */
+ * cell- and site-per-thread kernels on one or two streams, persistent kernel
+ * at 256/512/1024 threads, tile list), captured in one graph as pyro_sweep
+ * replays them, restoring the state after each trial, and keep the fastest.
+ * results (optional, 5 doubles per candidate, at most 8 candidates): mode,
+ * block, phase variant, streams, ms per sweep. */
 int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results);
 
 /* Lattice tables: links16 (Pyrochlore.cu:152-167) and IndependentSubLattices
@@ -244,6 +245,16 @@ int pyro_measure_fetch(pyro_ctx *ctx, double *rows, int *n_rows);
 /* Change the per-sweep schedule (Metropolis and over-relaxation passes; the
  * reference's --McSteps/--OverRelaxation options, Pyrochlore.cu:1161-1216). */
 int pyro_set_passes(pyro_ctx *ctx, int n_metro, int n_or);
+
+/* Sweeps of a uniform lattice with >= 2 replicas can run the phases of two
+ * replica halves on two streams (n = 2), joined every sweep, or on one
+ * stream (n = 1). Results are identical either way (no reference
+ * counterpart: a performance setting; two streams hide part of each short
+ * phase kernel's ramp). Default: 2 when each half spans >= 256 workgroups of
+ * 256 cells, else 1; pyro_autotune also tries both. Learned passes always
+ * run on one stream. */
+int pyro_set_streams(pyro_ctx *ctx, int n);
+int pyro_get_streams(pyro_ctx *ctx, int *n);
 
 /* Spin guard (no reference counterpart; the reference checks nothing): counts
  * the spins that are not finite or whose squared norm differs from 1 by more

@@ -130,8 +130,8 @@ def test_autotune_keeps_the_state():
         h.Mc(4)
         before = g.get_spins()
         res = g.autotune(3)
-        assert len(res) == 4 and np.all(res[:, 3] > 0)
-        say(f"autotune results (mode, block, variant, ms/sweep): {res.tolist()}")
+        assert len(res) == 4 and np.all(res[:, 3] == 1) and np.all(res[:, 4] > 0)
+        say(f"autotune results (mode, block, variant, streams, ms/sweep): {res.tolist()}")
         assert np.array_equal(g.get_spins(), before)
         assert g.sweep_counters() == h.sweep_counters()
         g.Mc(6)

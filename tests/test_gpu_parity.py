@@ -328,3 +328,111 @@ def test_post_rotation_matches_oracle(variant):
         oracle.set_post_rotate(False)
     assert np.array_equal(spins, o.spins)
     assert np.array_equal(acc, o.accepted)
+
+
+@pytest.mark.parametrize("variant,R,graph", [(0, 5, True), (1, 4, False), (0, 2, False)])
+def test_two_stream_sweeps_identical(variant, R, graph):
+    """Sweeps whose replica halves run on two streams equal one-stream sweeps
+    bit for bit (spins, rungs, acceptance, incremental energies), captured or
+    eager, odd replica counts included, and equal the oracle."""
+    L = 8
+    temps = list(np.geomspace(0.05, 2.0, R))
+    kw = dict(n_metro=1, n_or=3, pt_every=3, cos_cone=0.4)
+    with gpu(L, temps, variant=variant, **kw) as a, gpu(L, temps, variant=variant, **kw) as b:
+        a.set_streams(2)
+        b.set_streams(1)
+        a.Mc(12, use_graph=graph)
+        b.Mc(12, use_graph=graph)
+        assert np.array_equal(a.get_spins(), b.get_spins())
+        assert np.array_equal(a.temperatures()[1], b.temperatures()[1])
+        assert np.array_equal(a.acceptance()[0], b.acceptance()[0])
+        assert np.array_equal(a.energy_incremental(), b.energy_incremental())
+        spins = a.get_spins()
+    o = orc(L, temps, **kw)
+    o.run(12)
+    assert np.array_equal(spins, o.spins)
+
+
+def test_two_stream_sweeps_identical_matrix_model():
+    """The same for the anisotropic exchange model in a field."""
+    from cudacode_amd import pyro
+    L, temps = 4, [0.1, 0.4, 1.6]
+    C = pyro.exchange_matrices(*pyro.YB2TI2O7_J)
+    B = pyro.zeeman_fields([0.0, 0.0, 0.5])
+    kw = dict(n_metro=1, n_or=2, pt_every=2, cos_cone=0.4)
+    with gpu(L, temps, **kw) as a, gpu(L, temps, **kw) as b:
+        for g in (a, b):
+            g.set_couplings(C, B)
+        b.set_streams(1)
+        a.Mc(8)
+        b.Mc(8)
+        assert np.array_equal(a.get_spins(), b.get_spins())
+        assert np.array_equal(a.energy_incremental(), b.energy_incremental())
+
+
+def test_stream_default_and_autotune_on_uniform_lattice():
+    """Two streams by default only when each replica half spans >= 256
+    workgroups; the autotuner tries both on a uniform lattice, keeps the
+    state, and the sweeps after it equal an untuned ctx's."""
+    with gpu(4, [0.2, 0.5, 1.0, 2.0]) as small:
+        assert small.streams() == 1
+    temps = list(np.geomspace(0.05, 1.0, 32))
+    kw = dict(n_metro=1, n_or=2, pt_every=2, cos_cone=0.4)
+    with gpu(16, temps, **kw) as g, gpu(16, temps, **kw) as h:
+        assert g.streams() == 2  # 16 replicas x 16 workgroups per half
+        g.Mc(2)
+        h.Mc(2)
+        res = g.autotune(2)
+        say(f"autotune (mode, block, variant, streams, ms/sweep): {res.tolist()}")
+        assert {1.0, 2.0} <= set(res[res[:, 0] == 0, 3].tolist())
+        assert np.all(res[:, 4] > 0)
+        assert np.array_equal(g.get_spins(), h.get_spins())
+        g.Mc(4)
+        h.Mc(4)
+        assert np.array_equal(g.get_spins(), h.get_spins())
+        assert np.array_equal(g.energy_incremental(), h.energy_incremental())
+
+
+def test_full_size_c2_spins_agree_with_oracle():
+    """The L=16 x 64 parallel-tempering workload at its full size (T 0.01..1,
+    1 Metropolis + 10 over-relaxation, swaps every 10 sweeps, the fast
+    untracked kernels on two streams): after 100 sweeps the spins, the rungs
+    and the acceptance counts equal the oracle's, bit for bit."""
+    from cudacode_amd.pyro import geometric_ladder
+    temps = geometric_ladder(0.01, 1.0, 64)
+    kw = dict(n_metro=1, n_or=10, pt_every=10, cos_cone=0.5)
+    with gpu(16, temps, track=False, **kw) as g:
+        assert g.streams() == 2
+        g.Mc(100)
+        spins, at, acc = g.get_spins(), g.temperatures()[1], g.acceptance()[0]
+        assert g.sweep_counters() == (100, 10)
+    say("full c2 100 sweeps on the oracle")
+    o = orc(16, temps, **kw)
+    o.run(100)
+    assert np.array_equal(at, o.replica_at)
+    assert np.array_equal(acc, o.accepted)
+    assert np.array_equal(spins, o.spins)
+
+
+def test_full_size_large_lattice_replicas_agree_with_oracle():
+    """The L=64 x 32 ensemble at its full size (1.61 GB of spins): replicas 0
+    and 31 after 5 sweeps of 1 Metropolis + 10 over-relaxation equal the
+    oracle's replay of each replica alone (its own Philox counters), bit for
+    bit."""
+    from cudacode_amd.pyro import geometric_ladder
+    temps = geometric_ladder(0.02, 0.5, 32)
+    kw = dict(n_metro=1, n_or=10, pt_every=0, cos_cone=0.5)
+    with gpu(64, temps, track=False, **kw) as g:
+        g.Mc(5)
+        s = g.get_spins().reshape(32, -1)
+        picks = {r: s[r].copy() for r in (0, 31)}
+        del s
+    for r, sr in picks.items():
+        say(f"L=64 replica {r}: 5 sweeps on the oracle")
+        oracle.set_replica_base(r)
+        try:
+            o = orc(64, [temps[r]], **kw)
+            o.run(5)
+        finally:
+            oracle.set_replica_base(0)
+        assert np.array_equal(sr, o.spins.reshape(-1))
