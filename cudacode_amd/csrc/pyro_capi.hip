@@ -66,8 +66,7 @@ static int enqueue_phases(pyro_ctx *ctx) {
 }
 
 static bool split_sweep(const pyro_ctx *ctx) {
-  return ctx->split && ctx->stream2 && ctx->exec_mode == 0 && !ctx->hetero && ctx->n_learned == 0 &&
-         ctx->R >= 2;
+  return ctx->split && ctx->stream2 && ctx->exec_mode == 0 && !ctx->hetero && ctx->R >= 2;
 }
 
 // One sweep as kernel launches (also the body that gets captured). Replicas

@@ -163,6 +163,14 @@ struct pyro_ctx {
 
 namespace pyro {
 
+// The replicas and stream of the next phase launch: all replicas on the ctx
+// stream, unless a sweep split into replica halves on two streams set a range.
+static inline hipStream_t launch_stream(const pyro_ctx *c) {
+  return c->launch_stream ? c->launch_stream : c->stream;
+}
+static inline int launch_r0(const pyro_ctx *c) { return c->launch_nrep ? c->launch_r0 : 0; }
+static inline int launch_nrep(const pyro_ctx *c) { return c->launch_nrep ? c->launch_nrep : c->R; }
+
 int set_error(int code, const char *fmt, ...);
 
 // Kernel launchers (pyro_kernels.hip). All enqueue on ctx->stream.

@@ -279,10 +279,10 @@ __global__ __launch_bounds__(kPhaseBlock, 3) void k_learned_phase(
     float *__restrict__ spins, int L_rt, int ncell_rt, int colour, int pass,
     const float *__restrict__ beta, const uint32_t *__restrict__ ctl, uint32_t key0,
     uint32_t key1, float jeff, const uint16_t *__restrict__ pack, const float *__restrict__ w,
-    unsigned long long *__restrict__ acc, double *__restrict__ dE) {
+    unsigned long long *__restrict__ acc, double *__restrict__ dE, int r0) {
   const int L = LT > 0 ? LT : L_rt;
   const int ncell = LT > 0 ? LT * LT * LT : ncell_rt;
-  const int r = blockIdx.y;
+  const int r = r0 + (int)blockIdx.y;
   const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int site = colour + 4 * k;
@@ -716,18 +716,19 @@ constexpr int kLearnedGroups = 1;
 template <int LT>
 static void launch_learned_t(pyro_ctx *ctx, int colour, int pass) {
   constexpr int G = kLearnedGroups;
-  dim3 grid((ctx->ncell + kPhaseCells * G - 1) / (kPhaseCells * G), ctx->R);
+  dim3 grid((ctx->ncell + kPhaseCells * G - 1) / (kPhaseCells * G), launch_nrep(ctx));
   if (ctx->time_ev0) {
-    hipExtLaunchKernelGGL(k_learned_phase<LT, G>, grid, dim3(kPhaseBlock), 0, ctx->stream,
+    hipExtLaunchKernelGGL(k_learned_phase<LT, G>, grid, dim3(kPhaseBlock), 0, launch_stream(ctx),
                           ctx->time_ev0, ctx->time_ev1, 0, ctx->d_spins, ctx->L, ctx->ncell, colour,
                           pass, (const float *)ctx->d_beta, (const uint32_t *)ctx->d_ctl, klo(ctx),
                           khi(ctx), ctx->jeff, (const uint16_t *)ctx->d_mlp_pack,
-                          (const float *)ctx->d_mlp_w, ctx->d_acc_learned, ctx->d_dE);
+                          (const float *)ctx->d_mlp_w, ctx->d_acc_learned, ctx->d_dE,
+                          launch_r0(ctx));
     return;
   }
-  k_learned_phase<LT, G><<<grid, kPhaseBlock, 0, ctx->stream>>>(
+  k_learned_phase<LT, G><<<grid, kPhaseBlock, 0, launch_stream(ctx)>>>(
       ctx->d_spins, ctx->L, ctx->ncell, colour, pass, ctx->d_beta, ctx->d_ctl, klo(ctx), khi(ctx),
-      ctx->jeff, ctx->d_mlp_pack, ctx->d_mlp_w, ctx->d_acc_learned, ctx->d_dE);
+      ctx->jeff, ctx->d_mlp_pack, ctx->d_mlp_w, ctx->d_acc_learned, ctx->d_dE, launch_r0(ctx));
 }
 
 int launch_learned_phase(pyro_ctx *ctx, int colour, int pass) {

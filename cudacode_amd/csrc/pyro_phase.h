@@ -23,14 +23,6 @@
 
 namespace pyro {
 
-// The replicas and stream of the next phase launch: all replicas on the ctx
-// stream, unless a sweep split into replica halves on two streams set a range.
-static inline hipStream_t launch_stream(const pyro_ctx *c) {
-  return c->launch_stream ? c->launch_stream : c->stream;
-}
-static inline int launch_r0(const pyro_ctx *c) { return c->launch_nrep ? c->launch_r0 : 0; }
-static inline int launch_nrep(const pyro_ctx *c) { return c->launch_nrep ? c->launch_nrep : c->R; }
-
 static inline uint32_t key_lo(const pyro_ctx *c) { return (uint32_t)c->seed; }
 static inline uint32_t key_hi(const pyro_ctx *c) { return (uint32_t)(c->seed >> 32); }
 

@@ -251,8 +251,8 @@ int pyro_set_passes(pyro_ctx *ctx, int n_metro, int n_or);
  * stream (n = 1). Results are identical either way (no reference
  * counterpart: a performance setting; two streams hide part of each short
  * phase kernel's ramp). Default: 2 when each half spans >= 256 workgroups of
- * 256 cells, else 1; pyro_autotune also tries both. Learned passes always
- * run on one stream. */
+ * 256 cells, else 1; pyro_autotune also tries both. Learned passes take
+ * the same split. */
 int pyro_set_streams(pyro_ctx *ctx, int n);
 int pyro_get_streams(pyro_ctx *ctx, int *n);
 
