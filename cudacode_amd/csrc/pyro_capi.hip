@@ -54,14 +54,27 @@ int reset_model_state(pyro_ctx *ctx) {
   return 0;
 }
 
-// The colour phases of one sweep for the current launch range.
-static int enqueue_phases(pyro_ctx *ctx) {
+// The colour phases of one sweep for the current launch range. With record,
+// the range's energy-series entries after the sweep are written too: by the
+// first over-relaxation launch when its kernel can take them (untracked
+// over-relaxation leaves dE final after the Metropolis and learned phases),
+// else by a separate launch at the end.
+static int enqueue_phases(pyro_ctx *ctx, bool record = false) {
   for (int p = 0; p < ctx->n_metro; ++p)
     for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_metropolis_phase(ctx, c, p, nullptr));
   for (int p = 0; p < ctx->n_learned; ++p)
     for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_learned_phase(ctx, c, p));
+  ctx->launch_series_done = 0;
   for (int o = 0; o < ctx->n_or; ++o)
-    for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_overrelax_phase(ctx, c));
+    for (int c = 0; c < kColours; ++c) {
+      ctx->launch_series = record && o == 0 && c == 0 ? ctx->d_series : nullptr;
+      const int rc = launch_overrelax_phase(ctx, c);
+      ctx->launch_series = nullptr;
+      if (rc) return rc;
+    }
+  const bool recorded = ctx->launch_series_done != 0;
+  ctx->launch_series_done = 0;
+  if (record && !recorded) PYRO_TRY(launch_record_half(ctx));
   return 0;
 }
 
@@ -69,37 +82,71 @@ static bool split_sweep(const pyro_ctx *ctx) {
   return ctx->split && ctx->stream2 && ctx->exec_mode == 0 && !ctx->hetero && ctx->R >= 2;
 }
 
-// One sweep as kernel launches (also the body that gets captured). Replicas
-// are independent within a sweep, so the phases of two replica halves can run
-// on two streams (forked from and joined back into the ctx stream every
-// sweep): the second half's kernels fill the GPU while the first half's
-// drain, which hides most of the per-launch ramp of these short kernels.
-static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
-  if (ctx->exec_mode != 0) {
-    PYRO_TRY(launch_batch_sweep_body(ctx));
-  } else if (split_sweep(ctx)) {
-    const int half = ctx->R / 2;
+// Ends a split run: the second stream joins the ctx stream, and the sweeps
+// made since the fork are booked (sweep counter, series rows). With
+// record_last the series row of the last of them is written here (after an
+// exchange moved the rungs); otherwise every row came from k_record_half.
+static int split_join(pyro_ctx *ctx, int record_last) {
+  if (!ctx->forked) return 0;
+  PYRO_HIP(hipEventRecord(ctx->ev_join, ctx->stream2));
+  PYRO_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+  ctx->forked = 0;
+  const uint32_t n = ctx->unjoined;
+  ctx->unjoined = 0;
+  return launch_end_sweep(ctx, n, record_last);
+}
+
+// The phases of one sweep on two streams, one replica half each, plus each
+// half's series entries when the sweep is not followed by a join.
+static int enqueue_split_phases(pyro_ctx *ctx, bool join_after) {
+  if (!ctx->forked) {
     PYRO_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
     PYRO_HIP(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
-    ctx->launch_stream = ctx->stream2;
-    ctx->launch_r0 = half;
-    ctx->launch_nrep = ctx->R - half;
-    int rc = enqueue_phases(ctx);
-    ctx->launch_stream = ctx->stream;
-    ctx->launch_r0 = 0;
-    ctx->launch_nrep = half;
-    if (rc == 0) rc = enqueue_phases(ctx);
-    ctx->launch_stream = nullptr;
-    ctx->launch_nrep = 0;
-    PYRO_HIP(hipEventRecord(ctx->ev_join, ctx->stream2));
-    PYRO_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
-    if (rc) return rc;
-  } else {
-    PYRO_TRY(enqueue_phases(ctx));
+    ctx->forked = 1;
+    ctx->unjoined = 0;
   }
+  const int half = ctx->R / 2;
+  const struct { hipStream_t s; int r0, n; } parts[2] = {{ctx->stream2, half, ctx->R - half},
+                                                         {ctx->stream, 0, half}};
+  ctx->launch_delta = ctx->unjoined;
+  int rc = 0;
+  for (const auto &p : parts) {
+    ctx->launch_stream = p.s;
+    ctx->launch_r0 = p.r0;
+    ctx->launch_nrep = p.n;
+    if (rc == 0) rc = enqueue_phases(ctx, !join_after && ctx->d_series != nullptr);
+  }
+  ctx->launch_stream = nullptr;
+  ctx->launch_nrep = 0;
+  ctx->launch_delta = 0;
+  ++ctx->unjoined;
+  return rc;
+}
+
+// One sweep as kernel launches (also the body that gets captured). Replicas
+// are independent within a sweep, so the phases of two replica halves can run
+// on two streams: the second half's kernels fill the GPU while the first
+// half's drain, which hides most of the per-launch ramp of these short
+// kernels. The streams run apart until the next exchange or measurement
+// (which need every replica) or the end of the call (split_join).
+static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
   const bool exchange = ctx->pt_every > 0 && (sweep_index + 1u) % (uint32_t)ctx->pt_every == 0u;
   const bool measure =
       ctx->measure_every > 0 && (sweep_index + 1u) % (uint32_t)ctx->measure_every == 0u;
+  const bool split = ctx->exec_mode == 0 && split_sweep(ctx);
+  if (ctx->exec_mode != 0) {
+    PYRO_TRY(launch_batch_sweep_body(ctx));
+  } else if (split) {
+    PYRO_TRY(enqueue_split_phases(ctx, exchange || measure));
+    if (!(exchange || measure)) return 0;
+    // Join without booking yet: the counters advance after the exchange, as
+    // for an unsplit sweep.
+    PYRO_HIP(hipEventRecord(ctx->ev_join, ctx->stream2));
+    PYRO_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+    ctx->forked = 0;
+  } else {
+    PYRO_TRY(enqueue_phases(ctx));
+  }
   if (exchange || measure) {
     PYRO_TRY(launch_observables(ctx));
     // The full energies also re-anchor the incremental energy, so FP32
@@ -111,7 +158,9 @@ static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
   // simulated at, before the exchange moves temperatures.
   if (measure) PYRO_TRY(launch_measure_row(ctx));
   if (exchange) PYRO_TRY(launch_pt_swap(ctx, 0, 0, 1));
-  PYRO_TRY(launch_end_sweep(ctx));
+  const uint32_t n = split ? ctx->unjoined : 1u;
+  ctx->unjoined = 0;
+  PYRO_TRY(launch_end_sweep(ctx, n, 1));
   return 0;
 }
 
@@ -458,10 +507,18 @@ int pyro_overrelax_phase(pyro_ctx *ctx, int colour) {
   return 0;
 }
 
+static int run_sweeps(pyro_ctx *ctx, int n_sweeps, int use_graph);
+
 int pyro_sweep(pyro_ctx *ctx, int n_sweeps, int use_graph) {
   if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
   if (n_sweeps < 0) return set_error(PYRO_EINVAL, "n_sweeps %d < 0", n_sweeps);
   PYRO_TRY(device_guard(ctx));
+  const int rc = run_sweeps(ctx, n_sweeps, use_graph);
+  if (rc && ctx->forked) (void)split_join(ctx, 0);  // never leave the second stream running ahead
+  return rc;
+}
+
+static int run_sweeps(pyro_ctx *ctx, int n_sweeps, int use_graph) {
   const uint32_t block = graph_block(ctx);
   uint32_t left = (uint32_t)n_sweeps;
   if (!use_graph) {
@@ -479,11 +536,15 @@ int pyro_sweep(pyro_ctx *ctx, int n_sweeps, int use_graph) {
     --left;
   }
   if (left >= block) {
+    PYRO_TRY(split_join(ctx, 0));  // a captured block starts and ends joined
     if (!ctx->graph_exec || ctx->graph_sweeps != (int)block) {
       drop_graph(ctx);
       PYRO_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       int rc = 0;
       for (uint32_t i = 0; i < block && rc == 0; ++i) rc = enqueue_sweep(ctx, i);
+      if (rc == 0) rc = split_join(ctx, 0);
+      ctx->forked = 0;
+      ctx->unjoined = 0;
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(ctx->stream, &g);
       if (rc) {
@@ -506,6 +567,7 @@ int pyro_sweep(pyro_ctx *ctx, int n_sweeps, int use_graph) {
     account_sweeps(ctx, 1);
     --left;
   }
+  PYRO_TRY(split_join(ctx, 0));
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   return 0;
 }
@@ -795,6 +857,9 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
       break;
     }
     for (int s = 0; s < sweeps && !rc; ++s) rc = enqueue_sweep(ctx, ctx->h_sweep + (uint32_t)s);
+    if (!rc) rc = split_join(ctx, 0);
+    ctx->forked = 0;
+    ctx->unjoined = 0;
     const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
     if (!rc && ce != hipSuccess) rc = set_error(PYRO_EHIP, "pyro_autotune: capture failed");
     if (!rc && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)

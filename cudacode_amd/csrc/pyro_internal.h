@@ -142,6 +142,17 @@ struct pyro_ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipStream_t launch_stream = nullptr;
   int launch_r0 = 0, launch_nrep = 0;
+  // Split sweeps join the second stream only at exchanges, measurements and
+  // the end of a call: `forked` while it runs ahead, `unjoined` sweeps since
+  // the fork. Phase kernels key their draws on ctl[kCtlSweep] + launch_delta,
+  // because the counter itself advances only at the join.
+  int forked = 0;
+  uint32_t unjoined = 0;
+  uint32_t launch_delta = 0;
+  // Series hook offered to the next over-relaxation launch of a split sweep;
+  // launch_series_done is set if the launched kernel took it.
+  double *launch_series = nullptr;
+  int launch_series_done = 0;
   float *d_coup = nullptr;
   float h_coup[pyro::kCoupFloats] = {};
 
@@ -173,13 +184,16 @@ static inline int launch_nrep(const pyro_ctx *c) { return c->launch_nrep ? c->la
 
 int set_error(int code, const char *fmt, ...);
 
-// Kernel launchers (pyro_kernels.hip). All enqueue on ctx->stream.
+// Kernel launchers (pyro_kernels.hip). All enqueue on ctx->stream, except
+// that the phase launchers and launch_record_half follow the launch range
+// (launch_stream / launch_r0 / launch_nrep) while a split sweep sets one.
 int launch_init_spins(pyro_ctx *ctx);
 int launch_metropolis_phase(pyro_ctx *ctx, int colour, int pass, const uint32_t *sweep_override);
 int launch_overrelax_phase(pyro_ctx *ctx, int colour);
 int launch_observables(pyro_ctx *ctx);           // fills d_obs
 int launch_pt_swap(pyro_ctx *ctx, int parity, uint32_t round, int from_ctl);
-int launch_end_sweep(pyro_ctx *ctx);             // record row, advance sweep counter
+int launch_end_sweep(pyro_ctx *ctx, uint32_t n, int record_last);  // series row, sweep counter
+int launch_record_half(pyro_ctx *ctx);           // series entries of the launch range
 int launch_measure_row(pyro_ctx *ctx);           // append a measurement row from d_obs
 int launch_phase_mat(pyro_ctx *ctx, int kind, int colour, int pass);  // matrix field model
 int reset_model_state(pyro_ctx *ctx);  // drop captured sweeps, re-anchor the energy

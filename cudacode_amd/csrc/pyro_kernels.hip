@@ -99,27 +99,50 @@ __global__ void k_pt_swap(int R, const double *__restrict__ obs, double *__restr
   if (from_ctl && last_group) ctl[kCtlPtRound] = round + 1u;
 }
 
-// End of a sweep: append one row (incremental energy per rung) to the series
-// when recording, then advance the sweep counter. grid = 1, block = 256.
+// End of n sweeps (n = 1, or the sweeps a split run made since its fork):
+// when recording, the row of the last of them (incremental energy per rung)
+// is written here if record_last (the rows of the earlier ones were written
+// by k_record_half) and the row count advances by n up to the capacity; the
+// sweep counter advances by n. grid = 1, block = 256.
 __global__ __launch_bounds__(256) void k_end_sweep(int R, uint32_t *__restrict__ ctl,
                                                    const double *__restrict__ anchor,
                                                    const double *__restrict__ dE,
                                                    const int *__restrict__ replica_at,
-                                                   double *__restrict__ series) {
+                                                   double *__restrict__ series, uint32_t n,
+                                                   int record_last) {
   const uint32_t on = ctl[kCtlRecordOn];
   const uint32_t row = ctl[kCtlRecordRow];
   const uint32_t cap = ctl[kCtlRecordCap];
-  const bool rec = on && series != nullptr && row < cap;
+  const uint32_t last = row + n - 1u;
+  const bool rec = on && series != nullptr && record_last && last < cap;
   if (rec) {
     for (int k = threadIdx.x; k < R; k += blockDim.x) {
       const int rep = replica_at[k];
-      series[(size_t)row * R + k] = anchor[rep] + dE[rep];
+      series[(size_t)last * R + k] = anchor[rep] + dE[rep];
     }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    if (rec) ctl[kCtlRecordRow] = row + 1u;
-    ctl[kCtlSweep] = ctl[kCtlSweep] + 1u;
+    if (on && series != nullptr) ctl[kCtlRecordRow] = row + n < cap ? row + n : (row < cap ? cap : row);
+    ctl[kCtlSweep] = ctl[kCtlSweep] + n;
+  }
+}
+
+// Series entries of one replica half after sweep ctl[kCtlSweep] + delta of a
+// split run (the row count advances at the join): the rungs whose replica
+// lies in [r0, r0 + nrep). Rungs do not move between joins. grid = 1,
+// block = 256, on the half's own stream.
+__global__ __launch_bounds__(256) void k_record_half(int R, int r0, int nrep, uint32_t delta,
+                                                     const uint32_t *__restrict__ ctl,
+                                                     const double *__restrict__ anchor,
+                                                     const double *__restrict__ dE,
+                                                     const int *__restrict__ replica_at,
+                                                     double *__restrict__ series) {
+  const uint32_t row = ctl[kCtlRecordRow] + delta;
+  if (!ctl[kCtlRecordOn] || row >= ctl[kCtlRecordCap]) return;  // uniform over the block
+  for (int k = threadIdx.x; k < R; k += blockDim.x) {
+    const int rep = replica_at[k];
+    if (rep >= r0 && rep < r0 + nrep) series[(size_t)row * R + k] = anchor[rep] + dE[rep];
   }
 }
 
@@ -264,9 +287,18 @@ int launch_pt_swap(pyro_ctx *ctx, int parity, uint32_t round, int from_ctl) {
   return 0;
 }
 
-int launch_end_sweep(pyro_ctx *ctx) {
+int launch_end_sweep(pyro_ctx *ctx, uint32_t n, int record_last) {
   k_end_sweep<<<1, 256, 0, ctx->stream>>>(ctx->R, ctx->d_ctl, ctx->d_anchor, ctx->d_dE,
-                                          ctx->d_replica_at, ctx->d_series);
+                                          ctx->d_replica_at, ctx->d_series, n, record_last);
+  PYRO_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_record_half(pyro_ctx *ctx) {
+  if (!ctx->d_series) return 0;
+  k_record_half<<<1, 256, 0, launch_stream(ctx)>>>(ctx->R, launch_r0(ctx), launch_nrep(ctx),
+                                                   ctx->launch_delta, ctx->d_ctl, ctx->d_anchor,
+                                                   ctx->d_dE, ctx->d_replica_at, ctx->d_series);
   PYRO_HIP(hipGetLastError());
   return 0;
 }

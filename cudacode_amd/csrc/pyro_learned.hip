@@ -279,7 +279,7 @@ __global__ __launch_bounds__(kPhaseBlock, 3) void k_learned_phase(
     float *__restrict__ spins, int L_rt, int ncell_rt, int colour, int pass,
     const float *__restrict__ beta, const uint32_t *__restrict__ ctl, uint32_t key0,
     uint32_t key1, float jeff, const uint16_t *__restrict__ pack, const float *__restrict__ w,
-    unsigned long long *__restrict__ acc, double *__restrict__ dE, int r0) {
+    unsigned long long *__restrict__ acc, double *__restrict__ dE, int r0, uint32_t sweep_delta) {
   const int L = LT > 0 ? LT : L_rt;
   const int ncell = LT > 0 ? LT * LT * LT : ncell_rt;
   const int r = r0 + (int)blockIdx.y;
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(kPhaseBlock, 3) void k_learned_phase(
   float *Sy = Sx + plane;
   float *Sz = Sy + plane;
   const float bt = beta[r];
-  const uint32_t sweep = ctl[kCtlSweep];
+  const uint32_t sweep = ctl[kCtlSweep] + sweep_delta;
 
   MlpW W;
   mlp_load(pack, w, W);
@@ -723,12 +723,13 @@ static void launch_learned_t(pyro_ctx *ctx, int colour, int pass) {
                           pass, (const float *)ctx->d_beta, (const uint32_t *)ctx->d_ctl, klo(ctx),
                           khi(ctx), ctx->jeff, (const uint16_t *)ctx->d_mlp_pack,
                           (const float *)ctx->d_mlp_w, ctx->d_acc_learned, ctx->d_dE,
-                          launch_r0(ctx));
+                          launch_r0(ctx), ctx->launch_delta);
     return;
   }
   k_learned_phase<LT, G><<<grid, kPhaseBlock, 0, launch_stream(ctx)>>>(
       ctx->d_spins, ctx->L, ctx->ncell, colour, pass, ctx->d_beta, ctx->d_ctl, klo(ctx), khi(ctx),
-      ctx->jeff, ctx->d_mlp_pack, ctx->d_mlp_w, ctx->d_acc_learned, ctx->d_dE, launch_r0(ctx));
+      ctx->jeff, ctx->d_mlp_pack, ctx->d_mlp_w, ctx->d_acc_learned, ctx->d_dE, launch_r0(ctx),
+      ctx->launch_delta);
 }
 
 int launch_learned_phase(pyro_ctx *ctx, int colour, int pass) {

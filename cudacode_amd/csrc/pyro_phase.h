@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kPhaseBlock) void k_phase(
     const float *__restrict__ beta, const float *__restrict__ cosc,
     const uint32_t *__restrict__ ctl, uint32_t key0, uint32_t key1, float jeff,
     unsigned long long *__restrict__ acc, double *__restrict__ dE, const float *__restrict__ coup,
-    int post_rotate, int r0) {
+    int post_rotate, int r0, uint32_t sweep_delta) {
   const int L = LT > 0 ? LT : L_rt;
   const int ncell = LT > 0 ? LT * LT * LT : ncell_rt;
   const int r = r0 + (int)blockIdx.y;
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kPhaseBlock) void k_phase(
     if (KIND == 0) {
       const u32x4 w = philox4x32_10(
           rng_counter((uint32_t)cell, (uint32_t)site, (uint32_t)pass, kKindMetropolis,
-                      (uint32_t)r, ctl[kCtlSweep]),
+                      (uint32_t)r, ctl[kCtlSweep] + sweep_delta),
           key0, key1);
       const f3 h = field_h<MAT>(jeff, fx, fy, fz);
       sp = cone_proposal(s, cosc[r], u01(w.x), u01(w.y));
@@ -144,16 +144,36 @@ __global__ __launch_bounds__(kPhaseBlock) void k_phase(
 // z-1 / z+1 neighbour planes of a block then sit in the same XCD's L2.
 // grid = 1-D, n_blocks = R * ceil(ncell / 256), block = 256; a block covers
 // cells of one replica only, so one atomic per block.
+//
+// Series hook (over-relaxation without FP64 tracking only, which leaves dE
+// alone): with series != nullptr, block 0 also writes the energy-series
+// entries (anchor + dE) of the rungs whose replica is in this launch's range,
+// at row ctl[kCtlRecordRow] + sweep_delta. A split sweep passes it to its
+// first over-relaxation launch, whose blocks start after the sweep's
+// Metropolis or learned phases have finished changing dE, so no separate
+// recording launch breaks the stream.
 // ---------------------------------------------------------------------------
 template <int KIND, int COLOUR, int LT, bool TRACK, bool MAT>
 __global__ __launch_bounds__(256) void k_phase_cells(
     float *__restrict__ spins, int blocks_per_replica, int pass, const float *__restrict__ beta,
     const float *__restrict__ cosc, const uint32_t *__restrict__ ctl, uint32_t key0,
     uint32_t key1, float jeff, unsigned long long *__restrict__ acc, double *__restrict__ dE,
-    const float *__restrict__ coup, int post_rotate, int r0) {
+    const float *__restrict__ coup, int post_rotate, int r0, uint32_t sweep_delta,
+    double *__restrict__ series, int R_total, const int *__restrict__ replica_at,
+    const double *__restrict__ anchor) {
   constexpr int NC = LT * LT * LT;
   const int nb = gridDim.x;
   const int b = blockIdx.x;
+  if (KIND == 1 && !TRACK && series != nullptr && b == 0) {
+    const uint32_t row = ctl[kCtlRecordRow] + sweep_delta;
+    if (ctl[kCtlRecordOn] && row < ctl[kCtlRecordCap]) {
+      const int r1 = r0 + nb / blocks_per_replica;
+      for (int k = threadIdx.x; k < R_total; k += 256) {
+        const int rep = replica_at[k];
+        if (rep >= r0 && rep < r1) series[(size_t)row * R_total + k] = anchor[rep] + dE[rep];
+      }
+    }
+  }
   const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
   const int q = lb / blocks_per_replica;  // replica within the launch range
   const int r = r0 + q;
@@ -164,7 +184,7 @@ __global__ __launch_bounds__(256) void k_phase_cells(
     cell_update<KIND, COLOUR, LT, TRACK, MAT>(spins + (size_t)r * 48 * NC, cell, r, pass,
                                               KIND == 0 ? beta[r] : 0.0f,
                                               KIND == 0 ? cosc[r] : 0.0f,
-                                              KIND == 0 ? ctl[kCtlSweep] : 0u, key0, key1, jeff,
+                                              KIND == 0 ? ctl[kCtlSweep] + sweep_delta : 0u, key0, key1, jeff,
                                               de, nacc, coup, KIND == 0 && post_rotate != 0);
   if (KIND == 0 || TRACK) {
     block_sum<256>(de, nacc);
@@ -268,31 +288,39 @@ static void launch_phase_t(pyro_ctx *ctx, int colour, int pass) {
                           pass, (const float *)ctx->d_beta, (const float *)ctx->d_cosc,
                           (const uint32_t *)ctx->d_ctl, key_lo(ctx), key_hi(ctx), ctx->jeff,
                           ctx->d_acc, ctx->d_dE, (const float *)ctx->d_coup, (int)ctx->post_rotate,
-                          launch_r0(ctx));
+                          launch_r0(ctx), ctx->launch_delta);
     return;
   }
   k_phase<KIND, LT, MAT><<<grid, kPhaseBlock, 0, launch_stream(ctx)>>>(
       ctx->d_spins, ctx->L, ctx->ncell, colour, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl,
       key_lo(ctx), key_hi(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE, ctx->d_coup,
-      (int)ctx->post_rotate, launch_r0(ctx));
+      (int)ctx->post_rotate, launch_r0(ctx), ctx->launch_delta);
 }
 
 template <int KIND, int COLOUR, int LT, bool TRACK, bool MAT>
 static void launch_cells_t(pyro_ctx *ctx, int pass) {
   const int bpr = (LT * LT * LT + 255) / 256;
   const int nrep = launch_nrep(ctx);
+  double *series = nullptr;
+  if (KIND == 1 && !TRACK && ctx->launch_series) {
+    series = ctx->launch_series;
+    ctx->launch_series_done = 1;
+  }
   if (ctx->time_ev0) {
     hipExtLaunchKernelGGL(k_phase_cells<KIND, COLOUR, LT, TRACK, MAT>, dim3(bpr * nrep),
                           dim3(256), 0, launch_stream(ctx), ctx->time_ev0, ctx->time_ev1, 0,
                           ctx->d_spins, bpr, pass, (const float *)ctx->d_beta,
                           (const float *)ctx->d_cosc, (const uint32_t *)ctx->d_ctl, key_lo(ctx),
                           key_hi(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE,
-                          (const float *)ctx->d_coup, (int)ctx->post_rotate, launch_r0(ctx));
+                          (const float *)ctx->d_coup, (int)ctx->post_rotate, launch_r0(ctx),
+                          ctx->launch_delta, series, ctx->R, (const int *)ctx->d_replica_at,
+                          (const double *)ctx->d_anchor);
     return;
   }
   k_phase_cells<KIND, COLOUR, LT, TRACK, MAT><<<bpr * nrep, 256, 0, launch_stream(ctx)>>>(
       ctx->d_spins, bpr, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl, key_lo(ctx), key_hi(ctx),
-      ctx->jeff, ctx->d_acc, ctx->d_dE, ctx->d_coup, (int)ctx->post_rotate, launch_r0(ctx));
+      ctx->jeff, ctx->d_acc, ctx->d_dE, ctx->d_coup, (int)ctx->post_rotate, launch_r0(ctx),
+      ctx->launch_delta, series, ctx->R, ctx->d_replica_at, ctx->d_anchor);
 }
 
 template <int KIND, int LT, bool MAT>

@@ -264,7 +264,7 @@ def test_adam_step_closed_form_and_descent():
 def test_learned_sweeps_on_two_streams_identical(graph):
     """Learned sweeps whose replica halves run on two streams equal one-stream
     learned sweeps bit for bit (spins, learned acceptance, incremental
-    energies, rungs), with exchanges."""
+    energies to FP64 rounding: per-block atomic sums), with exchanges."""
     from cudacode_amd import pyro
     temps = [0.05, 0.2, 0.5, 1.0, 2.0]
     kw = dict(n_metro=0, n_or=2, pt_every=3, cos_cone=0.5)
@@ -280,5 +280,5 @@ def test_learned_sweeps_on_two_streams_identical(graph):
         assert np.array_equal(a.get_spins(), b.get_spins())
         assert np.array_equal(a.acceptance_learned()[0], b.acceptance_learned()[0])
         assert a.acceptance_learned()[0].sum() > 0
-        assert np.array_equal(a.energy_incremental(), b.energy_incremental())
+        assert np.allclose(a.energy_incremental(), b.energy_incremental(), rtol=1e-11, atol=0)
         assert np.array_equal(a.temperatures()[1], b.temperatures()[1])
