@@ -2,11 +2,15 @@
 // executions that serve them (no reference counterpart: the reference runs
 // one lattice size per process, Pyrochlore.cu:1251-1257).
 //
-//  exec_mode 1 -- one persistent kernel per sweep. Replicas are independent
-//    between exchanges, so a block that owns whole replicas runs every colour
-//    phase of every pass of the sweep with block barriers only: no grid-wide
-//    synchronisation, one launch per sweep. Replicas are packed into bins of
-//    balanced work (longest-processing-time first); bins = blocks.
+//  exec_mode 1 -- one persistent kernel per run of sweeps up to the next
+//    exchange or measurement. Replicas are independent between exchanges, so
+//    a block that owns whole replicas runs every colour phase of every pass of
+//    those sweeps with block barriers only: no grid-wide synchronisation, one
+//    launch per interval. Replicas are packed into bins of balanced work
+//    (longest-processing-time first); bins = blocks. A replica of L <= 8
+//    (at most 96 KiB of spins) is staged in LDS for the interval, so its
+//    neighbour gathers are LDS reads; larger ones work in global memory
+//    (L2-resident per block).
 //  exec_mode 2 -- per-phase kernels over a work list of 256-cell tiles of all
 //    replicas (perfect balance, one launch per colour phase), tiles ordered so
 //    each XCD gets a contiguous run of one replica's cells.
@@ -51,47 +55,132 @@ __device__ __forceinline__ void replica_pass(float *Sx, int r, int pass, float b
 #undef PYRO_COLOUR
 }
 
-template <int LT, int BLOCK, bool TRACK>
-__device__ __attribute__((noinline)) void replica_sweep(float *Sx, int r, int n_metro, int n_or, float beta_r,
-                                              float cosc_r, uint32_t sweep, uint32_t key0,
-                                              uint32_t key1, float jeff, double &de, unsigned &nacc) {
+// Replicas up to this size are staged in LDS (48 L^3 floats: 12 KiB at L=4,
+// 96 KiB at L=8). The array is referenced by name in the staged variant, so
+// the compiler knows those spins live in LDS and emits LDS instructions.
+constexpr int kLdsMaxL = 8;
+extern __shared__ float lds_spins[];
+
+// One sweep of one replica (spins at Sg, or their LDS copy when STAGED). One
+// out-of-line instantiation per variant keeps each within the register
+// budget of a 1024-thread block.
+template <int LT, int BLOCK, bool TRACK, bool STAGED>
+__device__ __attribute__((noinline)) void replica_sweep(float *Sg, int r, int n_metro, int n_or,
+                                                        float beta_r, float cosc_r, uint32_t sweep,
+                                                        uint32_t key0, uint32_t key1, float jeff,
+                                                        double &de, unsigned &nacc) {
+  float *Sx = STAGED ? lds_spins : Sg;
   for (int p = 0; p < n_metro; ++p)
     replica_pass<0, LT, BLOCK, TRACK>(Sx, r, p, beta_r, cosc_r, sweep, key0, key1, jeff, de, nacc);
   for (int o = 0; o < n_or; ++o)
     replica_pass<1, LT, BLOCK, TRACK>(Sx, r, 0, beta_r, cosc_r, sweep, key0, key1, jeff, de, nacc);
 }
 
-// grid = n_bins, block = BLOCK. Bin b owns replicas order[bin_start[b] ..
-// bin_start[b+1]); each replica is swept start to finish by this block only,
-// so its counters are updated without atomics.
+// The sweeps [0, n_sweeps) of one replica: after each, the block's FP64
+// energy change and acceptance count go to dE[r] and acc[r] (this block owns
+// the replica: no atomics), and, when recording, the series entry anchor + dE
+// of the replica's rung at row ctl[kCtlRecordRow] + s; the last sweep's entry
+// only if record_last (else the end-of-sweep kernel writes it after an
+// exchange). A replica of L <= kLdsMaxL is copied to LDS for the run when
+// staged.
+template <int LT, int BLOCK, bool TRACK>
+__device__ __forceinline__ void replica_run(float *Sg, bool staged, int r, int n_metro, int n_or,
+                                            float beta_r, float cosc_r,
+                                            const uint32_t *__restrict__ ctl, int n_sweeps,
+                                            int record_last, int rung, int R,
+                                            double *__restrict__ series,
+                                            const double *__restrict__ anchor, uint32_t key0,
+                                            uint32_t key1, float jeff,
+                                            unsigned long long *__restrict__ acc,
+                                            double *__restrict__ dE) {
+  constexpr int NF = 48 * LT * LT * LT;
+  bool lds = false;
+  if constexpr (LT <= kLdsMaxL) lds = staged;
+  if (lds) {
+    for (int i = threadIdx.x; i < NF; i += BLOCK) lds_spins[i] = Sg[i];
+    __syncthreads();
+  }
+  const uint32_t sweep0 = ctl[kCtlSweep];
+  const uint32_t row0 = ctl[kCtlRecordRow];
+  const bool rec = series != nullptr && rung >= 0 && ctl[kCtlRecordOn] != 0u;
+  const uint32_t cap = ctl[kCtlRecordCap];
+  for (int s = 0; s < n_sweeps; ++s) {
+    double de = 0.0;
+    unsigned nacc = 0;
+    const uint32_t sweep = sweep0 + (uint32_t)s;
+    if constexpr (LT <= kLdsMaxL) {
+      if (lds)
+        replica_sweep<LT, BLOCK, TRACK, true>(Sg, r, n_metro, n_or, beta_r, cosc_r, sweep, key0,
+                                              key1, jeff, de, nacc);
+      else
+        replica_sweep<LT, BLOCK, TRACK, false>(Sg, r, n_metro, n_or, beta_r, cosc_r, sweep, key0,
+                                               key1, jeff, de, nacc);
+    } else {
+      replica_sweep<LT, BLOCK, TRACK, false>(Sg, r, n_metro, n_or, beta_r, cosc_r, sweep, key0,
+                                             key1, jeff, de, nacc);
+    }
+    block_sum<BLOCK>(de, nacc);
+    if (threadIdx.x == 0) {
+      dE[r] += de;
+      acc[r] += nacc;
+      const uint32_t row = row0 + (uint32_t)s;
+      if (rec && (s + 1 < n_sweeps || record_last) && row < cap)
+        series[(size_t)row * R + rung] = anchor[r] + dE[r];
+    }
+  }
+  if (lds) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < NF; i += BLOCK) Sg[i] = lds_spins[i];
+    __syncthreads();  // the next replica of the bin reuses the LDS
+  }
+}
+
+// grid = n_bins, block = BLOCK, dynamic LDS = 48 L^3 floats of the largest
+// staged size in the batch (0: nothing staged). Bin b owns replicas
+// order[bin_start[b] .. bin_start[b+1]); each replica is swept through all
+// n_sweeps sweeps by this block only, so its counters are updated without
+// atomics. rung_of[r] is the replica's rung (for the series), or nullptr.
 template <int BLOCK, bool TRACK>
 __global__ __launch_bounds__(BLOCK) void k_batch_sweep(
     float *__restrict__ spins, const long long *__restrict__ off, const int *__restrict__ Ls,
     const int *__restrict__ order, const int *__restrict__ bin_start, int n_metro, int n_or,
     const float *__restrict__ beta, const float *__restrict__ cosc,
     const uint32_t *__restrict__ ctl, uint32_t key0, uint32_t key1, float jeff,
-    unsigned long long *__restrict__ acc, double *__restrict__ dE) {
-  const uint32_t sweep = ctl[kCtlSweep];
+    unsigned long long *__restrict__ acc, double *__restrict__ dE, int n_sweeps, int record_last,
+    int lds_max_L, int R, const int *__restrict__ replica_at, double *__restrict__ series,
+    const double *__restrict__ anchor) {
+  __shared__ int s_rung;
   for (int q = bin_start[blockIdx.x]; q < bin_start[blockIdx.x + 1]; ++q) {
     const int r = order[q];
     float *Sx = spins + off[r];
     const float b = beta[r], c = cosc[r];
-    double de = 0.0;
-    unsigned nacc = 0;
-    switch (Ls[r]) {
-      case 4: replica_sweep<4, BLOCK, TRACK>(Sx, r, n_metro, n_or, b, c, sweep, key0, key1, jeff, de, nacc); break;
-      case 8: replica_sweep<8, BLOCK, TRACK>(Sx, r, n_metro, n_or, b, c, sweep, key0, key1, jeff, de, nacc); break;
-      case 12: replica_sweep<12, BLOCK, TRACK>(Sx, r, n_metro, n_or, b, c, sweep, key0, key1, jeff, de, nacc); break;
-      case 16: replica_sweep<16, BLOCK, TRACK>(Sx, r, n_metro, n_or, b, c, sweep, key0, key1, jeff, de, nacc); break;
-      case 24: replica_sweep<24, BLOCK, TRACK>(Sx, r, n_metro, n_or, b, c, sweep, key0, key1, jeff, de, nacc); break;
-      case 32: replica_sweep<32, BLOCK, TRACK>(Sx, r, n_metro, n_or, b, c, sweep, key0, key1, jeff, de, nacc); break;
+    const int L = Ls[r];
+    const bool staged = L <= lds_max_L;
+    int rung = -1;
+    if (series != nullptr) {
+      if (threadIdx.x == 0) s_rung = -1;
+      __syncthreads();
+      for (int k = threadIdx.x; k < R; k += BLOCK)
+        if (replica_at[k] == r) s_rung = k;
+      __syncthreads();
+      rung = s_rung;
+      __syncthreads();
+    }
+#define PYRO_RUN(LV)                                                                          \
+  case LV:                                                                                    \
+    replica_run<LV, BLOCK, TRACK>(Sx, staged, r, n_metro, n_or, b, c, ctl, n_sweeps, record_last, \
+                                  rung, R, series, anchor, key0, key1, jeff, acc, dE);         \
+    break;
+    switch (L) {
+      PYRO_RUN(4)
+      PYRO_RUN(8)
+      PYRO_RUN(12)
+      PYRO_RUN(16)
+      PYRO_RUN(24)
+      PYRO_RUN(32)
       default: break;
     }
-    block_sum<BLOCK>(de, nacc);
-    if (threadIdx.x == 0) {
-      dE[r] += de;
-      acc[r] += nacc;
-    }
+#undef PYRO_RUN
   }
 }
 
@@ -297,11 +386,27 @@ int launch_batch_observables(pyro_ctx *ctx) {
   return 0;
 }
 
+// Largest replica size of the batch that the persistent kernel stages in LDS
+// (0: none), and the dynamic LDS bytes that takes.
+static int lds_staged_L(const pyro_ctx *ctx) {
+  int m = 0;
+  for (int L : ctx->h_L)
+    if (L <= kLdsMaxL && L > m) m = L;
+  return m;
+}
+
 template <int BLOCK, bool TRACK>
-static void launch_persistent_t(pyro_ctx *ctx) {
-  k_batch_sweep<BLOCK, TRACK><<<ctx->n_bins, BLOCK, 0, ctx->stream>>>(
+static int launch_persistent_t(pyro_ctx *ctx, int n_sweeps, int record_last) {
+  const int lmax = lds_staged_L(ctx);
+  const size_t lds = sizeof(float) * 48 * (size_t)lmax * lmax * lmax;
+  if (lds > 65536)
+    PYRO_HIP(hipFuncSetAttribute((const void *)k_batch_sweep<BLOCK, TRACK>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  k_batch_sweep<BLOCK, TRACK><<<ctx->n_bins, BLOCK, lds, ctx->stream>>>(
       ctx->d_spins, ctx->d_off, ctx->d_L, ctx->d_order, ctx->d_bin_start, ctx->n_metro, ctx->n_or,
-      ctx->d_beta, ctx->d_cosc, ctx->d_ctl, bk0(ctx), bk1(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE);
+      ctx->d_beta, ctx->d_cosc, ctx->d_ctl, bk0(ctx), bk1(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE,
+      n_sweeps, record_last, lmax, ctx->R, ctx->d_replica_at, ctx->d_series, ctx->d_anchor);
+  return 0;
 }
 
 template <int KIND, int COLOUR, bool TRACK>
@@ -319,19 +424,27 @@ static void launch_tiles_pass(pyro_ctx *ctx, int pass) {
   launch_tiles_t<KIND, 3, TRACK>(ctx, pass);
 }
 
+int launch_batch_sweeps(pyro_ctx *ctx, int n_sweeps, int record_last) {
+  const bool tr = ctx->track_or;
+  int rc;
+  if (ctx->batch_block == 1024)
+    rc = tr ? launch_persistent_t<1024, true>(ctx, n_sweeps, record_last)
+            : launch_persistent_t<1024, false>(ctx, n_sweeps, record_last);
+  else if (ctx->batch_block == 512)
+    rc = tr ? launch_persistent_t<512, true>(ctx, n_sweeps, record_last)
+            : launch_persistent_t<512, false>(ctx, n_sweeps, record_last);
+  else
+    rc = tr ? launch_persistent_t<256, true>(ctx, n_sweeps, record_last)
+            : launch_persistent_t<256, false>(ctx, n_sweeps, record_last);
+  if (rc) return rc;
+  PYRO_HIP(hipGetLastError());
+  return 0;
+}
+
 int launch_batch_sweep_body(pyro_ctx *ctx) {
   const bool tr = ctx->track_or;
   if (ctx->exec_mode == 1) {
-    if (ctx->batch_block == 1024) {
-      if (tr) launch_persistent_t<1024, true>(ctx);
-      else launch_persistent_t<1024, false>(ctx);
-    } else if (ctx->batch_block == 512) {
-      if (tr) launch_persistent_t<512, true>(ctx);
-      else launch_persistent_t<512, false>(ctx);
-    } else {
-      if (tr) launch_persistent_t<256, true>(ctx);
-      else launch_persistent_t<256, false>(ctx);
-    }
+    PYRO_TRY(launch_batch_sweeps(ctx, 1, 0));
   } else {
     for (int p = 0; p < ctx->n_metro; ++p) launch_tiles_pass<0, false>(ctx, p);
     for (int o = 0; o < ctx->n_or; ++o) {

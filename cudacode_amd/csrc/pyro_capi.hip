@@ -129,10 +129,33 @@ static int enqueue_split_phases(pyro_ctx *ctx, bool join_after) {
 // half's drain, which hides most of the per-launch ramp of these short
 // kernels. The streams run apart until the next exchange or measurement
 // (which need every replica) or the end of the call (split_join).
+static bool exchange_after(const pyro_ctx *ctx, uint32_t sweep_index) {
+  return ctx->pt_every > 0 && (sweep_index + 1u) % (uint32_t)ctx->pt_every == 0u;
+}
+static bool measure_after(const pyro_ctx *ctx, uint32_t sweep_index) {
+  return ctx->measure_every > 0 && (sweep_index + 1u) % (uint32_t)ctx->measure_every == 0u;
+}
+
+// What follows the last sweep of an interval: full energies (which also
+// re-anchor the incremental energy, so FP32 rounding in the untracked
+// over-relaxation passes cannot accumulate beyond one exchange or measurement
+// interval), the measurement row, the exchange, and the booking of the
+// interval's n sweeps with the series row of the last one.
+static int enqueue_interval_end(pyro_ctx *ctx, bool exchange, bool measure, uint32_t n) {
+  if (exchange || measure) {
+    PYRO_TRY(launch_observables(ctx));
+    PYRO_TRY(launch_set_anchor(ctx));
+  }
+  // The measurement sees the configuration of this sweep at the rungs it was
+  // simulated at, before the exchange moves temperatures.
+  if (measure) PYRO_TRY(launch_measure_row(ctx));
+  if (exchange) PYRO_TRY(launch_pt_swap(ctx, 0, 0, 1));
+  return launch_end_sweep(ctx, n, 1);
+}
+
 static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
-  const bool exchange = ctx->pt_every > 0 && (sweep_index + 1u) % (uint32_t)ctx->pt_every == 0u;
-  const bool measure =
-      ctx->measure_every > 0 && (sweep_index + 1u) % (uint32_t)ctx->measure_every == 0u;
+  const bool exchange = exchange_after(ctx, sweep_index);
+  const bool measure = measure_after(ctx, sweep_index);
   const bool split = ctx->exec_mode == 0 && split_sweep(ctx);
   if (ctx->exec_mode != 0) {
     PYRO_TRY(launch_batch_sweep_body(ctx));
@@ -147,20 +170,34 @@ static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
   } else {
     PYRO_TRY(enqueue_phases(ctx));
   }
-  if (exchange || measure) {
-    PYRO_TRY(launch_observables(ctx));
-    // The full energies also re-anchor the incremental energy, so FP32
-    // rounding in the untracked over-relaxation passes cannot accumulate
-    // beyond one exchange (or measurement) interval.
-    PYRO_TRY(launch_set_anchor(ctx));
-  }
-  // The measurement sees the configuration of this sweep at the rungs it was
-  // simulated at, before the exchange moves temperatures.
-  if (measure) PYRO_TRY(launch_measure_row(ctx));
-  if (exchange) PYRO_TRY(launch_pt_swap(ctx, 0, 0, 1));
   const uint32_t n = split ? ctx->unjoined : 1u;
   ctx->unjoined = 0;
-  PYRO_TRY(launch_end_sweep(ctx, n, 1));
+  return enqueue_interval_end(ctx, exchange, measure, n);
+}
+
+// Sweeps first .. first + count - 1 (first = the sweep index that sets the
+// exchange and measurement cadence). The persistent execution runs each
+// interval up to an exchange or measurement (or to the end) in one launch;
+// the others go sweep by sweep.
+static int enqueue_sweeps(pyro_ctx *ctx, uint32_t first, uint32_t count) {
+  if (ctx->exec_mode != 1) {
+    for (uint32_t i = 0; i < count; ++i) PYRO_TRY(enqueue_sweep(ctx, first + i));
+    return 0;
+  }
+  uint32_t i = 0;
+  while (i < count) {
+    uint32_t j = i;
+    while (j + 1 < count && !exchange_after(ctx, first + j) && !measure_after(ctx, first + j)) ++j;
+    const bool exchange = exchange_after(ctx, first + j), measure = measure_after(ctx, first + j);
+    const uint32_t n = j - i + 1;
+    PYRO_TRY(launch_batch_sweeps(ctx, (int)n, exchange || measure ? 0 : 1));
+    if (exchange || measure) {
+      PYRO_TRY(enqueue_interval_end(ctx, exchange, measure, n));
+    } else {
+      PYRO_TRY(launch_end_sweep(ctx, n, 0));
+    }
+    i = j + 1;
+  }
   return 0;
 }
 
@@ -521,27 +558,25 @@ int pyro_sweep(pyro_ctx *ctx, int n_sweeps, int use_graph) {
 static int run_sweeps(pyro_ctx *ctx, int n_sweeps, int use_graph) {
   const uint32_t block = graph_block(ctx);
   uint32_t left = (uint32_t)n_sweeps;
-  if (!use_graph) {
-    while (left > 0) {
-      PYRO_TRY(enqueue_sweep(ctx, ctx->h_sweep));
-      account_sweeps(ctx, 1);
-      --left;
-    }
+  if (!use_graph && left > 0) {
+    PYRO_TRY(enqueue_sweeps(ctx, ctx->h_sweep, left));
+    account_sweeps(ctx, left);
+    left = 0;
   }
   // A captured block always starts on a multiple of the exchange and
   // measurement cadences, so both land after the same sweeps as in eager mode.
-  while (left > 0 && ctx->h_sweep % block != 0u) {
-    PYRO_TRY(enqueue_sweep(ctx, ctx->h_sweep));
-    account_sweeps(ctx, 1);
-    --left;
+  if (left > 0 && ctx->h_sweep % block != 0u) {
+    const uint32_t k = std::min(left, block - ctx->h_sweep % block);
+    PYRO_TRY(enqueue_sweeps(ctx, ctx->h_sweep, k));
+    account_sweeps(ctx, k);
+    left -= k;
   }
   if (left >= block) {
     PYRO_TRY(split_join(ctx, 0));  // a captured block starts and ends joined
     if (!ctx->graph_exec || ctx->graph_sweeps != (int)block) {
       drop_graph(ctx);
       PYRO_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      int rc = 0;
-      for (uint32_t i = 0; i < block && rc == 0; ++i) rc = enqueue_sweep(ctx, i);
+      int rc = enqueue_sweeps(ctx, 0, block);
       if (rc == 0) rc = split_join(ctx, 0);
       ctx->forked = 0;
       ctx->unjoined = 0;
@@ -562,10 +597,9 @@ static int run_sweeps(pyro_ctx *ctx, int n_sweeps, int use_graph) {
       left -= block;
     }
   }
-  while (left > 0) {
-    PYRO_TRY(enqueue_sweep(ctx, ctx->h_sweep));
-    account_sweeps(ctx, 1);
-    --left;
+  if (left > 0) {
+    PYRO_TRY(enqueue_sweeps(ctx, ctx->h_sweep, left));
+    account_sweeps(ctx, left);
   }
   PYRO_TRY(split_join(ctx, 0));
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
@@ -856,7 +890,7 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
       rc = set_error(PYRO_EHIP, "pyro_autotune: capture failed");
       break;
     }
-    for (int s = 0; s < sweeps && !rc; ++s) rc = enqueue_sweep(ctx, ctx->h_sweep + (uint32_t)s);
+    rc = enqueue_sweeps(ctx, ctx->h_sweep, (uint32_t)sweeps);
     if (!rc) rc = split_join(ctx, 0);
     ctx->forked = 0;
     ctx->unjoined = 0;

@@ -209,6 +209,9 @@ int batch_setup(pyro_ctx *ctx, int n_bins);       // bins, tiles, device geometr
 int launch_batch_init_spins(pyro_ctx *ctx);
 int launch_batch_observables(pyro_ctx *ctx);      // fills d_obs
 int launch_batch_sweep_body(pyro_ctx *ctx);       // Metropolis + OR passes of one sweep
+// Persistent execution: n_sweeps sweeps in one launch, with the series rows of
+// all but the last (and the last too if record_last).
+int launch_batch_sweeps(pyro_ctx *ctx, int n_sweeps, int record_last);
 void batch_free(pyro_ctx *ctx);
 
 // Learned proposal (pyro_learned.hip).

@@ -87,9 +87,10 @@ int pyro_get_layout(pyro_ctx *ctx, int *L, long long *offset);
 
 /* Sweep execution. mode 0: per-phase kernels (uniform ctx only; the
  * reference's launch structure, Pyrochlore.cu:105-127); mode 1: one
- * persistent kernel per sweep in which a block of `block` threads (256, 512,
- * 1024) sweeps whole replicas through every phase with block barriers only,
- * replicas packed into n_bins balanced bins (<= 0: one replica per block);
+ * persistent kernel per run of sweeps up to the next exchange or measurement,
+ * in which a block of `block` threads (256, 512, 1024) sweeps whole replicas
+ * through every phase with block barriers only (replicas of L <= 8 staged in
+ * LDS), replicas packed into n_bins balanced bins (<= 0: one per block);
  * mode 2: per-phase kernels over a list of 256-cell tiles of all replicas.
  * Modes 1 and 2 need L in {4,8,12,16,24,32}. Results are identical in every
  * mode. Default: 0 for a uniform ctx, 2 for a batch. */

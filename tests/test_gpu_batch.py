@@ -137,3 +137,59 @@ def test_autotune_keeps_the_state():
         g.Mc(6)
         h.Mc(6)
         assert np.array_equal(g.get_spins(), h.get_spins())
+
+
+@pytest.mark.parametrize("block", [256, 1024])
+@pytest.mark.parametrize("graph", [True, False])
+def test_persistent_intervals_equal_tile_phases(block, graph):
+    """The persistent execution runs each interval up to an exchange or
+    measurement in one launch, replicas of L <= 8 staged in LDS: spins,
+    rungs, acceptance counts, sweep counters and measurement rows equal the
+    tile-list execution's bit for bit, and the energy series (exchange and
+    non-exchange rows, past its capacity) and incremental energies equal them
+    to FP64 rounding; calls that end mid-interval included."""
+    Ls = [4, 8, 12, 8, 4, 12]
+    temps = [0.2, 0.5, 0.9, 1.3, 0.35, 0.6]
+    kw = dict(n_metro=1, n_or=2, pt_every=3, cos_cone=0.4)
+    with batch(Ls, temps, **kw) as a, batch(Ls, temps, **kw) as b:
+        a.set_exec_mode(1, block)
+        b.set_exec_mode(2, 256)
+        for g in (a, b):
+            g.record_start(41)
+            g.measure_start(4, 8)
+            g.Mc(5, use_graph=graph)
+            g.Mc(45, use_graph=graph)
+        assert np.array_equal(a.get_spins(), b.get_spins())
+        assert np.array_equal(a.temperatures()[1], b.temperatures()[1])
+        assert np.array_equal(a.acceptance()[0], b.acceptance()[0])
+        assert a.sweep_counters() == b.sweep_counters() == (50, 16)
+        ma, mb = a.measure_fetch(), b.measure_fetch()
+        assert ma.shape[0] == 8 and np.array_equal(ma, mb)
+        ra, rb = a.record_fetch(), b.record_fetch()
+        say(f"series max |diff| {np.abs(ra - rb).max()}")
+        assert ra.shape == (41, 6) and np.allclose(ra, rb, rtol=1e-11, atol=0)
+        assert np.allclose(a.energy_incremental(), b.energy_incremental(), rtol=1e-11, atol=0)
+
+
+def test_persistent_single_small_lattice_equals_phases():
+    """The L=4 single-replica workload shape (T=0.5, 1 Metropolis + 5
+    over-relaxation, no exchanges) in the persistent execution (one launch per
+    16-sweep graph block, spins in LDS) equals the per-phase kernels bit for
+    bit, and the oracle; the series agrees to FP64 rounding."""
+    from cudacode_amd import pyro
+    kw = dict(n_metro=1, n_or=5, pt_every=0, cos_cone=0.5)
+    with pyro.Pyrochlore(4, [0.5], seed=SEED, **kw) as a, \
+            pyro.Pyrochlore(4, [0.5], seed=SEED, **kw) as b:
+        a.set_exec_mode(1, 256)
+        for g in (a, b):
+            g.record_start(100)
+            g.Mc(3)
+            g.Mc(37)
+        assert np.array_equal(a.get_spins(), b.get_spins())
+        assert np.array_equal(a.acceptance()[0], b.acceptance()[0])
+        ra, rb = a.record_fetch(), b.record_fetch()
+        assert ra.shape == (40, 1) and np.allclose(ra, rb, rtol=1e-11, atol=0)
+        spins = a.get_spins()
+    o = oracle.OracleRun(4, [0.5], seed=SEED, **kw)
+    o.run(40)
+    assert np.array_equal(spins, o.spins)
