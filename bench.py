@@ -292,7 +292,10 @@ def workload_c5(args):
                           "frac": achieved / HBM_PEAK_GBS,
                           "traffic": measured_traffic(OR_KERNEL_C5),
                           "avg_launch_ms": or_ms,
-                          "bytes_per_launch": BYTES_PER_UPDATE * n_total // 4}}
+                          "bytes_per_launch": BYTES_PER_UPDATE * n_total // 4,
+                          "sweep_achieved": BYTES_PER_UPDATE * 11 * n_total * steps / dt / 1e9,
+                          "sweep_frac": BYTES_PER_UPDATE * 11 * n_total * steps / dt / 1e9
+                                        / HBM_PEAK_GBS}}
     if not args.no_learned:
         log("c5: training the learned proposal on this ensemble")
         info = learned.train(g, snapshots=16, per_replica=2048, steps=args.train_steps,
@@ -455,6 +458,11 @@ def main():
             "bytes_per_launch": bytes_per_launch,
             "metropolis_avg_launch_ms": met_ms,
             "site_per_thread_or_avg_launch_ms": or_ms_v0,
+            # The whole timed sweep at the same 60 B per update (exchanges,
+            # bookkeeping and the two streams' overlap included).
+            "sweep_achieved": BYTES_PER_UPDATE * (N_METRO + N_OR) * n_total / (ms_per_step * 1e-3) / 1e9,
+            "sweep_frac": BYTES_PER_UPDATE * (N_METRO + N_OR) * n_total / (ms_per_step * 1e-3) / 1e9
+                          / HBM_PEAK_GBS,
         },
     }
     if learned is not None:
