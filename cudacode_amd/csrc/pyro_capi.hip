@@ -37,11 +37,7 @@ static void drop_graph(pyro_ctx *ctx) {
 }
 
 // Recompute the full energy and make it the anchor of the incremental one.
-static int reanchor(pyro_ctx *ctx) {
-  PYRO_TRY(launch_observables(ctx));
-  PYRO_TRY(launch_set_anchor(ctx));
-  return 0;
-}
+static int reanchor(pyro_ctx *ctx) { return launch_observables(ctx, true); }
 
 // After a change of the energy model: the captured sweeps hold the old
 // kernels, and the incremental energy restarts from a full recomputation.
@@ -142,14 +138,11 @@ static bool measure_after(const pyro_ctx *ctx, uint32_t sweep_index) {
 // interval), the measurement row, the exchange, and the booking of the
 // interval's n sweeps with the series row of the last one.
 static int enqueue_interval_end(pyro_ctx *ctx, bool exchange, bool measure, uint32_t n) {
-  if (exchange || measure) {
-    PYRO_TRY(launch_observables(ctx));
-    PYRO_TRY(launch_set_anchor(ctx));
-  }
+  if (exchange || measure) PYRO_TRY(launch_observables(ctx, true));
   // The measurement sees the configuration of this sweep at the rungs it was
   // simulated at, before the exchange moves temperatures.
   if (measure) PYRO_TRY(launch_measure_row(ctx));
-  if (exchange) PYRO_TRY(launch_pt_swap(ctx, 0, 0, 1));
+  if (exchange) return launch_pt_swap(ctx, 0, 0, 1, n);  // books the interval too
   return launch_end_sweep(ctx, n, 1);
 }
 

@@ -190,8 +190,11 @@ int set_error(int code, const char *fmt, ...);
 int launch_init_spins(pyro_ctx *ctx);
 int launch_metropolis_phase(pyro_ctx *ctx, int colour, int pass, const uint32_t *sweep_override);
 int launch_overrelax_phase(pyro_ctx *ctx, int colour);
-int launch_observables(pyro_ctx *ctx);           // fills d_obs
-int launch_pt_swap(pyro_ctx *ctx, int parity, uint32_t round, int from_ctl);
+// Fills d_obs; with anchor, also re-anchors the incremental energy (anchor = E, dE = 0).
+int launch_observables(pyro_ctx *ctx, bool anchor = false);
+// end_n > 0: the exchange also ends an interval of end_n sweeps (series row
+// of the last, sweep counter), as launch_end_sweep(ctx, end_n, 1) after it.
+int launch_pt_swap(pyro_ctx *ctx, int parity, uint32_t round, int from_ctl, uint32_t end_n = 0);
 int launch_end_sweep(pyro_ctx *ctx, uint32_t n, int record_last);  // series row, sweep counter
 int launch_record_half(pyro_ctx *ctx);           // series entries of the launch range
 int launch_measure_row(pyro_ctx *ctx);           // append a measurement row from d_obs

@@ -32,7 +32,9 @@ __global__ __launch_bounds__(256) void k_init_spins(float *__restrict__ spins, i
 // over blocks of component c of sublattice b. grid = R, block = 256.
 __global__ __launch_bounds__(256) void k_observables_final(const double *__restrict__ partials,
                                                            int nblk, double escale,
-                                                           double *__restrict__ obs) {
+                                                           double *__restrict__ obs,
+                                                           double *__restrict__ anchor,
+                                                           double *__restrict__ dE) {
   const int r = blockIdx.x;
   __shared__ double s_v[256];
   for (int q = 0; q < kObs; ++q) {
@@ -51,65 +53,27 @@ __global__ __launch_bounds__(256) void k_observables_final(const double *__restr
       if ((int)threadIdx.x < w) s_v[threadIdx.x] += s_v[threadIdx.x + w];
       __syncthreads();
     }
-    if (threadIdx.x == 0) obs[(size_t)r * kObs + q] = (q == 0) ? escale * s_v[0] : s_v[0];
+    if (threadIdx.x == 0) {
+      const double v = (q == 0) ? escale * s_v[0] : s_v[0];
+      obs[(size_t)r * kObs + q] = v;
+      if (q == 0 && anchor != nullptr) {  // re-anchor the incremental energy here
+        anchor[r] = v;
+        dE[r] = 0.0;
+      }
+    }
     __syncthreads();
   }
 }
 
-// Parallel-tempering exchange (TrySwapingTemperatures, MonteCarlo.cu:105-172)
-// on the energies in obs. Rungs are ordered by descending temperature;
-// pairs (c, c+1), c = parity, parity+2, ... One thread: R is small and the
-// pairs are visited in order, as in the reference.
-// A batch of mixed lattice sizes exchanges within groups of equal size: R is
-// the group's rung count, replica_at its rungs, `group` keys its draws, and
-// the exchange-round counter advances after the last group.
-__global__ void k_pt_swap(int R, const double *__restrict__ obs, double *__restrict__ temps,
-                          float *__restrict__ beta, float *__restrict__ cosc,
-                          int *__restrict__ replica_at, uint32_t *__restrict__ ctl,
-                          uint32_t key0, uint32_t key1, int parity, uint32_t round,
-                          int from_ctl, int group, int last_group) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (from_ctl) {
-    round = ctl[kCtlPtRound];
-    parity = (int)(round & 1u);
-  }
-  int nacc = 0;
-  for (int c = parity ? 1 : 0; c + 1 < R; c += 2) {
-    const int t1 = replica_at[c], t2 = replica_at[c + 1];
-    const double T1 = temps[t1], T2 = temps[t2];
-    const double arg = (1.0 / T1 - 1.0 / T2) * (obs[(size_t)t1 * kObs] - obs[(size_t)t2 * kObs]);
-    const u32x4 w = philox4x32_10(
-        rng_counter((uint32_t)c, 0, 0, kKindSwap, (uint32_t)group, round), key0, key1);
-    const double u = u01_53(w.x, w.y);
-    if (arg >= 0.0 || u < exp(arg)) {
-      temps[t1] = T2;
-      temps[t2] = T1;
-      const float b = beta[t1];
-      beta[t1] = beta[t2];
-      beta[t2] = b;
-      const float cc = cosc[t1];
-      cosc[t1] = cosc[t2];
-      cosc[t2] = cc;
-      replica_at[c] = t2;
-      replica_at[c + 1] = t1;
-      ++nacc;
-    }
-  }
-  ctl[kCtlSwapsAcc] += (uint32_t)nacc;
-  if (from_ctl && last_group) ctl[kCtlPtRound] = round + 1u;
-}
-
-// End of n sweeps (n = 1, or the sweeps a split run made since its fork):
-// when recording, the row of the last of them (incremental energy per rung)
-// is written here if record_last (the rows of the earlier ones were written
-// by k_record_half) and the row count advances by n up to the capacity; the
-// sweep counter advances by n. grid = 1, block = 256.
-__global__ __launch_bounds__(256) void k_end_sweep(int R, uint32_t *__restrict__ ctl,
-                                                   const double *__restrict__ anchor,
-                                                   const double *__restrict__ dE,
-                                                   const int *__restrict__ replica_at,
-                                                   double *__restrict__ series, uint32_t n,
-                                                   int record_last) {
+// End of n sweeps (n = 1, or the sweeps of an interval run without a
+// per-sweep booking): when recording, the row of the last of them
+// (incremental energy per rung) is written here if record_last (the rows of
+// the earlier ones were written by the phase kernels) and the row count
+// advances by n up to the capacity; the sweep counter advances by n. Called by
+// every thread of one block.
+__device__ void end_interval(int R, uint32_t *__restrict__ ctl, const double *__restrict__ anchor,
+                             const double *__restrict__ dE, const int *__restrict__ replica_at,
+                             double *__restrict__ series, uint32_t n, int record_last) {
   const uint32_t on = ctl[kCtlRecordOn];
   const uint32_t row = ctl[kCtlRecordRow];
   const uint32_t cap = ctl[kCtlRecordCap];
@@ -126,6 +90,74 @@ __global__ __launch_bounds__(256) void k_end_sweep(int R, uint32_t *__restrict__
     if (on && series != nullptr) ctl[kCtlRecordRow] = row + n < cap ? row + n : (row < cap ? cap : row);
     ctl[kCtlSweep] = ctl[kCtlSweep] + n;
   }
+}
+
+// Parallel-tempering exchange (TrySwapingTemperatures, MonteCarlo.cu:105-172)
+// on the energies in obs. Rungs are ordered by descending temperature; pairs
+// (c, c+1), c = parity, parity+2, ... are disjoint, so each thread decides its
+// own pairs (the reference visits them in order; no pair reads another's
+// result, so the outcome is the same). A batch of mixed lattice sizes
+// exchanges within groups of equal size: R is the group's rung count,
+// replica_at its rungs, `group` keys its draws, and the exchange-round
+// counter advances after the last group. With end_n > 0 the last group's
+// launch also ends the interval (end_interval over all R_all rungs, after the
+// exchange). grid = 1, block = 256.
+__global__ __launch_bounds__(256) void k_pt_swap(
+    int R, const double *__restrict__ obs, double *__restrict__ temps, float *__restrict__ beta,
+    float *__restrict__ cosc, int *__restrict__ replica_at, uint32_t *__restrict__ ctl,
+    uint32_t key0, uint32_t key1, int parity, uint32_t round, int from_ctl, int group,
+    int last_group, uint32_t end_n, int R_all, const int *__restrict__ replica_at_all,
+    const double *__restrict__ anchor, const double *__restrict__ dE,
+    double *__restrict__ series) {
+  __shared__ uint32_t s_round;
+  __shared__ int s_nacc;
+  if (threadIdx.x == 0) {
+    s_round = from_ctl ? ctl[kCtlPtRound] : round;
+    s_nacc = 0;
+  }
+  __syncthreads();
+  const uint32_t rnd = s_round;
+  const int par = from_ctl ? (int)(rnd & 1u) : parity;
+  int nacc = 0;
+  for (int c = par + 2 * (int)threadIdx.x; c + 1 < R; c += 2 * (int)blockDim.x) {
+    const int t1 = replica_at[c], t2 = replica_at[c + 1];
+    const double T1 = temps[t1], T2 = temps[t2];
+    const double arg = (1.0 / T1 - 1.0 / T2) * (obs[(size_t)t1 * kObs] - obs[(size_t)t2 * kObs]);
+    const u32x4 w = philox4x32_10(
+        rng_counter((uint32_t)c, 0, 0, kKindSwap, (uint32_t)group, rnd), key0, key1);
+    const double u = u01_53(w.x, w.y);
+    if (arg >= 0.0 || u < exp(arg)) {
+      temps[t1] = T2;
+      temps[t2] = T1;
+      const float b = beta[t1];
+      beta[t1] = beta[t2];
+      beta[t2] = b;
+      const float cc = cosc[t1];
+      cosc[t1] = cosc[t2];
+      cosc[t2] = cc;
+      replica_at[c] = t2;
+      replica_at[c + 1] = t1;
+      ++nacc;
+    }
+  }
+  if (nacc) atomicAdd(&s_nacc, nacc);
+  __syncthreads();  // also makes every thread's replica_at writes visible to the block
+  if (threadIdx.x == 0) {
+    ctl[kCtlSwapsAcc] += (uint32_t)s_nacc;
+    if (from_ctl && last_group) ctl[kCtlPtRound] = rnd + 1u;
+  }
+  if (last_group && end_n > 0u)
+    end_interval(R_all, ctl, anchor, dE, replica_at_all, series, end_n, 1);
+}
+
+// End of n sweeps without an exchange (see end_interval). grid = 1, block = 256.
+__global__ __launch_bounds__(256) void k_end_sweep(int R, uint32_t *__restrict__ ctl,
+                                                   const double *__restrict__ anchor,
+                                                   const double *__restrict__ dE,
+                                                   const int *__restrict__ replica_at,
+                                                   double *__restrict__ series, uint32_t n,
+                                                   int record_last) {
+  end_interval(R, ctl, anchor, dE, replica_at, series, n, record_last);
 }
 
 // Series entries of one replica half after sweep ctl[kCtlSweep] + delta of a
@@ -259,8 +291,11 @@ int launch_overrelax_phase(pyro_ctx *ctx, int colour) {
   return 0;
 }
 
-int launch_observables(pyro_ctx *ctx) {
-  if (ctx->hetero) return launch_batch_observables(ctx);
+int launch_observables(pyro_ctx *ctx, bool anchor) {
+  if (ctx->hetero) {
+    PYRO_TRY(launch_batch_observables(ctx));
+    return anchor ? launch_set_anchor(ctx) : 0;
+  }
   if (ctx->mat) {
     PYRO_TRY(launch_observables_partials_mat(ctx));
   } else {
@@ -268,20 +303,21 @@ int launch_observables(pyro_ctx *ctx) {
     PYRO_HIP(hipGetLastError());
   }
   const double escale = ctx->mat ? -1.0 : 0.5 * (double)ctx->jeff;
-  k_observables_final<<<ctx->R, 256, 0, ctx->stream>>>(ctx->d_partials, ctx->obs_blocks, escale,
-                                                       ctx->d_obs);
+  k_observables_final<<<ctx->R, 256, 0, ctx->stream>>>(
+      ctx->d_partials, ctx->obs_blocks, escale, ctx->d_obs, anchor ? ctx->d_anchor : nullptr,
+      anchor ? ctx->d_dE : nullptr);
   PYRO_HIP(hipGetLastError());
   return 0;
 }
 
-int launch_pt_swap(pyro_ctx *ctx, int parity, uint32_t round, int from_ctl) {
+int launch_pt_swap(pyro_ctx *ctx, int parity, uint32_t round, int from_ctl, uint32_t end_n) {
   const int ng = (int)ctx->h_group_off.size();
   for (int g = 0; g < ng; ++g) {
-    k_pt_swap<<<1, 64, 0, ctx->stream>>>(ctx->h_group_len[g], ctx->d_obs, ctx->d_temps,
-                                         ctx->d_beta, ctx->d_cosc,
-                                         ctx->d_replica_at + ctx->h_group_off[g], ctx->d_ctl,
-                                         key_lo(ctx), key_hi(ctx), parity, round, from_ctl, g,
-                                         g == ng - 1);
+    k_pt_swap<<<1, 256, 0, ctx->stream>>>(
+        ctx->h_group_len[g], ctx->d_obs, ctx->d_temps, ctx->d_beta, ctx->d_cosc,
+        ctx->d_replica_at + ctx->h_group_off[g], ctx->d_ctl, key_lo(ctx), key_hi(ctx), parity,
+        round, from_ctl, g, g == ng - 1, end_n, ctx->R, ctx->d_replica_at, ctx->d_anchor,
+        ctx->d_dE, ctx->d_series);
     PYRO_HIP(hipGetLastError());
   }
   return 0;
