@@ -191,14 +191,14 @@ __global__ __launch_bounds__(256) void k_batch_phase(
     float *__restrict__ spins, const long long *__restrict__ off, const int *__restrict__ Ls,
     const int *__restrict__ tiles, int pass, const float *__restrict__ beta,
     const float *__restrict__ cosc, const uint32_t *__restrict__ ctl, uint32_t key0, uint32_t key1,
-    float jeff, unsigned long long *__restrict__ acc, double *__restrict__ dE) {
+    float jeff, unsigned long long *__restrict__ acc, double *__restrict__ dE, uint32_t sweep_delta) {
   const int r = tiles[2 * blockIdx.x];
   if (r < 0) return;  // padding tile: the whole block leaves together
   const int cell = tiles[2 * blockIdx.x + 1] * 256 + threadIdx.x;
   const int L = Ls[r];
   float *Sx = spins + off[r];
   const float b = KIND == 0 ? beta[r] : 0.0f, c = KIND == 0 ? cosc[r] : 0.0f;
-  const uint32_t sweep = KIND == 0 ? ctl[kCtlSweep] : 0u;
+  const uint32_t sweep = KIND == 0 ? ctl[kCtlSweep] + sweep_delta : 0u;
   double de = 0.0;
   unsigned nacc = 0;
   if (cell < L * L * L) {
@@ -304,12 +304,40 @@ static inline uint32_t bk0(const pyro_ctx *c) { return (uint32_t)c->seed; }
 static inline uint32_t bk1(const pyro_ctx *c) { return (uint32_t)(c->seed >> 32); }
 
 void batch_free(pyro_ctx *ctx) {
-  void *ptrs[] = {ctx->d_L, ctx->d_off, ctx->d_order, ctx->d_bin_start, ctx->d_tiles};
+  void *ptrs[] = {ctx->d_L,   ctx->d_off, ctx->d_order, ctx->d_bin_start, ctx->d_tiles,
+                  ctx->d_tiles_half[0], ctx->d_tiles_half[1]};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   ctx->d_L = nullptr;
   ctx->d_off = nullptr;
   ctx->d_order = ctx->d_bin_start = ctx->d_tiles = nullptr;
+  ctx->d_tiles_half[0] = ctx->d_tiles_half[1] = nullptr;
+}
+
+// Tiles of replicas [r_begin, r_end) in replica order, dealt to the 8 XCDs in
+// contiguous chunks: hardware block b runs on XCD b % 8, so list slot b takes
+// logical tile (b % 8) * chunk + b / 8. Padded to a multiple of 8 with r = -1.
+static std::vector<int> tile_list(const pyro_ctx *ctx, int r_begin, int r_end) {
+  std::vector<int> logical;
+  for (int r = r_begin; r < r_end; ++r) {
+    const int nc = ctx->h_L[r] * ctx->h_L[r] * ctx->h_L[r];
+    for (int t = 0; t < (nc + 255) / 256; ++t) {
+      logical.push_back(r);
+      logical.push_back(t);
+    }
+  }
+  const int nt = (int)logical.size() / 2;
+  const int chunk = (nt + 7) / 8;
+  const int padded = chunk * 8;
+  std::vector<int> tiles(2 * (size_t)padded, -1);
+  for (int b = 0; b < padded; ++b) {
+    const int lt = (b % 8) * chunk + b / 8;
+    if (lt < nt) {
+      tiles[2 * b] = logical[2 * lt];
+      tiles[2 * b + 1] = logical[2 * lt + 1];
+    }
+  }
+  return tiles;
 }
 
 // Geometry tables, persistent-kernel bins (LPT packing of work ~ L^3 into
@@ -332,28 +360,8 @@ int batch_setup(pyro_ctx *ctx, int n_bins) {
     flat.insert(flat.end(), bv.begin(), bv.end());
     start.push_back((int)flat.size());
   }
-  // Tiles in replica order, dealt to the 8 XCDs in contiguous chunks:
-  // hardware block b runs on XCD b % 8, so list slot b takes logical tile
-  // (b % 8) * chunk + b / 8.
-  std::vector<int> logical;
-  for (int r = 0; r < R; ++r) {
-    const int nc = ctx->h_L[r] * ctx->h_L[r] * ctx->h_L[r];
-    for (int t = 0; t < (nc + 255) / 256; ++t) {
-      logical.push_back(r);
-      logical.push_back(t);
-    }
-  }
-  const int nt = (int)logical.size() / 2;
-  const int chunk = (nt + 7) / 8;
-  const int padded = chunk * 8;
-  std::vector<int> tiles(2 * (size_t)padded, -1);
-  for (int b = 0; b < padded; ++b) {
-    const int lt = (b % 8) * chunk + b / 8;
-    if (lt < nt) {
-      tiles[2 * b] = logical[2 * lt];
-      tiles[2 * b + 1] = logical[2 * lt + 1];
-    }
-  }
+  const std::vector<int> tiles = tile_list(ctx, 0, R);
+  const std::vector<int> halves[2] = {tile_list(ctx, 0, R / 2), tile_list(ctx, R / 2, R)};
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   batch_free(ctx);
   PYRO_HIP(hipMalloc((void **)&ctx->d_L, sizeof(int) * R));
@@ -366,8 +374,15 @@ int batch_setup(pyro_ctx *ctx, int n_bins) {
   PYRO_HIP(hipMemcpy(ctx->d_order, flat.data(), sizeof(int) * R, hipMemcpyHostToDevice));
   PYRO_HIP(hipMemcpy(ctx->d_bin_start, start.data(), sizeof(int) * (n_bins + 1), hipMemcpyHostToDevice));
   PYRO_HIP(hipMemcpy(ctx->d_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice));
+  for (int hf = 0; hf < 2; ++hf) {
+    if (halves[hf].empty()) continue;
+    PYRO_HIP(hipMalloc((void **)&ctx->d_tiles_half[hf], sizeof(int) * halves[hf].size()));
+    PYRO_HIP(hipMemcpy(ctx->d_tiles_half[hf], halves[hf].data(), sizeof(int) * halves[hf].size(),
+                       hipMemcpyHostToDevice));
+    ctx->n_tiles_half[hf] = (int)halves[hf].size() / 2;
+  }
   ctx->n_bins = n_bins;
-  ctx->n_tiles = padded;
+  ctx->n_tiles = (int)tiles.size() / 2;
   return 0;
 }
 
@@ -409,11 +424,21 @@ static int launch_persistent_t(pyro_ctx *ctx, int n_sweeps, int record_last) {
   return 0;
 }
 
+// All tiles, or one half's list while a split sweep has set a launch range
+// (the range is always one of the halves [0, R/2), [R/2, R)).
 template <int KIND, int COLOUR, bool TRACK>
 static void launch_tiles_t(pyro_ctx *ctx, int pass) {
-  k_batch_phase<KIND, COLOUR, TRACK><<<ctx->n_tiles, 256, 0, ctx->stream>>>(
-      ctx->d_spins, ctx->d_off, ctx->d_L, ctx->d_tiles, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl,
-      bk0(ctx), bk1(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE);
+  const int *tiles = ctx->d_tiles;
+  int n = ctx->n_tiles;
+  if (ctx->launch_nrep) {
+    const int hf = ctx->launch_r0 == 0 ? 0 : 1;
+    tiles = ctx->d_tiles_half[hf];
+    n = ctx->n_tiles_half[hf];
+    if (n == 0) return;
+  }
+  k_batch_phase<KIND, COLOUR, TRACK><<<n, 256, 0, launch_stream(ctx)>>>(
+      ctx->d_spins, ctx->d_off, ctx->d_L, tiles, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl,
+      bk0(ctx), bk1(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE, ctx->launch_delta);
 }
 
 template <int KIND, bool TRACK>

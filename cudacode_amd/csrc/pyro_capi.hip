@@ -56,6 +56,10 @@ int reset_model_state(pyro_ctx *ctx) {
 // over-relaxation leaves dE final after the Metropolis and learned phases),
 // else by a separate launch at the end.
 static int enqueue_phases(pyro_ctx *ctx, bool record = false) {
+  if (ctx->exec_mode == 2) {  // tile lists: no series hook in these kernels
+    PYRO_TRY(launch_batch_sweep_body(ctx));
+    return record ? launch_record_half(ctx) : 0;
+  }
   for (int p = 0; p < ctx->n_metro; ++p)
     for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_metropolis_phase(ctx, c, p, nullptr));
   for (int p = 0; p < ctx->n_learned; ++p)
@@ -75,7 +79,8 @@ static int enqueue_phases(pyro_ctx *ctx, bool record = false) {
 }
 
 static bool split_sweep(const pyro_ctx *ctx) {
-  return ctx->split && ctx->stream2 && ctx->exec_mode == 0 && !ctx->hetero && ctx->R >= 2;
+  return ctx->split && ctx->stream2 && ctx->R >= 2 &&
+         (ctx->exec_mode == 2 || (ctx->exec_mode == 0 && !ctx->hetero));
 }
 
 // Ends a split run: the second stream joins the ctx stream, and the sweeps
@@ -149,10 +154,8 @@ static int enqueue_interval_end(pyro_ctx *ctx, bool exchange, bool measure, uint
 static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
   const bool exchange = exchange_after(ctx, sweep_index);
   const bool measure = measure_after(ctx, sweep_index);
-  const bool split = ctx->exec_mode == 0 && split_sweep(ctx);
-  if (ctx->exec_mode != 0) {
-    PYRO_TRY(launch_batch_sweep_body(ctx));
-  } else if (split) {
+  const bool split = split_sweep(ctx);
+  if (split) {
     PYRO_TRY(enqueue_split_phases(ctx, exchange || measure));
     if (!(exchange || measure)) return 0;
     // Join without booking yet: the counters advance after the exchange, as
@@ -160,6 +163,8 @@ static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
     PYRO_HIP(hipEventRecord(ctx->ev_join, ctx->stream2));
     PYRO_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
     ctx->forked = 0;
+  } else if (ctx->exec_mode != 0) {
+    PYRO_TRY(launch_batch_sweep_body(ctx));
   } else {
     PYRO_TRY(enqueue_phases(ctx));
   }
@@ -332,7 +337,7 @@ static int create_ctx(const std::vector<int> &Ls, const double *temps, uint64_t 
   } while (0)
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(set_error(PYRO_EHIP, "pyro_create: stream creation failed"));
-  if (!ctx->hetero && R >= 2) {  // second stream for split sweeps (default priority)
+  if (R >= 2) {  // second stream for split sweeps (default priority)
     if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess)
@@ -340,8 +345,9 @@ static int create_ctx(const std::vector<int> &Ls, const double *temps, uint64_t 
     // Two streams pay when each half's launch still fills the chip; below
     // that, the fork and join cost more than the overlap hides (measured:
     // L=16 x 64 replicas 7% faster, L=4 x 64 2.2x slower).
-    const int bpr = (int)((ctx->ncell + 255) / 256);
-    ctx->split = (long long)bpr * (R / 2) >= kSplitMinBlocks;
+    long long half_blocks = 0;
+    for (int r = 0; r < R / 2; ++r) half_blocks += ((long long)Ls[r] * Ls[r] * Ls[r] + 255) / 256;
+    ctx->split = half_blocks >= kSplitMinBlocks;
   }
   ALLOC(ctx->d_spins, total);
   ALLOC(ctx->d_temps, R);
@@ -827,6 +833,7 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
     cands.push_back({1, 512, 1, 1});
     cands.push_back({1, 1024, 1, 1});
     cands.push_back({2, 256, 1, 1});
+    if (ctx->stream2) cands.push_back({2, 256, 1, 2});
   }
   if (cands.empty()) return 0;
   // Snapshot of everything a sweep changes.
@@ -1033,7 +1040,7 @@ int pyro_set_streams(pyro_ctx *ctx, int n) {
   if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
   if (n != 1 && n != 2) return set_error(PYRO_EINVAL, "pyro_set_streams: %d streams (1 or 2)", n);
   if (n == 2 && !ctx->stream2)
-    return set_error(PYRO_ESTATE, "pyro_set_streams: two streams need a uniform lattice and >= 2 replicas");
+    return set_error(PYRO_ESTATE, "pyro_set_streams: two streams need >= 2 replicas");
   PYRO_TRY(device_guard(ctx));
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   drop_graph(ctx);

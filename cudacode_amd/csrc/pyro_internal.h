@@ -111,6 +111,9 @@ struct pyro_ctx {
   int n_bins = 0;
   int *d_tiles = nullptr;           // 2 ints per tile: replica, tile index (-1: padding)
   int n_tiles = 0;
+  // The same lists for replicas [0, R/2) and [R/2, R) (split sweeps).
+  int *d_tiles_half[2] = {nullptr, nullptr};
+  int n_tiles_half[2] = {0, 0};
 
   // learned proposal
   int n_learned = 0;                    // learned passes per sweep (after Metropolis)

@@ -101,7 +101,7 @@ int pyro_get_exec_mode(pyro_ctx *ctx, int *mode, int *block, int *n_bins);
  * cell- and site-per-thread kernels on one or two streams, persistent kernel
  * at 256/512/1024 threads, tile list), captured in one graph as pyro_sweep
  * replays them, restoring the state after each trial, and keep the fastest.
- * results (optional, 5 doubles per candidate, at most 8 candidates): mode,
+ * results (optional, 5 doubles per candidate, at most 16 candidates): mode,
  * block, phase variant, streams, ms per sweep. */
 int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results);
 
@@ -247,13 +247,15 @@ int pyro_measure_fetch(pyro_ctx *ctx, double *rows, int *n_rows);
  * reference's --McSteps/--OverRelaxation options, Pyrochlore.cu:1161-1216). */
 int pyro_set_passes(pyro_ctx *ctx, int n_metro, int n_or);
 
-/* Sweeps of a uniform lattice with >= 2 replicas can run the phases of two
- * replica halves on two streams (n = 2), joined every sweep, or on one
- * stream (n = 1). Results are identical either way (no reference
- * counterpart: a performance setting; two streams hide part of each short
- * phase kernel's ramp). Default: 2 when each half spans >= 256 workgroups of
- * 256 cells, else 1; pyro_autotune also tries both. Learned passes take
- * the same split. */
+/* With >= 2 replicas, the phases of the two replica halves [0, R/2) and
+ * [R/2, R) can run on two streams (n = 2) that join at exchanges,
+ * measurements and the end of each call, or on one stream (n = 1), in
+ * execution modes 0 and 2. Spins, rungs and counters are identical either
+ * way; energies agree to FP64 rounding (no reference counterpart: a
+ * performance setting; two streams hide part of each short phase kernel's
+ * ramp). Default: 2 when the first half spans >= 256 workgroups of 256
+ * cells, else 1; pyro_autotune also tries both. Learned passes take the same
+ * split. */
 int pyro_set_streams(pyro_ctx *ctx, int n);
 int pyro_get_streams(pyro_ctx *ctx, int *n);
 

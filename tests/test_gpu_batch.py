@@ -130,7 +130,10 @@ def test_autotune_keeps_the_state():
         h.Mc(4)
         before = g.get_spins()
         res = g.autotune(3)
-        assert len(res) == 4 and np.all(res[:, 3] == 1) and np.all(res[:, 4] > 0)
+        # persistent at 256/512/1024 threads and tile lists, on one stream;
+        # tile lists on two streams
+        assert len(res) == 5 and np.all(res[:, 4] > 0)
+        assert res[:, 3].tolist() == [1, 1, 1, 1, 2] and res[4, 0] == 2
         say(f"autotune results (mode, block, variant, streams, ms/sweep): {res.tolist()}")
         assert np.array_equal(g.get_spins(), before)
         assert g.sweep_counters() == h.sweep_counters()
@@ -193,3 +196,45 @@ def test_persistent_single_small_lattice_equals_phases():
     o = oracle.OracleRun(4, [0.5], seed=SEED, **kw)
     o.run(40)
     assert np.array_equal(spins, o.spins)
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_tile_list_two_streams_identical(graph):
+    """Mixed lattice sizes in the tile-list execution with the replica halves
+    on two streams (each half its own XCD-dealt tile list, joined at
+    exchanges and measurements) equal one stream: spins, rungs, acceptance,
+    sweep counters and measurement rows bit for bit, the energy series to FP64
+    rounding; and each equal-size group (the L=8 group spans both halves)
+    equals the oracle."""
+    Ls = [4, 4, 4, 8, 8, 8, 12, 12]
+    temps = [0.2, 0.5, 1.1, 0.3, 0.9, 1.5, 0.4, 0.8]
+    kw = dict(n_metro=1, n_or=2, pt_every=3, cos_cone=0.4)
+    with batch(Ls, temps, **kw) as a, batch(Ls, temps, **kw) as b:
+        for g, n in ((a, 2), (b, 1)):
+            g.set_exec_mode(2, 256)
+            g.set_streams(n)
+            g.record_start(30)
+            g.measure_start(4, 8)
+            g.Mc(5, use_graph=graph)
+            g.Mc(31, use_graph=graph)
+        assert a.streams() == 2 and b.streams() == 1
+        assert np.array_equal(a.get_spins(), b.get_spins())
+        assert np.array_equal(a.temperatures()[1], b.temperatures()[1])
+        assert np.array_equal(a.acceptance()[0], b.acceptance()[0])
+        assert a.sweep_counters() == b.sweep_counters()
+        assert np.array_equal(a.measure_fetch(), b.measure_fetch())
+        ra, rb = a.record_fetch(), b.record_fetch()
+        assert ra.shape == (30, 8) and np.allclose(ra, rb, rtol=1e-11, atol=0)
+        s, (_, off) = a.get_spins(), a.layout()
+    for gi, L in enumerate((4, 8, 12)):
+        members = [r for r, l in enumerate(Ls) if l == L]
+        oracle.set_replica_base(members[0])
+        oracle.set_swap_group(gi)
+        try:
+            o = oracle.OracleRun(L, [temps[m] for m in members], seed=SEED, **kw)
+            o.run(36)
+        finally:
+            oracle.set_replica_base(0)
+            oracle.set_swap_group(0)
+        got = np.concatenate([s[off[m]: off[m] + 48 * L ** 3] for m in members])
+        assert np.array_equal(got, o.spins), gi
