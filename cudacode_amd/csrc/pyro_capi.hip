@@ -86,7 +86,8 @@ static bool split_sweep(const pyro_ctx *ctx) {
 // Ends a split run: the second stream joins the ctx stream, and the sweeps
 // made since the fork are booked (sweep counter, series rows). With
 // record_last the series row of the last of them is written here (after an
-// exchange moved the rungs); otherwise every row came from k_record_half.
+// exchange moved the rungs); otherwise the phase launches of each half wrote
+// every row (enqueue_phases with record).
 static int split_join(pyro_ctx *ctx, int record_last) {
   if (!ctx->forked) return 0;
   PYRO_HIP(hipEventRecord(ctx->ev_join, ctx->stream2));
@@ -124,12 +125,7 @@ static int enqueue_split_phases(pyro_ctx *ctx, bool join_after) {
   return rc;
 }
 
-// One sweep as kernel launches (also the body that gets captured). Replicas
-// are independent within a sweep, so the phases of two replica halves can run
-// on two streams: the second half's kernels fill the GPU while the first
-// half's drain, which hides most of the per-launch ramp of these short
-// kernels. The streams run apart until the next exchange or measurement
-// (which need every replica) or the end of the call (split_join).
+// Whether an exchange / a measurement follows sweep `sweep_index`.
 static bool exchange_after(const pyro_ctx *ctx, uint32_t sweep_index) {
   return ctx->pt_every > 0 && (sweep_index + 1u) % (uint32_t)ctx->pt_every == 0u;
 }
@@ -151,6 +147,12 @@ static int enqueue_interval_end(pyro_ctx *ctx, bool exchange, bool measure, uint
   return launch_end_sweep(ctx, n, 1);
 }
 
+// One sweep as kernel launches (also the body that gets captured). Replicas
+// are independent between exchanges, so the phases of two replica halves can
+// run on two streams: the second half's kernels fill the GPU while the first
+// half's drain, which hides most of the per-launch ramp of these short
+// kernels. The streams run apart until the next exchange or measurement
+// (which need every replica) or the end of the call (split_join).
 static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
   const bool exchange = exchange_after(ctx, sweep_index);
   const bool measure = measure_after(ctx, sweep_index);
