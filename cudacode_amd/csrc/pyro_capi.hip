@@ -65,7 +65,18 @@ static int enqueue_phases(pyro_ctx *ctx, bool record = false) {
   for (int p = 0; p < ctx->n_learned; ++p)
     for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_learned_phase(ctx, c, p));
   ctx->launch_series_done = 0;
-  for (int o = 0; o < ctx->n_or; ++o)
+  int o = 0;
+  if (or_fusable(ctx)) {
+    // An even number of fused passes (two colours per launch, alternating
+    // between the spin buffers) ends in d_spins; an odd last pass runs in place.
+    for (; o < (ctx->n_or & ~1); ++o) {
+      ctx->launch_series = record && o == 0 ? ctx->d_series : nullptr;
+      const int rc = launch_overrelax_pass_fused(ctx, o & 1);
+      ctx->launch_series = nullptr;
+      if (rc) return rc;
+    }
+  }
+  for (; o < ctx->n_or; ++o)
     for (int c = 0; c < kColours; ++c) {
       ctx->launch_series = record && o == 0 && c == 0 ? ctx->d_series : nullptr;
       const int rc = launch_overrelax_phase(ctx, c);
@@ -352,6 +363,12 @@ static int create_ctx(const std::vector<int> &Ls, const double *temps, uint64_t 
     ctx->split = half_blocks >= kSplitMinBlocks;
   }
   ALLOC(ctx->d_spins, total);
+  // The second buffer of the fused over-relaxation passes (uniform lattices;
+  // without it, or if it does not fit, the passes run one colour per launch).
+  if (!ctx->hetero && hipMalloc((void **)&ctx->d_spins_alt, sizeof(float) * total) != hipSuccess) {
+    ctx->d_spins_alt = nullptr;
+    (void)hipGetLastError();
+  }
   ALLOC(ctx->d_temps, R);
   ALLOC(ctx->d_beta, R);
   ALLOC(ctx->d_cosc, R);
@@ -415,15 +432,15 @@ int pyro_destroy(pyro_ctx *ctx) {
   if (!ctx) return 0;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
   drop_graph(ctx);
   mlp_free(ctx);
   batch_free(ctx);
   void *ptrs[] = {ctx->d_spins, ctx->d_temps, ctx->d_beta,     ctx->d_cosc, ctx->d_replica_at,
                   ctx->d_acc,   ctx->d_dE,    ctx->d_anchor,   ctx->d_partials, ctx->d_obs,
-                  ctx->d_ctl,   ctx->d_series, ctx->d_meas, ctx->d_coup};
+                  ctx->d_ctl,   ctx->d_series, ctx->d_meas, ctx->d_coup, ctx->d_spins_alt};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
-  if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
   if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
   if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
   if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
@@ -1050,6 +1067,15 @@ int pyro_set_streams(pyro_ctx *ctx, int n) {
   return 0;
 }
 
+int pyro_set_or_fusion(pyro_ctx *ctx, int on) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  drop_graph(ctx);
+  ctx->fuse_or = on ? 1 : 0;
+  return 0;
+}
+
 int pyro_get_streams(pyro_ctx *ctx, int *n) {
   if (!ctx || !n) return set_error(PYRO_EINVAL, "null argument");
   *n = ctx->split ? 2 : 1;
@@ -1148,8 +1174,10 @@ int pyro_measure_fetch(pyro_ctx *ctx, double *rows, int *n_rows) {
 
 int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, double *span_ms) {
   if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
-  if (kind < 0 || kind > 2 || n_launches < 1 || n_launches > 100000)
+  if (kind < 0 || kind > 3 || n_launches < 1 || n_launches > 100000)
     return set_error(PYRO_EINVAL, "pyro_time_phase: kind %d / n_launches %d", kind, n_launches);
+  if (kind == 3 && !or_fusable(ctx))
+    return set_error(PYRO_ESTATE, "pyro_time_phase: no fused over-relaxation for this ctx");
   PYRO_TRY(device_guard(ctx));
   std::vector<hipEvent_t> ev(2 * (size_t)n_launches, nullptr);
   int rc = 0;
@@ -1163,9 +1191,14 @@ int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, dou
   for (int i = 0; i < n_launches && rc == 0; ++i) {
     ctx->time_ev0 = ev[2 * i];
     ctx->time_ev1 = ev[2 * i + 1];
-    rc = kind == 0   ? launch_metropolis_phase(ctx, i & 3, 0, nullptr)
-         : kind == 1 ? launch_overrelax_phase(ctx, i & 3)
-                     : launch_learned_phase(ctx, i & 3, 0);
+    if (kind == 3) {  // fused pairs (0,1) and (2,3) from d_spins into the second buffer:
+                      // d_spins, the state, is left as it was
+      rc = launch_overrelax_pair(ctx, 2 * (i & 1), ctx->d_spins, ctx->d_spins_alt);
+    } else {
+      rc = kind == 0   ? launch_metropolis_phase(ctx, i & 3, 0, nullptr)
+           : kind == 1 ? launch_overrelax_phase(ctx, i & 3)
+                       : launch_learned_phase(ctx, i & 3, 0);
+    }
     ctx->time_ev0 = ctx->time_ev1 = nullptr;
   }
   if (rc == 0 && hipStreamSynchronize(ctx->stream) != hipSuccess)

@@ -168,6 +168,111 @@ __device__ __forceinline__ void cell_update(float *__restrict__ Sx, int cell, in
   }
 }
 
+// Two over-relaxation colour phases CA, CA + 1 of one cell in one pass over
+// the data, scalar model (cur/oth: this replica's spins in the two buffers).
+// Colours below CA have already been updated this pass and are read from
+// oth; the others are read from cur; the two new colours are written to oth,
+// so no thread reads what another thread of the launch writes. Phase CA + 1
+// needs the new colour-CA values of its neighbours: those in this cell come
+// from the first phase, those in the neighbouring cells are recomputed here
+// with the owner's float expressions in the owner's order (over-relaxation is
+// deterministic), so every result equals two separate phase launches bit for
+// bit. TRACK: FP64 energy changes of this cell's updates added to de.
+template <int CA, int LT, bool TRACK>
+__device__ __forceinline__ void or_pair_cell(const float *__restrict__ cur,
+                                             float *__restrict__ oth, int cell, float jeff,
+                                             double &de) {
+  static_assert(CA == 0 || CA == 2, "colour pairs (0, 1) and (2, 3)");
+  constexpr int CB = CA + 1, L = LT, LL = LT * LT, NC = LT * LT * LT, P = 16 * NC;
+  const int z = cell / LL;
+  const int y = (cell / L) % L;
+  const int x = cell % L;
+  int X[5], Y[5], Z[5];  // wrapped coordinates at offsets -2 .. 2
+#pragma unroll
+  for (int d = -2; d <= 2; ++d) {
+    X[d + 2] = (x + d + L) % L;
+    Y[d + 2] = ((y + d + L) % L) * L;
+    Z[d + 2] = ((z + d + L) % L) * LL;
+  }
+  // Spin of `site` in the cell at offset (dx, dy, dz), from the buffer that
+  // holds its current value (resolved at compile time once unrolled).
+  auto ld = [&](int site, int dx, int dy, int dz) -> f3 {
+    const int j = site * NC + Z[dz + 2] + Y[dy + 2] + X[dx + 2];
+    const float *b = (site & 3) < CA ? oth : cur;
+    return {b[j], b[P + j], b[2 * P + j]};
+  };
+  // Phase-CA update of colour-CA site `s` of the cell at offset (ox, oy, oz),
+  // with its FP64 neighbour sum.
+  auto upd_a = [&](int s, int ox, int oy, int oz, double &gx, double &gy, double &gz) -> f3 {
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    gx = gy = gz = 0.0;
+#pragma unroll
+    for (int n = 0; n < 6; ++n) {
+      const Link lk = kLinks[s][n];
+      const f3 v = ld(lk.site, ox + lk.dx, oy + lk.dy, oz + lk.dz);
+      fx = fx + v.x;
+      fy = fy + v.y;
+      fz = fz + v.z;
+      if (TRACK) {
+        gx += (double)v.x;
+        gy += (double)v.y;
+        gz += (double)v.z;
+      }
+    }
+    return overrelax({fx, fy, fz}, ld(s, ox, oy, oz));
+  };
+  f3 na[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double gx, gy, gz;
+    na[k] = upd_a(CA + 4 * k, 0, 0, 0, gx, gy, gz);
+    if (TRACK) de += field_de<false>(jeff, nullptr, CA, ld(CA + 4 * k, 0, 0, 0), na[k], gx, gy, gz);
+  }
+  f3 nb[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int t = CB + 4 * k;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+    for (int n = 0; n < 6; ++n) {
+      const Link lk = kLinks[t][n];
+      f3 v;
+      if ((lk.site & 3) == CA) {
+        if (lk.dx == 0 && lk.dy == 0 && lk.dz == 0) {
+          v = na[lk.site >> 2];
+        } else {
+          double hx, hy, hz;
+          v = upd_a(lk.site, lk.dx, lk.dy, lk.dz, hx, hy, hz);
+        }
+      } else {
+        v = ld(lk.site, lk.dx, lk.dy, lk.dz);
+      }
+      fx = fx + v.x;
+      fy = fy + v.y;
+      fz = fz + v.z;
+      if (TRACK) {
+        gx += (double)v.x;
+        gy += (double)v.y;
+        gz += (double)v.z;
+      }
+    }
+    const f3 told = ld(t, 0, 0, 0);
+    nb[k] = overrelax({fx, fy, fz}, told);
+    if (TRACK) de += field_de<false>(jeff, nullptr, CB, told, nb[k], gx, gy, gz);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int ia = (CA + 4 * k) * NC + cell, ib = (CB + 4 * k) * NC + cell;
+    oth[ia] = na[k].x;
+    oth[P + ia] = na[k].y;
+    oth[2 * P + ia] = na[k].z;
+    oth[ib] = nb[k].x;
+    oth[P + ib] = nb[k].y;
+    oth[2 * P + ib] = nb[k].z;
+  }
+}
+
 // One site's update (site = colour + 4k, runtime), for kernels that cannot
 // afford the registers of cell_update (its 4 concurrent targets): the same
 // float expressions in the same order, so the same results bit for bit.

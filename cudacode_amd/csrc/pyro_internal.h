@@ -156,6 +156,11 @@ struct pyro_ctx {
   // launch_series_done is set if the launched kernel took it.
   double *launch_series = nullptr;
   int launch_series_done = 0;
+  // Second spin buffer for the fused over-relaxation passes (k_or_pair): a
+  // pass reads one buffer and writes the other, so passes alternate and an
+  // even number of them ends back in d_spins. nullptr: no fusion.
+  float *d_spins_alt = nullptr;
+  int fuse_or = 1;
   float *d_coup = nullptr;
   float h_coup[pyro::kCoupFloats] = {};
 
@@ -193,6 +198,12 @@ int set_error(int code, const char *fmt, ...);
 int launch_init_spins(pyro_ctx *ctx);
 int launch_metropolis_phase(pyro_ctx *ctx, int colour, int pass, const uint32_t *sweep_override);
 int launch_overrelax_phase(pyro_ctx *ctx, int colour);
+// Whether the over-relaxation passes of a sweep run fused (two colours per
+// launch between the spin buffers), and one fused pass: parity 0 reads
+// d_spins and writes d_spins_alt, parity 1 the reverse.
+bool or_fusable(const pyro_ctx *ctx);
+int launch_overrelax_pass_fused(pyro_ctx *ctx, int parity);
+int launch_overrelax_pair(pyro_ctx *ctx, int ca, const float *cur, float *oth);
 // Fills d_obs; with anchor, also re-anchors the incremental energy (anchor = E, dE = 0).
 int launch_observables(pyro_ctx *ctx, bool anchor = false);
 // end_n > 0: the exchange also ends an interval of end_n sweeps (series row

@@ -283,6 +283,32 @@ int launch_metropolis_phase(pyro_ctx *ctx, int colour, int pass, const uint32_t 
   return 0;
 }
 
+bool or_fusable(const pyro_ctx *ctx) {
+  if (!ctx->fuse_or || !ctx->d_spins_alt || ctx->hetero || ctx->mat || ctx->exec_mode != 0 ||
+      ctx->phase_variant != 1)
+    return false;
+  switch (ctx->L) {
+    case 4: case 8: case 12: case 16: case 24: case 32: case 64: return true;
+    default: return false;
+  }
+}
+
+int launch_overrelax_pair(pyro_ctx *ctx, int ca, const float *cur, float *oth) {
+  if (!launch_or_pair(ctx, ca, cur, oth))
+    return set_error(PYRO_ESTATE, "fused over-relaxation: no compiled copy for L=%d", ctx->L);
+  PYRO_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_overrelax_pass_fused(pyro_ctx *ctx, int parity) {
+  const float *cur = parity ? ctx->d_spins_alt : ctx->d_spins;
+  float *oth = parity ? ctx->d_spins : ctx->d_spins_alt;
+  if (!launch_or_pair(ctx, 0, cur, oth) || !launch_or_pair(ctx, 2, cur, oth))
+    return set_error(PYRO_ESTATE, "fused over-relaxation: no compiled copy for L=%d", ctx->L);
+  PYRO_HIP(hipGetLastError());
+  return 0;
+}
+
 int launch_overrelax_phase(pyro_ctx *ctx, int colour) {
   if (ctx->hetero) return set_error(PYRO_ESTATE, "single colour phases need a uniform lattice size");
   if (ctx->mat) return launch_phase_mat(ctx, 1, colour, 0);

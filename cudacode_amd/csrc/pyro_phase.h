@@ -195,6 +195,45 @@ __global__ __launch_bounds__(256) void k_phase_cells(
   }
 }
 
+// Two over-relaxation colour phases (CA, CA + 1) in one launch between the two
+// spin buffers (or_pair_cell), scalar model. Same block layout, XCD remap and
+// series hook as k_phase_cells; a pass is k_or_pair<0> then k_or_pair<2>,
+// after which `oth` holds every colour and the buffers swap roles.
+// Algorithmic bytes: 12 N read + 6 N written for N/2 updates = 36 B/update.
+template <int CA, int LT, bool TRACK>
+__global__ __launch_bounds__(256) void k_or_pair(
+    const float *__restrict__ cur, float *__restrict__ oth, int blocks_per_replica, float jeff,
+    double *__restrict__ dE, int r0, const uint32_t *__restrict__ ctl, uint32_t sweep_delta,
+    double *__restrict__ series, int R_total, const int *__restrict__ replica_at,
+    const double *__restrict__ anchor) {
+  constexpr int NC = LT * LT * LT;
+  const int nb = gridDim.x;
+  const int b = blockIdx.x;
+  const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+  const int q = lb / blocks_per_replica;
+  const int r = r0 + q;
+  const int cell = (lb - q * blocks_per_replica) * 256 + threadIdx.x;
+  if (!TRACK && series != nullptr && b == 0) {
+    const uint32_t row = ctl[kCtlRecordRow] + sweep_delta;
+    if (ctl[kCtlRecordOn] && row < ctl[kCtlRecordCap]) {
+      const int r1 = r0 + nb / blocks_per_replica;
+      for (int k = threadIdx.x; k < R_total; k += 256) {
+        const int rep = replica_at[k];
+        if (rep >= r0 && rep < r1) series[(size_t)row * R_total + k] = anchor[rep] + dE[rep];
+      }
+    }
+  }
+  double de = 0.0;
+  if (cell < NC)
+    or_pair_cell<CA, LT, TRACK>(cur + (size_t)r * 48 * NC, oth + (size_t)r * 48 * NC, cell, jeff,
+                                de);
+  if (TRACK) {
+    unsigned nacc = 0;
+    block_sum<256>(de, nacc);
+    if (threadIdx.x == 0 && de != 0.0) atomicAdd(&dE[r], de);
+  }
+}
+
 // Per-block FP64 partial sums of the energy and the sublattice-b magnetisation
 // (TotalEnergyGPU, Pyrochlore.cu:344-389; MagnetizationGPU, :277-342;
 // Measurement, :526-578). grid = (obs_blocks, 4, R): blockIdx.y is the basis
@@ -352,6 +391,54 @@ static bool launch_cells(pyro_ctx *ctx, int colour, int pass) {
     case 24: launch_cells_l<KIND, 24, MAT>(ctx, colour, pass); return true;
     case 32: launch_cells_l<KIND, 32, MAT>(ctx, colour, pass); return true;
     case 64: launch_cells_l<KIND, 64, MAT>(ctx, colour, pass); return true;
+    default: return false;
+  }
+}
+
+template <int CA, int LT, bool TRACK>
+static void launch_or_pair_t(pyro_ctx *ctx, const float *cur, float *oth) {
+  const int bpr = (LT * LT * LT + 255) / 256;
+  const int nrep = launch_nrep(ctx);
+  double *series = nullptr;
+  if (CA == 0 && !TRACK && ctx->launch_series) {
+    series = ctx->launch_series;
+    ctx->launch_series_done = 1;
+  }
+  if (ctx->time_ev0) {
+    hipExtLaunchKernelGGL(k_or_pair<CA, LT, TRACK>, dim3(bpr * nrep), dim3(256), 0,
+                          launch_stream(ctx), ctx->time_ev0, ctx->time_ev1, 0, cur, oth, bpr,
+                          ctx->jeff, ctx->d_dE, launch_r0(ctx), (const uint32_t *)ctx->d_ctl,
+                          ctx->launch_delta, series, ctx->R, (const int *)ctx->d_replica_at,
+                          (const double *)ctx->d_anchor);
+    return;
+  }
+  k_or_pair<CA, LT, TRACK><<<bpr * nrep, 256, 0, launch_stream(ctx)>>>(
+      cur, oth, bpr, ctx->jeff, ctx->d_dE, launch_r0(ctx), ctx->d_ctl, ctx->launch_delta, series,
+      ctx->R, ctx->d_replica_at, ctx->d_anchor);
+}
+
+template <int LT>
+static void launch_or_pair_l(pyro_ctx *ctx, int ca, const float *cur, float *oth) {
+  if (ctx->track_or) {
+    if (ca == 0) launch_or_pair_t<0, LT, true>(ctx, cur, oth);
+    else launch_or_pair_t<2, LT, true>(ctx, cur, oth);
+  } else {
+    if (ca == 0) launch_or_pair_t<0, LT, false>(ctx, cur, oth);
+    else launch_or_pair_t<2, LT, false>(ctx, cur, oth);
+  }
+}
+
+// The fused over-relaxation colour pair ca (0 or 2) from cur into oth, for
+// the compile-time lattice sizes; false when L has no copy.
+static inline bool launch_or_pair(pyro_ctx *ctx, int ca, const float *cur, float *oth) {
+  switch (ctx->L) {
+    case 4: launch_or_pair_l<4>(ctx, ca, cur, oth); return true;
+    case 8: launch_or_pair_l<8>(ctx, ca, cur, oth); return true;
+    case 12: launch_or_pair_l<12>(ctx, ca, cur, oth); return true;
+    case 16: launch_or_pair_l<16>(ctx, ca, cur, oth); return true;
+    case 24: launch_or_pair_l<24>(ctx, ca, cur, oth); return true;
+    case 32: launch_or_pair_l<32>(ctx, ca, cur, oth); return true;
+    case 64: launch_or_pair_l<64>(ctx, ca, cur, oth); return true;
     default: return false;
   }
 }

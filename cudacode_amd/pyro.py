@@ -34,7 +34,7 @@ EXPORTS = [
     "pyro_energy_incremental",
     "pyro_acceptance", "pyro_reset_acceptance", "pyro_adapt_cone", "pyro_record_start",
     "pyro_record_fetch", "pyro_set_couplings", "pyro_get_couplings", "pyro_exchange_matrices",
-    "pyro_zeeman_fields", "pyro_set_streams", "pyro_get_streams", "pyro_check_spins", "pyro_set_post_rotate", "pyro_anneal", "pyro_measure_start", "pyro_measure_stop", "pyro_measure_fetch",
+    "pyro_zeeman_fields", "pyro_set_streams", "pyro_get_streams", "pyro_set_or_fusion", "pyro_check_spins", "pyro_set_post_rotate", "pyro_anneal", "pyro_measure_start", "pyro_measure_stop", "pyro_measure_fetch",
     "pyro_set_passes", "pyro_set_kernel_options", "pyro_create_batch",
     "pyro_get_layout", "pyro_set_exec_mode", "pyro_get_exec_mode", "pyro_autotune",
     "pyro_checkpoint_save", "pyro_checkpoint_load", "pyro_export_configurations",
@@ -130,6 +130,7 @@ def lib():
         L.pyro_set_post_rotate.argtypes = [vp, C.c_int]
         L.pyro_set_streams.argtypes = [vp, C.c_int]
         L.pyro_get_streams.argtypes = [vp, C.POINTER(C.c_int)]
+        L.pyro_set_or_fusion.argtypes = [vp, C.c_int]
         L.pyro_check_spins.argtypes = [vp, C.c_double, C.POINTER(C.c_longlong),
                                        C.POINTER(C.c_longlong)]
         L.pyro_set_couplings.argtypes = [vp, vp, vp]
@@ -419,6 +420,10 @@ class Pyrochlore:
         """1 or 2 streams for the phases of a sweep (identical results)."""
         _check(lib().pyro_set_streams(self._ctx, int(n)))
 
+    def set_or_fusion(self, on: bool) -> None:
+        """Two over-relaxation colours per launch (default) or one (identical results)."""
+        _check(lib().pyro_set_or_fusion(self._ctx, 1 if on else 0))
+
     def streams(self) -> int:
         n = C.c_int()
         _check(lib().pyro_get_streams(self._ctx, C.byref(n)))
@@ -580,7 +585,8 @@ class Pyrochlore:
 
     def time_phase(self, kind: int, n_launches: int):
         """(mean ms per launch, first-to-last ms) of n back-to-back phase launches
-        (kind 0 Metropolis, 1 over-relaxation, 2 learned), timed with HIP events on the ctx stream."""
+        (kind 0 Metropolis, 1 over-relaxation, 2 learned, 3 fused over-relaxation
+        pair), timed with HIP events on the ctx stream."""
         avg, span = C.c_double(), C.c_double()
         _check(lib().pyro_time_phase(self._ctx, kind, n_launches, C.byref(avg), C.byref(span)))
         return avg.value, span.value

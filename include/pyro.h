@@ -259,6 +259,15 @@ int pyro_set_passes(pyro_ctx *ctx, int n_metro, int n_or);
 int pyro_set_streams(pyro_ctx *ctx, int n);
 int pyro_get_streams(pyro_ctx *ctx, int *n);
 
+/* Over-relaxation passes fused two colours per launch (on = 1, default where
+ * it applies: uniform lattice, execution mode 0, cell-per-thread kernels,
+ * scalar model) or one colour per launch (on = 0). A fused launch reads the
+ * spins from one buffer and writes the two updated colours to a second one
+ * (allocated with the ctx), recomputing the neighbouring first-colour values
+ * that the second colour needs, so a pass moves 36 N instead of 60 N bytes;
+ * results are identical (no reference counterpart: a performance setting). */
+int pyro_set_or_fusion(pyro_ctx *ctx, int on);
+
 /* Spin guard (no reference counterpart; the reference checks nothing): counts
  * the spins that are not finite or whose squared norm differs from 1 by more
  * than tol (FP32 updates keep it within a few 1e-7), and returns the
@@ -341,11 +350,12 @@ int pyro_import_configurations(pyro_ctx *ctx, const char *prefix, int rep, int b
 
 /* Time n_launches back-to-back colour-phase launches of one kind (0 =
  * Metropolis, 1 = over-relaxation, 2 = learned proposal, which needs weights
- * set; colours cycle 0..3) on the ctx stream,
+ * set; colours cycle 0..3; 3 = fused over-relaxation pairs (0,1), (2,3)
+ * alternating) on the ctx stream,
  * each launch carrying its own start/stop HIP events in its dispatch
  * (hipExtLaunchKernelGGL). avg_ms: mean kernel time per launch; span_ms:
- * first start to last stop. The phases run for real (the state
- * advances). The reference's counterpart is the ENABLE_TIMING cudaEvent block
+ * first start to last stop. Kinds 0-2 run for real (the state advances);
+ * kind 3 writes only the second buffer and leaves the state as it was. The reference's counterpart is the ENABLE_TIMING cudaEvent block
  * around Mc (MonteCarlo.cu:36-63). */
 int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, double *span_ms);
 

@@ -456,3 +456,39 @@ def test_full_size_large_lattice_replicas_agree_with_oracle():
         finally:
             oracle.set_replica_base(0)
         assert np.array_equal(sr, o.spins.reshape(-1))
+
+
+@pytest.mark.parametrize("L,n_or,track,streams,graph", [
+    (4, 3, False, 1, True), (8, 4, True, 1, False), (8, 5, False, 2, True),
+    (12, 2, True, 2, True), (16, 10, False, 2, False)])
+def test_fused_overrelaxation_identical(L, n_or, track, streams, graph):
+    """Over-relaxation passes fused two colours per launch (alternating
+    between the two spin buffers, an odd last pass in place) equal one colour
+    per launch bit for bit: spins, rungs, acceptance, counters, measurement
+    rows; incremental energies and the series to FP64 rounding; and the
+    oracle. The fused timing launches leave the state as it was."""
+    R = 4
+    temps = list(np.geomspace(0.05, 2.0, R))
+    kw = dict(n_metro=1, n_or=n_or, pt_every=3, cos_cone=0.4)
+    with gpu(L, temps, track=track, **kw) as a, gpu(L, temps, track=track, **kw) as b:
+        b.set_or_fusion(False)
+        for g in (a, b):
+            g.set_streams(streams)
+            g.record_start(20)
+            g.measure_start(4, 8)
+            g.Mc(3, use_graph=graph)
+            g.Mc(17, use_graph=graph)
+        assert np.array_equal(a.get_spins(), b.get_spins())
+        assert np.array_equal(a.temperatures()[1], b.temperatures()[1])
+        assert np.array_equal(a.acceptance()[0], b.acceptance()[0])
+        assert a.sweep_counters() == b.sweep_counters()
+        assert np.array_equal(a.measure_fetch(), b.measure_fetch())
+        ra, rb = a.record_fetch(), b.record_fetch()
+        assert ra.shape == (20, R) and np.allclose(ra, rb, rtol=1e-11, atol=0)
+        assert np.allclose(a.energy_incremental(), b.energy_incremental(), rtol=1e-11, atol=0)
+        spins = a.get_spins()
+        a.time_phase(3, 6)
+        assert np.array_equal(a.get_spins(), spins)
+    o = orc(L, temps, **kw)
+    o.run(20)
+    assert np.array_equal(spins, o.spins)
