@@ -168,6 +168,56 @@ __device__ __forceinline__ void cell_update(float *__restrict__ Sx, int cell, in
   }
 }
 
+// The spins one fused over-relaxation pair (colours CA, CA + 1) reads for a
+// cell: (site, dx, dy, dz) with offsets in -1..1, built at compile time from
+// the link table (41 of them for either pair), and the position in that list
+// of every value the two phases use, so the kernel indexes its register copy
+// with constants only.
+struct PairNeed { int8_t site, dx, dy, dz; };
+template <int CA>
+struct PairNeeds {
+  PairNeed v[64] = {};
+  int n = 0;
+  int a_self[4] = {}, a_nb[4][6] = {};         // colour-CA sites of the cell and their links
+  int t_self[4] = {}, t_nb[4][6] = {};         // colour-CA+1 sites and their links
+  int f_self[4][6] = {}, f_nb[4][6][6] = {};   // a foreign colour-CA neighbour and its links
+  constexpr int find(int site, int dx, int dy, int dz) const {
+    for (int i = 0; i < n; ++i)
+      if (v[i].site == site && v[i].dx == dx && v[i].dy == dy && v[i].dz == dz) return i;
+    return -1;
+  }
+  constexpr int add(int site, int dx, int dy, int dz) {
+    const int i = find(site, dx, dy, dz);
+    if (i >= 0) return i;
+    v[n] = {(int8_t)site, (int8_t)dx, (int8_t)dy, (int8_t)dz};
+    return n++;
+  }
+  constexpr PairNeeds() {
+    for (int k = 0; k < 4; ++k) {
+      const int s = CA + 4 * k;
+      a_self[k] = add(s, 0, 0, 0);
+      for (int m = 0; m < 6; ++m)
+        a_nb[k][m] = add(kLinks[s][m].site, kLinks[s][m].dx, kLinks[s][m].dy, kLinks[s][m].dz);
+    }
+    for (int k = 0; k < 4; ++k) {
+      const int t = CA + 1 + 4 * k;
+      t_self[k] = add(t, 0, 0, 0);
+      for (int m = 0; m < 6; ++m) {
+        const Link lk = kLinks[t][m];
+        t_nb[k][m] = add(lk.site, lk.dx, lk.dy, lk.dz);
+        f_self[k][m] = -1;
+        if ((lk.site & 3) == CA && (lk.dx | lk.dy | lk.dz) != 0) {
+          f_self[k][m] = t_nb[k][m];
+          for (int q = 0; q < 6; ++q) {
+            const Link l2 = kLinks[lk.site][q];
+            f_nb[k][m][q] = add(l2.site, lk.dx + l2.dx, lk.dy + l2.dy, lk.dz + l2.dz);
+          }
+        }
+      }
+    }
+  }
+};
+
 // Two over-relaxation colour phases CA, CA + 1 of one cell in one pass over
 // the data, scalar model (cur/oth: this replica's spins in the two buffers).
 // Colours below CA have already been updated this pass and are read from
@@ -177,56 +227,62 @@ __device__ __forceinline__ void cell_update(float *__restrict__ Sx, int cell, in
 // from the first phase, those in the neighbouring cells are recomputed here
 // with the owner's float expressions in the owner's order (over-relaxation is
 // deterministic), so every result equals two separate phase launches bit for
-// bit. TRACK: FP64 energy changes of this cell's updates added to de.
+// bit. All 41 spins the cell needs are loaded in one burst before any
+// arithmetic (one memory round trip, and the halo lines are used while they
+// are still in L2). TRACK: FP64 energy changes of this cell's updates added
+// to de.
 template <int CA, int LT, bool TRACK>
 __device__ __forceinline__ void or_pair_cell(const float *__restrict__ cur,
                                              float *__restrict__ oth, int cell, float jeff,
                                              double &de) {
   static_assert(CA == 0 || CA == 2, "colour pairs (0, 1) and (2, 3)");
   constexpr int CB = CA + 1, L = LT, LL = LT * LT, NC = LT * LT * LT, P = 16 * NC;
+  constexpr PairNeeds<CA> nd{};
+  constexpr int NN = nd.n;
   const int z = cell / LL;
   const int y = (cell / L) % L;
   const int x = cell % L;
-  int X[5], Y[5], Z[5];  // wrapped coordinates at offsets -2 .. 2
+  const int X[3] = {x == 0 ? L - 1 : x - 1, x, x == L - 1 ? 0 : x + 1};
+  const int Y[3] = {(y == 0 ? L - 1 : y - 1) * L, y * L, (y == L - 1 ? 0 : y + 1) * L};
+  const int Z[3] = {(z == 0 ? L - 1 : z - 1) * LL, z * LL, (z == L - 1 ? 0 : z + 1) * LL};
+  float vx[NN], vy[NN], vz[NN];
 #pragma unroll
-  for (int d = -2; d <= 2; ++d) {
-    X[d + 2] = (x + d + L) % L;
-    Y[d + 2] = ((y + d + L) % L) * L;
-    Z[d + 2] = ((z + d + L) % L) * LL;
+  for (int i = 0; i < NN; ++i) {
+    const PairNeed q = nd.v[i];
+    const int j = q.site * NC + Z[q.dz + 1] + Y[q.dy + 1] + X[q.dx + 1];
+    const float *b = (q.site & 3) < CA ? oth : cur;
+    vx[i] = b[j];
+    vy[i] = b[P + j];
+    vz[i] = b[2 * P + j];
   }
-  // Spin of `site` in the cell at offset (dx, dy, dz), from the buffer that
-  // holds its current value (resolved at compile time once unrolled).
-  auto ld = [&](int site, int dx, int dy, int dz) -> f3 {
-    const int j = site * NC + Z[dz + 2] + Y[dy + 2] + X[dx + 2];
-    const float *b = (site & 3) < CA ? oth : cur;
-    return {b[j], b[P + j], b[2 * P + j]};
-  };
-  // Phase-CA update of colour-CA site `s` of the cell at offset (ox, oy, oz),
-  // with its FP64 neighbour sum.
-  auto upd_a = [&](int s, int ox, int oy, int oz, double &gx, double &gy, double &gz) -> f3 {
+  // Phase-CA update of a colour-CA site from its own value (index self) and
+  // its 6 links (indices nb), with its FP64 neighbour sum.
+  auto upd_a = [&](int self, const int *nb, double &gx, double &gy, double &gz) -> f3 {
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     gx = gy = gz = 0.0;
 #pragma unroll
-    for (int n = 0; n < 6; ++n) {
-      const Link lk = kLinks[s][n];
-      const f3 v = ld(lk.site, ox + lk.dx, oy + lk.dy, oz + lk.dz);
-      fx = fx + v.x;
-      fy = fy + v.y;
-      fz = fz + v.z;
+    for (int m = 0; m < 6; ++m) {
+      const int i = nb[m];
+      fx = fx + vx[i];
+      fy = fy + vy[i];
+      fz = fz + vz[i];
       if (TRACK) {
-        gx += (double)v.x;
-        gy += (double)v.y;
-        gz += (double)v.z;
+        gx += (double)vx[i];
+        gy += (double)vy[i];
+        gz += (double)vz[i];
       }
     }
-    return overrelax({fx, fy, fz}, ld(s, ox, oy, oz));
+    return overrelax({fx, fy, fz}, {vx[self], vy[self], vz[self]});
   };
   f3 na[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     double gx, gy, gz;
-    na[k] = upd_a(CA + 4 * k, 0, 0, 0, gx, gy, gz);
-    if (TRACK) de += field_de<false>(jeff, nullptr, CA, ld(CA + 4 * k, 0, 0, 0), na[k], gx, gy, gz);
+    na[k] = upd_a(nd.a_self[k], nd.a_nb[k], gx, gy, gz);
+    if (TRACK) {
+      const int i = nd.a_self[k];
+      de += field_de<false>(jeff, nullptr, CA, {vx[i], vy[i], vz[i]}, na[k], gx, gy, gz);
+    }
   }
   f3 nb[4];
 #pragma unroll
@@ -235,29 +291,31 @@ __device__ __forceinline__ void or_pair_cell(const float *__restrict__ cur,
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     double gx = 0.0, gy = 0.0, gz = 0.0;
 #pragma unroll
-    for (int n = 0; n < 6; ++n) {
-      const Link lk = kLinks[t][n];
-      f3 v;
+    for (int m = 0; m < 6; ++m) {
+      const Link lk = kLinks[t][m];
+      f3 w;
       if ((lk.site & 3) == CA) {
         if (lk.dx == 0 && lk.dy == 0 && lk.dz == 0) {
-          v = na[lk.site >> 2];
+          w = na[lk.site >> 2];
         } else {
           double hx, hy, hz;
-          v = upd_a(lk.site, lk.dx, lk.dy, lk.dz, hx, hy, hz);
+          w = upd_a(nd.f_self[k][m], nd.f_nb[k][m], hx, hy, hz);
         }
       } else {
-        v = ld(lk.site, lk.dx, lk.dy, lk.dz);
+        const int i = nd.t_nb[k][m];
+        w = {vx[i], vy[i], vz[i]};
       }
-      fx = fx + v.x;
-      fy = fy + v.y;
-      fz = fz + v.z;
+      fx = fx + w.x;
+      fy = fy + w.y;
+      fz = fz + w.z;
       if (TRACK) {
-        gx += (double)v.x;
-        gy += (double)v.y;
-        gz += (double)v.z;
+        gx += (double)w.x;
+        gy += (double)w.y;
+        gz += (double)w.z;
       }
     }
-    const f3 told = ld(t, 0, 0, 0);
+    const int it = nd.t_self[k];
+    const f3 told = {vx[it], vy[it], vz[it]};
     nb[k] = overrelax({fx, fy, fz}, told);
     if (TRACK) de += field_de<false>(jeff, nullptr, CB, told, nb[k], gx, gy, gz);
   }

@@ -4,6 +4,7 @@ kept under profiles/.
   python -m cudacode_amd.rocprof_summary trace DB [N]        per-kernel time table (top N)
   python -m cudacode_amd.rocprof_summary grid DB [N]         per (kernel, grid size) time table
   python -m cudacode_amd.rocprof_summary pmc DB [DB ...]      mean counter value per kernel
+  python -m cudacode_amd.rocprof_summary pmcgrid DB [DB ...]  the same per (kernel, grid size)
 
 Counter passes are scaled by a copy kernel of known size run under the same
 counter (see bench.py --calibrate-copy): bytes = KiB * 1024 * scale.
@@ -51,6 +52,15 @@ def pmc(db: str) -> dict:
     return res
 
 
+def pmc_grid(db: str) -> list:
+    """Mean counter value per (kernel, grid size, counter): dispatches of one
+    kernel over different replica ranges kept apart."""
+    c = sqlite3.connect(db)
+    return c.execute("select kernel_name, grid_size, counter_name, count(*), avg(value) "
+                     "from counters_collection group by kernel_name, grid_size, counter_name "
+                     "order by avg(value) desc").fetchall()
+
+
 def short(name: str) -> str:
     return name.split("(")[0].replace("void ", "")
 
@@ -60,6 +70,10 @@ def main(argv):
         print(trace(argv[1], int(argv[2]) if len(argv) > 2 else 12))
     elif len(argv) >= 2 and argv[0] == "grid":
         print(grid(argv[1], int(argv[2]) if len(argv) > 2 else 16))
+    elif len(argv) >= 2 and argv[0] == "pmcgrid":
+        for db in argv[1:]:
+            for k, grid, cnt, n, avg in pmc_grid(db)[:16]:
+                print(f"| `{short(k)}` | {grid} | {cnt} | {n} | {avg:.1f} KiB |")
     elif len(argv) >= 2 and argv[0] == "pmc":
         for db in argv[1:]:
             for (k, cnt), (n, avg) in sorted(pmc(db).items(), key=lambda t: -t[1][1])[:8]:
