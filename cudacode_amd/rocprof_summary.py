@@ -72,8 +72,8 @@ def main(argv):
         print(grid(argv[1], int(argv[2]) if len(argv) > 2 else 16))
     elif len(argv) >= 2 and argv[0] == "pmcgrid":
         for db in argv[1:]:
-            for k, grid, cnt, n, avg in pmc_grid(db)[:16]:
-                print(f"| `{short(k)}` | {grid} | {cnt} | {n} | {avg:.1f} KiB |")
+            for k, threads, cnt, n, avg in pmc_grid(db)[:16]:
+                print(f"| `{short(k)}` | {threads} | {cnt} | {n} | {avg:.1f} KiB |")
     elif len(argv) >= 2 and argv[0] == "pmc":
         for db in argv[1:]:
             for (k, cnt), (n, avg) in sorted(pmc(db).items(), key=lambda t: -t[1][1])[:8]:
