@@ -221,6 +221,38 @@ __global__ __launch_bounds__(256) void k_batch_phase(
   }
 }
 
+// Fused over-relaxation pair (colours CA, CA + 1, or_pair_cell) over the tile
+// list, between the two spin buffers. grid = n_tiles, block = 256.
+template <int CA, bool TRACK>
+__global__ __launch_bounds__(256) void k_batch_or_pair(
+    const float *__restrict__ cur, float *__restrict__ oth, const long long *__restrict__ off,
+    const int *__restrict__ Ls, const int *__restrict__ tiles, float jeff,
+    double *__restrict__ dE) {
+  const int r = tiles[2 * blockIdx.x];
+  if (r < 0) return;  // padding tile: the whole block leaves together
+  const int cell = tiles[2 * blockIdx.x + 1] * 256 + threadIdx.x;
+  const int L = Ls[r];
+  const float *c = cur + off[r];
+  float *o = oth + off[r];
+  double de = 0.0;
+  if (cell < L * L * L) {
+    switch (L) {
+      case 4: or_pair_cell<CA, 4, TRACK>(c, o, cell, jeff, de); break;
+      case 8: or_pair_cell<CA, 8, TRACK>(c, o, cell, jeff, de); break;
+      case 12: or_pair_cell<CA, 12, TRACK>(c, o, cell, jeff, de); break;
+      case 16: or_pair_cell<CA, 16, TRACK>(c, o, cell, jeff, de); break;
+      case 24: or_pair_cell<CA, 24, TRACK>(c, o, cell, jeff, de); break;
+      case 32: or_pair_cell<CA, 32, TRACK>(c, o, cell, jeff, de); break;
+      default: break;
+    }
+  }
+  if (TRACK) {
+    unsigned nacc = 0;
+    block_sum<256>(de, nacc);
+    if (threadIdx.x == 0 && de != 0.0) atomicAdd(&dE[r], de);
+  }
+}
+
 // --- initial spins and observables ------------------------------------------
 // grid = (ceil(max ncell / 256), R), block = 256; the same draws as k_init_spins.
 __global__ __launch_bounds__(256) void k_batch_init(float *__restrict__ spins,
@@ -441,6 +473,26 @@ static void launch_tiles_t(pyro_ctx *ctx, int pass) {
       bk0(ctx), bk1(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE, ctx->launch_delta);
 }
 
+// The fused over-relaxation pass `parity` (0: d_spins -> second buffer,
+// 1: back) over the tile list of the current launch range.
+template <bool TRACK>
+static void launch_tiles_or_pass(pyro_ctx *ctx, int parity) {
+  const int *tiles = ctx->d_tiles;
+  int n = ctx->n_tiles;
+  if (ctx->launch_nrep) {
+    const int hf = ctx->launch_r0 == 0 ? 0 : 1;
+    tiles = ctx->d_tiles_half[hf];
+    n = ctx->n_tiles_half[hf];
+    if (n == 0) return;
+  }
+  const float *cur = parity ? ctx->d_spins_alt : ctx->d_spins;
+  float *oth = parity ? ctx->d_spins : ctx->d_spins_alt;
+  k_batch_or_pair<0, TRACK><<<n, 256, 0, launch_stream(ctx)>>>(cur, oth, ctx->d_off, ctx->d_L,
+                                                                 tiles, ctx->jeff, ctx->d_dE);
+  k_batch_or_pair<2, TRACK><<<n, 256, 0, launch_stream(ctx)>>>(cur, oth, ctx->d_off, ctx->d_L,
+                                                                 tiles, ctx->jeff, ctx->d_dE);
+}
+
 template <int KIND, bool TRACK>
 static void launch_tiles_pass(pyro_ctx *ctx, int pass) {
   launch_tiles_t<KIND, 0, TRACK>(ctx, pass);
@@ -472,7 +524,16 @@ int launch_batch_sweep_body(pyro_ctx *ctx) {
     PYRO_TRY(launch_batch_sweeps(ctx, 1, 0));
   } else {
     for (int p = 0; p < ctx->n_metro; ++p) launch_tiles_pass<0, false>(ctx, p);
-    for (int o = 0; o < ctx->n_or; ++o) {
+    int o = 0;
+    if (ctx->fuse_or && ctx->d_spins_alt && !ctx->mat) {
+      // Two colours per launch, alternating between the spin buffers; an even
+      // number of fused passes ends in d_spins, an odd last pass runs in place.
+      for (; o < (ctx->n_or & ~1); ++o) {
+        if (tr) launch_tiles_or_pass<true>(ctx, o & 1);
+        else launch_tiles_or_pass<false>(ctx, o & 1);
+      }
+    }
+    for (; o < ctx->n_or; ++o) {
       if (tr) launch_tiles_pass<1, true>(ctx, 0);
       else launch_tiles_pass<1, false>(ctx, 0);
     }

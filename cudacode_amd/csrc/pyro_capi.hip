@@ -363,9 +363,9 @@ static int create_ctx(const std::vector<int> &Ls, const double *temps, uint64_t 
     ctx->split = half_blocks >= kSplitMinBlocks;
   }
   ALLOC(ctx->d_spins, total);
-  // The second buffer of the fused over-relaxation passes (uniform lattices;
-  // without it, or if it does not fit, the passes run one colour per launch).
-  if (!ctx->hetero && hipMalloc((void **)&ctx->d_spins_alt, sizeof(float) * total) != hipSuccess) {
+  // The second buffer of the fused over-relaxation passes (without it, if it
+  // does not fit, the passes run one colour per launch).
+  if (hipMalloc((void **)&ctx->d_spins_alt, sizeof(float) * total) != hipSuccess) {
     ctx->d_spins_alt = nullptr;
     (void)hipGetLastError();
   }

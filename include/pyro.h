@@ -260,8 +260,9 @@ int pyro_set_streams(pyro_ctx *ctx, int n);
 int pyro_get_streams(pyro_ctx *ctx, int *n);
 
 /* Over-relaxation passes fused two colours per launch (on = 1, default where
- * it applies: uniform lattice, execution mode 0, cell-per-thread kernels,
- * scalar model) or one colour per launch (on = 0). A fused launch reads the
+ * it applies: execution mode 0 with the cell-per-thread kernels on a uniform
+ * lattice, or the tile-list execution mode 2; scalar model) or one colour per
+ * launch (on = 0). A fused launch reads the
  * spins from one buffer and writes the two updated colours to a second one
  * (allocated with the ctx), recomputing the neighbouring first-colour values
  * that the second colour needs, so a pass moves 36 N instead of 60 N bytes;

@@ -238,3 +238,25 @@ def test_tile_list_two_streams_identical(graph):
             oracle.set_swap_group(0)
         got = np.concatenate([s[off[m]: off[m] + 48 * L ** 3] for m in members])
         assert np.array_equal(got, o.spins), gi
+
+
+@pytest.mark.parametrize("n_or,track", [(3, False), (4, True)])
+def test_tile_list_fused_overrelaxation_identical(n_or, track):
+    """The tile-list execution with over-relaxation fused two colours per
+    launch (mixed sizes, alternating spin buffers, an odd last pass in place)
+    equals one colour per launch: spins, rungs, acceptance bit for bit,
+    incremental energies to FP64 rounding."""
+    Ls = [4, 8, 12, 8, 16, 4]
+    temps = [0.3, 0.05, 0.8, 1.5, 0.2, 0.6]
+    kw = dict(n_metro=1, n_or=n_or, pt_every=2, cos_cone=0.4)
+    with batch(Ls, temps, **kw) as a, batch(Ls, temps, **kw) as b:
+        for g in (a, b):
+            g.set_exec_mode(2, 256)
+            g.set_kernel_options(1, track)
+        b.set_or_fusion(False)
+        a.Mc(9)
+        b.Mc(9)
+        assert np.array_equal(a.get_spins(), b.get_spins())
+        assert np.array_equal(a.temperatures()[1], b.temperatures()[1])
+        assert np.array_equal(a.acceptance()[0], b.acceptance()[0])
+        assert np.allclose(a.energy_incremental(), b.energy_incremental(), rtol=1e-11, atol=0)
