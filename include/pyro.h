@@ -118,7 +118,9 @@ int pyro_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
  * (GenerateRandomVectorsOnDevice, MonteCarlo.cu:257-293; Generate3DVectorList,
  * Math.cu:132-143). set/get copy R*48*L^3 floats host<->device
  * (CopyResultsOnGPU / CopyResultsOnCPU, missing glue, Pyrochlore.cu:1530,1766).
- * device_ptr exposes the resident buffer (no copy). All reset the energy anchor. */
+ * device_ptr exposes the resident buffer (no copy; it holds the state between
+ * calls: fused over-relaxation passes use a second buffer only inside a
+ * sweep). All reset the energy anchor. */
 int pyro_init_random_spins(pyro_ctx *ctx);
 int pyro_set_spins(pyro_ctx *ctx, const float *soa);
 int pyro_get_spins(pyro_ctx *ctx, float *soa);
