@@ -482,7 +482,7 @@ def main():
             "metropolis_avg_launch_ms": met_ms,
             "site_per_thread_or_avg_launch_ms": or_ms_v0,
             # The whole timed sweep at its algorithmic bytes (exchanges,
-            # bookkeeping and the two streams' overlap included).
+            # bookkeeping and the replica groups' overlap included).
             "sweep_achieved": sweep_bytes / (ms_per_step * 1e-3) / 1e9,
             "sweep_frac": sweep_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },

@@ -336,14 +336,19 @@ static inline uint32_t bk0(const pyro_ctx *c) { return (uint32_t)c->seed; }
 static inline uint32_t bk1(const pyro_ctx *c) { return (uint32_t)(c->seed >> 32); }
 
 void batch_free(pyro_ctx *ctx) {
-  void *ptrs[] = {ctx->d_L,   ctx->d_off, ctx->d_order, ctx->d_bin_start, ctx->d_tiles,
-                  ctx->d_tiles_half[0], ctx->d_tiles_half[1]};
+  void *ptrs[] = {ctx->d_L, ctx->d_off, ctx->d_order, ctx->d_bin_start, ctx->d_tiles};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  for (auto &row : ctx->d_tiles_part)
+    for (int *&p : row) {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+    }
+  for (auto &row : ctx->n_tiles_part)
+    for (int &n : row) n = 0;
   ctx->d_L = nullptr;
   ctx->d_off = nullptr;
   ctx->d_order = ctx->d_bin_start = ctx->d_tiles = nullptr;
-  ctx->d_tiles_half[0] = ctx->d_tiles_half[1] = nullptr;
 }
 
 // Tiles of replicas [r_begin, r_end) in replica order, dealt to the 8 XCDs in
@@ -393,7 +398,10 @@ int batch_setup(pyro_ctx *ctx, int n_bins) {
     start.push_back((int)flat.size());
   }
   const std::vector<int> tiles = tile_list(ctx, 0, R);
-  const std::vector<int> halves[2] = {tile_list(ctx, 0, R / 2), tile_list(ctx, R / 2, R)};
+  // parts[n][p]: group p of a split into n groups (n = 2 .. kMaxParts, n <= R).
+  std::vector<int> parts[kMaxParts + 1][kMaxParts];
+  for (int n = 2; n <= kMaxParts && n <= R; ++n)
+    for (int p = 0; p < n; ++p) parts[n][p] = tile_list(ctx, p * R / n, (p + 1) * R / n);
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   batch_free(ctx);
   PYRO_HIP(hipMalloc((void **)&ctx->d_L, sizeof(int) * R));
@@ -406,13 +414,15 @@ int batch_setup(pyro_ctx *ctx, int n_bins) {
   PYRO_HIP(hipMemcpy(ctx->d_order, flat.data(), sizeof(int) * R, hipMemcpyHostToDevice));
   PYRO_HIP(hipMemcpy(ctx->d_bin_start, start.data(), sizeof(int) * (n_bins + 1), hipMemcpyHostToDevice));
   PYRO_HIP(hipMemcpy(ctx->d_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice));
-  for (int hf = 0; hf < 2; ++hf) {
-    if (halves[hf].empty()) continue;
-    PYRO_HIP(hipMalloc((void **)&ctx->d_tiles_half[hf], sizeof(int) * halves[hf].size()));
-    PYRO_HIP(hipMemcpy(ctx->d_tiles_half[hf], halves[hf].data(), sizeof(int) * halves[hf].size(),
-                       hipMemcpyHostToDevice));
-    ctx->n_tiles_half[hf] = (int)halves[hf].size() / 2;
-  }
+  for (int n = 2; n <= kMaxParts; ++n)
+    for (int p = 0; p < n; ++p) {
+      const std::vector<int> &t = parts[n][p];
+      if (t.empty()) continue;
+      PYRO_HIP(hipMalloc((void **)&ctx->d_tiles_part[n][p], sizeof(int) * t.size()));
+      PYRO_HIP(hipMemcpy(ctx->d_tiles_part[n][p], t.data(), sizeof(int) * t.size(),
+                         hipMemcpyHostToDevice));
+      ctx->n_tiles_part[n][p] = (int)t.size() / 2;
+    }
   ctx->n_bins = n_bins;
   ctx->n_tiles = (int)tiles.size() / 2;
   return 0;
@@ -456,18 +466,23 @@ static int launch_persistent_t(pyro_ctx *ctx, int n_sweeps, int record_last) {
   return 0;
 }
 
-// All tiles, or one half's list while a split sweep has set a launch range
-// (the range is always one of the halves [0, R/2), [R/2, R)).
+// All tiles, or the list of one replica group while a split sweep has set a
+// launch range (always group launch_part of n_parts).
+static void launch_tile_range(const pyro_ctx *ctx, const int *&tiles, int &n) {
+  tiles = ctx->d_tiles;
+  n = ctx->n_tiles;
+  if (ctx->launch_nrep) {
+    tiles = ctx->d_tiles_part[ctx->n_parts][ctx->launch_part];
+    n = ctx->n_tiles_part[ctx->n_parts][ctx->launch_part];
+  }
+}
+
 template <int KIND, int COLOUR, bool TRACK>
 static void launch_tiles_t(pyro_ctx *ctx, int pass) {
-  const int *tiles = ctx->d_tiles;
-  int n = ctx->n_tiles;
-  if (ctx->launch_nrep) {
-    const int hf = ctx->launch_r0 == 0 ? 0 : 1;
-    tiles = ctx->d_tiles_half[hf];
-    n = ctx->n_tiles_half[hf];
-    if (n == 0) return;
-  }
+  const int *tiles;
+  int n;
+  launch_tile_range(ctx, tiles, n);
+  if (n == 0) return;
   k_batch_phase<KIND, COLOUR, TRACK><<<n, 256, 0, launch_stream(ctx)>>>(
       ctx->d_spins, ctx->d_off, ctx->d_L, tiles, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl,
       bk0(ctx), bk1(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE, ctx->launch_delta);
@@ -477,14 +492,10 @@ static void launch_tiles_t(pyro_ctx *ctx, int pass) {
 // 1: back) over the tile list of the current launch range.
 template <bool TRACK>
 static void launch_tiles_or_pass(pyro_ctx *ctx, int parity) {
-  const int *tiles = ctx->d_tiles;
-  int n = ctx->n_tiles;
-  if (ctx->launch_nrep) {
-    const int hf = ctx->launch_r0 == 0 ? 0 : 1;
-    tiles = ctx->d_tiles_half[hf];
-    n = ctx->n_tiles_half[hf];
-    if (n == 0) return;
-  }
+  const int *tiles;
+  int n;
+  launch_tile_range(ctx, tiles, n);
+  if (n == 0) return;
   const float *cur = parity ? ctx->d_spins_alt : ctx->d_spins;
   float *oth = parity ? ctx->d_spins : ctx->d_spins_alt;
   k_batch_or_pair<0, TRACK><<<n, 256, 0, launch_stream(ctx)>>>(cur, oth, ctx->d_off, ctx->d_L,

@@ -58,7 +58,7 @@ int reset_model_state(pyro_ctx *ctx) {
 static int enqueue_phases(pyro_ctx *ctx, bool record = false) {
   if (ctx->exec_mode == 2) {  // tile lists: no series hook in these kernels
     PYRO_TRY(launch_batch_sweep_body(ctx));
-    return record ? launch_record_half(ctx) : 0;
+    return record ? launch_record_part(ctx) : 0;
   }
   for (int p = 0; p < ctx->n_metro; ++p)
     for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_metropolis_phase(ctx, c, p, nullptr));
@@ -85,52 +85,62 @@ static int enqueue_phases(pyro_ctx *ctx, bool record = false) {
     }
   const bool recorded = ctx->launch_series_done != 0;
   ctx->launch_series_done = 0;
-  if (record && !recorded) PYRO_TRY(launch_record_half(ctx));
+  if (record && !recorded) PYRO_TRY(launch_record_part(ctx));
   return 0;
 }
 
 static bool split_sweep(const pyro_ctx *ctx) {
-  return ctx->split && ctx->stream2 && ctx->R >= 2 &&
+  return ctx->n_parts > 1 && ctx->n_parts <= ctx->n_streams && ctx->n_parts <= ctx->R &&
          (ctx->exec_mode == 2 || (ctx->exec_mode == 0 && !ctx->hetero));
 }
 
-// Ends a split run: the second stream joins the ctx stream, and the sweeps
+// The other groups' streams join the ctx stream (no booking).
+static int split_join_streams(pyro_ctx *ctx) {
+  for (int p = 1; p < ctx->n_parts; ++p) {
+    PYRO_HIP(hipEventRecord(ctx->ev_join[p], ctx->part_stream[p]));
+    PYRO_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join[p], 0));
+  }
+  ctx->forked = 0;
+  return 0;
+}
+
+// Ends a split run: the other streams join the ctx stream, and the sweeps
 // made since the fork are booked (sweep counter, series rows). With
 // record_last the series row of the last of them is written here (after an
-// exchange moved the rungs); otherwise the phase launches of each half wrote
-// every row (enqueue_phases with record).
+// exchange moved the rungs); otherwise the phase launches of each group
+// wrote every row (enqueue_phases with record).
 static int split_join(pyro_ctx *ctx, int record_last) {
   if (!ctx->forked) return 0;
-  PYRO_HIP(hipEventRecord(ctx->ev_join, ctx->stream2));
-  PYRO_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
-  ctx->forked = 0;
+  PYRO_TRY(split_join_streams(ctx));
   const uint32_t n = ctx->unjoined;
   ctx->unjoined = 0;
   return launch_end_sweep(ctx, n, record_last);
 }
 
-// The phases of one sweep on two streams, one replica half each, plus each
-// half's series entries when the sweep is not followed by a join.
+// The phases of one sweep on n_parts streams, one replica group each (the
+// highest group first, group 0 on the ctx stream last), plus each group's
+// series entries when the sweep is not followed by a join.
 static int enqueue_split_phases(pyro_ctx *ctx, bool join_after) {
+  const int np = ctx->n_parts;
   if (!ctx->forked) {
     PYRO_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
-    PYRO_HIP(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+    for (int p = 1; p < np; ++p) PYRO_HIP(hipStreamWaitEvent(ctx->part_stream[p], ctx->ev_fork, 0));
     ctx->forked = 1;
     ctx->unjoined = 0;
   }
-  const int half = ctx->R / 2;
-  const struct { hipStream_t s; int r0, n; } parts[2] = {{ctx->stream2, half, ctx->R - half},
-                                                         {ctx->stream, 0, half}};
   ctx->launch_delta = ctx->unjoined;
   int rc = 0;
-  for (const auto &p : parts) {
-    ctx->launch_stream = p.s;
-    ctx->launch_r0 = p.r0;
-    ctx->launch_nrep = p.n;
-    if (rc == 0) rc = enqueue_phases(ctx, !join_after && ctx->d_series != nullptr);
+  for (int p = np - 1; p >= 0 && rc == 0; --p) {
+    const int r0 = p * ctx->R / np, r1 = (p + 1) * ctx->R / np;
+    ctx->launch_stream = ctx->part_stream[p];
+    ctx->launch_part = p;
+    ctx->launch_r0 = r0;
+    ctx->launch_nrep = r1 - r0;
+    rc = enqueue_phases(ctx, !join_after && ctx->d_series != nullptr);
   }
   ctx->launch_stream = nullptr;
   ctx->launch_nrep = 0;
+  ctx->launch_part = 0;
   ctx->launch_delta = 0;
   ++ctx->unjoined;
   return rc;
@@ -159,9 +169,9 @@ static int enqueue_interval_end(pyro_ctx *ctx, bool exchange, bool measure, uint
 }
 
 // One sweep as kernel launches (also the body that gets captured). Replicas
-// are independent between exchanges, so the phases of two replica halves can
-// run on two streams: the second half's kernels fill the GPU while the first
-// half's drain, which hides most of the per-launch ramp of these short
+// are independent between exchanges, so the phases of n_parts replica groups
+// can run on as many streams: one group's kernels fill the GPU while
+// another's drain, which hides most of the per-launch ramp of these short
 // kernels. The streams run apart until the next exchange or measurement
 // (which need every replica) or the end of the call (split_join).
 static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
@@ -173,9 +183,7 @@ static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
     if (!(exchange || measure)) return 0;
     // Join without booking yet: the counters advance after the exchange, as
     // for an unsplit sweep.
-    PYRO_HIP(hipEventRecord(ctx->ev_join, ctx->stream2));
-    PYRO_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
-    ctx->forked = 0;
+    PYRO_TRY(split_join_streams(ctx));
   } else if (ctx->exec_mode != 0) {
     PYRO_TRY(launch_batch_sweep_body(ctx));
   } else {
@@ -350,17 +358,32 @@ static int create_ctx(const std::vector<int> &Ls, const double *temps, uint64_t 
   } while (0)
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(set_error(PYRO_EHIP, "pyro_create: stream creation failed"));
-  if (R >= 2) {  // second stream for split sweeps (default priority)
-    if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess)
-      return fail(set_error(PYRO_EHIP, "pyro_create: second stream creation failed"));
-    // Two streams pay when each half's launch still fills the chip; below
-    // that, the fork and join cost more than the overlap hides (measured:
-    // L=16 x 64 replicas 7% faster, L=4 x 64 2.2x slower).
-    long long half_blocks = 0;
-    for (int r = 0; r < R / 2; ++r) half_blocks += ((long long)Ls[r] * Ls[r] * Ls[r] + 255) / 256;
-    ctx->split = half_blocks >= kSplitMinBlocks;
+  ctx->part_stream[0] = ctx->stream;
+  if (R >= 2) {  // streams for split sweeps (default priority)
+    ctx->n_streams = std::min(R, kMaxParts);
+    if (hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess)
+      return fail(set_error(PYRO_EHIP, "pyro_create: event creation failed"));
+    for (int p = 1; p < ctx->n_streams; ++p)
+      if (hipStreamCreateWithFlags(&ctx->part_stream[p], hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&ctx->ev_join[p], hipEventDisableTiming) != hipSuccess)
+        return fail(set_error(PYRO_EHIP, "pyro_create: stream creation failed"));
+    // Splitting pays when each group's launch still spans a few hundred
+    // workgroups; below that, the fork and join cost more than the overlap
+    // hides (measured: L=16 x 64 replicas 0.396 ms per sweep on one stream,
+    // 0.342 in two groups, 0.335 in three; L=4 x 64 2.2x slower in two).
+    // Four groups ran 0.328-0.363 ms from one captured graph to the next
+    // (four streams of this ctx on the default four hardware queues), so
+    // the default is the larger of 3 and 2 groups whose smallest group spans
+    // kSplitMinBlocks; pyro_set_streams allows 4.
+    for (int n : {3, 2}) {
+      if (n > ctx->n_streams) continue;
+      long long blocks = 0;  // group 0 is the smallest: R / n replicas
+      for (int r = 0; r < R / n; ++r) blocks += ((long long)Ls[r] * Ls[r] * Ls[r] + 255) / 256;
+      if (blocks >= kSplitMinBlocks) {
+        ctx->n_parts = n;
+        break;
+      }
+    }
   }
   ALLOC(ctx->d_spins, total);
   // The second buffer of the fused over-relaxation passes (without it, if it
@@ -431,8 +454,9 @@ int pyro_create_batch(const pyro_batch_config *cfg, pyro_ctx **out) {
 int pyro_destroy(pyro_ctx *ctx) {
   if (!ctx) return 0;
   (void)hipSetDevice(ctx->device);
+  for (hipStream_t st : ctx->part_stream)
+    if (st) (void)hipStreamSynchronize(st);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
   drop_graph(ctx);
   mlp_free(ctx);
   batch_free(ctx);
@@ -442,8 +466,10 @@ int pyro_destroy(pyro_ctx *ctx) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
-  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
-  if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
+  for (int p = 1; p < kMaxParts; ++p) {
+    if (ctx->ev_join[p]) (void)hipEventDestroy(ctx->ev_join[p]);
+    if (ctx->part_stream[p]) (void)hipStreamDestroy(ctx->part_stream[p]);
+  }
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return 0;
@@ -569,7 +595,7 @@ int pyro_sweep(pyro_ctx *ctx, int n_sweeps, int use_graph) {
   if (n_sweeps < 0) return set_error(PYRO_EINVAL, "n_sweeps %d < 0", n_sweeps);
   PYRO_TRY(device_guard(ctx));
   const int rc = run_sweeps(ctx, n_sweeps, use_graph);
-  if (rc && ctx->forked) (void)split_join(ctx, 0);  // never leave the second stream running ahead
+  if (rc && ctx->forked) (void)split_join(ctx, 0);  // never leave the other streams running ahead
   return rc;
 }
 
@@ -842,17 +868,16 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
   struct Cand { int mode, block, variant, streams; };
   std::vector<Cand> cands;
   if (!ctx->hetero) {
-    for (int v : {1, 0}) {
-      cands.push_back({0, 0, v, 1});
-      if (ctx->stream2) cands.push_back({0, 0, v, 2});
-    }
+    for (int v : {1, 0})
+      for (int n : {1, 2, 3})
+        if (n <= ctx->n_streams) cands.push_back({0, 0, v, n});
   }
   if (batch_size_supported(ctx->hetero ? 4 : ctx->L)) {
     cands.push_back({1, 256, 1, 1});
     cands.push_back({1, 512, 1, 1});
     cands.push_back({1, 1024, 1, 1});
-    cands.push_back({2, 256, 1, 1});
-    if (ctx->stream2) cands.push_back({2, 256, 1, 2});
+    for (int n : {1, 2, 3})
+      if (n <= ctx->n_streams) cands.push_back({2, 256, 1, n});
   }
   if (cands.empty()) return 0;
   // Snapshot of everything a sweep changes.
@@ -885,9 +910,9 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
   const uint32_t h_sweep = ctx->h_sweep, h_round = ctx->h_pt_round;
   const std::vector<uint64_t> h_prop = ctx->h_proposed;
   const int mode0 = ctx->exec_mode, block0 = ctx->batch_block, variant0 = ctx->phase_variant;
-  const int split0 = ctx->split;
+  const int parts0 = ctx->n_parts;
   double best = 1e30;
-  Cand best_c = {mode0, block0, variant0, split0 ? 2 : 1};
+  Cand best_c = {mode0, block0, variant0, parts0};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     rc = set_error(PYRO_EHIP, "pyro_autotune: event creation failed");
@@ -900,7 +925,7 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
     ctx->exec_mode = c.mode;
     ctx->batch_block = c.block ? c.block : block0;
     ctx->phase_variant = c.variant;
-    ctx->split = c.streams == 2;
+    ctx->n_parts = c.streams;
     // Time the candidate as pyro_sweep runs it: `sweeps` sweeps captured in
     // one graph, replayed once to warm up (code load, caches) and once timed.
     hipGraph_t graph = nullptr;
@@ -952,13 +977,13 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
   ctx->exec_mode = best_c.mode;
   ctx->batch_block = best_c.block ? best_c.block : block0;
   ctx->phase_variant = best_c.variant;
-  ctx->split = best_c.streams == 2;
+  ctx->n_parts = best_c.streams;
   if (n_results) *n_results = nres;
   if (rc) {
     ctx->exec_mode = mode0;
     ctx->batch_block = block0;
     ctx->phase_variant = variant0;
-    ctx->split = split0;
+    ctx->n_parts = parts0;
   }
   return rc;
 }
@@ -1057,13 +1082,14 @@ int pyro_check_spins(pyro_ctx *ctx, double tol, long long *n_bad, long long *fir
 
 int pyro_set_streams(pyro_ctx *ctx, int n) {
   if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
-  if (n != 1 && n != 2) return set_error(PYRO_EINVAL, "pyro_set_streams: %d streams (1 or 2)", n);
-  if (n == 2 && !ctx->stream2)
-    return set_error(PYRO_ESTATE, "pyro_set_streams: two streams need >= 2 replicas");
+  if (n < 1 || n > kMaxParts)
+    return set_error(PYRO_EINVAL, "pyro_set_streams: %d streams (1 to %d)", n, kMaxParts);
+  if (n > ctx->n_streams)
+    return set_error(PYRO_ESTATE, "pyro_set_streams: %d streams need >= %d replicas", n, n);
   PYRO_TRY(device_guard(ctx));
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   drop_graph(ctx);
-  ctx->split = n == 2;
+  ctx->n_parts = n;
   return 0;
 }
 
@@ -1078,7 +1104,7 @@ int pyro_set_or_fusion(pyro_ctx *ctx, int on) {
 
 int pyro_get_streams(pyro_ctx *ctx, int *n) {
   if (!ctx || !n) return set_error(PYRO_EINVAL, "null argument");
-  *n = ctx->split ? 2 : 1;
+  *n = ctx->n_parts;
   return 0;
 }
 

@@ -30,7 +30,9 @@ enum CtlWord : int {
 constexpr int kObs = 13;  // energy + 4 sublattices x 3 components
 constexpr int kObsPartials = 16;
 constexpr int kCoupFloats = 156;  // matrix field model: 16 x 9 exchange + 4 x 3 Zeeman
-// Workgroups each replica half must span before a sweep's phases go to two streams.
+// Replica groups of a split sweep (one stream each), and the workgroups each
+// group's launch must span for the split to be chosen by default.
+constexpr int kMaxParts = 4;
 constexpr long long kSplitMinBlocks = 256;
 // Measurement row, per rung: E, |m_A2|, |m_E|, |m_T1|, |m_F| (= |M| / N), |m_T2|.
 constexpr int kMeasQ = 6;  // per observables block: 4 bases x (energy, mx, my, mz)
@@ -111,9 +113,10 @@ struct pyro_ctx {
   int n_bins = 0;
   int *d_tiles = nullptr;           // 2 ints per tile: replica, tile index (-1: padding)
   int n_tiles = 0;
-  // The same lists for replicas [0, R/2) and [R/2, R) (split sweeps).
-  int *d_tiles_half[2] = {nullptr, nullptr};
-  int n_tiles_half[2] = {0, 0};
+  // The same lists for each replica group of a split sweep:
+  // d_tiles_part[n][p] covers group p of n (replicas [p R / n, (p + 1) R / n)).
+  int *d_tiles_part[pyro::kMaxParts + 1][pyro::kMaxParts] = {};
+  int n_tiles_part[pyro::kMaxParts + 1][pyro::kMaxParts] = {};
 
   // learned proposal
   int n_learned = 0;                    // learned passes per sweep (after Metropolis)
@@ -137,16 +140,21 @@ struct pyro_ctx {
   bool mat = false;
   bool post_rotate = false;  // rotate about h after each Metropolis test
 
-  // Sweeps split into two replica halves on two streams (overlaps one
-  // kernel's tail with the next one's start); the launch range below is set
-  // by the sweep driver around each half and is empty otherwise.
-  int split = 0;
-  hipStream_t stream2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // Sweeps split into n_parts replica groups, each on its own stream (one
+  // group's kernel ramps up while another's drains): group p holds replicas
+  // [p R / n_parts, (p + 1) R / n_parts) and runs on part_stream[p], where
+  // part_stream[0] is the ctx stream and the others exist for p < n_streams.
+  // The launch range below (and launch_part) is set by the sweep driver
+  // around each group and is empty otherwise.
+  int n_parts = 1;
+  int n_streams = 1;
+  hipStream_t part_stream[pyro::kMaxParts] = {};
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[pyro::kMaxParts] = {};
   hipStream_t launch_stream = nullptr;
-  int launch_r0 = 0, launch_nrep = 0;
-  // Split sweeps join the second stream only at exchanges, measurements and
-  // the end of a call: `forked` while it runs ahead, `unjoined` sweeps since
+  int launch_r0 = 0, launch_nrep = 0, launch_part = 0;
+  // Split sweeps join the other streams only at exchanges, measurements and
+  // the end of a call: `forked` while they run ahead, `unjoined` sweeps since
   // the fork. Phase kernels key their draws on ctl[kCtlSweep] + launch_delta,
   // because the counter itself advances only at the join.
   int forked = 0;
@@ -183,7 +191,8 @@ struct pyro_ctx {
 namespace pyro {
 
 // The replicas and stream of the next phase launch: all replicas on the ctx
-// stream, unless a sweep split into replica halves on two streams set a range.
+// stream, unless a sweep split into replica groups on several streams set a
+// range.
 static inline hipStream_t launch_stream(const pyro_ctx *c) {
   return c->launch_stream ? c->launch_stream : c->stream;
 }
@@ -193,7 +202,7 @@ static inline int launch_nrep(const pyro_ctx *c) { return c->launch_nrep ? c->la
 int set_error(int code, const char *fmt, ...);
 
 // Kernel launchers (pyro_kernels.hip). All enqueue on ctx->stream, except
-// that the phase launchers and launch_record_half follow the launch range
+// that the phase launchers and launch_record_part follow the launch range
 // (launch_stream / launch_r0 / launch_nrep) while a split sweep sets one.
 int launch_init_spins(pyro_ctx *ctx);
 int launch_metropolis_phase(pyro_ctx *ctx, int colour, int pass, const uint32_t *sweep_override);
@@ -210,7 +219,7 @@ int launch_observables(pyro_ctx *ctx, bool anchor = false);
 // of the last, sweep counter), as launch_end_sweep(ctx, end_n, 1) after it.
 int launch_pt_swap(pyro_ctx *ctx, int parity, uint32_t round, int from_ctl, uint32_t end_n = 0);
 int launch_end_sweep(pyro_ctx *ctx, uint32_t n, int record_last);  // series row, sweep counter
-int launch_record_half(pyro_ctx *ctx);           // series entries of the launch range
+int launch_record_part(pyro_ctx *ctx);           // series entries of the launch range
 int launch_measure_row(pyro_ctx *ctx);           // append a measurement row from d_obs
 int launch_phase_mat(pyro_ctx *ctx, int kind, int colour, int pass);  // matrix field model
 int reset_model_state(pyro_ctx *ctx);  // drop captured sweeps, re-anchor the energy

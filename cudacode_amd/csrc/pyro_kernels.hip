@@ -160,11 +160,11 @@ __global__ __launch_bounds__(256) void k_end_sweep(int R, uint32_t *__restrict__
   end_interval(R, ctl, anchor, dE, replica_at, series, n, record_last);
 }
 
-// Series entries of one replica half after sweep ctl[kCtlSweep] + delta of a
-// split run (the row count advances at the join): the rungs whose replica
+// Series entries of one replica group after sweep ctl[kCtlSweep] + delta of
+// a split run (the row count advances at the join): the rungs whose replica
 // lies in [r0, r0 + nrep). Rungs do not move between joins. grid = 1,
-// block = 256, on the half's own stream.
-__global__ __launch_bounds__(256) void k_record_half(int R, int r0, int nrep, uint32_t delta,
+// block = 256, on the group's own stream.
+__global__ __launch_bounds__(256) void k_record_part(int R, int r0, int nrep, uint32_t delta,
                                                      const uint32_t *__restrict__ ctl,
                                                      const double *__restrict__ anchor,
                                                      const double *__restrict__ dE,
@@ -356,9 +356,9 @@ int launch_end_sweep(pyro_ctx *ctx, uint32_t n, int record_last) {
   return 0;
 }
 
-int launch_record_half(pyro_ctx *ctx) {
+int launch_record_part(pyro_ctx *ctx) {
   if (!ctx->d_series) return 0;
-  k_record_half<<<1, 256, 0, launch_stream(ctx)>>>(ctx->R, launch_r0(ctx), launch_nrep(ctx),
+  k_record_part<<<1, 256, 0, launch_stream(ctx)>>>(ctx->R, launch_r0(ctx), launch_nrep(ctx),
                                                    ctx->launch_delta, ctx->d_ctl, ctx->d_anchor,
                                                    ctx->d_dE, ctx->d_replica_at, ctx->d_series);
   PYRO_HIP(hipGetLastError());

@@ -417,7 +417,8 @@ class Pyrochlore:
         _check(lib().pyro_set_couplings(self._ctx, _p(c), None if b is None else _p(b)))
 
     def set_streams(self, n: int) -> None:
-        """1 or 2 streams for the phases of a sweep (identical results)."""
+        """1 to 4 streams, one replica group each, for the phases of a sweep
+        (identical results)."""
         _check(lib().pyro_set_streams(self._ctx, int(n)))
 
     def set_or_fusion(self, on: bool) -> None:

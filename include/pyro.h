@@ -98,8 +98,9 @@ int pyro_set_exec_mode(pyro_ctx *ctx, int mode, int block, int n_bins);
 int pyro_get_exec_mode(pyro_ctx *ctx, int *mode, int *block, int *n_bins);
 
 /* Autotuner: time `sweeps` sweeps of each candidate execution (per-phase
- * cell- and site-per-thread kernels on one or two streams, persistent kernel
- * at 256/512/1024 threads, tile list), captured in one graph as pyro_sweep
+ * cell- and site-per-thread kernels on 1, 2 or 3 streams, persistent kernel
+ * at 256/512/1024 threads, tile list on 1, 2 or 3 streams), captured in one
+ * graph as pyro_sweep
  * replays them, restoring the state after each trial, and keep the fastest.
  * results (optional, 5 doubles per candidate, at most 16 candidates): mode,
  * block, phase variant, streams, ms per sweep. */
@@ -249,15 +250,17 @@ int pyro_measure_fetch(pyro_ctx *ctx, double *rows, int *n_rows);
  * reference's --McSteps/--OverRelaxation options, Pyrochlore.cu:1161-1216). */
 int pyro_set_passes(pyro_ctx *ctx, int n_metro, int n_or);
 
-/* With >= 2 replicas, the phases of the two replica halves [0, R/2) and
- * [R/2, R) can run on two streams (n = 2) that join at exchanges,
- * measurements and the end of each call, or on one stream (n = 1), in
- * execution modes 0 and 2. Spins, rungs and counters are identical either
- * way; energies agree to FP64 rounding (no reference counterpart: a
- * performance setting; two streams hide part of each short phase kernel's
- * ramp). Default: 2 when the first half spans >= 256 workgroups of 256
- * cells, else 1; pyro_autotune also tries both. Learned passes take the same
- * split. */
+/* The phases of n replica groups [p R / n, (p + 1) R / n), p < n, can run
+ * on n streams (1 <= n <= 4, n <= R) that join at exchanges, measurements
+ * and the end of each call, in execution modes 0 and 2; n = 1 runs every
+ * replica on the ctx stream. Spins, rungs and counters are identical for
+ * every n; energies agree to FP64 rounding (no reference counterpart: a
+ * performance setting; each group's kernels fill the GPU while another's
+ * drain, hiding part of each short phase kernel's ramp). Default: 3 when
+ * group 0 of 3 spans >= 256 workgroups of 256 cells, else 2 when the first
+ * half does, else 1; pyro_autotune tries 1, 2 and 3 (4 streams share the
+ * default four hardware queues with the runtime and time erratically). Learned
+ * passes take the same split. */
 int pyro_set_streams(pyro_ctx *ctx, int n);
 int pyro_get_streams(pyro_ctx *ctx, int *n);
 

@@ -130,10 +130,10 @@ def test_autotune_keeps_the_state():
         h.Mc(4)
         before = g.get_spins()
         res = g.autotune(3)
-        # persistent at 256/512/1024 threads and tile lists, on one stream;
-        # tile lists on two streams
-        assert len(res) == 5 and np.all(res[:, 4] > 0)
-        assert res[:, 3].tolist() == [1, 1, 1, 1, 2] and res[4, 0] == 2
+        # persistent at 256/512/1024 threads on one stream; tile lists on
+        # 1, 2 and 3 streams
+        assert len(res) == 6 and np.all(res[:, 4] > 0)
+        assert res[:, 3].tolist() == [1, 1, 1, 1, 2, 3] and res[3:, 0].tolist() == [2, 2, 2]
         say(f"autotune results (mode, block, variant, streams, ms/sweep): {res.tolist()}")
         assert np.array_equal(g.get_spins(), before)
         assert g.sweep_counters() == h.sweep_counters()
@@ -198,26 +198,26 @@ def test_persistent_single_small_lattice_equals_phases():
     assert np.array_equal(spins, o.spins)
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_tile_list_two_streams_identical(graph):
-    """Mixed lattice sizes in the tile-list execution with the replica halves
-    on two streams (each half its own XCD-dealt tile list, joined at
+@pytest.mark.parametrize("graph,streams", [(True, 2), (False, 2), (True, 4), (False, 3)])
+def test_tile_list_split_streams_identical(graph, streams):
+    """Mixed lattice sizes in the tile-list execution with the replica groups
+    on 2, 3 or 4 streams (each group its own XCD-dealt tile list, joined at
     exchanges and measurements) equal one stream: spins, rungs, acceptance,
     sweep counters and measurement rows bit for bit, the energy series to FP64
-    rounding; and each equal-size group (the L=8 group spans both halves)
-    equals the oracle."""
+    rounding; and each equal-size lattice group (they straddle the stream
+    groups) equals the oracle."""
     Ls = [4, 4, 4, 8, 8, 8, 12, 12]
     temps = [0.2, 0.5, 1.1, 0.3, 0.9, 1.5, 0.4, 0.8]
     kw = dict(n_metro=1, n_or=2, pt_every=3, cos_cone=0.4)
     with batch(Ls, temps, **kw) as a, batch(Ls, temps, **kw) as b:
-        for g, n in ((a, 2), (b, 1)):
+        for g, n in ((a, streams), (b, 1)):
             g.set_exec_mode(2, 256)
             g.set_streams(n)
             g.record_start(30)
             g.measure_start(4, 8)
             g.Mc(5, use_graph=graph)
             g.Mc(31, use_graph=graph)
-        assert a.streams() == 2 and b.streams() == 1
+        assert a.streams() == streams and b.streams() == 1
         assert np.array_equal(a.get_spins(), b.get_spins())
         assert np.array_equal(a.temperatures()[1], b.temperatures()[1])
         assert np.array_equal(a.acceptance()[0], b.acceptance()[0])

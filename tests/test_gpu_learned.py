@@ -260,23 +260,24 @@ def test_adam_step_closed_form_and_descent():
         assert loss1 < 0.8 * loss0
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_learned_sweeps_on_two_streams_identical(graph):
-    """Learned sweeps whose replica halves run on two streams equal one-stream
-    learned sweeps bit for bit (spins, learned acceptance, incremental
-    energies to FP64 rounding: per-block atomic sums), with exchanges."""
+@pytest.mark.parametrize("graph,streams", [(True, 2), (False, 2), (True, 4)])
+def test_learned_sweeps_on_split_streams_identical(graph, streams):
+    """Learned sweeps whose replica groups run on 2 or 4 streams equal
+    one-stream learned sweeps bit for bit (spins, learned acceptance,
+    incremental energies to FP64 rounding: per-block atomic sums), with
+    exchanges."""
     from cudacode_amd import pyro
     temps = [0.05, 0.2, 0.5, 1.0, 2.0]
     kw = dict(n_metro=0, n_or=2, pt_every=3, cos_cone=0.5)
     w = random_weights(11, out_scale=0.3).astype(np.float32)
     with ctx(8, temps, **kw) as a, ctx(8, temps, **kw) as b:
-        for g, n in ((a, 2), (b, 1)):
+        for g, n in ((a, streams), (b, 1)):
             g.set_streams(n)
             g.mlp_set_weights(w)
             g.set_learned_passes(1)
         a.Mc(12, use_graph=graph)
         b.Mc(12, use_graph=graph)
-        assert a.streams() == 2 and b.streams() == 1
+        assert a.streams() == streams and b.streams() == 1
         assert np.array_equal(a.get_spins(), b.get_spins())
         assert np.array_equal(a.acceptance_learned()[0], b.acceptance_learned()[0])
         assert a.acceptance_learned()[0].sum() > 0

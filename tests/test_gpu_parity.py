@@ -330,16 +330,19 @@ def test_post_rotation_matches_oracle(variant):
     assert np.array_equal(acc, o.accepted)
 
 
-@pytest.mark.parametrize("variant,R,graph,pt_every,track", [
-    (0, 5, True, 3, True), (1, 4, False, 3, False), (0, 2, False, 3, True),
-    (1, 4, True, 0, False), (1, 3, False, 0, True), (1, 5, True, 3, False)])
-def test_two_stream_sweeps_identical(variant, R, graph, pt_every, track):
-    """Sweeps whose replica halves run on two streams (joined only at
+@pytest.mark.parametrize("variant,R,graph,pt_every,track,streams", [
+    (0, 5, True, 3, True, 2), (1, 4, False, 3, False, 2), (0, 2, False, 3, True, 2),
+    (1, 4, True, 0, False, 2), (1, 3, False, 0, True, 2), (1, 5, True, 3, False, 2),
+    (1, 4, False, 3, False, 4), (0, 5, True, 3, True, 4), (1, 7, True, 3, False, 4),
+    (1, 3, False, 0, True, 3), (0, 5, False, 3, False, 3)])
+def test_split_sweeps_identical(variant, R, graph, pt_every, track, streams):
+    """Sweeps whose replica groups run on 2, 3 or 4 streams (joined only at
     exchanges, measurements and the end of a call) equal one-stream sweeps
     bit for bit: spins, rungs, acceptance, incremental energies, sweep
     counters, the energy series (including rows past its capacity) and the
-    measurement rows; captured or eager, odd replica counts, with and without
-    exchanges, calls that end mid-interval; and equal the oracle. Incremental
+    measurement rows; captured or eager, odd replica counts and uneven groups,
+    with and without exchanges, calls that end mid-interval; and equal the
+    oracle. Incremental
     energies (and so the series) agree to FP64 rounding only: each block adds
     its partial sum with one atomic, in whatever order the blocks finish.
     Untracked over-relaxation in the cell kernels writes the series from its
@@ -349,8 +352,9 @@ def test_two_stream_sweeps_identical(variant, R, graph, pt_every, track):
     kw = dict(n_metro=1, n_or=3, pt_every=pt_every, cos_cone=0.4)
     with gpu(L, temps, variant=variant, track=track, **kw) as a, \
             gpu(L, temps, variant=variant, track=track, **kw) as b:
-        a.set_streams(2)
+        a.set_streams(streams)
         b.set_streams(1)
+        assert a.streams() == streams
         for g in (a, b):
             g.record_start(37)
             g.measure_start(4, 12)
@@ -391,11 +395,19 @@ def test_two_stream_sweeps_identical_matrix_model():
 
 
 def test_stream_default_and_autotune_on_uniform_lattice():
-    """Two streams by default only when each replica half spans >= 256
-    workgroups; the autotuner tries both on a uniform lattice, keeps the
-    state, and the sweeps after it equal an untuned ctx's."""
+    """Replica groups by default: 3 when a third of the replicas spans >= 256
+    workgroups, else 2 when half of them does, else 1; more streams than
+    replicas are refused. The autotuner tries 1, 2 and 3 on a uniform
+    lattice, keeps the state, and the sweeps after it equal an untuned ctx's."""
     with gpu(4, [0.2, 0.5, 1.0, 2.0]) as small:
         assert small.streams() == 1
+    with gpu(4, [0.2, 0.5]) as two:
+        with pytest.raises(RuntimeError):
+            two.set_streams(3)
+        with pytest.raises(RuntimeError):
+            two.set_streams(5)
+    with gpu(16, list(np.geomspace(0.05, 1.0, 64))) as big:
+        assert big.streams() == 3  # 21 replicas x 16 workgroups in group 0
     temps = list(np.geomspace(0.05, 1.0, 32))
     kw = dict(n_metro=1, n_or=2, pt_every=2, cos_cone=0.4)
     with gpu(16, temps, **kw) as g, gpu(16, temps, **kw) as h:
@@ -404,7 +416,7 @@ def test_stream_default_and_autotune_on_uniform_lattice():
         h.Mc(2)
         res = g.autotune(2)
         say(f"autotune (mode, block, variant, streams, ms/sweep): {res.tolist()}")
-        assert {1.0, 2.0} <= set(res[res[:, 0] == 0, 3].tolist())
+        assert set(res[res[:, 0] == 0, 3].tolist()) == {1.0, 2.0, 3.0}
         assert np.all(res[:, 4] > 0)
         assert np.array_equal(g.get_spins(), h.get_spins())
         g.Mc(4)
@@ -416,13 +428,13 @@ def test_stream_default_and_autotune_on_uniform_lattice():
 def test_full_size_c2_spins_agree_with_oracle():
     """The L=16 x 64 parallel-tempering workload at its full size (T 0.01..1,
     1 Metropolis + 10 over-relaxation, swaps every 10 sweeps, the fast
-    untracked kernels on two streams): after 100 sweeps the spins, the rungs
+    untracked kernels on three streams): after 100 sweeps the spins, the rungs
     and the acceptance counts equal the oracle's, bit for bit."""
     from cudacode_amd.pyro import geometric_ladder
     temps = geometric_ladder(0.01, 1.0, 64)
     kw = dict(n_metro=1, n_or=10, pt_every=10, cos_cone=0.5)
     with gpu(16, temps, track=False, **kw) as g:
-        assert g.streams() == 2
+        assert g.streams() == 3
         g.Mc(100)
         spins, at, acc = g.get_spins(), g.temperatures()[1], g.acceptance()[0]
         assert g.sweep_counters() == (100, 10)
