@@ -377,7 +377,7 @@ static int create_ctx(const std::vector<int> &Ls, const double *temps, uint64_t 
     // kSplitMinBlocks; pyro_set_streams allows 4.
     for (int n : {3, 2}) {
       if (n > ctx->n_streams) continue;
-      long long blocks = 0;  // group 0 is the smallest: R / n replicas
+      long long blocks = 0;  // group 0: the first R / n replicas (the fewest)
       for (int r = 0; r < R / n; ++r) blocks += ((long long)Ls[r] * Ls[r] * Ls[r] + 255) / 256;
       if (blocks >= kSplitMinBlocks) {
         ctx->n_parts = n;
