@@ -134,9 +134,40 @@ def calibrate_copy(mib: int = 512, reps: int = 5, sweep=()):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / timed
+    del a, b
     print(json.dumps({"calibration_copy_bytes_read": n * 4, "bytes_written": n * 4, "reps": reps,
                       "copy_ms": ms, "copy_GBps_read_plus_write": 2 * n * 4 / (ms * 1e-3) / 1e9,
-                      "size_sweep": rates}))
+                      "size_sweep": rates, "streams": stream_rates()}))
+
+
+def stream_rates(gib: int = 1, reps: int = 20):
+    """Practical HBM rates of torch's own streaming kernels, far beyond the
+    Infinity Cache: a read-only sum of 2 GiB, and c = a + b over 1 GiB each
+    (2 bytes read per byte written, about the fused over-relaxation kernel's
+    12 N : 6 N). GB/s of bytes moved, torch events on torch's stream."""
+    import torch
+    n = gib * (1 << 30) // 4
+    x = torch.ones(2 * n, device="cuda", dtype=torch.float32)
+    a, b = x[:n], x[n:]
+    c = torch.empty(n, device="cuda", dtype=torch.float32)
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    t_sum = timed(lambda: x.sum())
+    t_add = timed(lambda: torch.add(a, b, out=c))
+    del x, a, b, c
+    return {"read_only_sum_ms": t_sum, "read_only_sum_GBps": 8 * n / (t_sum * 1e-3) / 1e9,
+            "add_2read_1write_ms": t_add, "add_2read_1write_GBps": 12 * n / (t_add * 1e-3) / 1e9}
 
 
 def learned_leg(g, args, plain):
