@@ -266,11 +266,12 @@ def test_large_lattice_properties():
         assert np.allclose(g.energy_incremental(), g.TotalEnergyv2(), rtol=1e-10)
 
 
-@pytest.mark.parametrize("L", [4, 8, 12, 16, 24])
+@pytest.mark.parametrize("L", [4, 8, 12, 16, 24, 32])
 def test_kernel_variants_identical(L):
     """The cell-per-thread kernels (compile-time colour and L, XCD-remapped
     blocks) and the site-per-thread kernels give the same spins bit for bit,
-    and both equal the oracle."""
+    and both equal the oracle, for every lattice size with compiled copies
+    below the full-size L=64 tests."""
     temps = [0.05, 0.3, 1.5]
     kw = dict(n_metro=1, n_or=3, pt_every=2, cos_cone=0.4)
     with gpu(L, temps, variant=0, **kw) as a, gpu(L, temps, variant=1, **kw) as b:
@@ -279,10 +280,9 @@ def test_kernel_variants_identical(L):
         sa, sb = a.get_spins(), b.get_spins()
         assert np.array_equal(sa, sb)
         assert np.array_equal(a.acceptance()[0], b.acceptance()[0])
-    if L <= 12:
-        o = orc(L, temps, **kw)
-        o.run(6)
-        assert np.array_equal(sb, o.spins)
+    o = orc(L, temps, **kw)
+    o.run(6)
+    assert np.array_equal(sb, o.spins)
 
 
 def test_untracked_overrelaxation_drift_is_rounding_only():
@@ -472,7 +472,8 @@ def test_full_size_large_lattice_replicas_agree_with_oracle():
 
 @pytest.mark.parametrize("L,n_or,track,streams,graph", [
     (4, 3, False, 1, True), (8, 4, True, 1, False), (8, 5, False, 2, True),
-    (12, 2, True, 2, True), (16, 10, False, 2, False)])
+    (12, 2, True, 2, True), (16, 10, False, 2, False), (24, 3, False, 3, True),
+    (32, 2, True, 2, False)])
 def test_fused_overrelaxation_identical(L, n_or, track, streams, graph):
     """Over-relaxation passes fused two colours per launch (alternating
     between the two spin buffers, an odd last pass in place) equal one colour
