@@ -47,8 +47,10 @@ MODES = [(1, 256), (1, 512), (1, 1024), (2, 256)]
 
 @pytest.mark.parametrize("mode,block", MODES)
 def test_batch_replicas_match_oracle(mode, block):
-    Ls = [8, 4, 12, 8, 16, 4]
-    temps = [0.3, 0.05, 0.8, 1.5, 0.2, 0.6]
+    """Every replica of a batch holding each supported size (4 to 32) follows
+    the oracle's run of that replica alone, in every execution."""
+    Ls = [8, 4, 12, 8, 16, 4, 24, 32]
+    temps = [0.3, 0.05, 0.8, 1.5, 0.2, 0.6, 0.4, 0.1]
     kw = dict(n_metro=1, n_or=3, cos_cone=0.4)
     with batch(Ls, temps, **kw) as g:
         g.set_exec_mode(mode, block)
