@@ -498,7 +498,7 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": OR_PAIR_KERNEL + " (two over-relaxation colours per launch, one thread "
-                                       "per cell; 20 of 24 launches per sweep and stream)",
+                                       "per cell; 20 of 24 launches per sweep and replica group)",
             "achieved": pair_achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
