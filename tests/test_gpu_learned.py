@@ -139,13 +139,14 @@ def replica_mean(es, burn):
     return m.mean(), m.std(ddof=1) / math.sqrt(len(m))
 
 
-@pytest.mark.parametrize("trained", [False, True])
-def test_learned_sweeps_sample_the_boltzmann_law(trained):
+@pytest.mark.parametrize("trained,L", [(False, 4), (True, 4), (True, 6)])
+def test_learned_sweeps_sample_the_boltzmann_law(trained, L):
     """16 independent replicas at T = 0.35: the mean energy of learned-only
     sweeps equals that of plain Metropolis sweeps within 4 sigma. With an
     untrained network the acceptance is low but the law is the same: that is
-    the exactness of the Metropolis-Hastings correction."""
-    L, T, R = 4, 0.35, 16
+    the exactness of the Metropolis-Hastings correction. L=6 (216 cells) runs
+    the runtime-size kernel with a partly filled last wave of cells."""
+    T, R = 0.35, 16
     with ctx(L, [T] * R, n_metro=1, n_or=0, cos_cone=0.5, seed=11) as plain:
         say("plain reference")
         es_p = equilibrium_energy(plain, 300)
