@@ -77,7 +77,26 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_examples()
     return LIB
+
+
+EXAMPLES = ["pt_driver"]
+
+
+def build_examples() -> None:
+    """Host programs in examples/ that drive the library through the C ABI
+    (include/pyro.h), linked next to libpyro.so; a C++ compiler is enough."""
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if cxx is None:
+        raise RuntimeError("no C++ compiler for the examples")
+    for name in EXAMPLES:
+        out = os.path.join(HERE, name)
+        cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"),
+               os.path.join(HERE, "examples", name + ".cpp"), "-L", HERE, "-lpyro",
+               "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+        subprocess.run(cmd, check=True)
+        os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":
