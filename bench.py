@@ -65,13 +65,38 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """Logical CPUs of the host, the CPUs this process may run on, the thread
+    budget (OMP_NUM_THREADS when the environment sets one: a shared host's
+    share for one GPU's job) and the CPU model."""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    budget = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or allowed
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": allowed, "threads": min(budget, allowed),
+            "model": model}
+
+
 def cpu_baseline(target_s: float = 15.0):
     """The oracle on the host cores: the same workload for as many whole sweeps
-    as fit in about target_s seconds (at least one), timed end to end."""
+    as fit in about target_s seconds (at least one) on every core this process
+    may use (capped by OMP_NUM_THREADS where the environment sets one), timed
+    end to end; then one thread for about a fifth of that, for the per-core
+    rate."""
     from cudacode_amd.pyro import geometric_ladder
     from oracle import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
-    oracle.set_num_threads(threads)
+    cpu = host_cpu()
+    oracle.set_num_threads(cpu["threads"])
     used = oracle.num_threads()
     temps = geometric_ladder(T_MIN, T_MAX, R)
     run = oracle.OracleRun(L, temps, seed=SEED, n_metro=N_METRO, n_or=N_OR, pt_every=PT_EVERY,
@@ -84,14 +109,30 @@ def cpu_baseline(target_s: float = 15.0):
     t0 = time.perf_counter()
     run.run(n)
     dt = time.perf_counter() - t0
-    updates = n * (N_METRO + N_OR) * 16 * L ** 3 * R
+    per_sweep = (N_METRO + N_OR) * 16 * L ** 3 * R
+    updates = n * per_sweep
+    # One thread: whole sweeps, continuing the same chain.
+    oracle.set_num_threads(1)
+    t0 = time.perf_counter()
+    run.run(1)
+    one1 = time.perf_counter() - t0
+    n1 = max(0, min(20, int(0.2 * target_s / max(one1, 1e-3)) - 1))
+    t0 = time.perf_counter()
+    if n1:
+        run.run(n1)
+    dt1 = time.perf_counter() - t0
+    single = per_sweep * (n1 + 1) / (dt1 + one1)
+    oracle.set_num_threads(used)
     return {
         "value": updates / dt,
         "unit": "spin updates/s",
         "cores": used,
         "kind": "port",
         "sample": f"{n} sweeps of the same L=16 x 64-replica workload "
-                  f"({updates:.3g} spin updates, {dt:.1f} s), C oracle with OpenMP",
+                  f"({updates:.3g} spin updates, {dt:.1f} s), C oracle with OpenMP on {used} "
+                  f"threads; single thread: {n1 + 1} sweeps",
+        "single_thread_value": single,
+        "host": cpu,
     }
 
 
@@ -170,41 +211,44 @@ def stream_rates(gib: int = 1, reps: int = 20):
             "add_2read_1write_ms": t_add, "add_2read_1write_GBps": 12 * n / (t_add * 1e-3) / 1e9}
 
 
-def learned_leg(g, args, plain):
+def learned_leg(g, args):
     """The learned local proposal on the same ensemble (cudacode_amd.learned):
     environments (beta h, s) gathered from the plain sampler across the
     temperature ladder (16 snapshots x 64 replicas x 1024 sites = 1,048,576),
-    the network trained on the GPU, then sweeps of 1 learned pass + 10
-    over-relaxation passes timed like the plain ones, then a plain control leg
-    of the same length. Reports sweeps/s, tau_int, independent samples/s and
-    their ratio to the plain sampler, the agreement of the mean energy per
-    rung, the learned phase's MFMA roofline, and which sampler the
-    fallback rule keeps."""
+    the network trained on the GPU, the chain thermalised by the plain
+    sampler, then alternating legs of a fixed length (plain, learned, plain,
+    learned; --series-sweeps each, independent of --steps) that give each
+    sampler's sweeps/s and per-rung tau_int of the energy with its error,
+    hence the tau_int-corrected samples/s ratio with a bootstrap interval;
+    observables legs (C/N, <|m_A2|^2>) of each sampler; the learned phase's
+    MFMA roofline; and which sampler the fallback rule keeps.
+    Returns (record, pooled plain leg)."""
     from cudacode_amd import learned, pyro
     log("learned: gathering environments and training on the GPU")
     info = learned.train(g, steps=args.train_steps, init_seed=SEED)
-    prev = learned.use_learned(g, 1)
-    log(f"learned: timed {args.steps} sweeps")
-    leg = learned.timed_leg(g, args.steps, warmup=args.warmup, learned=True)
+    log(f"learned: {args.therm_sweeps} plain thermalisation sweeps")
+    g.Mc(args.therm_sweeps)
+    log(f"learned: 2 x (plain, learned) legs of {args.series_sweeps} timed sweeps")
+    plain_legs, learned_legs = learned.paired_legs(g, args.series_sweeps, rounds=2)
     log("learned: phase timing, 200 learned launches")
     learned_ms, _ = g.time_phase(2, 200)
-    log(f"learned: observables, {args.obs_sweeps} measured sweeps")
+    log(f"learned: observables, {args.obs_sweeps} measured sweeps per sampler")
+    prev = learned.use_learned(g, 1)
     learned_obs = learned.observables_leg(g, args.obs_sweeps)
     learned.use_plain(g, prev)
-    log(f"learned: plain control {args.steps} sweeps")
-    control = learned.timed_leg(g, args.steps)
-    n_spins = 16 * L ** 3
-    log(f"learned: plain observables, {args.obs_sweeps} measured sweeps")
     plain_obs = learned.observables_leg(g, args.obs_sweeps)
-    verdict = learned.judge(plain, leg, control, n_spins, plain_obs, learned_obs)
+    n_spins = 16 * L ** 3
+    rung_T = np.sort(pyro.geometric_ladder(T_MIN, T_MAX, R))[::-1]  # rungs: descending T
+    verdict = learned.judge_paired(plain_legs, learned_legs, n_spins, rung_T, plain_obs,
+                                   learned_obs)
     choice = learned.choose(g, verdict, prev)
     learned.use_plain(g, prev)
     # Dense work of one learned colour phase: two forward passes per proposal
     # of 2 (6*64 + 64*64 + 64*4) = 9,472 FLOP each, N/4 proposals per replica.
     proposals = n_spins // 4 * R
     flop = proposals * 2 * FLOP_PER_FORWARD
-    rung_T = np.sort(pyro.geometric_ladder(T_MIN, T_MAX, R))[::-1]  # rungs: descending T
-    out = {"sweep": "1 learned pass + 10 over-relaxation passes (plain: 1 Metropolis + 10)"}
+    out = {"sweep": "1 learned pass + 10 over-relaxation passes (plain: 1 Metropolis + 10)",
+           "thermalisation_sweeps": args.therm_sweeps}
     out.update(info)
     out["phase"] = {
         "kernel": "k_learned_phase<16, 1>", "avg_launch_ms": learned_ms,
@@ -213,11 +257,19 @@ def learned_leg(g, args, plain):
         "frac_mfma": flop / (learned_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
         "achieved_gbs": proposals * BYTES_PER_UPDATE / (learned_ms * 1e-3) / 1e9,
     }
-    out["spin_updates_per_s"] = leg.sweeps_per_s * (1 + N_OR) * n_spins * R
+    out["spin_updates_per_s"] = verdict["sweeps_per_s"] * (1 + N_OR) * n_spins * R
     out.update(verdict)
     out["worst_rungs_T"] = [float(rung_T[k]) for k in verdict.pop("worst_rungs")]
     out["fallback_choice"] = choice
-    return out
+    return out, learned.pool(plain_legs)
+
+
+def plain_tau_leg(g, args):
+    """Without the learned leg: the plain sampler's tau_int from two legs of
+    --series-sweeps after --therm-sweeps thermalisation sweeps."""
+    from cudacode_amd import learned
+    g.Mc(args.therm_sweeps)
+    return learned.pool([learned.timed_leg(g, args.series_sweeps, warmup=20) for _ in range(2)])
 
 
 def line(metric_value, steps, warmup, dt, workload, extra):
@@ -386,6 +438,10 @@ def main():
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--obs-sweeps", type=int, default=2000,
                     help="measured sweeps per observables leg (C/N, |m|^2 agreement and tau_int)")
+    ap.add_argument("--therm-sweeps", type=int, default=2000,
+                    help="plain sweeps before the tau_int legs (independent of --warmup)")
+    ap.add_argument("--series-sweeps", type=int, default=3000,
+                    help="timed sweeps per tau_int leg (independent of --steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the replicas of "
@@ -421,12 +477,7 @@ def main():
     ms_per_step = 1e3 * dt / args.steps
     acc = float(np.mean(g.CalculateRatio()))
 
-    series = g.record_fetch()  # rows: sweeps, columns: rungs (descending T)
-    taus = np.array([stats.tau_int(series[:, k])[0] for k in range(R)])
     sweeps_per_s = args.steps / dt
-    samples = np.array([stats.independent_samples_per_s(sweeps_per_s, t) for t in taus])
-    from cudacode_amd.learned import Leg
-    plain = Leg(series, sweeps_per_s, taus, samples, acc)
 
     log(f"phase timing: {args.phase_launches} fused and single over-relaxation launches")
     pair_ms, _ = g.time_phase(3, args.phase_launches)
@@ -460,10 +511,18 @@ def main():
         "host_inclusive_updates_per_s": (N_METRO + N_OR) * n_total / (
             (ms_per_step + get_ms + set_ms) * 1e-3),
     }
+    # tau_int of the plain sampler from fixed-length legs after a fixed
+    # thermalisation (a series as short as --steps cannot resolve it).
     learned = None
     if not args.no_learned:
-        learned = learned_leg(g, args, plain)
+        learned, plain = learned_leg(g, args)
+    else:
+        log(f"plain tau_int: {args.therm_sweeps} thermalisation sweeps, 2 legs of "
+            f"{args.series_sweeps}")
+        plain = plain_tau_leg(g, args)
     g.close()
+    taus, tau_err = plain.taus, plain.tau_err
+    samples = np.array([stats.independent_samples_per_s(sweeps_per_s, t) for t in taus])
     traffic = measured_traffic(OR_KERNEL)
 
     out = {
@@ -487,11 +546,15 @@ def main():
             "seed": SEED, "parallelism": "replicas resident on one GPU",
         },
         "indep_samples_per_s": {
+            # sweeps/s of the timed sweeps / (2 tau_int), tau_int per rung of
+            # the energy from the plain sampler's fixed-length legs
             "median_over_rungs": float(np.median(samples)),
             "lowest_T": float(samples[-1]),
             "tau_int_sweeps_median": float(np.median(taus)),
             "tau_int_sweeps_lowest_T": float(taus[-1]),
-            "series_sweeps": int(series.shape[0]),
+            "tau_int_sweeps_lowest_T_err": float(tau_err[-1]),
+            "series_sweeps": int(plain.series.shape[0]),
+            "thermalisation_sweeps": args.therm_sweeps,
         },
         "acceptance": acc,
         "host_transfer": host_transfer,

@@ -880,12 +880,13 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
       if (n <= ctx->n_streams) cands.push_back({2, 256, 1, n});
   }
   if (cands.empty()) return 0;
-  // Snapshot of everything a sweep changes.
+  // Snapshot of everything a sweep changes (an exchange or measurement
+  // inside a trial also re-anchors the incremental energy: anchor and dE).
   const int R = ctx->R;
   float *s_spins = nullptr;
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   PYRO_HIP(hipMalloc((void **)&s_spins, sizeof(float) * ctx->total_floats));
-  std::vector<double> temps(R), dE(R);
+  std::vector<double> temps(R), dE(R), anchor(R);
   std::vector<float> beta(R), cosc(R);
   std::vector<int> at(R);
   std::vector<unsigned long long> acc(R);
@@ -898,6 +899,7 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
                   hipMemcpyDeviceToDevice) != hipSuccess ||
         hipMemcpy(save ? (void *)temps.data() : (void *)ctx->d_temps, save ? (void *)ctx->d_temps : (void *)temps.data(), sizeof(double) * R, k) != hipSuccess ||
         hipMemcpy(save ? (void *)dE.data() : (void *)ctx->d_dE, save ? (void *)ctx->d_dE : (void *)dE.data(), sizeof(double) * R, k) != hipSuccess ||
+        hipMemcpy(save ? (void *)anchor.data() : (void *)ctx->d_anchor, save ? (void *)ctx->d_anchor : (void *)anchor.data(), sizeof(double) * R, k) != hipSuccess ||
         hipMemcpy(save ? (void *)beta.data() : (void *)ctx->d_beta, save ? (void *)ctx->d_beta : (void *)beta.data(), sizeof(float) * R, k) != hipSuccess ||
         hipMemcpy(save ? (void *)cosc.data() : (void *)ctx->d_cosc, save ? (void *)ctx->d_cosc : (void *)cosc.data(), sizeof(float) * R, k) != hipSuccess ||
         hipMemcpy(save ? (void *)at.data() : (void *)ctx->d_replica_at, save ? (void *)ctx->d_replica_at : (void *)at.data(), sizeof(int) * R, k) != hipSuccess ||

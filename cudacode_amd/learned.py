@@ -6,10 +6,15 @@ plain sampler when it does not pay.
   temperature ladder and trains the network on the GPU (the library's own
   FP32 forward/backward kernels and Adam, pyro_mlp_train).
 - `timed_leg`: runs and times sweeps with an energy series recorded, and
-  returns sweeps/s, per-rung tau_int and tau_int-corrected samples/s.
-- `judge`: compares a learned leg with a plain leg and a plain control leg:
-  the samples/s ratio per rung and the agreement of the mean energy per rung
-  (z-scores, read against the plain-vs-plain spread of the same run lengths).
+  returns sweeps/s, per-rung tau_int with its Madras-Sokal error and
+  tau_int-corrected samples/s.
+- `paired_legs`: alternating plain and learned legs of a fixed length from
+  one equilibrated chain (plain, learned, plain, learned, ...), so that both
+  samplers are measured at the same point of the chain and neither carries
+  the other's equilibration; `pool` joins the legs of one sampler.
+- `judge_paired`: the samples/s ratio per rung with a bootstrap interval of
+  its median over rungs, the agreement of the mean energy and C/N per rung,
+  and the plain-vs-plain and learned-vs-learned controls of the same lengths.
 - `choose`: keeps the learned proposal only if it is faster in samples/s and
   agrees with the plain sampler; otherwise restores the plain schedule.
 
@@ -55,6 +60,13 @@ class Leg:
     taus: np.ndarray          # tau_int per rung, in sweeps
     samples: np.ndarray       # tau_int-corrected independent samples/s per rung
     acceptance: float
+    tau_err: np.ndarray = None  # Madras-Sokal error of tau_int per rung
+    seconds: float = 0.0      # wall time of the timed sweeps
+
+
+def _taus(series: np.ndarray):
+    t = [stats.tau_int(series[:, k]) for k in range(series.shape[1])]
+    return np.array([v[0] for v in t]), np.array([v[2] for v in t])
 
 
 def timed_leg(g, sweeps: int, warmup: int = 0, learned: bool = False) -> Leg:
@@ -71,9 +83,55 @@ def timed_leg(g, sweeps: int, warmup: int = 0, learned: bool = False) -> Leg:
     series = g.record_fetch()
     a, p = g.acceptance_learned() if learned else g.acceptance()
     sps = sweeps / dt
-    taus = np.array([stats.tau_int(series[:, k])[0] for k in range(series.shape[1])])
+    taus, err = _taus(series)
     samples = np.array([stats.independent_samples_per_s(sps, t) for t in taus])
-    return Leg(series, sps, taus, samples, float(a.sum() / max(p.sum(), 1)))
+    return Leg(series, sps, taus, samples, float(a.sum() / max(p.sum(), 1)), err, dt)
+
+
+def paired_legs(g, sweeps: int, rounds: int = 2, warmup: int = 20, n_learned: int = 1):
+    """`rounds` rounds of a plain leg then a learned leg, each `sweeps` timed
+    sweeps (after `warmup` untimed ones of the same sampler, which also
+    capture its graph), continuing one chain. Both samplers leave the
+    Boltzmann law stationary, so a chain equilibrated by one is equilibrated
+    for the other, and alternating the legs puts both at the same point of
+    the chain. Returns (plain legs, learned legs); the ctx ends on its
+    previous (plain) schedule."""
+    plain, learned = [], []
+    prev = (g.n_metro, g.n_or)
+    for _ in range(rounds):
+        plain.append(timed_leg(g, sweeps, warmup=warmup))
+        use_learned(g, n_learned)
+        learned.append(timed_leg(g, sweeps, warmup=warmup, learned=True))
+        use_plain(g, prev)
+    return plain, learned
+
+
+def pool(legs) -> Leg:
+    """One sampler's legs together: sweeps/s over all of them, tau_int the
+    mean of the legs' estimates (error: their Madras-Sokal errors combined),
+    and the series concatenated (for means and binning errors only)."""
+    n = sum(leg.series.shape[0] for leg in legs)
+    secs = sum(leg.seconds for leg in legs)
+    sps = n / secs
+    taus = np.mean([leg.taus for leg in legs], axis=0)
+    err = np.sqrt(np.sum([leg.tau_err ** 2 for leg in legs], axis=0)) / len(legs)
+    samples = sps / (2.0 * taus)
+    acc = float(np.mean([leg.acceptance for leg in legs]))
+    return Leg(np.concatenate([leg.series for leg in legs]), sps, taus, samples, acc, err, secs)
+
+
+def ratio_interval(plain: Leg, learned: Leg, n_boot: int = 4000, seed: int = 0):
+    """Percentiles (2.5, 16, 50, 84, 97.5) of the median over rungs of the
+    samples/s ratio learned / plain, from a parametric bootstrap that draws
+    each rung's tau_int of both samplers from a normal law with its
+    Madras-Sokal error (clipped at 1/2, the uncorrelated value); the
+    sweeps/s ratio is taken as exact (its run-to-run spread is a few %)."""
+    rng = np.random.default_rng(seed)
+    R = plain.taus.size
+    tp = np.maximum(plain.taus + rng.standard_normal((n_boot, R)) * plain.tau_err, 0.5)
+    tl = np.maximum(learned.taus + rng.standard_normal((n_boot, R)) * learned.tau_err, 0.5)
+    med = np.median((learned.sweeps_per_s / plain.sweeps_per_s) * tp / tl, axis=1)
+    return np.percentile(med, [2.5, 16, 50, 84, 97.5])
 
 
 def observables_leg(g, sweeps: int, n_bins: int = 20) -> measure.Summary:
@@ -117,36 +175,76 @@ def use_plain(g, prev: tuple) -> None:
     g.set_passes(*prev)
 
 
-def judge(plain: Leg, learned: Leg, control: Leg, n_spins, plain_obs=None,
-          learned_obs=None) -> dict:
-    """Speed and agreement of a learned leg against the plain sampler; with
-    observables legs, also the agreement of C/N and <|m_A2|^2> per rung and
-    tau_int per rung for E and |m_A2|^2."""
-    ratio = learned.samples / plain.samples
-    z = zscores(plain.series, learned.series, n_spins)
-    z_ctl = zscores(plain.series, control.series, n_spins)
-    z_lc = zscores(learned.series, control.series, n_spins)
-    return {
-        "sweeps_per_s": learned.sweeps_per_s, "plain_sweeps_per_s": plain.sweeps_per_s,
-        "acceptance": learned.acceptance,
-        "tau_int_sweeps_median": float(np.median(learned.taus)),
-        "tau_int_sweeps_lowest_T": float(learned.taus[-1]),
-        "plain_tau_int_sweeps_median": float(np.median(plain.taus)),
-        "indep_samples_per_s_median": float(np.median(learned.samples)),
-        "indep_samples_per_s_lowest_T": float(learned.samples[-1]),
+def specific_heats(series: np.ndarray, temps_desc, n_spins, n_bins: int = 20) -> np.ndarray:
+    """C/N per rung with its jackknife error from an energy series (sweeps, R)."""
+    N = np.broadcast_to(np.asarray(n_spins, np.float64), (series.shape[1],))
+    return np.array([stats.specific_heat(series[:, k], float(temps_desc[k]), float(N[k]), n_bins)
+                     for k in range(series.shape[1])])
+
+
+def judge_paired(plain_legs, learned_legs, n_spins, temps_desc, plain_obs=None,
+                 learned_obs=None, min_length_taus: float = 50.0) -> dict:
+    """Speed and agreement of the learned sampler against the plain one from
+    the legs of `paired_legs` (at least two of each): the samples/s ratio per
+    rung and the bootstrap interval of its median, the agreement per rung of
+    the mean energy and of C/N between the pooled samplers, and the same
+    statistics between the first two legs of each sampler (plain vs plain,
+    learned vs learned) as controls of what chance gives at these lengths.
+    `series_long_enough` is False when a leg is shorter than min_length_taus
+    times a rung's tau_int (the window estimate is then biased low)."""
+    P, Q = pool(plain_legs), pool(learned_legs)
+    ratio = Q.samples / P.samples
+    rel = np.hypot(P.tau_err / P.taus, Q.tau_err / Q.taus)
+    lo95, lo68, med, hi68, hi95 = ratio_interval(P, Q)
+    n_leg = min(leg.series.shape[0] for leg in plain_legs + learned_legs)
+    long_enough = bool(n_leg >= min_length_taus * max(P.taus.max(), Q.taus.max()))
+    z = zscores(P.series, Q.series, n_spins)
+    z_pp = zscores(plain_legs[0].series, plain_legs[1].series, n_spins)
+    z_ll = zscores(learned_legs[0].series, learned_legs[1].series, n_spins)
+    cP = specific_heats(P.series, temps_desc, n_spins)
+    cQ = specific_heats(Q.series, temps_desc, n_spins)
+    zc = zdiff(cP, cQ)
+    zc_pp = zdiff(specific_heats(plain_legs[0].series, temps_desc, n_spins),
+                  specific_heats(plain_legs[1].series, temps_desc, n_spins))
+    out = {
+        "legs": f"{len(plain_legs)} plain and {len(learned_legs)} learned legs of {n_leg} "
+                "timed sweeps, alternating on one equilibrated chain",
+        "series_sweeps_per_leg": int(n_leg), "series_long_enough": long_enough,
+        "sweeps_per_s": Q.sweeps_per_s, "plain_sweeps_per_s": P.sweeps_per_s,
+        "acceptance": Q.acceptance, "plain_acceptance": P.acceptance,
+        "tau_int_sweeps_median": float(np.median(Q.taus)),
+        "tau_int_sweeps_lowest_T": float(Q.taus[-1]),
+        "plain_tau_int_sweeps_median": float(np.median(P.taus)),
+        "plain_tau_int_sweeps_lowest_T": float(P.taus[-1]),
+        "indep_samples_per_s_median": float(np.median(Q.samples)),
+        "indep_samples_per_s_lowest_T": float(Q.samples[-1]),
+        "plain_indep_samples_per_s_median": float(np.median(P.samples)),
         "samples_ratio_median": float(np.median(ratio)),
+        "samples_ratio_median_68": [float(lo68), float(hi68)],
+        "samples_ratio_median_95": [float(lo95), float(hi95)],
         "samples_ratio_lowest_T": float(ratio[-1]),
+        "samples_ratio_lowest_T_err": float(ratio[-1] * rel[-1]),
         "samples_ratio_min": float(ratio.min()),
         "energy_agreement_max_z": float(z.max()),
         "energy_agreement_rungs_within_2sigma": int((z < 2).sum()),
-        "control_plain_vs_plain_max_z": float(z_ctl.max()),
-        "control_plain_vs_plain_within_2sigma": int((z_ctl < 2).sum()),
-        "learned_vs_control_max_z": float(z_lc.max()),
-        "learned_vs_control_within_2sigma": int((z_lc < 2).sum()),
+        "control_plain_vs_plain_max_z": float(z_pp.max()),
+        "control_plain_vs_plain_within_2sigma": int((z_pp < 2).sum()),
+        "control_learned_vs_learned_max_z": float(z_ll.max()),
+        "control_learned_vs_learned_within_2sigma": int((z_ll < 2).sum()),
+        "specific_heat_series_max_z": float(np.nanmax(zc)),
+        "specific_heat_series_within_2sigma": int((zc < 2).sum()),
+        "control_specific_heat_plain_vs_plain_max_z": float(np.nanmax(zc_pp)),
+        "control_specific_heat_plain_vs_plain_within_2sigma": int((zc_pp < 2).sum()),
         "worst_rungs": [int(k) for k in np.argsort(-z)[:3]],
-        "tau_int_E_per_rung": [float(t) for t in learned.taus],
-        "plain_tau_int_E_per_rung": [float(t) for t in plain.taus],
-    } | (_obs_verdict(plain_obs, learned_obs) if plain_obs is not None else {})
+        "tau_int_E_per_rung": [float(t) for t in Q.taus],
+        "tau_int_E_err_per_rung": [float(t) for t in Q.tau_err],
+        "plain_tau_int_E_per_rung": [float(t) for t in P.taus],
+        "plain_tau_int_E_err_per_rung": [float(t) for t in P.tau_err],
+        "samples_ratio_per_rung": [float(r) for r in ratio],
+    }
+    if plain_obs is not None:
+        out.update(_obs_verdict(plain_obs, learned_obs))
+    return out
 
 
 def _obs_verdict(p: measure.Summary, q: measure.Summary) -> dict:

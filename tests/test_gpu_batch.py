@@ -124,13 +124,17 @@ def test_batch_observables_and_order_params():
 
 
 def test_autotune_keeps_the_state():
+    """Autotuning trials (3 sweeps each, crossing exchanges that re-anchor
+    the incremental energy) leave the chain as an untuned twin's: spins,
+    counters, the incremental energy and the energy series afterwards."""
     Ls = [8, 12, 16, 8, 24, 12, 16, 24]
     temps = np.linspace(0.1, 1.0, 8)
     with batch(Ls, temps, n_metro=1, n_or=4, pt_every=2, cos_cone=0.4) as g, \
             batch(Ls, temps, n_metro=1, n_or=4, pt_every=2, cos_cone=0.4) as h:
-        g.Mc(4)
-        h.Mc(4)
+        g.Mc(5)
+        h.Mc(5)
         before = g.get_spins()
+        inc_before = g.energy_incremental()
         res = g.autotune(3)
         # persistent at 256/512/1024 threads on one stream; tile lists on
         # 1, 2 and 3 streams
@@ -139,9 +143,16 @@ def test_autotune_keeps_the_state():
         say(f"autotune results (mode, block, variant, streams, ms/sweep): {res.tolist()}")
         assert np.array_equal(g.get_spins(), before)
         assert g.sweep_counters() == h.sweep_counters()
-        g.Mc(6)
+        assert np.array_equal(g.energy_incremental(), inc_before)
+        assert np.allclose(g.energy_incremental(), h.energy_incremental(), rtol=1e-12, atol=0)
+        for x in (g, h):
+            x.record_start(6)
+        g.Mc(6)  # an odd start: the first sweep has no exchange after it
         h.Mc(6)
         assert np.array_equal(g.get_spins(), h.get_spins())
+        assert np.allclose(g.record_fetch(), h.record_fetch(), rtol=1e-12, atol=0)
+        assert np.allclose(g.energy_incremental(), h.energy_incremental(), rtol=1e-12, atol=0)
+        assert np.allclose(g.energy_incremental(), g.TotalEnergyv2(), rtol=1e-7, atol=0)
 
 
 @pytest.mark.parametrize("block", [256, 1024])

@@ -284,3 +284,112 @@ def test_learned_sweeps_on_split_streams_identical(graph, streams):
         assert a.acceptance_learned()[0].sum() > 0
         assert np.allclose(a.energy_incremental(), b.energy_incremental(), rtol=1e-11, atol=0)
         assert np.array_equal(a.temperatures()[1], b.temperatures()[1])
+
+
+def check_trained_network(g, n_rows=4096, seed=0):
+    """The library's forward pass and MH log ratio on n_rows of the
+    environments the ctx gathered (the trained network's real inputs), each
+    with a uniform proposal s': the BF16 MFMA outputs within BF16 tolerance of
+    the FP64 oracle forward, the device log ratio equal to the FP64 host
+    formula evaluated on the device's own outputs."""
+    env = g.environments()
+    assert env.shape[0] >= n_rows
+    pick = np.random.default_rng(seed).choice(env.shape[0], n_rows, replace=False)
+    e = env[pick].astype(np.float32)
+    sp = env_rows(n_rows)[:, 3:]
+    rows = np.concatenate([e, sp], axis=1)
+    out = g.mlp_eval(rows)
+    w = g.mlp_get_weights().astype(np.float64)
+    ref1 = np.array([oracle.mlp_forward(w, oracle.mlp_features(r[:3].astype(np.float64),
+                                                               r[3:6].astype(np.float64)))
+                     for r in rows])
+    ref2 = np.array([oracle.mlp_forward(w, oracle.mlp_features(r[:3].astype(np.float64),
+                                                               r[6:9].astype(np.float64)))
+                     for r in rows])
+    scale = np.sqrt(np.mean(ref1 ** 2))
+    err = max(np.abs(out[:, :4] - ref1).max(), np.abs(out[:, 4:8] - ref2).max()) / scale
+    say(f"trained forward vs FP64 oracle on {n_rows} gathered environments: max err {err:.4f} x rms")
+    assert err < 0.05
+    worst = 0.0
+    for r, o in zip(rows, out):
+        ref = host_log_ratio(o[:4], o[4:8], r[:3], r[3:6], r[6:9])
+        k = max(math.exp(min(o[3], 8.3)), math.exp(min(o[7], 8.3)))
+        d = abs(o[8] - ref) / (1 + k + abs(ref))
+        worst = max(worst, d)
+    say(f"device MH log ratio vs FP64 host formula: max scaled err {worst:.2e}")
+    assert worst < 1e-4
+
+
+def check_books(g, tag):
+    n_bad, first = g.check_spins(1e-5)
+    E, inc = g.TotalEnergyv2(), g.energy_incremental()
+    rel = np.abs(inc - E).max() / np.abs(E).max()
+    say(f"{tag}: non-unit spins {n_bad}, max |incremental - recomputed| / |E| = {rel:.2e}")
+    assert n_bad == 0, first
+    assert np.allclose(inc, E, rtol=1e-6, atol=1e-3)
+
+
+def test_c4_workload_learned_proposal():
+    """The learned-proposal workload at its size: pyrochlore L=16 x 64
+    replicas on the geometric ladder 0.01..1, the network trained by the
+    library on 1,048,576 environments (beta h, s) gathered from the plain
+    sampler across the ladder, then 46 to 55 sweeps of 1 learned + 10
+    over-relaxation passes with five exchanges (every 10 sweeps). The trained
+    forward pass on 4,096 of the gathered environments matches the FP64
+    oracle within BF16 tolerance, the device MH log ratio equals the FP64
+    formula, acceptance exceeds 0.8, spins stay unit and the incremental
+    energy equals the recomputed one (5 sweeps after the last re-anchoring
+    exchange). The move replaced is the Metropolis phase of
+    Pyrochlore.cu:829-885."""
+    from cudacode_amd import learned, pyro
+    temps = pyro.geometric_ladder(0.01, 1.0, 64)
+    with ctx(16, temps, n_metro=1, n_or=10, pt_every=10, cos_cone=0.5, seed=56081376) as g:
+        g.CalculateSolidAngle(0.5, 5, 10)
+        g.Mc(100)
+        info = learned.train(g, snapshots=16, per_replica=1024, steps=2000, init_seed=1)
+        say(f"trained: {info}")
+        assert info["environments"] == 1 << 20 and g.environments().shape[0] == 1 << 20
+        assert info["loss_last"] < 0.1 * info["loss_first"]
+        check_trained_network(g)
+        learned.use_learned(g, 1)
+        g.reset_acceptance()
+        s0, r0 = g.sweep_counters()
+        g.Mc(55 - s0 % 10)  # 5 exchanges, ending 5 sweeps after the last
+        a, p = g.acceptance_learned()
+        acc = a.sum() / p.sum()
+        say(f"c4 learned sweeps: acceptance {acc:.3f}, counters {g.sweep_counters()}")
+        assert g.sweep_counters()[1] - r0 == 5
+        assert acc > 0.8
+        assert np.all(a > 0.5 * p)  # every rung
+        check_books(g, "c4")
+
+
+def test_c5_workload_learned_pass_interleaved():
+    """The large-lattice workload at its size: L=64 (4,194,304 spins) x 32
+    replicas on the geometric ladder 0.02..0.5 (1.61 GB of spins), the
+    network trained by the library on 1,048,576 environments of this
+    ensemble, then 15 to 24 sweeps of 1 Metropolis + 1 learned + 10
+    over-relaxation passes with two exchanges (every 10 sweeps): the trained
+    forward pass and MH log ratio as for the L=16 workload, learned
+    acceptance > 0.8, unit spins, incremental == recomputed energy."""
+    from cudacode_amd import learned, pyro
+    temps = pyro.geometric_ladder(0.02, 0.5, 32)
+    with ctx(64, temps, n_metro=1, n_or=10, pt_every=10, cos_cone=0.5, seed=56081376) as g:
+        g.CalculateSolidAngle(0.5, 3, 5)
+        g.Mc(20)
+        info = learned.train(g, snapshots=16, per_replica=2048, steps=2000, init_seed=1)
+        say(f"trained: {info}")
+        assert info["environments"] == 1 << 20
+        check_trained_network(g, seed=1)
+        g.set_learned_passes(1)  # interleaved: Metropolis, learned, over-relaxation
+        g.reset_acceptance()
+        s0, r0 = g.sweep_counters()
+        g.Mc(25 - s0 % 10)  # 2 exchanges, ending 5 sweeps after the last
+        a, p = g.acceptance_learned()
+        am, pm = g.acceptance()
+        say(f"c5 interleaved sweeps: learned acceptance {a.sum() / p.sum():.3f}, "
+            f"Metropolis {am.sum() / pm.sum():.3f}, counters {g.sweep_counters()}")
+        assert g.sweep_counters()[1] - r0 == 2
+        assert a.sum() > 0.8 * p.sum()
+        assert am.sum() > 0
+        check_books(g, "c5")
