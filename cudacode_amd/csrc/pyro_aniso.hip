@@ -11,6 +11,7 @@
 // true, in a translation unit of their own.
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "pyro_phase.h"
 
@@ -54,6 +55,11 @@ static const int kLocalAxis[4][3] = {{1, 1, 1}, {1, 1, -1}, {1, -1, -1}, {1, -1,
 // (Pyrochlore.cu:447-449).
 constexpr double kMuBmeVperT = 0.0578838261305;
 
+// Component (i, j) of the g-tensor of basis site b.
+static double g_tensor(int b, int i, int j, double gxy, double gz) {
+  return (i == j ? gxy : 0.0) + (gz - gxy) * (double)(kLocalAxis[b][i] * kLocalAxis[b][j]) / 3.0;
+}
+
 }  // namespace pyro
 
 using namespace pyro;
@@ -82,14 +88,37 @@ int pyro_zeeman_fields(const double *B3, double gxy, double gz, double *B12) {
   for (int b = 0; b < 4; ++b)
     for (int i = 0; i < 3; ++i) {
       double t = 0.0;
-      for (int j = 0; j < 3; ++j) {
-        const double g = (i == j ? gxy : 0.0) +
-                         (gz - gxy) * (double)(kLocalAxis[b][i] * kLocalAxis[b][j]) / 3.0;
-        t += g * B3[j];
-      }
+      for (int j = 0; j < 3; ++j) t += g_tensor(b, i, j, gxy, gz) * B3[j];
       B12[3 * b + i] = t * kMuBmeVperT;
     }
   return 0;
+}
+
+int pyro_fold_magnetization_g(const double *sub12, int R, double gxy, double gz, double *M3) {
+  if (!sub12 || !M3 || R < 0 || !std::isfinite(gxy) || !std::isfinite(gz))
+    return set_error(PYRO_EINVAL, "pyro_fold_magnetization_g: bad argument");
+  for (int r = 0; r < R; ++r) {
+    const double *sb = sub12 + (size_t)12 * r;
+    for (int i = 0; i < 3; ++i) {
+      // MagnetizationGPU's order: basis sites 0..3 in turn, each a row of
+      // its g-tensor dotted with that site's sum (Pyrochlore.cu:318-336).
+      double m = 0.0;
+      for (int b = 0; b < 4; ++b) {
+        double t = 0.0;
+        for (int j = 0; j < 3; ++j) t += g_tensor(b, i, j, gxy, gz) * sb[3 * b + j];
+        m += t;
+      }
+      M3[(size_t)3 * r + i] = m;
+    }
+  }
+  return 0;
+}
+
+int pyro_magnetization_g(pyro_ctx *ctx, double gxy, double gz, double *M3) {
+  if (!ctx || !M3) return set_error(PYRO_EINVAL, "pyro_magnetization_g: null argument");
+  std::vector<double> sub((size_t)ctx->R * 12);
+  PYRO_TRY(pyro_sublattice_sums(ctx, sub.data()));
+  return pyro_fold_magnetization_g(sub.data(), ctx->R, gxy, gz, M3);
 }
 
 int pyro_set_couplings(pyro_ctx *ctx, const double *C144, const double *B12) {

@@ -34,7 +34,7 @@ EXPORTS = [
     "pyro_energy_incremental",
     "pyro_acceptance", "pyro_reset_acceptance", "pyro_adapt_cone", "pyro_record_start",
     "pyro_record_fetch", "pyro_set_couplings", "pyro_get_couplings", "pyro_exchange_matrices",
-    "pyro_zeeman_fields", "pyro_set_streams", "pyro_get_streams", "pyro_set_or_fusion", "pyro_check_spins", "pyro_set_post_rotate", "pyro_anneal", "pyro_measure_start", "pyro_measure_stop", "pyro_measure_fetch",
+    "pyro_zeeman_fields", "pyro_magnetization_g", "pyro_fold_magnetization_g", "pyro_set_streams", "pyro_get_streams", "pyro_set_or_fusion", "pyro_check_spins", "pyro_set_post_rotate", "pyro_anneal", "pyro_measure_start", "pyro_measure_stop", "pyro_measure_fetch",
     "pyro_set_passes", "pyro_set_kernel_options", "pyro_create_batch",
     "pyro_get_layout", "pyro_set_exec_mode", "pyro_get_exec_mode", "pyro_autotune",
     "pyro_checkpoint_save", "pyro_checkpoint_load", "pyro_export_configurations",
@@ -42,7 +42,8 @@ EXPORTS = [
     "pyro_mlp_set_weights", "pyro_mlp_get_weights",
     "pyro_mlp_eval", "pyro_vmf_draw", "pyro_mlp_loss_grad", "pyro_gather_environments",
     "pyro_get_environments", "pyro_clear_environments", "pyro_mlp_train", "pyro_learned_phase",
-    "pyro_set_learned_passes", "pyro_acceptance_learned", "pyro_mlp_adam_step", "pyro_time_phase", "pyro_sync",
+    "pyro_set_learned_passes", "pyro_acceptance_learned", "pyro_mlp_adam_step", "pyro_time_phase",
+    "pyro_sync",
 ]
 
 MLP_PARAMS = 4868
@@ -137,6 +138,8 @@ def lib():
         L.pyro_get_couplings.argtypes = [vp, C.POINTER(C.c_int), vp]
         L.pyro_exchange_matrices.argtypes = [C.c_double] * 4 + [vp]
         L.pyro_zeeman_fields.argtypes = [vp, C.c_double, C.c_double, vp]
+        L.pyro_magnetization_g.argtypes = [vp, C.c_double, C.c_double, vp]
+        L.pyro_fold_magnetization_g.argtypes = [vp, C.c_int, C.c_double, C.c_double, vp]
         L.pyro_measure_start.argtypes = [vp, C.c_int, C.c_int]
         L.pyro_measure_stop.argtypes = [vp]
         L.pyro_measure_fetch.argtypes = [vp, vp, C.POINTER(C.c_int)]
@@ -208,6 +211,15 @@ def fold_order_params(sub12, n_spins: float) -> np.ndarray:
     s = np.ascontiguousarray(sub12, np.float64).reshape(-1, 12)
     out = np.empty_like(s)
     _check(lib().pyro_fold_order_params(_p(s), s.shape[0], float(n_spins), _p(out)))
+    return out
+
+
+def fold_magnetization_g(sub12, gxy: float, gz: float) -> np.ndarray:
+    """g-tensor-weighted magnetisation (R, 3) from sublattice sums (R, 12);
+    host arithmetic only (MagnetizationGPU, Pyrochlore.cu:277-342)."""
+    s = np.ascontiguousarray(sub12, np.float64).reshape(-1, 12)
+    out = np.empty((s.shape[0], 3), np.float64)
+    _check(lib().pyro_fold_magnetization_g(_p(s), s.shape[0], float(gxy), float(gz), _p(out)))
     return out
 
 
@@ -370,6 +382,13 @@ class Pyrochlore:
 
     def Magnetization(self) -> np.ndarray:
         return self._vec(lib().pyro_magnetization, 3)
+
+    def MagnetizationG(self, gxy: float, gz: float) -> np.ndarray:
+        """Sum over spins of g_b s_i, g_b the g-tensor of the spin's basis
+        site (MagnetizationGPU, Pyrochlore.cu:277-342), (R, 3)."""
+        out = np.empty(self.R * 3, np.float64)
+        _check(lib().pyro_magnetization_g(self._ctx, float(gxy), float(gz), _p(out)))
+        return out.reshape(self.R, 3)
 
     def DoMeasurement(self) -> np.ndarray:
         return self._vec(lib().pyro_order_params, 12)

@@ -158,7 +158,8 @@ int pyro_pt_swap(pyro_ctx *ctx, int parity, uint32_t round, int *n_accepted);
 
 /* Observables, FP64, per replica (replica order). energy: R doubles
  * (TotalEnergyv2, Math.cu:309-330); magnetization: R*3 (Magnetization,
- * Pyrochlore.cu:453-466, g = 1); order_params: R*12 q=0 components
+ * Pyrochlore.cu:453-466, g = 1; pyro_magnetization_g weighs by the
+ * g-tensors); order_params: R*12 q=0 components
  * normalised so the ideal ordered state has modulus 1 (Measurement +
  * DoMeasurement, Pyrochlore.cu:526-641,1081-1092); sublattice: R*12 raw
  * sublattice sums (b, xyz). */
@@ -218,6 +219,17 @@ int pyro_exchange_matrices(double J1, double J2, double J3, double J4, double *C
  * components g_xy, g_z about each site's local [111] axis, in meV
  * (InitializeMagneticField, Pyrochlore.cu:414-450). */
 int pyro_zeeman_fields(const double *B3, double gxy, double gz, double *B12);
+
+/* Magnetisation weighted by the g-tensor of each basis site, M = sum_b g_b
+ * s_b over the sublattice sums s_b, with g_b = g_xy I + (g_z - g_xy) n_b
+ * n_b^T about the site's local [111] axis n_b: MagnetizationGPU
+ * (Pyrochlore.cu:277-342, its g1..g4 built from g1 = (2 g_xy + g_z)/3 and
+ * g2 = (g_z - g_xy)/3 as InitializeMagneticField does, :416-420) followed by
+ * its sum over cells (Magnetization, :453-466). M3: R*3 doubles, replica
+ * order; g_xy = g_z = 1 gives pyro_magnetization. The fold takes R rows of
+ * pyro_sublattice_sums and needs no device. */
+int pyro_magnetization_g(pyro_ctx *ctx, double gxy, double gz, double *M3);
+int pyro_fold_magnetization_g(const double *sub12, int R, double gxy, double gz, double *M3);
 
 /* Simulated annealing (SimulatedAnnealing / SimulatedAnnealingThread,
  * MonteCarlo.cu:68-104,295-344): `steps` steps; at step m = 0..steps-1 every

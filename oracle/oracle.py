@@ -63,6 +63,8 @@ def lib():
         L.orc_set_post_rotate.argtypes = [C.c_int]
         L.orc_rotate_about.argtypes = [_f32p, _f32p, C.c_float, _f32p]
         L.orc_set_couplings.argtypes = [C.c_void_p]
+        L.orc_magnetization_g.argtypes = [_f64p, C.c_double, C.c_double, _f64p]
+        L.orc_adapt_cone_step.argtypes = [C.c_int, C.c_double, _u64p, _u64p, _f32p]
         L.orc_num_threads.restype = C.c_int
         L.orc_set_num_threads.argtypes = [C.c_int]
         L.orc_set_replica_base.argtypes = [C.c_int]
@@ -188,6 +190,17 @@ class OracleRun:
     def observables(self):
         return observables(self.spins, self.L, self.R, self.jeff)
 
+    def adapt_cone(self, target, rounds, sweeps_per_round):
+        """Cone adaptation: rounds of sweeps_per_round sweeps, each followed by
+        one adapt_cone_step on the acceptance of that round."""
+        per_sweep = np.uint64(self.n_metro * 16 * self.L ** 3)
+        for _ in range(rounds):
+            self.accepted[:] = 0
+            self.run(sweeps_per_round)
+            proposed = np.full(self.R, per_sweep * np.uint64(sweeps_per_round), np.uint64)
+            adapt_cone_step(target, self.accepted, proposed, self.cosc)
+        self.accepted[:] = 0
+
 
 _order_keep = None
 
@@ -229,6 +242,22 @@ def set_couplings(coup156) -> None:
     _coup_keep = np.ascontiguousarray(coup156, np.float32).copy()
     assert _coup_keep.size == 156
     lib().orc_set_couplings(_coup_keep.ctypes.data_as(C.c_void_p))
+
+
+def magnetization_g(sub12, gxy: float, gz: float) -> np.ndarray:
+    """g-tensor-weighted magnetisation (3,) from one replica's 12 sublattice sums."""
+    out = np.zeros(3, np.float64)
+    lib().orc_magnetization_g(np.ascontiguousarray(sub12, np.float64).reshape(12), float(gxy),
+                              float(gz), out)
+    return out
+
+
+def adapt_cone_step(target: float, accepted, proposed, cos_cone) -> None:
+    """One round of the cone rule on cos_cone (float32, updated in place)."""
+    assert cos_cone.dtype == np.float32 and cos_cone.flags.c_contiguous
+    lib().orc_adapt_cone_step(cos_cone.size, float(target),
+                              np.ascontiguousarray(accepted, np.uint64),
+                              np.ascontiguousarray(proposed, np.uint64), cos_cone)
 
 
 def set_replica_base(base: int) -> None:

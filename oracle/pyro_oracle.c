@@ -471,6 +471,60 @@ void orc_observables(const float *spins, int L, int R, float jeff, double *out) 
   }
 }
 
+/* MagnetizationGPU, Pyrochlore.cu:277-342: the sum over spins of g_b s_i,
+ * from the four sublattice sums s_b (sub12[3 b + k]). The four g-tensors are
+ * the reference's g1..g4 (:283-286), whose diagonal is g1 = (2 gxy + gz)/3
+ * and whose off-diagonal entries are +-g2, g2 = (gz - gxy)/3, with the sign
+ * patterns of :283-286 (the same numbers InitializeMagneticField uses,
+ * :416-420). The sum runs as the kernel's: site 0's row products, then
+ * sites 1, 2, 3 added in turn (:318-336). */
+void orc_magnetization_g(const double sub12[12], double gxy, double gz, double out[3]) {
+  const double g1 = (2.0 * gxy + gz) / 3.0, g2 = (gz - gxy) / 3.0;
+  /* off-diagonal signs of (0,1), (0,2), (1,2) per basis site, Pyrochlore.cu:283-286 */
+  static const int sgn[4][3] = {{1, 1, 1}, {1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}};
+  for (int i = 0; i < 3; i++) out[i] = 0.0;
+  for (int b = 0; b < 4; b++) {
+    double g[3][3];
+    g[0][0] = g[1][1] = g[2][2] = g1;
+    g[0][1] = g[1][0] = sgn[b][0] * g2;
+    g[0][2] = g[2][0] = sgn[b][1] * g2;
+    g[1][2] = g[2][1] = sgn[b][2] * g2;
+    const double *sb = sub12 + 3 * b;
+    for (int i = 0; i < 3; i++) out[i] += g[i][0] * sb[0] + g[i][1] * sb[1] + g[i][2] * sb[2];
+  }
+}
+
+/* One round of the cone adaptation, after a round of sweeps with acceptance
+ * counts accepted[r] of proposed[r]. The reference's CalculateSolidAngle
+ * (MonteCarlo.cu:371-426) multiplies the half-angle theta = acos(cos_cone) by
+ * 0.9 while the ratio is below the target, never enlarges it, and stops when
+ * replica 0 reaches the target; its acceptance counters are never filled by
+ * the production kernels (MonteCarlo.cu:346-367 reads DevResults, which
+ * MCMultipleTemperaturesPerSystem leaves alone). This build's rule, restated
+ * here: every replica's theta is multiplied by 0.9 below the target and
+ * divided by 0.9 at or above it, clamped to [1e-4, pi], and cos_cone =
+ * (float)cos(theta), kept below 1. */
+void orc_adapt_cone_step(int R, double target, const uint64_t *accepted,
+                         const uint64_t *proposed, float *cos_cone) {
+  for (int r = 0; r < R; r++) {
+    double ratio = proposed[r] ? (double)accepted[r] / (double)proposed[r] : 0.0;
+    double c = (double)cos_cone[r];
+    if (c > 1.0) c = 1.0;
+    if (c < -1.0) c = -1.0;
+    double theta = acos(c);
+    if (ratio < target)
+      theta *= 0.9;
+    else
+      theta *= 1.0 / 0.9;
+    if (theta > 3.14159265358979323846) theta = 3.14159265358979323846;
+    if (theta < 1e-4) theta = 1e-4;
+    float f = (float)cos(theta);
+    if (f > 0.99999994f) f = 0.99999994f;
+    if (f < -1.0f) f = -1.0f;
+    cos_cone[r] = f;
+  }
+}
+
 /* TrySwapingTemperatures, MonteCarlo.cu:105-172. The MKL uniforms become
  * Philox draws (counter = pair index, kind = swap, round), and the gsl sort by
  * descending temperature is kept incrementally in replica_at[]. */

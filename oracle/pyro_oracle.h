@@ -67,6 +67,9 @@ int orc_pt_swap(int R, int parity, uint32_t round, uint64_t seed, const double *
  * spin_len2, 9 per ordered basis pair) and Zeeman fields (times 2 spin_len2,
  * 3 per basis site). The array must outlive its use. */
 void orc_set_couplings(const float *coup156);
+void orc_magnetization_g(const double sub12[12], double gxy, double gz, double out[3]);
+void orc_adapt_cone_step(int R, double target, const uint64_t *accepted,
+                         const uint64_t *proposed, float *cos_cone);
 
 /* Rotation of s about h by 2 pi u after each Metropolis test (Math.cu:168-197):
  * the move itself, and the switch for every following Metropolis phase. */

@@ -98,3 +98,67 @@ def test_zeeman_fields_follow_the_local_axes():
     n0 = VERTS[0] / np.linalg.norm(VERTS[0])
     F = pyro.zeeman_fields(n0, gxy, gz)
     assert np.allclose(F[0], mu * gz * n0, rtol=1e-13)
+
+
+def test_exchange_matrices_equal_the_reference_table():
+    """The library's J1..J4 placement equals the reference's couplings_const
+    table (Pyrochlore.cu:169-184) evaluated at unit, Heisenberg, default and
+    generic J sets (fixture written by tests/golden/make_couplings_fixture.py
+    from the table's own entries): every sign and position is pinned."""
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "couplings_const.json")))
+    assert len(fx["J_sets"]) == len(fx["C"]) >= 5
+    for J, ref in zip(fx["J_sets"], fx["C"]):
+        C = pyro.exchange_matrices(*J).reshape(144)
+        assert np.array_equal(C, np.asarray(ref, np.float64)), J
+
+
+def sublattice_state(dirs, n_per_site=4):
+    """Sublattice sums (12,) of a state whose spins on basis site b all point
+    along dirs[b] (n_per_site spins per basis site)."""
+    return np.concatenate([n_per_site * np.asarray(d, np.float64) for d in dirs])
+
+
+@pytest.mark.parametrize("fold", ["library", "oracle"])
+def test_g_magnetization_closed_forms(fold):
+    """M = sum_b g_b s_b: spins along each site's own local axis n_b are scaled
+    by g_z, spins perpendicular to it by g_xy, an isotropic g is a plain
+    scaling, and a uniform state along z gives N (2 g_xy + g_z)/3 z (the
+    trace average of the four tensors)."""
+    from oracle import oracle
+
+    def M(sub, gxy, gz):
+        if fold == "library":
+            return pyro.fold_magnetization_g(sub[None, :], gxy, gz)[0]
+        return oracle.magnetization_g(sub, gxy, gz)
+
+    gxy, gz = 4.18, 1.77
+    n = -VERTS / np.linalg.norm(VERTS, axis=1, keepdims=True)  # any sign: n n^T is even
+    along = sublattice_state(n)
+    assert np.allclose(M(along, gxy, gz), gz * along.reshape(4, 3).sum(axis=0), rtol=1e-14, atol=1e-14)
+    perp = []
+    for b in range(4):
+        v = np.cross(n[b], [1.0, 0.0, 0.0] if b % 2 else [0.0, 1.0, 0.0])
+        perp.append(v / np.linalg.norm(v))
+    ps = sublattice_state(perp)
+    assert np.allclose(M(ps, gxy, gz), gxy * ps.reshape(4, 3).sum(axis=0), rtol=1e-13, atol=1e-13)
+    rnd = np.random.default_rng(3).normal(size=12)
+    assert np.allclose(M(rnd, 2.5, 2.5), 2.5 * rnd.reshape(4, 3).sum(axis=0), rtol=1e-14)
+    up = sublattice_state([[0, 0, 1]] * 4)
+    assert np.allclose(M(up, gxy, gz), [0, 0, 16 * (2 * gxy + gz) / 3], rtol=1e-14, atol=1e-13)
+
+
+def test_g_magnetization_fold_equals_oracle():
+    """The library's fold (g-tensors from the local axes) and the oracle's
+    (the reference's g1/g2 sign patterns, Pyrochlore.cu:283-286) agree to
+    rounding on random sublattice sums; at g = 1 the fold is the plain sum
+    bit for bit."""
+    from oracle import oracle
+    sub = np.random.default_rng(5).normal(size=(20, 12)) * 1e3
+    for gxy, gz in [(4.18, 1.77), (0.3, 7.0), (1.0, 1.0)]:
+        got = pyro.fold_magnetization_g(sub, gxy, gz)
+        ref = np.array([oracle.magnetization_g(s, gxy, gz) for s in sub])
+        assert np.allclose(got, ref, rtol=1e-13, atol=1e-9)
+    plain = ((sub[:, 0:3] + sub[:, 3:6]) + sub[:, 6:9]) + sub[:, 9:12]
+    assert np.array_equal(pyro.fold_magnetization_g(sub, 1.0, 1.0), plain)

@@ -217,6 +217,26 @@ def test_cone_adaptation_reaches_target():
         assert r[2] > 0.4  # at high T even the whole sphere is accepted often
 
 
+def test_cone_adaptation_equals_oracle():
+    """The library's adaptation loop (pyro_adapt_cone: rounds of sweeps,
+    per-replica acceptance, the 0.9 / 1/0.9 rule, with exchanges carrying the
+    cones between replicas) equals the oracle's restatement bit for bit:
+    cones, spins, rungs after 12 rounds from the whole sphere."""
+    L = 4
+    temps = [0.05, 0.2, 0.5, 5.0]
+    kw = dict(n_metro=1, n_or=1, pt_every=2, cos_cone=-1.0)
+    with gpu(L, temps, **kw) as g:
+        cone = g.CalculateSolidAngle(target=0.5, rounds=12, sweeps_per_round=4)
+        s = g.get_spins()
+        T, at = g.temperatures()
+    o = oracle.OracleRun(L, temps, seed=SEED, **kw)
+    o.adapt_cone(0.5, 12, 4)
+    assert np.array_equal(cone, o.cosc), (cone, o.cosc)
+    assert np.array_equal(s, o.spins)
+    assert np.array_equal(T, o.temps) and np.array_equal(at, o.replica_at)
+    assert len(set(cone.tolist())) > 1  # the replicas' cones really differ
+
+
 def test_bad_arguments_raise():
     from cudacode_amd import pyro
     with gpu(4, [1.0]) as g:

@@ -261,3 +261,42 @@ def test_phase_order_independence():
     assert np.array_equal(a.spins, b.spins)
     assert np.array_equal(a.accepted, b.accepted)
     assert np.allclose(a.dE, b.dE, rtol=1e-12, atol=1e-9)
+
+
+def test_cone_rule_moves_toward_the_target_and_stays_bounded():
+    """The oracle's cone rule, pinned by what it is for rather than by its
+    formula: all rejected shrinks the cone (cos grows), all accepted widens it
+    (cos falls), and cos_cone never leaves [-1, 1) however many rounds push
+    it against either end."""
+    c = np.array([0.2, 0.2, 0.99999994, -1.0], np.float32)
+    prop = np.full(4, 1000, np.uint64)
+    acc = np.array([0, 1000, 0, 1000], np.uint64)
+    c0 = c.copy()
+    oracle.adapt_cone_step(0.5, acc, prop, c)
+    assert c[0] > c0[0] and c[1] < c0[1]
+    for _ in range(200):
+        oracle.adapt_cone_step(0.5, acc, prop, c)
+    assert c[0] < 1.0 and c[2] < 1.0 and c[1] == -1.0 and c[3] == -1.0
+    assert c[0] > 0.9999 and c[2] > 0.9999
+
+
+def test_cone_rule_converges_from_both_sides():
+    """From the whole sphere at low temperature (most proposals rejected) and
+    from a tiny cone at moderate temperature (almost all accepted), rounds of
+    the rule bring each replica's Metropolis acceptance to the 50% target;
+    where even the whole sphere is accepted more than half the time (high T)
+    the cone opens to the whole sphere."""
+    def ratio(o, sweeps=20):
+        o.accepted[:] = 0
+        o.run(sweeps)
+        return o.accepted / (sweeps * 16 * o.L ** 3)
+
+    o = oracle.OracleRun(4, [0.05, 0.3], seed=5, n_metro=1, n_or=1, cos_cone=-1.0)
+    o.adapt_cone(0.5, 40, 3)
+    r = ratio(o)
+    assert np.all(np.abs(r - 0.5) < 0.1), r
+    o = oracle.OracleRun(4, [0.3, 20.0], seed=6, n_metro=1, n_or=1, cos_cone=0.9999)
+    o.adapt_cone(0.5, 80, 3)
+    r = ratio(o)
+    assert abs(r[0] - 0.5) < 0.1, r
+    assert r[1] > 0.5 and o.cosc[1] == -1.0, (r, o.cosc)
