@@ -413,6 +413,11 @@ static int create_ctx(const std::vector<int> &Ls, const double *temps, uint64_t 
       hipMemset(ctx->d_anchor, 0, sizeof(double) * R) != hipSuccess ||
       hipMemcpy(ctx->d_cosc, cosc.data(), sizeof(float) * R, hipMemcpyHostToDevice) != hipSuccess)
     return fail(set_error(PYRO_EHIP, "pyro_create: initial upload failed"));
+  // Resolve this device's no-allocation scratch now, outside any capture
+  // (pyro_observables_noalloc resolves it on its first call otherwise, and
+  // reports a failure itself: the ctx does not need it).
+  double *scratch_ptr = nullptr;
+  (void)noalloc_scratch(device, &scratch_ptr);
   int rc = batch_setup(ctx, 0);
   if (rc) return fail(rc);
   rc = pyro_set_temperatures(ctx, temps);

@@ -228,6 +228,8 @@ int launch_observables_partials_mat(pyro_ctx *ctx);
 int launch_anneal_beta(pyro_ctx *ctx, double T0, int m, int steps, int restore);
 int launch_set_anchor(pyro_ctx *ctx);            // d_anchor = d_obs energy, d_dE = 0
 int launch_set_sweep(pyro_ctx *ctx, uint32_t sweep, uint32_t round);
+// Address of the no-allocation scratch array on `device` (the current device).
+int noalloc_scratch(int device, double **ptr);
 
 // Batches and alternative sweep executions (pyro_batch.hip).
 bool batch_size_supported(int L);

@@ -2,6 +2,8 @@
 // (Heisenberg J) colour phases and observables from pyro_phase.h, spin
 // initialisation, the observables' second stage, parallel-tempering
 // exchanges, the per-sweep bookkeeping and the measurement rows.
+#include <mutex>
+
 #include "pyro_phase.h"
 
 namespace pyro {
@@ -405,4 +407,73 @@ int launch_set_sweep(pyro_ctx *ctx, uint32_t sweep, uint32_t round) {
   return 0;
 }
 
+// Scratch of pyro_observables_noalloc: the first-stage partial sums of one
+// call, R * 4 * ceil(L^3 / 256) * 4 doubles, at most kNoAllocScratchDoubles
+// (4 MiB: L = 16 x 64 replicas needs 128 KiB, L = 64 x 32 all of it). Each GPU
+// holds its own copy until the process ends; every call on a GPU shares it.
+// Host code uses the per-device addresses from hipGetSymbolAddress, never the
+// array's name (a host-side stand-in).
+constexpr size_t kNoAllocScratchDoubles = size_t(1) << 19;
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static double *g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mutex;
+
+// The device address of `scratch` on `device`, which must be the current
+// device; resolved once per device.
+int noalloc_scratch(int device, double **ptr) {
+  if (device < 0 || device >= kMaxDevices)
+    return set_error(PYRO_EINVAL, "no-allocation scratch: device %d outside [0, %d)", device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(PYRO_EHIP, "no-allocation scratch: hipGetSymbolAddress failed on device %d: %s",
+                       device, hipGetErrorString(e));
+    g_scratch_ptr[device] = (double *)p;
+  }
+  *ptr = g_scratch_ptr[device];
+  return 0;
+}
+
 }  // namespace pyro
+
+using namespace pyro;
+
+extern "C" int pyro_observables_noalloc(int device, const float *spins, int L, int R, float jeff,
+                                        double *obs13, void *stream) {
+  if (!spins || !obs13) return set_error(PYRO_EINVAL, "pyro_observables_noalloc: null pointer");
+  if (L < 2 || L > 256 || (size_t)L * L * L * 16 > (size_t)INT32_MAX)
+    return set_error(PYRO_EINVAL, "pyro_observables_noalloc: L=%d outside [2, 256]", L);
+  if (R < 1 || R > 65535)
+    return set_error(PYRO_EINVAL, "pyro_observables_noalloc: R=%d outside [1, 65535]", R);
+  const int ncell = L * L * L;
+  const int nblk = (ncell + kObsBlock - 1) / kObsBlock;
+  const size_t need = (size_t)R * 4 * nblk * 4;
+  if (need > kNoAllocScratchDoubles)
+    return set_error(PYRO_EINVAL,
+                     "pyro_observables_noalloc: L=%d x %d replicas need %zu bytes of scratch, above "
+                     "the %zu-byte maximum",
+                     L, R, need * sizeof(double), kNoAllocScratchDoubles * sizeof(double));
+  int ndev = 0, prev = 0;
+  PYRO_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev)
+    return set_error(PYRO_EINVAL, "pyro_observables_noalloc: device %d of %d", device, ndev);
+  PYRO_HIP(hipGetDevice(&prev));
+  if (prev != device) PYRO_HIP(hipSetDevice(device));
+  double *part = nullptr;
+  int rc = noalloc_scratch(device, &part);
+  if (rc == 0) {
+    hipStream_t st = (hipStream_t)stream;
+    // Every partial is written by its block, so nothing needs clearing.
+    k_observables<false><<<dim3(nblk, 4, R), kObsBlock, 0, st>>>(spins, L, ncell, part, nullptr);
+    k_observables_final<<<R, 256, 0, st>>>(part, nblk, 0.5 * (double)jeff, obs13, nullptr, nullptr);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      rc = set_error(PYRO_EHIP, "pyro_observables_noalloc: launch failed: %s", hipGetErrorString(e));
+  }
+  if (prev != device) (void)hipSetDevice(prev);
+  return rc;
+}

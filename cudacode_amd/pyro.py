@@ -43,7 +43,7 @@ EXPORTS = [
     "pyro_mlp_eval", "pyro_vmf_draw", "pyro_mlp_loss_grad", "pyro_gather_environments",
     "pyro_get_environments", "pyro_clear_environments", "pyro_mlp_train", "pyro_learned_phase",
     "pyro_set_learned_passes", "pyro_acceptance_learned", "pyro_mlp_adam_step", "pyro_time_phase",
-    "pyro_sync",
+    "pyro_observables_noalloc", "pyro_sync",
 ]
 
 MLP_PARAMS = 4868
@@ -171,6 +171,7 @@ def lib():
         L.pyro_learned_phase.argtypes = [vp, C.c_int, C.c_uint32, C.c_int]
         L.pyro_set_learned_passes.argtypes = [vp, C.c_int]
         L.pyro_acceptance_learned.argtypes = [vp, vp, vp]
+        L.pyro_observables_noalloc.argtypes = [C.c_int, vp, C.c_int, C.c_int, C.c_float, vp, vp]
         L.pyro_sync.argtypes = [vp]
         _lib = L
     return _lib
@@ -221,6 +222,16 @@ def fold_magnetization_g(sub12, gxy: float, gz: float) -> np.ndarray:
     out = np.empty((s.shape[0], 3), np.float64)
     _check(lib().pyro_fold_magnetization_g(_p(s), s.shape[0], float(gxy), float(gz), _p(out)))
     return out
+
+
+def observables_noalloc(device: int, spins_ptr: int, L: int, R: int, jeff: float, out_ptr: int,
+                        stream_ptr: int = 0) -> None:
+    """Enqueue the energies and sublattice sums (R x 13 doubles at device
+    address out_ptr) of R replicas of spins at device address spins_ptr on the
+    HIP stream stream_ptr (0: the null stream); allocates nothing."""
+    _check(lib().pyro_observables_noalloc(int(device), C.c_void_p(spins_ptr), int(L), int(R),
+                                          float(jeff), C.c_void_p(out_ptr),
+                                          C.c_void_p(stream_ptr or None)))
 
 
 # Exchange constants of the reference's defaults (Pyrochlore.cu:139-142).

@@ -377,6 +377,24 @@ int pyro_import_configurations(pyro_ctx *ctx, const char *prefix, int rep, int b
  * around Mc (MonteCarlo.cu:36-63). */
 int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, double *span_ms);
 
+/* Energies and sublattice sums of R replicas of L^3-cell spins that the caller
+ * holds in device memory (uniform layout, scalar model, jeff = J * spin_len2),
+ * into the caller's device buffer obs13: per replica E, then the 12
+ * sublattice sums (b, xyz) -- the values pyro_energy and pyro_sublattice_sums
+ * return for a ctx holding the same spins, bit for bit. Enqueued on `stream`
+ * (a hipStream_t; NULL: the null stream) of `device`, nothing synchronised,
+ * nothing allocated: the partial sums go to a fixed scratch array in the
+ * library, one per GPU, of at most 4 MiB (R * 4 * ceil(L^3/256) * 4 doubles;
+ * L = 64 x 32 replicas fits exactly); larger requests are PYRO_EINVAL before
+ * any device work. Calls that overlap on one GPU race on that scratch. The
+ * scratch address is looked up once per device, at pyro_create or at the
+ * first call on that device: a caller that captures graphs makes that first
+ * call (or creates a ctx) before capturing.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops). */
+int pyro_observables_noalloc(int device, const float *spins, int L, int R, float jeff,
+                             double *obs13, void *stream);
+
 /* Synchronise the ctx stream. */
 int pyro_sync(pyro_ctx *ctx);
 
