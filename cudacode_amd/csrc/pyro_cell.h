@@ -16,6 +16,18 @@ namespace pyro {
 
 constexpr Link kLinks[16][6] = PYRO_LINKS16_INIT;
 
+// Access at a 32-bit byte offset from a base that is uniform over the wave
+// (a replica's component plane): the base stays in scalar registers and the
+// global load or store takes the offset as one 32-bit vector register, with
+// no 64-bit address arithmetic per access. Offsets stay below 2^31: a
+// replica of L <= 64 spans 48 L^3 floats = 50 MB.
+__device__ __forceinline__ float ld_at(const float *__restrict__ base, uint32_t off) {
+  return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + off);
+}
+__device__ __forceinline__ void st_at(float *__restrict__ base, uint32_t off, float v) {
+  *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
 // Local-field models. Scalar (the Heisenberg J of pyro_create): a site's
 // field is -jeff times the plain FP32 neighbour sum f. Matrix (anisotropic
 // exchange and Zeeman term, pyro_set_couplings; the reference's
@@ -125,13 +137,13 @@ __device__ __forceinline__ void cell_update(float *__restrict__ Sx, int cell, in
 #pragma unroll
     for (int n = 0; n < 6; ++n) {
       const Link lk = kLinks[site][n];
-      const int j = lk.site * NC + Z[lk.dz + 1] + Y[lk.dy + 1] + X[lk.dx + 1];
-      const float a = Sx[j], b = Sy[j], c = Sz[j];
+      const uint32_t j = 4u * (uint32_t)(lk.site * NC + Z[lk.dz + 1] + Y[lk.dy + 1] + X[lk.dx + 1]);
+      const float a = ld_at(Sx, j), b = ld_at(Sy, j), c = ld_at(Sz, j);
       field_add<MAT, KIND == 0 || TRACK>(coup, COLOUR, lk.site & 3, a, b, c, fx[k], fy[k], fz[k],
                                          gx[k], gy[k], gz[k]);
     }
-    const int i = site * NC + cell;
-    s[k] = {Sx[i], Sy[i], Sz[i]};
+    const uint32_t i = 4u * (uint32_t)(site * NC + cell);
+    s[k] = {ld_at(Sx, i), ld_at(Sy, i), ld_at(Sz, i)};
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -160,10 +172,10 @@ __device__ __forceinline__ void cell_update(float *__restrict__ Sx, int cell, in
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     if (wr[k]) {
-      const int i = (COLOUR + 4 * k) * NC + cell;
-      Sx[i] = sp[k].x;
-      Sy[i] = sp[k].y;
-      Sz[i] = sp[k].z;
+      const uint32_t i = 4u * (uint32_t)((COLOUR + 4 * k) * NC + cell);
+      st_at(Sx, i, sp[k].x);
+      st_at(Sy, i, sp[k].y);
+      st_at(Sz, i, sp[k].z);
     }
   }
 }
@@ -246,14 +258,16 @@ __device__ __forceinline__ void or_pair_cell(const float *__restrict__ cur,
   const int Y[3] = {(y == 0 ? L - 1 : y - 1) * L, y * L, (y == L - 1 ? 0 : y + 1) * L};
   const int Z[3] = {(z == 0 ? L - 1 : z - 1) * LL, z * LL, (z == L - 1 ? 0 : z + 1) * LL};
   float vx[NN], vy[NN], vz[NN];
+  const float *cy = cur + P, *cz = cur + 2 * P;
+  float *oy = oth + P, *oz = oth + 2 * P;
 #pragma unroll
   for (int i = 0; i < NN; ++i) {
     const PairNeed q = nd.v[i];
-    const int j = q.site * NC + Z[q.dz + 1] + Y[q.dy + 1] + X[q.dx + 1];
-    const float *b = (q.site & 3) < CA ? oth : cur;
-    vx[i] = b[j];
-    vy[i] = b[P + j];
-    vz[i] = b[2 * P + j];
+    const uint32_t j = 4u * (uint32_t)(q.site * NC + Z[q.dz + 1] + Y[q.dy + 1] + X[q.dx + 1]);
+    const bool o = (q.site & 3) < CA;
+    vx[i] = ld_at(o ? oth : cur, j);
+    vy[i] = ld_at(o ? oy : cy, j);
+    vz[i] = ld_at(o ? oz : cz, j);
   }
   // Phase-CA update of a colour-CA site from its own value (index self) and
   // its 6 links (indices nb), with its FP64 neighbour sum.
@@ -321,13 +335,14 @@ __device__ __forceinline__ void or_pair_cell(const float *__restrict__ cur,
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const int ia = (CA + 4 * k) * NC + cell, ib = (CB + 4 * k) * NC + cell;
-    oth[ia] = na[k].x;
-    oth[P + ia] = na[k].y;
-    oth[2 * P + ia] = na[k].z;
-    oth[ib] = nb[k].x;
-    oth[P + ib] = nb[k].y;
-    oth[2 * P + ib] = nb[k].z;
+    const uint32_t ia = 4u * (uint32_t)((CA + 4 * k) * NC + cell);
+    const uint32_t ib = 4u * (uint32_t)((CB + 4 * k) * NC + cell);
+    st_at(oth, ia, na[k].x);
+    st_at(oy, ia, na[k].y);
+    st_at(oz, ia, na[k].z);
+    st_at(oth, ib, nb[k].x);
+    st_at(oy, ib, nb[k].y);
+    st_at(oz, ib, nb[k].z);
   }
 }
 

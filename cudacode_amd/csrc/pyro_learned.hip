@@ -246,11 +246,14 @@ __device__ __forceinline__ f3 vmf_sample(const Vmf &v, float u1, float u2) {
   const float d = up ? 1.0f + m.z : 1.0f - m.z;
   const float scal = (N.x * r.x + N.y * r.y + N.z * r.z) * rcp_f(d);
   f3 o = {N.x * scal - r.x, N.y * scal - r.y, N.z * scal - r.z};
-  // The stored spin is normalised with correctly rounded sqrt and divide, as
-  // every other update does: a hardware rsq would leave |s| off 1 by a biased
-  // ulp, and the over-relaxation passes that renormalise the spin afterwards
-  // would turn that bias into an untracked energy drift.
-  return normalise(o);
+  // The stored spin is renormalised as every other update does, so that its
+  // length has the same (unbiased) statistics: the over-relaxation passes that
+  // renormalise it afterwards would otherwise turn any difference into an
+  // untracked energy drift (a hardware rsq here did, by an order of
+  // magnitude). This vector also carries the hardware sin, cos and reciprocal
+  // errors, larger than rounding, so it takes two steps: the first leaves
+  // |q - 1| at the square of its deviation.
+  return renormalise(renormalise(o));
 }
 
 // log of the Metropolis-Hastings ratio for s -> sp at fixed local field h:

@@ -206,6 +206,27 @@ PYRO_HD f3 normalise(f3 v) {
   return o;
 }
 
+// Renormalisation of a vector whose squared length q is within 1e-4 of 1:
+// every caller's vector is a reflection or rotation of a unit vector, off by
+// FP32 rounding only (|q - 1| of order 1e-7). One Newton step for 1/sqrt(q)
+// from 1 gives the factor 1 + c with c = (1 - q)/2, whose error 3/8 (q - 1)^2
+// is below 4e-9, far under an FP32 ulp. Both 1 - q (Sterbenz) and the halving
+// are exact, and each component is scaled as v + v c, so its only rounding of
+// consequence is the last one: the length comes out unbiased to about 3e-9
+// (a factor 1 + c rounded to the float grid next to 1 would be coarser above
+// 1 than below and bias it), in four IEEE operations and three per component
+// instead of a correctly rounded square root and divide. Deterministic, so
+// host and device round identically.
+PYRO_HD f3 renormalise(f3 v) {
+  float q = v.x * v.x + v.y * v.y + v.z * v.z;
+  float c = 0.5f * (1.0f - q);
+  f3 o;
+  o.x = v.x + v.x * c;
+  o.y = v.y + v.y * c;
+  o.z = v.z + v.z * c;
+  return o;
+}
+
 // Uniform point on the unit sphere from two uniforms: azimuth 2*pi*u_phi,
 // cos(theta) = 2*u_cos - 1 (Math.cu:120-130).
 PYRO_HD f3 uniform_sphere(float u_phi, float u_cos) {
@@ -246,7 +267,7 @@ PYRO_HD f3 cone_proposal(f3 s, float cos_cone, float u_cos, float u_phi) {
   o.x = N.x * scal - r.x;
   o.y = N.y * scal - r.y;
   o.z = N.z * scal - r.z;
-  return normalise(o);
+  return renormalise(o);
 }
 
 // Metropolis test (Math.cu:968-986): local energies -h.s before and after,
@@ -271,7 +292,7 @@ PYRO_HD f3 overrelax(f3 h, f3 s) {
   v.x = t * h.x - s.x;
   v.y = t * h.y - s.y;
   v.z = t * h.z - s.z;
-  return normalise(v);
+  return renormalise(v);
 }
 
 // Rotation of s about the local field h by the angle 2 pi u (Rodrigues'
@@ -294,7 +315,7 @@ PYRO_HD f3 rotate_about(f3 h, f3 s, float u) {
   v.x = (s.x * ct + (n.y * s.z - n.z * s.y) * st) + n.x * k;
   v.y = (s.y * ct + (n.z * s.x - n.x * s.z) * st) + n.y * k;
   v.z = (s.z * ct + (n.x * s.y - n.y * s.x) * st) + n.z * k;
-  return normalise(v);
+  return renormalise(v);
 }
 
 // The 12 order-parameter components of Measurement (Pyrochlore.cu:580-624)

@@ -200,8 +200,13 @@ __global__ __launch_bounds__(256) void k_phase_cells(
 // series hook as k_phase_cells; a pass is k_or_pair<0> then k_or_pair<2>,
 // after which `oth` holds every colour and the buffers swap roles.
 // Algorithmic bytes: 12 N read + 6 N written for N/2 updates = 36 B/update.
+// Occupancy: the first pair fits 128 VGPRs (4 waves per SIMD) without spills;
+// the second, reading two buffers, would spill one register in the middle of
+// its load burst at that bound (a wait for every outstanding load), so it
+// runs at 3 waves per SIMD (measured: L=16 x 64 sweep 0.322 ms against 0.337
+// with both at 4 and 0.335 with both at 3).
 template <int CA, int LT, bool TRACK>
-__global__ __launch_bounds__(256) void k_or_pair(
+__global__ __launch_bounds__(256, TRACK ? 1 : (CA == 0 ? 4 : 3)) void k_or_pair(
     const float *__restrict__ cur, float *__restrict__ oth, int blocks_per_replica, float jeff,
     double *__restrict__ dE, int r0, const uint32_t *__restrict__ ctl, uint32_t sweep_delta,
     double *__restrict__ series, int R_total, const int *__restrict__ replica_at,

@@ -44,6 +44,7 @@ def lib():
         L.orc_sincospi.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.orc_cone_proposal.argtypes = [_f32p, C.c_float, C.c_float, C.c_float, _f32p]
         L.orc_overrelax.argtypes = [_f32p, _f32p, _f32p]
+        L.orc_renormalise.argtypes = [_f32p, _f32p]
         L.orc_metropolis_accept.argtypes = [_f32p, _f32p, _f32p, C.c_float, C.c_float]
         L.orc_metropolis_accept.restype = C.c_int
         L.orc_init_spins.argtypes = [_f32p, C.c_int, C.c_int, C.c_uint64]
@@ -125,6 +126,12 @@ def cone_proposal(s, cos_cone, u_cos, u_phi) -> np.ndarray:
 def overrelax(h, s) -> np.ndarray:
     out = np.zeros(3, np.float32)
     lib().orc_overrelax(np.asarray(h, np.float32), np.asarray(s, np.float32), out)
+    return out
+
+
+def renormalise(v) -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    lib().orc_renormalise(np.ascontiguousarray(v, np.float32), out)
     return out
 
 

@@ -152,11 +152,25 @@ void orc_sincospi(float a, float *s, float *c) {
   }
 }
 
+/* Renormalisation of a nearly unit vector (every caller's is a reflection or
+ * rotation of a unit spin, off by rounding): one Newton step for 1/sqrt(q)
+ * from 1, the factor 1 + c with c = (1 - q)/2 (exact), applied as v + v c
+ * (error 3/8 (q - 1)^2, below an FP32 ulp for |q - 1| < 1e-4; the length
+ * unbiased to rounding). The reference renormalises with the hardware RSQRT
+ * (Math.cu:238); this is a deterministic replacement. */
 static void renorm(float v[3]) {
-  float n = 1.0f / sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-  v[0] = v[0] * n;
-  v[1] = v[1] * n;
-  v[2] = v[2] * n;
+  float q = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  float c = 0.5f * (1.0f - q);
+  v[0] = v[0] + v[0] * c;
+  v[1] = v[1] + v[1] * c;
+  v[2] = v[2] + v[2] * c;
+}
+
+void orc_renormalise(const float v[3], float out[3]) {
+  out[0] = v[0];
+  out[1] = v[1];
+  out[2] = v[2];
+  renorm(out);
 }
 
 /* Cone proposal, Math.cu:244-281: uniform on the cap of half-angle acos(cos_cone)
@@ -197,7 +211,8 @@ int orc_metropolis_accept(const float h[3], const float s[3], const float sp[3],
 }
 
 /* Over-relaxation by pi, Math.cu:220-242: s' = 2 (s.h)/|h|^2 h - s, renormalised
- * (IEEE 1/sqrt instead of RSQRT). Unchanged when h = 0 (the reference divides by 0). */
+ * (one Newton step from 1 instead of RSQRT). Unchanged when h = 0 (the reference
+ * divides by 0). */
 void orc_overrelax(const float h[3], const float s[3], float out[3]) {
   float hh = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
   if (!(hh > 0.0f)) {

@@ -33,6 +33,7 @@ float orc_exp_neg(float x);
 void orc_sincospi(float a, float *s, float *c);
 void orc_cone_proposal(const float s[3], float cos_cone, float u_cos, float u_phi, float out[3]);
 void orc_overrelax(const float h[3], const float s[3], float out[3]);
+void orc_renormalise(const float v[3], float out[3]);
 int orc_metropolis_accept(const float h[3], const float s[3], const float sp[3], float beta,
                           float u);
 

@@ -132,3 +132,29 @@ def test_single_spin_langevin_law():
     bm = vals[: len(vals) // 100 * 100].reshape(100, -1).mean(axis=1)
     err = bm.std(ddof=1) / np.sqrt(len(bm))
     assert abs(vals.mean() - expect) < 4 * err + 1e-3, (vals.mean(), expect, err)
+
+
+def test_renormalise_near_unit_vectors():
+    """The near-unit renormalisation (one Newton step for 1/sqrt from 1) used
+    after every reflection and rotation: on vectors whose squared length is
+    within 1e-5 of 1 the result has unit length to 2.5e-7 and the direction
+    of the double-precision normalisation to 2e-7 (a few FP32 ulp), with no
+    systematic bias of the length (mean deviation below 2e-9), also for the
+    few-ulp deviations that rounding leaves."""
+    rng = np.random.default_rng(11)
+    v = rng.normal(size=(20000, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    v *= np.sqrt(1.0 + rng.uniform(-1e-5, 1e-5, size=(20000, 1)))
+    v = v.astype(np.float32)
+    out = np.array([oracle.renormalise(x) for x in v]).astype(np.float64)
+    n = np.linalg.norm(out, axis=1)
+    exact = v.astype(np.float64) / np.linalg.norm(v.astype(np.float64), axis=1, keepdims=True)
+    assert np.abs(n - 1).max() < 2.5e-7
+    assert np.abs(out - exact).max() < 2e-7
+    assert abs((n - 1).mean()) < 2e-9
+    # at the deviations rounding leaves (a few ulp) the length is unbiased too
+    w = rng.normal(size=(20000, 3))
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    w = (w * (1 + rng.uniform(-3e-7, 3e-7, size=(20000, 1)))).astype(np.float32)
+    m = np.linalg.norm(np.array([oracle.renormalise(x) for x in w]).astype(np.float64), axis=1)
+    assert abs((m - 1).mean()) < 1e-8 and np.abs(m - 1).max() < 1.5e-7
