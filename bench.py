@@ -272,6 +272,36 @@ def plain_tau_leg(g, args):
     return learned.pool([learned.timed_leg(g, args.series_sweeps, warmup=20) for _ in range(2)])
 
 
+def phase_only(args):
+    """The L=16 x 64 ensemble after the cone adaptation and --warmup sweeps;
+    for the learned phase also the network trained as in learned_leg; then
+    --phase-launches launches of one phase kernel over all 64 replicas, each
+    timed by start/stop events in its own dispatch (pyro_time_phase)."""
+    from cudacode_amd import learned, pyro
+    g = pyro.Pyrochlore(L, pyro.geometric_ladder(T_MIN, T_MAX, R), seed=SEED, n_metro=N_METRO,
+                        n_or=N_OR, pt_every=PT_EVERY, cos_cone=0.5)
+    g.CalculateSolidAngle(target=0.5, rounds=10, sweeps_per_round=10)
+    g.Mc(args.warmup)
+    kind = {"learned": 2, "or_pair": 3, "metropolis": 0}[args.phase_only]
+    if kind == 2:
+        learned.train(g, steps=args.train_steps, init_seed=SEED)
+    avg_ms, span_ms = g.time_phase(kind, args.phase_launches)
+    g.close()
+    n_total = 16 * L ** 3 * R
+    out = {"phase": args.phase_only, "launches": args.phase_launches, "avg_launch_ms": avg_ms,
+           "span_ms": span_ms}
+    if kind == 2:
+        flop = n_total // 4 * 2 * FLOP_PER_FORWARD
+        out.update({"flop_per_launch": flop,
+                    "achieved_tflops": flop / (avg_ms * 1e-3) / 1e12,
+                    "frac_bf16_peak": flop / (avg_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS})
+    else:
+        b = (BYTES_PER_UPDATE_FUSED * n_total // 2) if kind == 3 else BYTES_PER_UPDATE * n_total // 4
+        out.update({"bytes_per_launch": b, "achieved_gbs": b / (avg_ms * 1e-3) / 1e9,
+                    "frac_hbm_peak": b / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+    return out
+
+
 def line(metric_value, steps, warmup, dt, workload, extra):
     out = {
         "metric": "spin updates/s (plain sweep); tau_int-corrected independent samples/s",
@@ -442,12 +472,19 @@ def main():
                     help="plain sweeps before the tau_int legs (independent of --warmup)")
     ap.add_argument("--series-sweeps", type=int, default=3000,
                     help="timed sweeps per tau_int leg (independent of --steps)")
+    ap.add_argument("--phase-only", choices=["learned", "or_pair", "metropolis"],
+                    help="only time N launches of one colour-phase kernel on the L=16 x 64 "
+                         "workload, eagerly (no captured sweeps): the command the kernel-trace "
+                         "summaries of these kernels are made with")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the replicas of "
               "this workload are all resident on one MI355X.", file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
+    if args.phase_only:
+        print(json.dumps(phase_only(args)), flush=True)
+        return
     if args.workload != "c2":
         fn = {"c1": workload_c1, "c3": workload_c3, "c5": workload_c5}[args.workload]
         print(json.dumps(fn(args)), flush=True)

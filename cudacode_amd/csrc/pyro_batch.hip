@@ -279,26 +279,36 @@ __global__ __launch_bounds__(256) void k_batch_init(float *__restrict__ spins,
 
 __device__ __forceinline__ int wrap_b(int v, int L) { return v < 0 ? v + L : (v >= L ? v - L : v); }
 
-// One block per replica: FP64 energy (each bond from both ends, halved) and
-// the 12 sublattice sums, in a fixed summation order. grid = R, block = 256.
-__global__ __launch_bounds__(256) void k_batch_observables(const float *__restrict__ spins,
-                                                           const long long *__restrict__ off,
-                                                           const int *__restrict__ Ls, float jeff,
-                                                           double *__restrict__ obs) {
-  const int r = blockIdx.x;
+// Exchange and measurement observables of a batch, first stage: the FP64
+// energy partial (each bond from both ends) and the 12 sublattice sums of one
+// 256-cell tile, one block per tile of the tile list, written to the tile's
+// slot in its replica's run of partials, part[(part_off[r] + t) * kObs + q].
+// A block per tile spreads every replica over many CUs (one block per
+// replica took 1.26 ms per call for 256 mixed replicas of L up to 24).
+// grid = the padded tile list, block = 256.
+__global__ __launch_bounds__(256) void k_batch_obs_tiles(const float *__restrict__ spins,
+                                                         const long long *__restrict__ off,
+                                                         const int *__restrict__ Ls,
+                                                         const int *__restrict__ tiles,
+                                                         const int *__restrict__ part_off,
+                                                         double *__restrict__ part) {
+  const int r = tiles[2 * blockIdx.x];
+  if (r < 0) return;  // padding tile: the whole block leaves together
+  const int t = tiles[2 * blockIdx.x + 1];
   const int L = Ls[r], LL = L * L, nc = L * L * L;
+  const int cell = t * 256 + threadIdx.x;
   const float *Sx = spins + off[r];
   const float *Sy = Sx + 16 * nc;
   const float *Sz = Sy + 16 * nc;
   double v[kObs];
 #pragma unroll
   for (int q = 0; q < kObs; ++q) v[q] = 0.0;
-  for (int cell = threadIdx.x; cell < nc; cell += 256) {
+  if (cell < nc) {
     const int z = cell / LL, rem = cell - z * LL, y = rem / L, x = rem - y * L;
-    for (int t = 0; t < 4; ++t) {
+    for (int tt = 0; tt < 4; ++tt) {
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        const int site = 4 * t + b;
+        const int site = 4 * tt + b;
         double gx = 0.0, gy = 0.0, gz = 0.0;
 #pragma unroll
         for (int n = 0; n < 6; ++n) {
@@ -318,9 +328,33 @@ __global__ __launch_bounds__(256) void k_batch_observables(const float *__restri
       }
     }
   }
+  __shared__ double s_v[4][kObs];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < kObs; ++q) {
+    const double w = wave_sum_d(v[q]);
+    if (lane == 0) s_v[wv][q] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < kObs) {
+    const int q = threadIdx.x;
+    part[((size_t)part_off[r] + t) * kObs + q] = (s_v[0][q] + s_v[1][q]) + (s_v[2][q] + s_v[3][q]);
+  }
+}
+
+// Second stage: per replica, the tiles' partials in tile order (fixed, so the
+// result does not depend on which tile finished first); the energy scaled by
+// jeff / 2. grid = R, block = 256.
+__global__ __launch_bounds__(256) void k_batch_obs_final(const double *__restrict__ part,
+                                                         const int *__restrict__ part_off,
+                                                         float jeff, double *__restrict__ obs) {
+  const int r = blockIdx.x;
+  const int k0 = part_off[r], n = part_off[r + 1] - k0;
   __shared__ double s_v[256];
   for (int q = 0; q < kObs; ++q) {
-    s_v[threadIdx.x] = v[q];
+    double t = 0.0;
+    for (int k = threadIdx.x; k < n; k += 256) t += part[((size_t)k0 + k) * kObs + q];
+    s_v[threadIdx.x] = t;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
       if ((int)threadIdx.x < w) s_v[threadIdx.x] += s_v[threadIdx.x + w];
@@ -336,7 +370,8 @@ static inline uint32_t bk0(const pyro_ctx *c) { return (uint32_t)c->seed; }
 static inline uint32_t bk1(const pyro_ctx *c) { return (uint32_t)(c->seed >> 32); }
 
 void batch_free(pyro_ctx *ctx) {
-  void *ptrs[] = {ctx->d_L, ctx->d_off, ctx->d_order, ctx->d_bin_start, ctx->d_tiles};
+  void *ptrs[] = {ctx->d_L,     ctx->d_off,      ctx->d_order,   ctx->d_bin_start,
+                  ctx->d_tiles, ctx->d_part_off, ctx->d_tile_part};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &row : ctx->d_tiles_part)
@@ -348,7 +383,8 @@ void batch_free(pyro_ctx *ctx) {
     for (int &n : row) n = 0;
   ctx->d_L = nullptr;
   ctx->d_off = nullptr;
-  ctx->d_order = ctx->d_bin_start = ctx->d_tiles = nullptr;
+  ctx->d_order = ctx->d_bin_start = ctx->d_tiles = ctx->d_part_off = nullptr;
+  ctx->d_tile_part = nullptr;
 }
 
 // Tiles of replicas [r_begin, r_end) in replica order, dealt to the 8 XCDs in
@@ -398,6 +434,10 @@ int batch_setup(pyro_ctx *ctx, int n_bins) {
     start.push_back((int)flat.size());
   }
   const std::vector<int> tiles = tile_list(ctx, 0, R);
+  // Each replica's run of tile partials for the observables.
+  std::vector<int> part_off(R + 1, 0);
+  for (int r = 0; r < R; ++r)
+    part_off[r + 1] = part_off[r] + (ctx->h_L[r] * ctx->h_L[r] * ctx->h_L[r] + 255) / 256;
   // parts[n][p]: group p of a split into n groups (n = 2 .. kMaxParts, n <= R).
   std::vector<int> parts[kMaxParts + 1][kMaxParts];
   for (int n = 2; n <= kMaxParts && n <= R; ++n)
@@ -414,6 +454,9 @@ int batch_setup(pyro_ctx *ctx, int n_bins) {
   PYRO_HIP(hipMemcpy(ctx->d_order, flat.data(), sizeof(int) * R, hipMemcpyHostToDevice));
   PYRO_HIP(hipMemcpy(ctx->d_bin_start, start.data(), sizeof(int) * (n_bins + 1), hipMemcpyHostToDevice));
   PYRO_HIP(hipMemcpy(ctx->d_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice));
+  PYRO_HIP(hipMalloc((void **)&ctx->d_part_off, sizeof(int) * (R + 1)));
+  PYRO_HIP(hipMemcpy(ctx->d_part_off, part_off.data(), sizeof(int) * (R + 1), hipMemcpyHostToDevice));
+  PYRO_HIP(hipMalloc((void **)&ctx->d_tile_part, sizeof(double) * kObs * (size_t)part_off[R]));
   for (int n = 2; n <= kMaxParts; ++n)
     for (int p = 0; p < n; ++p) {
       const std::vector<int> &t = parts[n][p];
@@ -437,10 +480,25 @@ int launch_batch_init_spins(pyro_ctx *ctx) {
 }
 
 int launch_batch_observables(pyro_ctx *ctx) {
-  k_batch_observables<<<ctx->R, 256, 0, ctx->stream>>>(ctx->d_spins, ctx->d_off, ctx->d_L, ctx->jeff,
-                                                       ctx->d_obs);
+  k_batch_obs_tiles<<<ctx->n_tiles, 256, 0, ctx->stream>>>(ctx->d_spins, ctx->d_off, ctx->d_L,
+                                                           ctx->d_tiles, ctx->d_part_off,
+                                                           ctx->d_tile_part);
+  k_batch_obs_final<<<ctx->R, 256, 0, ctx->stream>>>(ctx->d_tile_part, ctx->d_part_off, ctx->jeff,
+                                                     ctx->d_obs);
   PYRO_HIP(hipGetLastError());
   return 0;
+}
+
+// Whether the autotuner tries the persistent execution: only when every
+// replica is staged in LDS (L <= kLdsMaxL). A block owns whole replicas, so a
+// larger replica serialises on one CU: 256 mixed replicas of L in
+// {8, 12, 16, 24} took 5.9-12 ms per sweep persistent against 1.58 ms as
+// tile-list phases, while one L=4 replica took 21.6 us persistent against
+// 70 us as per-phase kernels.
+bool persistent_candidate(const pyro_ctx *ctx) {
+  for (int L : ctx->h_L)
+    if (L > kLdsMaxL) return false;
+  return true;
 }
 
 // Largest replica size of the batch that the persistent kernel stages in LDS

@@ -878,9 +878,11 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
         if (n <= ctx->n_streams) cands.push_back({0, 0, v, n});
   }
   if (batch_size_supported(ctx->hetero ? 4 : ctx->L)) {
-    cands.push_back({1, 256, 1, 1});
-    cands.push_back({1, 512, 1, 1});
-    cands.push_back({1, 1024, 1, 1});
+    if (persistent_candidate(ctx)) {
+      cands.push_back({1, 256, 1, 1});
+      cands.push_back({1, 512, 1, 1});
+      cands.push_back({1, 1024, 1, 1});
+    }
     for (int n : {1, 2, 3})
       if (n <= ctx->n_streams) cands.push_back({2, 256, 1, n});
   }

@@ -16,18 +16,6 @@ namespace pyro {
 
 constexpr Link kLinks[16][6] = PYRO_LINKS16_INIT;
 
-// Access at a 32-bit byte offset from a base that is uniform over the wave
-// (a replica's component plane): the base stays in scalar registers and the
-// global load or store takes the offset as one 32-bit vector register, with
-// no 64-bit address arithmetic per access. Offsets stay below 2^31: a
-// replica of L <= 64 spans 48 L^3 floats = 50 MB.
-__device__ __forceinline__ float ld_at(const float *__restrict__ base, uint32_t off) {
-  return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + off);
-}
-__device__ __forceinline__ void st_at(float *__restrict__ base, uint32_t off, float v) {
-  *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + off) = v;
-}
-
 // Local-field models. Scalar (the Heisenberg J of pyro_create): a site's
 // field is -jeff times the plain FP32 neighbour sum f. Matrix (anisotropic
 // exchange and Zeeman term, pyro_set_couplings; the reference's

@@ -60,6 +60,19 @@ constexpr int kPackSize = kPackP3 + 32 * 64;
 constexpr float kLogKappaMin = -4.60517019f;  // ln 0.01
 constexpr float kLogKappaMax = 8.31776617f;   // ln 4096
 
+
+// Access at a 32-bit byte offset from a base that is uniform over the wave
+// (a replica's component plane): the base stays in scalar registers and the
+// global load or store takes the offset as one 32-bit vector register, with
+// no 64-bit address arithmetic per access. Offsets stay below 2^31: a
+// replica of L <= 64 spans 48 L^3 floats = 50 MB.
+__device__ __forceinline__ float ld_at(const float *__restrict__ base, uint32_t off) {
+  return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + off);
+}
+__device__ __forceinline__ void st_at(float *__restrict__ base, uint32_t off, float v) {
+  *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
 }  // namespace pyro
 
 struct pyro_ctx {
@@ -113,6 +126,8 @@ struct pyro_ctx {
   int n_bins = 0;
   int *d_tiles = nullptr;           // 2 ints per tile: replica, tile index (-1: padding)
   int n_tiles = 0;
+  int *d_part_off = nullptr;        // R + 1: first observables partial of each replica
+  double *d_tile_part = nullptr;    // kObs doubles per tile: observables partials
   // The same lists for each replica group of a split sweep:
   // d_tiles_part[n][p] covers group p of n (replicas [p R / n, (p + 1) R / n)).
   int *d_tiles_part[pyro::kMaxParts + 1][pyro::kMaxParts] = {};
@@ -233,6 +248,7 @@ int noalloc_scratch(int device, double **ptr);
 
 // Batches and alternative sweep executions (pyro_batch.hip).
 bool batch_size_supported(int L);
+bool persistent_candidate(const pyro_ctx *ctx);  // autotuner tries the persistent execution
 int batch_setup(pyro_ctx *ctx, int n_bins);       // bins, tiles, device geometry arrays
 int launch_batch_init_spins(pyro_ctx *ctx);
 int launch_batch_observables(pyro_ctx *ctx);      // fills d_obs

@@ -47,18 +47,45 @@ __device__ __forceinline__ uint32_t cvt2(float lo, float hi) {
 
 __device__ __forceinline__ float relu(float x) { return fmaxf(x, 0.0f); }
 
-// ReLU of 8 consecutive accumulator registers, converted to the 8 bf16 of
-// one B operand of a 32x32x16 MFMA. Rounding to bf16 keeps the sign, so the
-// ReLU is applied after it, as a signed 16-bit max with 0 on the bit patterns
-// (negative values and -0 have the sign bit set): two values per packed
-// integer instruction, and no NaN canonicalisation of the accumulators.
+// Exact power-of-two headroom of the hidden pre-activations: the packed
+// layer-1 operand (weights and bias column) and the layer-2 bias are scaled by
+// 2^-kHeadroomLog2 and the packed layer-3 weights by 2^kHeadroomLog2 (layer
+// 2's weights are unscaled: its input and output carry the same factor).
+// Scaling by a power of two is exact in FP32 and BF16, so every activation is
+// the unscaled one times 2^-16, and the network's output is unchanged, as
+// long as the hidden pre-activations stay below 2^16 (the trained network's
+// are O(10)); beyond that they saturate, which changes the proposal but not
+// the exactness of the Metropolis-Hastings test (both directions evaluate the
+// same function).
+constexpr int kHeadroomLog2 = 16;
+constexpr float kHeadroomDown = 1.0f / (float)(1 << kHeadroomLog2);
+constexpr float kHeadroomUp = (float)(1 << kHeadroomLog2);
+
+// ReLU and BF16 rounding of two scaled pre-activations in one instruction:
+// v_cvt_pk_bf16_f32 with the clamp modifier returns min(max(bf16(x), 0), 1)
+// (checked on gfx950 over negative, zero, denormal and large inputs), which
+// is ReLU for the scaled values below 1.
+__device__ __forceinline__ uint32_t relu_cvt2(float lo, float hi) {
+  uint32_t r;
+  asm("v_cvt_pk_bf16_f32 %0, %1, %2 clamp" : "=v"(r) : "v"(lo), "v"(hi));
+  return r;
+}
+
+// The compiler does not see the reads of an accumulator by the conversions
+// above (inline asm), so it inserts no wait states between an MFMA and them:
+// this guard, placed after the last MFMA of an accumulator chain, "updates"
+// the accumulator (ordering every later read after it) and waits the 13
+// states that a VALU read of a 32x32 MFMA result needs on gfx950 (12, as the
+// compiler itself inserts for its own reads).
+__device__ __forceinline__ void mfma_read_guard(f32x16 &d) { asm volatile("s_nop 12" : "+v"(d)); }
+
+// ReLU of 8 consecutive (guarded) accumulator registers, converted to the 8
+// bf16 of one B operand of a 32x32x16 MFMA: four instructions.
 __device__ __forceinline__ bf16x8 act8(const f32x16 &d, int base) {
   u32x4v p;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) p[j] = cvt2(d[base + 2 * j], d[base + 2 * j + 1]);
-  const s16x8 b = __builtin_bit_cast(s16x8, p);
-  const s16x8 z = {};
-  return __builtin_bit_cast(bf16x8, __builtin_elementwise_max(b, z));
+  for (int j = 0; j < 4; ++j) p[j] = relu_cvt2(d[base + 2 * j], d[base + 2 * j + 1]);
+  return __builtin_bit_cast(bf16x8, p);
 }
 
 // Hidden unit that 32x32x16 k-step t, lane half h, element j stands for. The
@@ -102,7 +129,8 @@ __device__ __forceinline__ void mlp_load(const uint16_t *__restrict__ pack,
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-    for (int v = 0; v < 16; ++v) W.b2[ti][v] = w[kOffB2 + 32 * ti + 8 * (v >> 2) + 4 * h + (v & 3)];
+    for (int v = 0; v < 16; ++v)
+      W.b2[ti][v] = w[kOffB2 + 32 * ti + 8 * (v >> 2) + 4 * h + (v & 3)] * kHeadroomDown;
 #pragma unroll
   for (int q = 0; q < 4; ++q) W.b3[q] = w[kOffB3 + q];
 }
@@ -129,7 +157,8 @@ __device__ __forceinline__ void mlp_forward(const MlpW &W, const float x[6], flo
   for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
     for (int tn = 0; tn < 2; ++tn) {
-      const f32x16 d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W.a1[tj], b1[tn], zero, 0, 0, 0);
+      f32x16 d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W.a1[tj], b1[tn], zero, 0, 0, 0);
+      mfma_read_guard(d);
       b2[tn][2 * tj] = act8(d, 0);
       b2[tn][2 * tj + 1] = act8(d, 8);
     }
@@ -142,6 +171,7 @@ __device__ __forceinline__ void mlp_forward(const MlpW &W, const float x[6], flo
 #pragma unroll
       for (int t = 0; t < 4; ++t)
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.a2[ti][t], b2[tn][t], acc, 0, 0, 0);
+      mfma_read_guard(acc);
       b3[tn][2 * ti] = act8(acc, 0);
       b3[tn][2 * ti + 1] = act8(acc, 8);
     }
@@ -299,15 +329,17 @@ __global__ __launch_bounds__(kPhaseBlock, 3) void k_learned_phase(
   MlpW W;
   mlp_load(pack, w, W);
 
-  double de = 0.0;
+  // Energy change of this lane's accepted proposals in FP32 (from the FP32
+  // neighbour sum the move itself uses: about 1e-7 of |s' - s| |h| per move,
+  // unbiased); summed over the block in FP64.
+  float de_f = 0.0f;
   unsigned nacc = 0;
   for (int g = 0; g < G; ++g) {
     const int cell = (blockIdx.x * G + g) * kPhaseCells + lane;
     const bool valid = cell < ncell;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    double gx = 0.0, gy = 0.0, gz = 0.0;
     f3 s = {0.0f, 0.0f, 1.0f};
-    int i = 0;
+    uint32_t i = 0;
     if (valid) {
       const int LL = L * L;
       const int z = cell / LL;
@@ -317,18 +349,15 @@ __global__ __launch_bounds__(kPhaseBlock, 3) void k_learned_phase(
 #pragma unroll
       for (int nb = 0; nb < 6; ++nb) {
         const Link lk = c_links_learned[site][nb];
-        const int j = lk.site * ncell + (wrap_t<LT>(z + lk.dz, L) * L + wrap_t<LT>(y + lk.dy, L)) * L +
-                      wrap_t<LT>(x + lk.dx, L);
-        const float a = Sx[j], b = Sy[j], c = Sz[j];
-        fx = fx + a;
-        fy = fy + b;
-        fz = fz + c;
-        gx += (double)a;
-        gy += (double)b;
-        gz += (double)c;
+        const uint32_t j =
+            4u * (uint32_t)(lk.site * ncell + (wrap_t<LT>(z + lk.dz, L) * L + wrap_t<LT>(y + lk.dy, L)) * L +
+                            wrap_t<LT>(x + lk.dx, L));
+        fx = fx + ld_at(Sx, j);
+        fy = fy + ld_at(Sy, j);
+        fz = fz + ld_at(Sz, j);
       }
-      i = site * ncell + cell;
-      s = {Sx[i], Sy[i], Sz[i]};
+      i = 4u * (uint32_t)(site * ncell + cell);
+      s = {ld_at(Sx, i), ld_at(Sy, i), ld_at(Sz, i)};
     }
     const f3 hf = {-jeff * fx, -jeff * fy, -jeff * fz};
     const f3 u = {bt * hf.x, bt * hf.y, bt * hf.z};
@@ -346,14 +375,14 @@ __global__ __launch_bounds__(kPhaseBlock, 3) void k_learned_phase(
     const float la = mh_log_ratio(hf, bt, s, sp, fwd, rev);
     const bool accept = valid && (u01(rnd.z) < exp_f(fminf(la, 0.0f)));
     if (accept) {
-      de += (double)jeff * (((double)sp.x - (double)s.x) * gx + ((double)sp.y - (double)s.y) * gy +
-                            ((double)sp.z - (double)s.z) * gz);
-      Sx[i] = sp.x;
-      Sy[i] = sp.y;
-      Sz[i] = sp.z;
+      de_f = de_f + jeff * (((sp.x - s.x) * fx + (sp.y - s.y) * fy) + (sp.z - s.z) * fz);
+      st_at(Sx, i, sp.x);
+      st_at(Sy, i, sp.y);
+      st_at(Sz, i, sp.z);
     }
     nacc += (unsigned)__popcll(__ballot(accept));
   }
+  double de = (double)de_f;
   __shared__ double s_de[4];
   __shared__ unsigned s_acc[4];
 #pragma unroll
@@ -659,13 +688,14 @@ __global__ void k_mlp_pack(const float *__restrict__ w, uint16_t *__restrict__ p
   if (p < kPackP2) {  // [64][8]: W1 columns, bias, 0
     const int row = p / 8, col = p % 8;
     val = col < 6 ? w[kOffW1 + row * 6 + col] : (col == 6 ? w[kOffB1 + row] : 0.0f);
+    val *= kHeadroomDown;
   } else if (p < kPackP3) {  // [ti][t][lane][j]
     const int q = p - kPackP2, j = q & 7, lane = (q >> 3) & 63, t = (q >> 9) & 3, ti = q >> 11;
     val = w[kOffW2 + (32 * ti + (lane & 31)) * 64 + kstep_unit(t, lane >> 5, j)];
   } else {  // [t][lane][j], rows 4..31 zero
     const int q = p - kPackP3, j = q & 7, lane = (q >> 3) & 63, t = q >> 9;
     const int row = lane & 31;
-    val = row < 4 ? w[kOffW3 + row * 64 + kstep_unit(t, lane >> 5, j)] : 0.0f;
+    val = row < 4 ? w[kOffW3 + row * 64 + kstep_unit(t, lane >> 5, j)] * kHeadroomUp : 0.0f;
   }
   pack[p] = (uint16_t)bf16_bits(val);
 }

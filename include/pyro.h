@@ -99,8 +99,10 @@ int pyro_get_exec_mode(pyro_ctx *ctx, int *mode, int *block, int *n_bins);
 
 /* Autotuner: time `sweeps` sweeps of each candidate execution (per-phase
  * cell- and site-per-thread kernels on 1, 2 or 3 streams, persistent kernel
- * at 256/512/1024 threads, tile list on 1, 2 or 3 streams), captured in one
- * graph as pyro_sweep
+ * at 256/512/1024 threads when every replica has L <= 8 and is staged in LDS
+ * -- a block owns whole replicas, so larger ones serialise: 5.9-12 ms per
+ * sweep for 256 mixed replicas of L up to 24 against 1.58 ms as tile lists
+ * -- and tile list on 1, 2 or 3 streams), captured in one graph as pyro_sweep
  * replays them, restoring the state after each trial, and keep the fastest.
  * results (optional, 5 doubles per candidate, at most 16 candidates): mode,
  * block, phase variant, streams, ms per sweep. */

@@ -136,10 +136,10 @@ def test_autotune_keeps_the_state():
         before = g.get_spins()
         inc_before = g.energy_incremental()
         res = g.autotune(3)
-        # persistent at 256/512/1024 threads on one stream; tile lists on
-        # 1, 2 and 3 streams
-        assert len(res) == 6 and np.all(res[:, 4] > 0)
-        assert res[:, 3].tolist() == [1, 1, 1, 1, 2, 3] and res[3:, 0].tolist() == [2, 2, 2]
+        # tile lists on 1, 2 and 3 streams; no persistent trials, replicas of
+        # L > 8 being too large for one block
+        assert len(res) == 3 and np.all(res[:, 4] > 0)
+        assert res[:, 3].tolist() == [1, 2, 3] and res[:, 0].tolist() == [2, 2, 2]
         say(f"autotune results (mode, block, variant, streams, ms/sweep): {res.tolist()}")
         assert np.array_equal(g.get_spins(), before)
         assert g.sweep_counters() == h.sweep_counters()
@@ -301,3 +301,22 @@ def test_full_size_mixed_batch_replicas_match_oracle():
         say(f"oracle replica {r} (L={Ls[r]}, T={T[r]:.4f})")
         o = oracle_replica(int(Ls[r]), float(T[r]), r, 4, **kw)
         assert np.array_equal(got[r], o.spins), r
+
+
+def test_autotune_tries_persistent_when_replicas_fit_lds():
+    """A batch whose replicas all stage in LDS (L <= 8) gets the persistent
+    execution among the autotuner's candidates (256/512/1024 threads) beside
+    the tile lists, and keeps its state."""
+    Ls = [4, 8, 4, 8]
+    temps = [0.2, 0.4, 0.7, 1.1]
+    with batch(Ls, temps, n_metro=1, n_or=2, pt_every=2, cos_cone=0.4) as g, \
+            batch(Ls, temps, n_metro=1, n_or=2, pt_every=2, cos_cone=0.4) as h:
+        g.Mc(3)
+        h.Mc(3)
+        res = g.autotune(2)
+        say(f"autotune results (mode, block, variant, streams, ms/sweep): {res.tolist()}")
+        assert res[res[:, 0] == 1, 1].tolist() == [256, 512, 1024]
+        assert res[res[:, 0] == 2, 3].tolist() == [1, 2, 3]
+        g.Mc(5)
+        h.Mc(5)
+        assert np.array_equal(g.get_spins(), h.get_spins())
