@@ -65,19 +65,35 @@ constexpr float kHeadroomUp = (float)(1 << kHeadroomLog2);
 // v_cvt_pk_bf16_f32 with the clamp modifier returns min(max(bf16(x), 0), 1)
 // (checked on gfx950 over negative, zero, denormal and large inputs), which
 // is ReLU for the scaled values below 1.
+//
+// The compiler's hazard recognizer does not look inside inline asm, so it
+// inserts no wait states between this instruction and the MFMAs around it.
+// Three guards supply them (gfx950's rules, as the compiler applies them to
+// its own instructions):
+//  - a read of an MFMA result needs 12 states: the accumulator is read only
+//    after mfma_read_guard;
+//  - a write into the destination of an MFMA in flight needs 12 states: the
+//    result register is one the compiler sees as free, which excludes every
+//    live accumulator register but not the unread rows of an accumulator
+//    (layer 3 reads 4 of its 32), so those are kept live by the same guard;
+//  - an MFMA reading the result needs 2 states: mfma_operand_guard.
+// tests/test_learned_isa.py checks these distances in the built assembly.
 __device__ __forceinline__ uint32_t relu_cvt2(float lo, float hi) {
   uint32_t r;
   asm("v_cvt_pk_bf16_f32 %0, %1, %2 clamp" : "=v"(r) : "v"(lo), "v"(hi));
   return r;
 }
 
-// The compiler does not see the reads of an accumulator by the conversions
-// above (inline asm), so it inserts no wait states between an MFMA and them:
-// this guard, placed after the last MFMA of an accumulator chain, "updates"
-// the accumulator (ordering every later read after it) and waits the 13
-// states that a VALU read of a 32x32 MFMA result needs on gfx950 (12, as the
-// compiler itself inserts for its own reads).
+// Placed after the last MFMA of an accumulator chain: "updates" the whole
+// accumulator (ordering every later read after it and keeping all of its
+// registers allocated) and waits the 13 states that a VALU access to a 32x32
+// MFMA result needs on gfx950 (12, as the compiler itself inserts).
 __device__ __forceinline__ void mfma_read_guard(f32x16 &d) { asm volatile("s_nop 12" : "+v"(d)); }
+
+// Placed after the conversions that build an MFMA operand: a VALU write of a
+// VGPR that an MFMA reads as SrcA/B/C needs 2 wait states on gfx950 (the
+// compiler's own s_nop 1 for its conversions).
+__device__ __forceinline__ void mfma_operand_guard(u32x4v &p) { asm volatile("s_nop 1" : "+v"(p)); }
 
 // ReLU of 8 consecutive (guarded) accumulator registers, converted to the 8
 // bf16 of one B operand of a 32x32x16 MFMA: four instructions.
@@ -85,6 +101,7 @@ __device__ __forceinline__ bf16x8 act8(const f32x16 &d, int base) {
   u32x4v p;
 #pragma unroll
   for (int j = 0; j < 4; ++j) p[j] = relu_cvt2(d[base + 2 * j], d[base + 2 * j + 1]);
+  mfma_operand_guard(p);
   return __builtin_bit_cast(bf16x8, p);
 }
 
@@ -182,6 +199,7 @@ __device__ __forceinline__ void mlp_forward(const MlpW &W, const float x[6], flo
 #pragma unroll
     for (int t = 0; t < 4; ++t)
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.a3[t], b3[tn][t], acc, 0, 0, 0);
+    mfma_read_guard(acc);
     d3[tn] = acc;
   }
   // Rows 0..3 of column n sit in lane n (upper half zero-padded rows elsewhere).
@@ -741,9 +759,15 @@ int mlp_pack(pyro_ctx *ctx) {
   return 0;
 }
 
-// 64-cell groups per block. One group and at most 168 VGPRs (3 waves per
-// SIMD, a few spilled values) measured fastest: 48 us per L=16 x 64 launch,
-// against 55 us for two groups at 226 VGPRs (2 waves per SIMD).
+// 64-cell groups per block. Several groups per wave share the weight
+// fragments a wave loads (22.5 KB) and, at 216 VGPRs (2 waves per SIMD), make
+// a launch that runs alone faster: 47.0 / 43.0 / 40.6 us per L=16 x 64 launch
+// for 1 / 4 / 8 groups, 1609 / 1249 / 1215 us at L=64 x 32, bit-identical
+// spins. Inside a sweep, where one replica group's learned launches overlap
+// the other groups' over-relaxation launches, they are not: 0.449 / 0.450 /
+// 0.458 ms per sweep at L=16 x 64, 15.66 / 15.95 / 15.98 ms at L=64 x 32 (two
+// 216-VGPR waves leave no room for an over-relaxation wave on their SIMD;
+// one group at 166 VGPRs does). One group.
 constexpr int kLearnedGroups = 1;
 
 template <int LT>
