@@ -218,53 +218,30 @@ struct PairNeeds {
   }
 };
 
-// Two over-relaxation colour phases CA, CA + 1 of one cell in one pass over
-// the data, scalar model (cur/oth: this replica's spins in the two buffers).
-// Colours below CA have already been updated this pass and are read from
-// oth; the others are read from cur; the two new colours are written to oth,
-// so no thread reads what another thread of the launch writes. Phase CA + 1
-// needs the new colour-CA values of its neighbours: those in this cell come
-// from the first phase, those in the neighbouring cells are recomputed here
-// with the owner's float expressions in the owner's order (over-relaxation is
-// deterministic), so every result equals two separate phase launches bit for
-// bit. All 41 spins the cell needs are loaded in one burst before any
-// arithmetic (one memory round trip, and the halo lines are used while they
-// are still in L2). TRACK: FP64 energy changes of this cell's updates added
-// to de.
-template <int CA, int LT, bool TRACK>
-__device__ __forceinline__ void or_pair_cell(const float *__restrict__ cur,
-                                             float *__restrict__ oth, int cell, float jeff,
-                                             double &de) {
-  static_assert(CA == 0 || CA == 2, "colour pairs (0, 1) and (2, 3)");
-  constexpr int CB = CA + 1, L = LT, LL = LT * LT, NC = LT * LT * LT, P = 16 * NC;
+// The arithmetic of two over-relaxation colour phases CA, CA + 1 of one cell,
+// scalar model, on the cell's PairNeeds<CA> values vx/vy/vz (old spins; the
+// colours below CA already updated this pass). Phase CA + 1 needs the new
+// colour-CA values of its neighbours: those in this cell come from the first
+// phase, those in the neighbouring cells are recomputed here with the owner's
+// float expressions in the owner's order (over-relaxation is deterministic),
+// so every result equals two separate phase launches bit for bit. na, nb: the
+// new colour-CA and colour-CA+1 spins of sites CA + 4k, CA + 1 + 4k. TRACK:
+// FP64 energy changes of this cell's updates added to de.
+template <int CA, bool TRACK, int NN>
+__device__ __forceinline__ void or_pair_math(const float (&vx)[NN], const float (&vy)[NN],
+                                             const float (&vz)[NN], float jeff, double &de,
+                                             f3 (&na)[4], f3 (&nb)[4]) {
+  constexpr int CB = CA + 1;
   constexpr PairNeeds<CA> nd{};
-  constexpr int NN = nd.n;
-  const int z = cell / LL;
-  const int y = (cell / L) % L;
-  const int x = cell % L;
-  const int X[3] = {x == 0 ? L - 1 : x - 1, x, x == L - 1 ? 0 : x + 1};
-  const int Y[3] = {(y == 0 ? L - 1 : y - 1) * L, y * L, (y == L - 1 ? 0 : y + 1) * L};
-  const int Z[3] = {(z == 0 ? L - 1 : z - 1) * LL, z * LL, (z == L - 1 ? 0 : z + 1) * LL};
-  float vx[NN], vy[NN], vz[NN];
-  const float *cy = cur + P, *cz = cur + 2 * P;
-  float *oy = oth + P, *oz = oth + 2 * P;
-#pragma unroll
-  for (int i = 0; i < NN; ++i) {
-    const PairNeed q = nd.v[i];
-    const uint32_t j = 4u * (uint32_t)(q.site * NC + Z[q.dz + 1] + Y[q.dy + 1] + X[q.dx + 1]);
-    const bool o = (q.site & 3) < CA;
-    vx[i] = ld_at(o ? oth : cur, j);
-    vy[i] = ld_at(o ? oy : cy, j);
-    vz[i] = ld_at(o ? oz : cz, j);
-  }
+  static_assert(NN == nd.n, "value arrays hold the pair's needs");
   // Phase-CA update of a colour-CA site from its own value (index self) and
   // its 6 links (indices nb), with its FP64 neighbour sum.
-  auto upd_a = [&](int self, const int *nb, double &gx, double &gy, double &gz) -> f3 {
+  auto upd_a = [&](int self, const int *nbr, double &gx, double &gy, double &gz) -> f3 {
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     gx = gy = gz = 0.0;
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
-      const int i = nb[m];
+      const int i = nbr[m];
       fx = fx + vx[i];
       fy = fy + vy[i];
       fz = fz + vz[i];
@@ -276,7 +253,6 @@ __device__ __forceinline__ void or_pair_cell(const float *__restrict__ cur,
     }
     return overrelax({fx, fy, fz}, {vx[self], vy[self], vz[self]});
   };
-  f3 na[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     double gx, gy, gz;
@@ -286,7 +262,6 @@ __device__ __forceinline__ void or_pair_cell(const float *__restrict__ cur,
       de += field_de<false>(jeff, nullptr, CA, {vx[i], vy[i], vz[i]}, na[k], gx, gy, gz);
     }
   }
-  f3 nb[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int t = CB + 4 * k;
@@ -321,6 +296,14 @@ __device__ __forceinline__ void or_pair_cell(const float *__restrict__ cur,
     nb[k] = overrelax({fx, fy, fz}, told);
     if (TRACK) de += field_de<false>(jeff, nullptr, CB, told, nb[k], gx, gy, gz);
   }
+}
+
+// Store of the pair's 8 new spins of one cell into the second buffer.
+template <int CA, int NC>
+__device__ __forceinline__ void or_pair_store(float *__restrict__ oth, int cell, const f3 (&na)[4],
+                                              const f3 (&nb)[4]) {
+  constexpr int CB = CA + 1, P = 16 * NC;
+  float *oy = oth + P, *oz = oth + 2 * P;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const uint32_t ia = 4u * (uint32_t)((CA + 4 * k) * NC + cell);
@@ -332,6 +315,44 @@ __device__ __forceinline__ void or_pair_cell(const float *__restrict__ cur,
     st_at(oy, ib, nb[k].y);
     st_at(oz, ib, nb[k].z);
   }
+}
+
+// Two over-relaxation colour phases CA, CA + 1 of one cell in one pass over
+// the data, scalar model (cur/oth: this replica's spins in the two buffers).
+// Colours below CA have already been updated this pass and are read from
+// oth; the others are read from cur; the two new colours are written to oth,
+// so no thread reads what another thread of the launch writes. All 41 spins
+// the cell needs are loaded in one burst before any arithmetic (one memory
+// round trip, and the halo lines are used while they are still in L2).
+template <int CA, int LT, bool TRACK>
+__device__ __forceinline__ void or_pair_cell(const float *__restrict__ cur,
+                                             float *__restrict__ oth, int cell, float jeff,
+                                             double &de) {
+  static_assert(CA == 0 || CA == 2, "colour pairs (0, 1) and (2, 3)");
+  constexpr int L = LT, LL = LT * LT, NC = LT * LT * LT, P = 16 * NC;
+  constexpr PairNeeds<CA> nd{};
+  constexpr int NN = nd.n;
+  const int z = cell / LL;
+  const int y = (cell / L) % L;
+  const int x = cell % L;
+  const int X[3] = {x == 0 ? L - 1 : x - 1, x, x == L - 1 ? 0 : x + 1};
+  const int Y[3] = {(y == 0 ? L - 1 : y - 1) * L, y * L, (y == L - 1 ? 0 : y + 1) * L};
+  const int Z[3] = {(z == 0 ? L - 1 : z - 1) * LL, z * LL, (z == L - 1 ? 0 : z + 1) * LL};
+  float vx[NN], vy[NN], vz[NN];
+  const float *cy = cur + P, *cz = cur + 2 * P;
+  const float *oy = oth + P, *oz = oth + 2 * P;
+#pragma unroll
+  for (int i = 0; i < NN; ++i) {
+    const PairNeed q = nd.v[i];
+    const uint32_t j = 4u * (uint32_t)(q.site * NC + Z[q.dz + 1] + Y[q.dy + 1] + X[q.dx + 1]);
+    const bool o = (q.site & 3) < CA;
+    vx[i] = ld_at(o ? oth : cur, j);
+    vy[i] = ld_at(o ? oy : cy, j);
+    vz[i] = ld_at(o ? oz : cz, j);
+  }
+  f3 na[4], nb[4];
+  or_pair_math<CA, TRACK>(vx, vy, vz, jeff, de, na, nb);
+  or_pair_store<CA, NC>(oth, cell, na, nb);
 }
 
 // One site's update (site = colour + 4k, runtime), for kernels that cannot

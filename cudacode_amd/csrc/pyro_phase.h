@@ -239,6 +239,205 @@ __global__ __launch_bounds__(256, TRACK ? 1 : (CA == 0 ? 4 : 3)) void k_or_pair(
   }
 }
 
+// The site-planes (site, dz) that the fused pair of a z-plane of cells reads:
+// the 16 sites of the plane itself and the 4 sites of one adjacent plane that
+// the pair's links reach (z - 1 for CA = 0, z + 1 for CA = 2), 20 in all;
+// slot[site][dz + 1] is a plane's position in that list (or -1). A plane is
+// held in one of three ways (MODE 0: all staged in LDS; MODE 1: LDS or own;
+// MODE 2: all three):
+//  - kPlaneLds: staged in LDS, read at in-plane offsets that change y;
+//  - kPlaneOwn: read only at the cell's own position: loaded into registers;
+//  - kPlaneRow: read only at x offsets -1, 0, 1: loaded into registers at the
+//    own position and rotated along x within the 16-lane DPP row, which at
+//    L = 16 is exactly one periodic row of cells.
+enum : int8_t { kPlaneLds = 0, kPlaneOwn = 1, kPlaneRow = 2 };
+
+template <int CA, int MODE>
+struct PairPlanes {
+  int8_t site[24] = {}, dz[24] = {}, kind[24] = {}, lds[24] = {};
+  int8_t slot[16][3] = {};
+  // LDS row k = 3 * lds + component: the row's float offset in a replica's
+  // buffer without its z term (component * 16 N + site * N, N = 4096),
+  // (dz + 1) << 20 and (read from the second buffer) << 22.
+  uint32_t row[72] = {};
+  int n = 0, n_lds = 0;
+  constexpr PairPlanes() {
+    constexpr PairNeeds<CA> nd{};
+    for (int s = 0; s < 16; ++s)
+      for (int d = 0; d < 3; ++d) slot[s][d] = -1;
+    bool dx_used[24] = {}, dy_used[24] = {};
+    for (int i = 0; i < nd.n; ++i) {
+      const int s = nd.v[i].site, d = nd.v[i].dz + 1;
+      if (slot[s][d] < 0) {
+        slot[s][d] = (int8_t)n;
+        site[n] = (int8_t)s;
+        dz[n] = (int8_t)(d - 1);
+        ++n;
+      }
+      dx_used[slot[s][d]] |= nd.v[i].dx != 0;
+      dy_used[slot[s][d]] |= nd.v[i].dy != 0;
+    }
+    for (int sl = 0; sl < n; ++sl) {
+      kind[sl] = kPlaneLds;
+      if (MODE >= 1 && !dy_used[sl] && !dx_used[sl]) kind[sl] = kPlaneOwn;
+      if (MODE >= 2 && !dy_used[sl] && dx_used[sl]) kind[sl] = kPlaneRow;
+      lds[sl] = -1;
+      if (kind[sl] == kPlaneLds) {
+        for (int comp = 0; comp < 3; ++comp)
+          row[3 * n_lds + comp] = (uint32_t)(comp * 16 * 4096 + site[sl] * 4096) |
+                                  (uint32_t)(dz[sl] + 1) << 20 | (uint32_t)((site[sl] & 3) < CA) << 22;
+        lds[sl] = (int8_t)n_lds++;
+      }
+    }
+  }
+};
+
+// The LDS staging rows as a device constant (a wave indexes it with a uniform
+// value: one scalar load per row).
+template <int CA, int MODE>
+__constant__ constexpr uint32_t kPairRows[72] = {
+#define PYRO_ROW(k) PairPlanes<CA, MODE>{}.row[k]
+    PYRO_ROW(0),  PYRO_ROW(1),  PYRO_ROW(2),  PYRO_ROW(3),  PYRO_ROW(4),  PYRO_ROW(5),
+    PYRO_ROW(6),  PYRO_ROW(7),  PYRO_ROW(8),  PYRO_ROW(9),  PYRO_ROW(10), PYRO_ROW(11),
+    PYRO_ROW(12), PYRO_ROW(13), PYRO_ROW(14), PYRO_ROW(15), PYRO_ROW(16), PYRO_ROW(17),
+    PYRO_ROW(18), PYRO_ROW(19), PYRO_ROW(20), PYRO_ROW(21), PYRO_ROW(22), PYRO_ROW(23),
+    PYRO_ROW(24), PYRO_ROW(25), PYRO_ROW(26), PYRO_ROW(27), PYRO_ROW(28), PYRO_ROW(29),
+    PYRO_ROW(30), PYRO_ROW(31), PYRO_ROW(32), PYRO_ROW(33), PYRO_ROW(34), PYRO_ROW(35),
+    PYRO_ROW(36), PYRO_ROW(37), PYRO_ROW(38), PYRO_ROW(39), PYRO_ROW(40), PYRO_ROW(41),
+    PYRO_ROW(42), PYRO_ROW(43), PYRO_ROW(44), PYRO_ROW(45), PYRO_ROW(46), PYRO_ROW(47),
+    PYRO_ROW(48), PYRO_ROW(49), PYRO_ROW(50), PYRO_ROW(51), PYRO_ROW(52), PYRO_ROW(53),
+    PYRO_ROW(54), PYRO_ROW(55), PYRO_ROW(56), PYRO_ROW(57), PYRO_ROW(58), PYRO_ROW(59),
+#undef PYRO_ROW
+};
+
+// Value of the cell x + DX (DX = -1 or 1, periodic within the row of 16) from
+// the lanes of the same 16-lane DPP row: row_ror:n gives lane i the value of
+// lane (i - n) mod 16.
+template <int DX>
+__device__ __forceinline__ float row_neighbour(float v) {
+  constexpr int ctrl = 0x120 + (DX < 0 ? 1 : 15);  // row_ror:1 or row_ror:15
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), ctrl, 0xF, 0xF, false));
+}
+
+// Blocks per CU that the staged pair's LDS allows (160 KB per CU).
+constexpr int or_plane_blocks(int mode) { return mode == 0 ? 2 : mode == 1 ? 3 : 4; }
+
+// The fused pair at L = 16, one z-plane of a replica per block (256 cells, a
+// cell per thread), with the planes it reads at neighbouring positions staged
+// in LDS: each wave copies whole rows of 256 floats (one site-plane component)
+// with 16-byte loads, all issued in one burst with the thread's own register
+// loads, and after a barrier every thread takes its cell's 41 values from LDS
+// (in-plane neighbours wrap within the staged rows), from its registers or from
+// its DPP row. Same arithmetic (or_pair_math) and stores as k_or_pair, so the
+// same results bit for bit. k_or_pair issues 123 four-byte loads per thread,
+// twice what a wave may have in flight, i.e. two memory round trips. LDS per
+// block: 20 / 17 / 13 planes x 3 KB for MODE 0 / 1 / 2, or_plane_blocks(MODE)
+// blocks per CU. Block order: replica-major z-planes behind the XCD remap, so
+// the planes of one replica run on one XCD and share its L2 for the halo.
+template <int CA, int MODE>
+__global__ __launch_bounds__(256, or_plane_blocks(MODE)) void k_or_pair_plane(
+    const float *__restrict__ cur, float *__restrict__ oth, float jeff, int r0,
+    const uint32_t *__restrict__ ctl, uint32_t sweep_delta, double *__restrict__ series,
+    int R_total, const int *__restrict__ replica_at, const double *__restrict__ anchor,
+    const double *__restrict__ dE) {
+  constexpr int LT = 16, NC = LT * LT * LT, P = 16 * NC;
+  constexpr PairPlanes<CA, MODE> pp{};
+  constexpr PairNeeds<CA> nd{};
+  constexpr int NS = pp.n, NN = nd.n, ROWS = 3 * pp.n_lds;
+  static_assert(NS == 20 && ROWS <= 72, "20 site-planes");
+  __shared__ float4 sm[ROWS * 64];
+  const int nb = gridDim.x;
+  const int b = blockIdx.x;
+  const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+  const int q = lb >> 4;  // 16 planes per replica
+  const int r = r0 + q;
+  const int z = lb & 15;
+  if (series != nullptr && b == 0) {
+    const uint32_t row = ctl[kCtlRecordRow] + sweep_delta;
+    if (ctl[kCtlRecordOn] && row < ctl[kCtlRecordCap]) {
+      const int r1 = r0 + (nb >> 4);
+      for (int k = threadIdx.x; k < R_total; k += 256) {
+        const int rep = replica_at[k];
+        if (rep >= r0 && rep < r1) series[(size_t)row * R_total + k] = anchor[rep] + dE[rep];
+      }
+    }
+  }
+  const float *rc = cur + (size_t)r * 48 * NC;
+  float *ro = oth + (size_t)r * 48 * NC;
+  const int t = threadIdx.x, x = t & 15, y = t >> 4;
+  const int lane = t & 63;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  // The planes held in registers: this cell's value.
+  float ox[NS], oy[NS], oz[NS];
+#pragma unroll
+  for (int sl = 0; sl < NS; ++sl) {
+    if (pp.kind[sl] == kPlaneLds) continue;
+    const float *src = (pp.site[sl] & 3) < CA ? ro : rc;
+    const uint32_t j = 4u * (uint32_t)(pp.site[sl] * NC + ((z + pp.dz[sl]) & 15) * 256 + t);
+    ox[sl] = ld_at(src, j);
+    oy[sl] = ld_at(src + P, j);
+    oz[sl] = ld_at(src + 2 * P, j);
+  }
+  // LDS row k = 3 * lds + component; wave w copies rows w, w + 4, ... (uniform).
+  // Every load is issued before the first LDS write, and no write sits in a
+  // branch (a write that waited for its own load inside one would serialise
+  // the round trips): a wave with no row left in the last round copies the
+  // last row again, writing the same bytes to the same place as its owner.
+  constexpr int NI = (ROWS + 3) / 4;
+  float4 st[NI];
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int k = min(4 * i + w, ROWS - 1);
+    const uint32_t e = kPairRows<CA, MODE>[k];
+    const float *src = (e >> 22) ? ro : rc;
+    const int zz = (z + (int)((e >> 20) & 3) - 1) & 15;
+    const uint32_t off = 4u * ((e & 0xFFFFFu) + (uint32_t)zz * 256u) + 16u * (uint32_t)lane;
+    st[i] = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(src) + off);
+  }
+#pragma unroll
+  for (int i = 0; i < NI; ++i) sm[min(4 * i + w, ROWS - 1) * 64 + lane] = st[i];
+  __syncthreads();
+  const float *sf = reinterpret_cast<const float *>(sm);
+  const int X[3] = {(x + 15) & 15, x, (x + 1) & 15};
+  const int Y[3] = {((y + 15) & 15) * 16, y * 16, ((y + 1) & 15) * 16};
+  float vx[NN], vy[NN], vz[NN];
+#pragma unroll
+  for (int i = 0; i < NN; ++i) {
+    const PairNeed qn = nd.v[i];
+    const int sl = pp.slot[qn.site][qn.dz + 1];
+    if (pp.kind[sl] == kPlaneLds) {
+      const int at = Y[qn.dy + 1] + X[qn.dx + 1];
+      const int l = pp.lds[sl];
+      vx[i] = sf[(3 * l + 0) * 256 + at];
+      vy[i] = sf[(3 * l + 1) * 256 + at];
+      vz[i] = sf[(3 * l + 2) * 256 + at];
+    } else if (qn.dx == 0) {
+      vx[i] = ox[sl];
+      vy[i] = oy[sl];
+      vz[i] = oz[sl];
+    } else if (qn.dx < 0) {
+      vx[i] = row_neighbour<-1>(ox[sl]);
+      vy[i] = row_neighbour<-1>(oy[sl]);
+      vz[i] = row_neighbour<-1>(oz[sl]);
+    } else {
+      vx[i] = row_neighbour<1>(ox[sl]);
+      vy[i] = row_neighbour<1>(oy[sl]);
+      vz[i] = row_neighbour<1>(oz[sl]);
+    }
+  }
+  f3 na[4], nbv[4];
+  double de = 0.0;
+  or_pair_math<CA, false>(vx, vy, vz, jeff, de, na, nbv);
+  or_pair_store<CA, NC>(ro, z * 256 + t, na, nbv);
+}
+
+// The staging the L = 16 fused pair uses. Measured over L = 16 x 64 replicas
+// (one launch over all replicas; sweep on three streams), against 15.65 us and
+// 0.326 ms for k_or_pair: MODE 0 16.3 us, 0.329 ms; MODE 1 14.5 us, 0.321 ms;
+// MODE 2 13.2 us (5.7 TB/s at 36 B per update), 0.321 ms.
+constexpr int kOrPlaneMode = 2;
+
 // Per-block FP64 partial sums of the energy and the sublattice-b magnetisation
 // (TotalEnergyGPU, Pyrochlore.cu:344-389; MagnetizationGPU, :277-342;
 // Measurement, :526-578). grid = (obs_blocks, 4, R): blockIdx.y is the basis
@@ -408,6 +607,20 @@ static void launch_or_pair_t(pyro_ctx *ctx, const float *cur, float *oth) {
   if (CA == 0 && !TRACK && ctx->launch_series) {
     series = ctx->launch_series;
     ctx->launch_series_done = 1;
+  }
+  if constexpr (LT == 16 && !TRACK) {  // planes staged in LDS
+    if (ctx->time_ev0) {
+      hipExtLaunchKernelGGL(k_or_pair_plane<CA, kOrPlaneMode>, dim3(16 * nrep), dim3(256), 0, launch_stream(ctx),
+                            ctx->time_ev0, ctx->time_ev1, 0, cur, oth, ctx->jeff, launch_r0(ctx),
+                            (const uint32_t *)ctx->d_ctl, ctx->launch_delta, series, ctx->R,
+                            (const int *)ctx->d_replica_at, (const double *)ctx->d_anchor,
+                            (const double *)ctx->d_dE);
+      return;
+    }
+    k_or_pair_plane<CA, kOrPlaneMode><<<16 * nrep, 256, 0, launch_stream(ctx)>>>(
+        cur, oth, ctx->jeff, launch_r0(ctx), ctx->d_ctl, ctx->launch_delta, series, ctx->R,
+        ctx->d_replica_at, ctx->d_anchor, ctx->d_dE);
+    return;
   }
   if (ctx->time_ev0) {
     hipExtLaunchKernelGGL(k_or_pair<CA, LT, TRACK>, dim3(bpr * nrep), dim3(256), 0,
