@@ -372,8 +372,10 @@ static int create_ctx(const std::vector<int> &Ls, const double *temps, uint64_t 
     // hides (measured: L=16 x 64 replicas 0.396 ms per sweep on one stream,
     // 0.342 in two groups, 0.335 in three; L=4 x 64 2.2x slower in two).
     // Four groups ran 0.328-0.363 ms from one captured graph to the next
-    // (four streams of this ctx on the default four hardware queues), so
-    // the default is the larger of 3 and 2 groups whose smallest group spans
+    // (four streams of this ctx on the default four hardware queues); with
+    // the plane-staged fused pair, 0.3115-0.3125 ms on one box and
+    // 0.43-0.45 ms on another, against 0.319-0.325 for three on both. So the
+    // default is the larger of 3 and 2 groups whose smallest group spans
     // kSplitMinBlocks; pyro_set_streams allows 4.
     for (int n : {3, 2}) {
       if (n > ctx->n_streams) continue;
