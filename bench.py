@@ -41,7 +41,7 @@ BF16_PEAK_TFLOPS = 2500.0  # dense bf16 MFMA
 
 
 OR_KERNEL = "k_phase_cells<1, colour, 16, false, false>"
-OR_PAIR_KERNEL = "k_or_pair_plane<pair, 2>"
+OR_PAIR_KERNEL = "k_or_pair_plane<pair, 3>"
 OR_PAIR_KERNEL_C5 = "k_or_pair<pair, 64, false>"
 OR_KERNEL_C5 = "k_phase_cells<1, colour, 64, false, false>"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -600,7 +600,8 @@ def main():
             "kernel": OR_PAIR_KERNEL + " (two over-relaxation colours per launch, one thread "
                                        "per cell, one z-plane of a replica per block: the "
                                        "planes read at y offsets staged in LDS, x offsets from "
-                                       "the DPP row; 20 of 24 launches per sweep and replica "
+                                       "the DPP row, the neighbours' first-colour values "
+                                       "exchanged; 20 of 24 launches per sweep and replica "
                                        "group)",
             "achieved": pair_achieved,
             "peak": HBM_PEAK_GBS,

@@ -170,17 +170,24 @@ __device__ __forceinline__ void cell_update(float *__restrict__ Sx, int cell, in
 
 // The spins one fused over-relaxation pair (colours CA, CA + 1) reads for a
 // cell: (site, dx, dy, dz) with offsets in -1..1, built at compile time from
-// the link table (41 of them for either pair), and the position in that list
-// of every value the two phases use, so the kernel indexes its register copy
-// with constants only.
+// the link table, and the position in that list of every value the two phases
+// use, so the kernel indexes its register copy with constants only. Phase
+// CA + 1 needs the new colour-CA values of its neighbours: those of the cell
+// come from the first phase, and 3 are of colour-CA sites of neighbouring
+// cells ("foreign", listed in fv). With RECOMP they are recomputed from their
+// own links, which are then needs too (41 spins for either pair); without it
+// the caller supplies them (29 spins).
 struct PairNeed { int8_t site, dx, dy, dz; };
-template <int CA>
+template <int CA, bool RECOMP = true>
 struct PairNeeds {
   PairNeed v[64] = {};
   int n = 0;
-  int a_self[4] = {}, a_nb[4][6] = {};         // colour-CA sites of the cell and their links
-  int t_self[4] = {}, t_nb[4][6] = {};         // colour-CA+1 sites and their links
-  int f_self[4][6] = {}, f_nb[4][6][6] = {};   // a foreign colour-CA neighbour and its links
+  int a_self[4] = {}, a_nb[4][6] = {};  // colour-CA sites of the cell and their links
+  int t_self[4] = {}, t_nb[4][6] = {};  // colour-CA+1 sites and their links (-1: colour CA, new)
+  int f_idx[4][6] = {};                 // the link's position in fv, or -1
+  PairNeed fv[8] = {};                  // the foreign colour-CA neighbours
+  int nf = 0;
+  int f_self[8] = {}, f_nb[8][6] = {};  // RECOMP: a foreign neighbour's value and its links
   constexpr int find(int site, int dx, int dy, int dz) const {
     for (int i = 0; i < n; ++i)
       if (v[i].site == site && v[i].dx == dx && v[i].dy == dy && v[i].dz == dz) return i;
@@ -204,64 +211,90 @@ struct PairNeeds {
       t_self[k] = add(t, 0, 0, 0);
       for (int m = 0; m < 6; ++m) {
         const Link lk = kLinks[t][m];
-        t_nb[k][m] = add(lk.site, lk.dx, lk.dy, lk.dz);
-        f_self[k][m] = -1;
-        if ((lk.site & 3) == CA && (lk.dx | lk.dy | lk.dz) != 0) {
-          f_self[k][m] = t_nb[k][m];
-          for (int q = 0; q < 6; ++q) {
-            const Link l2 = kLinks[lk.site][q];
-            f_nb[k][m][q] = add(l2.site, lk.dx + l2.dx, lk.dy + l2.dy, lk.dz + l2.dz);
+        const bool ca = (lk.site & 3) == CA;
+        t_nb[k][m] = (!ca || RECOMP) ? add(lk.site, lk.dx, lk.dy, lk.dz) : -1;
+        f_idx[k][m] = -1;
+        if (ca && (lk.dx | lk.dy | lk.dz) != 0) {
+          f_idx[k][m] = nf;
+          fv[nf] = {lk.site, lk.dx, lk.dy, lk.dz};
+          if (RECOMP) {
+            f_self[nf] = t_nb[k][m];
+            for (int q = 0; q < 6; ++q) {
+              const Link l2 = kLinks[lk.site][q];
+              f_nb[nf][q] = add(l2.site, lk.dx + l2.dx, lk.dy + l2.dy, lk.dz + l2.dz);
+            }
           }
+          ++nf;
         }
       }
     }
   }
 };
 
-// The arithmetic of two over-relaxation colour phases CA, CA + 1 of one cell,
-// scalar model, on the cell's PairNeeds<CA> values vx/vy/vz (old spins; the
-// colours below CA already updated this pass). Phase CA + 1 needs the new
-// colour-CA values of its neighbours: those in this cell come from the first
-// phase, those in the neighbouring cells are recomputed here with the owner's
-// float expressions in the owner's order (over-relaxation is deterministic),
-// so every result equals two separate phase launches bit for bit. na, nb: the
-// new colour-CA and colour-CA+1 spins of sites CA + 4k, CA + 1 + 4k. TRACK:
-// FP64 energy changes of this cell's updates added to de.
-template <int CA, bool TRACK, int NN>
-__device__ __forceinline__ void or_pair_math(const float (&vx)[NN], const float (&vy)[NN],
-                                             const float (&vz)[NN], float jeff, double &de,
-                                             f3 (&na)[4], f3 (&nb)[4]) {
-  constexpr int CB = CA + 1;
-  constexpr PairNeeds<CA> nd{};
-  static_assert(NN == nd.n, "value arrays hold the pair's needs");
-  // Phase-CA update of a colour-CA site from its own value (index self) and
-  // its 6 links (indices nb), with its FP64 neighbour sum.
-  auto upd_a = [&](int self, const int *nbr, double &gx, double &gy, double &gz) -> f3 {
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    gx = gy = gz = 0.0;
+// Over-relaxation of the spin at index self from its 6 links (indices nbr)
+// among the values vx/vy/vz, with its FP64 neighbour sum when TRACK.
+template <bool TRACK, int NN>
+__device__ __forceinline__ f3 pair_relax(const float (&vx)[NN], const float (&vy)[NN],
+                                         const float (&vz)[NN], int self, const int *nbr,
+                                         double &gx, double &gy, double &gz) {
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  gx = gy = gz = 0.0;
 #pragma unroll
-    for (int m = 0; m < 6; ++m) {
-      const int i = nbr[m];
-      fx = fx + vx[i];
-      fy = fy + vy[i];
-      fz = fz + vz[i];
-      if (TRACK) {
-        gx += (double)vx[i];
-        gy += (double)vy[i];
-        gz += (double)vz[i];
-      }
+  for (int m = 0; m < 6; ++m) {
+    const int i = nbr[m];
+    fx = fx + vx[i];
+    fy = fy + vy[i];
+    fz = fz + vz[i];
+    if (TRACK) {
+      gx += (double)vx[i];
+      gy += (double)vy[i];
+      gz += (double)vz[i];
     }
-    return overrelax({fx, fy, fz}, {vx[self], vy[self], vz[self]});
-  };
+  }
+  return overrelax({fx, fy, fz}, {vx[self], vy[self], vz[self]});
+}
+
+// Phase CA of a cell's pair: na[k] = new spin of site CA + 4k.
+template <int CA, bool TRACK, bool RECOMP, int NN>
+__device__ __forceinline__ void or_pair_phase_a(const float (&vx)[NN], const float (&vy)[NN],
+                                                const float (&vz)[NN], float jeff, double &de,
+                                                f3 (&na)[4]) {
+  constexpr PairNeeds<CA, RECOMP> nd{};
+  static_assert(NN == nd.n, "value arrays hold the pair's needs");
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     double gx, gy, gz;
-    na[k] = upd_a(nd.a_self[k], nd.a_nb[k], gx, gy, gz);
+    na[k] = pair_relax<TRACK>(vx, vy, vz, nd.a_self[k], nd.a_nb[k], gx, gy, gz);
     if (TRACK) {
       const int i = nd.a_self[k];
       de += field_de<false>(jeff, nullptr, CA, {vx[i], vy[i], vz[i]}, na[k], gx, gy, gz);
     }
   }
+}
+
+// The new values of the foreign colour-CA neighbours, recomputed with their
+// owners' float expressions in their owners' order (over-relaxation is
+// deterministic), so they equal what the owners compute bit for bit.
+template <int CA, int NN>
+__device__ __forceinline__ void or_pair_foreign(const float (&vx)[NN], const float (&vy)[NN],
+                                                const float (&vz)[NN], f3 (&fo)[3]) {
+  constexpr PairNeeds<CA, true> nd{};
+  static_assert(nd.nf == 3, "3 foreign neighbours per pair");
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    double hx, hy, hz;
+    fo[f] = pair_relax<false>(vx, vy, vz, nd.f_self[f], nd.f_nb[f], hx, hy, hz);
+  }
+}
+
+// Phase CA + 1 of a cell's pair from the old values, the cell's new colour-CA
+// spins na and the foreign ones fo: nb[k] = new spin of site CA + 1 + 4k.
+template <int CA, bool TRACK, bool RECOMP, int NN>
+__device__ __forceinline__ void or_pair_phase_b(const float (&vx)[NN], const float (&vy)[NN],
+                                                const float (&vz)[NN], float jeff, double &de,
+                                                const f3 (&na)[4], const f3 (&fo)[3], f3 (&nb)[4]) {
+  constexpr int CB = CA + 1;
+  constexpr PairNeeds<CA, RECOMP> nd{};
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int t = CB + 4 * k;
@@ -272,12 +305,7 @@ __device__ __forceinline__ void or_pair_math(const float (&vx)[NN], const float 
       const Link lk = kLinks[t][m];
       f3 w;
       if ((lk.site & 3) == CA) {
-        if (lk.dx == 0 && lk.dy == 0 && lk.dz == 0) {
-          w = na[lk.site >> 2];
-        } else {
-          double hx, hy, hz;
-          w = upd_a(nd.f_self[k][m], nd.f_nb[k][m], hx, hy, hz);
-        }
+        w = nd.f_idx[k][m] < 0 ? na[lk.site >> 2] : fo[nd.f_idx[k][m]];
       } else {
         const int i = nd.t_nb[k][m];
         w = {vx[i], vy[i], vz[i]};
@@ -296,6 +324,22 @@ __device__ __forceinline__ void or_pair_math(const float (&vx)[NN], const float 
     nb[k] = overrelax({fx, fy, fz}, told);
     if (TRACK) de += field_de<false>(jeff, nullptr, CB, told, nb[k], gx, gy, gz);
   }
+}
+
+// The arithmetic of two over-relaxation colour phases CA, CA + 1 of one cell,
+// scalar model, on the cell's PairNeeds<CA> values vx/vy/vz (old spins; the
+// colours below CA already updated this pass), the foreign neighbours
+// recomputed: every result equals two separate phase launches bit for bit.
+// na, nb: the new colour-CA and colour-CA+1 spins of sites CA + 4k,
+// CA + 1 + 4k. TRACK: FP64 energy changes of this cell's updates added to de.
+template <int CA, bool TRACK, int NN>
+__device__ __forceinline__ void or_pair_math(const float (&vx)[NN], const float (&vy)[NN],
+                                             const float (&vz)[NN], float jeff, double &de,
+                                             f3 (&na)[4], f3 (&nb)[4]) {
+  or_pair_phase_a<CA, TRACK, true>(vx, vy, vz, jeff, de, na);
+  f3 fo[3];
+  or_pair_foreign<CA>(vx, vy, vz, fo);
+  or_pair_phase_b<CA, TRACK, true>(vx, vy, vz, jeff, de, na, fo, nb);
 }
 
 // Store of the pair's 8 new spins of one cell into the second buffer.

@@ -242,9 +242,11 @@ __global__ __launch_bounds__(256, TRACK ? 1 : (CA == 0 ? 4 : 3)) void k_or_pair(
 // The site-planes (site, dz) that the fused pair of a z-plane of cells reads:
 // the 16 sites of the plane itself and the 4 sites of one adjacent plane that
 // the pair's links reach (z - 1 for CA = 0, z + 1 for CA = 2), 20 in all;
-// slot[site][dz + 1] is a plane's position in that list (or -1). A plane is
-// held in one of three ways (MODE 0: all staged in LDS; MODE 1: LDS or own;
-// MODE 2: all three):
+// slot[site][dz + 1] is a plane's position in that list (or -1). MODE 3
+// takes the 3 foreign colour-CA values from their owner lanes (all three lie
+// in the block's plane) instead of recomputing them, which leaves 29 values
+// a cell reads. A plane is held in one of three ways (MODE 0: all staged in
+// LDS; MODE 1: LDS or own; MODE 2 and 3: all three):
 //  - kPlaneLds: staged in LDS, read at in-plane offsets that change y;
 //  - kPlaneOwn: read only at the cell's own position: loaded into registers;
 //  - kPlaneRow: read only at x offsets -1, 0, 1: loaded into registers at the
@@ -262,7 +264,7 @@ struct PairPlanes {
   uint32_t row[72] = {};
   int n = 0, n_lds = 0;
   constexpr PairPlanes() {
-    constexpr PairNeeds<CA> nd{};
+    constexpr PairNeeds<CA, (MODE < 3)> nd{};
     for (int s = 0; s < 16; ++s)
       for (int d = 0; d < 3; ++d) slot[s][d] = -1;
     bool dx_used[24] = {}, dy_used[24] = {};
@@ -320,8 +322,29 @@ __device__ __forceinline__ float row_neighbour(float v) {
       float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), ctrl, 0xF, 0xF, false));
 }
 
+// The colour-CA sites of a cell whose new values its neighbours in other rows
+// of the plane need (foreign neighbours at dy != 0): k[e] = the site's index
+// k (site CA + 4k) in the exchange, slot[k] = its exchange row (or -1).
+template <int CA>
+struct PairExchange {
+  int k[4] = {}, slot[4] = {-1, -1, -1, -1};
+  int n = 0;
+  constexpr PairExchange() {
+    constexpr PairNeeds<CA, false> nd{};
+    for (int f = 0; f < nd.nf; ++f) {
+      const int kk = nd.fv[f].site >> 2;
+      if (nd.fv[f].dy != 0 && slot[kk] < 0) {
+        slot[kk] = n;
+        k[n++] = kk;
+      }
+    }
+  }
+};
+template <int CA>
+constexpr PairExchange<CA> kPairExchange{};
+
 // Blocks per CU that the staged pair's LDS allows (160 KB per CU).
-constexpr int or_plane_blocks(int mode) { return mode == 0 ? 2 : mode == 1 ? 3 : 4; }
+constexpr int or_plane_blocks(int mode) { return mode == 0 ? 2 : mode == 1 ? 3 : mode == 2 ? 4 : 5; }
 
 // The fused pair at L = 16, one z-plane of a replica per block (256 cells, a
 // cell per thread), with the planes it reads at neighbouring positions staged
@@ -329,11 +352,11 @@ constexpr int or_plane_blocks(int mode) { return mode == 0 ? 2 : mode == 1 ? 3 :
 // with 16-byte loads, all issued in one burst with the thread's own register
 // loads, and after a barrier every thread takes its cell's 41 values from LDS
 // (in-plane neighbours wrap within the staged rows), from its registers or from
-// its DPP row. Same arithmetic (or_pair_math) and stores as k_or_pair, so the
-// same results bit for bit. k_or_pair issues 123 four-byte loads per thread,
-// twice what a wave may have in flight, i.e. two memory round trips. LDS per
-// block: 20 / 17 / 13 planes x 3 KB for MODE 0 / 1 / 2, or_plane_blocks(MODE)
-// blocks per CU. Block order: replica-major z-planes behind the XCD remap, so
+// its DPP row. Same arithmetic (or_pair_phase_a / _b) and stores as k_or_pair,
+// so the same results bit for bit. k_or_pair issues 123 four-byte loads per
+// thread, twice what a wave may have in flight, i.e. two memory round trips.
+// LDS per block: 20 / 17 / 13 / 6 planes x 3 KB for MODE 0 / 1 / 2 / 3 (and
+// MODE 3's 6 KB exchange rows), or_plane_blocks(MODE) blocks per CU. Block order: replica-major z-planes behind the XCD remap, so
 // the planes of one replica run on one XCD and share its L2 for the halo.
 template <int CA, int MODE>
 __global__ __launch_bounds__(256, or_plane_blocks(MODE)) void k_or_pair_plane(
@@ -342,8 +365,9 @@ __global__ __launch_bounds__(256, or_plane_blocks(MODE)) void k_or_pair_plane(
     int R_total, const int *__restrict__ replica_at, const double *__restrict__ anchor,
     const double *__restrict__ dE) {
   constexpr int LT = 16, NC = LT * LT * LT, P = 16 * NC;
+  constexpr bool RECOMP = MODE < 3;
   constexpr PairPlanes<CA, MODE> pp{};
-  constexpr PairNeeds<CA> nd{};
+  constexpr PairNeeds<CA, RECOMP> nd{};
   constexpr int NS = pp.n, NN = nd.n, ROWS = 3 * pp.n_lds;
   static_assert(NS == 20 && ROWS <= 72, "20 site-planes");
   __shared__ float4 sm[ROWS * 64];
@@ -428,15 +452,54 @@ __global__ __launch_bounds__(256, or_plane_blocks(MODE)) void k_or_pair_plane(
   }
   f3 na[4], nbv[4];
   double de = 0.0;
-  or_pair_math<CA, false>(vx, vy, vz, jeff, de, na, nbv);
+  if constexpr (RECOMP) {
+    or_pair_math<CA, false>(vx, vy, vz, jeff, de, na, nbv);
+  } else {
+    // The foreign colour-CA neighbours' new values from their owners: along
+    // the row by DPP, across rows through LDS (a row per component of each
+    // such site), written after phase CA and read after a barrier.
+    static_assert(nd.nf == 3, "3 foreign neighbours per pair");
+    constexpr int NX = kPairExchange<CA>.n;
+    __shared__ float xs[(NX > 0 ? NX : 1) * 3 * 256];
+    or_pair_phase_a<CA, false, false>(vx, vy, vz, jeff, de, na);
+#pragma unroll
+    for (int e = 0; e < NX; ++e) {
+      const f3 v = na[kPairExchange<CA>.k[e]];
+      xs[(3 * e + 0) * 256 + t] = v.x;
+      xs[(3 * e + 1) * 256 + t] = v.y;
+      xs[(3 * e + 2) * 256 + t] = v.z;
+    }
+    __syncthreads();
+    f3 fo[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const PairNeed q = nd.fv[f];
+      static_assert(nd.fv[0].dz == 0 && nd.fv[1].dz == 0 && nd.fv[2].dz == 0,
+                    "foreign neighbours in the block's plane");
+      const f3 own = na[q.site >> 2];
+      if (q.dy == 0 && q.dx < 0) {
+        fo[f] = {row_neighbour<-1>(own.x), row_neighbour<-1>(own.y), row_neighbour<-1>(own.z)};
+      } else if (q.dy == 0) {
+        fo[f] = {row_neighbour<1>(own.x), row_neighbour<1>(own.y), row_neighbour<1>(own.z)};
+      } else {
+        const int e = kPairExchange<CA>.slot[q.site >> 2];
+        const int at = Y[q.dy + 1] + X[q.dx + 1];
+        fo[f] = {xs[(3 * e + 0) * 256 + at], xs[(3 * e + 1) * 256 + at],
+                 xs[(3 * e + 2) * 256 + at]};
+      }
+    }
+    or_pair_phase_b<CA, false, false>(vx, vy, vz, jeff, de, na, fo, nbv);
+  }
   or_pair_store<CA, NC>(ro, z * 256 + t, na, nbv);
 }
 
 // The staging the L = 16 fused pair uses. Measured over L = 16 x 64 replicas
 // (one launch over all replicas; sweep on three streams), against 15.65 us and
 // 0.326 ms for k_or_pair: MODE 0 16.3 us, 0.329 ms; MODE 1 14.5 us, 0.321 ms;
-// MODE 2 13.2 us (5.7 TB/s at 36 B per update), 0.321 ms.
-constexpr int kOrPlaneMode = 2;
+// MODE 2 13.2 us (5.7 TB/s at 36 B per update), 0.322 ms; MODE 3 (the foreign
+// values exchanged: 29 values a cell, 6 planes in LDS, ~600 instead of ~780
+// VALU per wave, 94 VGPRs, 5 blocks per CU) 12.4 us (6.1 TB/s), 0.308 ms.
+constexpr int kOrPlaneMode = 3;
 
 // Per-block FP64 partial sums of the energy and the sublattice-b magnetisation
 // (TotalEnergyGPU, Pyrochlore.cu:344-389; MagnetizationGPU, :277-342;
