@@ -51,11 +51,14 @@ constexpr int kOffW3 = kOffB2 + kMlpH;
 constexpr int kOffB3 = kOffW3 + kMlpOut * kMlpH;
 constexpr int kMlpParams = kOffB3 + kMlpOut;  // 4868
 // bf16 operand images: W1|b1 as [64][8] (column 6 = bias, 7 = 0), W2 [64][64],
-// W3 padded to [32][64] (rows 4..31 zero).
+// W3 padded to [32][64] (rows 4..31 zero); then b2 as 64 floats (two entries
+// each, 16-byte aligned), already scaled as the forward pass uses it.
 constexpr int kPackP1 = 0;
 constexpr int kPackP2 = kPackP1 + 64 * 8;
 constexpr int kPackP3 = kPackP2 + 64 * 64;
-constexpr int kPackSize = kPackP3 + 32 * 64;
+constexpr int kPackB2 = kPackP3 + 32 * 64;
+constexpr int kPackSize = kPackB2 + 2 * 64;
+static_assert(kPackB2 % 8 == 0, "b2 floats 16-byte aligned");
 // Per-sample training buffers, [unit][batch]: x(6) a1(64) a2(64) d3(4) d2(64) d1(64) loss(1)
 constexpr float kLogKappaMin = -4.60517019f;  // ln 0.01
 constexpr float kLogKappaMax = 8.31776617f;   // ln 4096

@@ -143,11 +143,18 @@ __device__ __forceinline__ void mlp_load(const uint16_t *__restrict__ pack,
   for (int t = 0; t < 4; ++t)
     W.a3[t] = __builtin_bit_cast(
         bf16x8, *reinterpret_cast<const u32x4v *>(pack + kPackP3 + (t * 64 + lane) * 8));
+  // b2 of the rows this lane's accumulators hold, 4 consecutive rows a load
+  const float4 *b2 = reinterpret_cast<const float4 *>(pack + kPackB2);
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-    for (int v = 0; v < 16; ++v)
-      W.b2[ti][v] = w[kOffB2 + 32 * ti + 8 * (v >> 2) + 4 * h + (v & 3)] * kHeadroomDown;
+    for (int blk = 0; blk < 4; ++blk) {
+      const float4 q = b2[8 * ti + 2 * blk + h];  // rows 32 ti + 8 blk + 4 h + (0..3)
+      W.b2[ti][4 * blk + 0] = q.x;
+      W.b2[ti][4 * blk + 1] = q.y;
+      W.b2[ti][4 * blk + 2] = q.z;
+      W.b2[ti][4 * blk + 3] = q.w;
+    }
 #pragma unroll
   for (int q = 0; q < 4; ++q) W.b3[q] = w[kOffB3 + q];
 }
@@ -219,6 +226,16 @@ __device__ __forceinline__ float rsq_f(float x) { return __builtin_amdgcn_rsqf(x
 __device__ __forceinline__ float exp_f(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504089f); }
 __device__ __forceinline__ float log_f(float x) { return __builtin_amdgcn_logf(x) * 0.693147180560f; }
 
+// log(1 + x) for x in [-1, 0] from the hardware log, as log(u) x / (u - 1)
+// with u = 1 + x rounded (D. Goldberg, "What every computer scientist should
+// know about floating-point arithmetic", 1991, theorem 4): a few ulps where
+// the library's log1pf costs about a hundred instructions; x at u = 1;
+// -inf at x = -1 like log1p.
+__device__ __forceinline__ float log1p_f(float x) {
+  const float u = 1.0f + x;
+  return u == 1.0f ? x : log_f(u) * (x * rcp_f(u - 1.0f));
+}
+
 // expm1(-2 kappa) for kappa >= 0: Taylor to the 7th power below kappa = 1/4
 // (truncation < 3e-7 relative), exp(-2 kappa) - 1 above (no cancellation:
 // the result is below -0.39).
@@ -281,7 +298,7 @@ __device__ __forceinline__ float vmf_log_density(const Vmf &v, f3 y) {
 __device__ __forceinline__ f3 vmf_sample(const Vmf &v, float u1, float u2) {
   const float k = v.kappa;
   const float p = 1.0f - u1;  // (0, 1]
-  float w = 1.0f + log1pf((1.0f - p) * v.em2k) * rcp_f(k);
+  float w = 1.0f + log1p_f((1.0f - p) * v.em2k) * rcp_f(k);
   w = fminf(1.0f, fmaxf(-1.0f, w));
   const float st = sqrtf(fmaxf(0.0f, 1.0f - w * w));
   float ps, pc;
@@ -710,10 +727,15 @@ __global__ void k_mlp_pack(const float *__restrict__ w, uint16_t *__restrict__ p
   } else if (p < kPackP3) {  // [ti][t][lane][j]
     const int q = p - kPackP2, j = q & 7, lane = (q >> 3) & 63, t = (q >> 9) & 3, ti = q >> 11;
     val = w[kOffW2 + (32 * ti + (lane & 31)) * 64 + kstep_unit(t, lane >> 5, j)];
-  } else {  // [t][lane][j], rows 4..31 zero
+  } else if (p < kPackB2) {  // [t][lane][j], rows 4..31 zero
     const int q = p - kPackP3, j = q & 7, lane = (q >> 3) & 63, t = q >> 9;
     const int row = lane & 31;
     val = row < 4 ? w[kOffW3 + row * 64 + kstep_unit(t, lane >> 5, j)] * kHeadroomUp : 0.0f;
+  } else {  // b2 * 2^-16 as FP32, low then high 16 bits of each
+    const int q = p - kPackB2;
+    const uint32_t bits = __builtin_bit_cast(uint32_t, w[kOffB2 + (q >> 1)] * kHeadroomDown);
+    pack[p] = (uint16_t)((q & 1) ? bits >> 16 : bits & 0xFFFFu);
+    return;
   }
   pack[p] = (uint16_t)bf16_bits(val);
 }
