@@ -282,7 +282,7 @@ def phase_only(args):
                         n_or=N_OR, pt_every=PT_EVERY, cos_cone=0.5)
     g.CalculateSolidAngle(target=0.5, rounds=10, sweeps_per_round=10)
     g.Mc(args.warmup)
-    kind = {"learned": 2, "or_pair": 3, "metropolis": 0}[args.phase_only]
+    kind = {"learned": 2, "or_pair": 3, "metropolis": 0, "metropolis_pair": 4}[args.phase_only]
     if kind == 2:
         learned.train(g, steps=args.train_steps, init_seed=SEED)
     avg_ms, span_ms = g.time_phase(kind, args.phase_launches)
@@ -296,7 +296,7 @@ def phase_only(args):
                     "achieved_tflops": flop / (avg_ms * 1e-3) / 1e12,
                     "frac_bf16_peak": flop / (avg_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS})
     else:
-        b = (BYTES_PER_UPDATE_FUSED * n_total // 2) if kind == 3 else BYTES_PER_UPDATE * n_total // 4
+        b = (BYTES_PER_UPDATE_FUSED * n_total // 2) if kind in (3, 4) else BYTES_PER_UPDATE * n_total // 4
         out.update({"bytes_per_launch": b, "achieved_gbs": b / (avg_ms * 1e-3) / 1e9,
                     "frac_hbm_peak": b / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
     return out
@@ -472,7 +472,7 @@ def main():
                     help="plain sweeps before the tau_int legs (independent of --warmup)")
     ap.add_argument("--series-sweeps", type=int, default=3000,
                     help="timed sweeps per tau_int leg (independent of --steps)")
-    ap.add_argument("--phase-only", choices=["learned", "or_pair", "metropolis"],
+    ap.add_argument("--phase-only", choices=["learned", "or_pair", "metropolis", "metropolis_pair"],
                     help="only time N launches of one colour-phase kernel on the L=16 x 64 "
                          "workload, eagerly (no captured sweeps): the command the kernel-trace "
                          "summaries of these kernels are made with")
@@ -520,6 +520,7 @@ def main():
     pair_ms, _ = g.time_phase(3, args.phase_launches)
     or_ms, _ = g.time_phase(1, args.phase_launches)
     met_ms, _ = g.time_phase(0, args.phase_launches // 4)
+    met_pair_ms, _ = g.time_phase(4, args.phase_launches // 4)
     g.set_kernel_options(0, False)  # the site-per-thread kernel, for comparison
     or_ms_v0, _ = g.time_phase(1, args.phase_launches // 4)
     g.set_kernel_options(1, False)
@@ -529,7 +530,8 @@ def main():
     pair_achieved = pair_bytes / (pair_ms * 1e-3) / 1e9
     # Algorithmic bytes of the whole sweep as it runs: 60 B per Metropolis
     # update, 36 B per fused over-relaxation update.
-    sweep_bytes = (BYTES_PER_UPDATE * N_METRO + BYTES_PER_UPDATE_FUSED * N_OR) * n_total
+    # L=16: Metropolis and over-relaxation both run two colours per launch
+    sweep_bytes = BYTES_PER_UPDATE_FUSED * (N_METRO + N_OR) * n_total
     # A caller that hands host buffers over every sweep pays the PCIe copies
     # too (pyro_set_spins re-anchors the energy as well): the host-inclusive
     # rate, reported beside the HBM-resident value.
@@ -615,6 +617,10 @@ def main():
                 "achieved": achieved, "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "avg_launch_ms": or_ms, "bytes_per_launch": bytes_per_launch},
             "metropolis_avg_launch_ms": met_ms,
+            # the sweep's Metropolis: two colours per launch (k_metro_pair_plane)
+            "metropolis_pair_avg_launch_ms": met_pair_ms,
+            "metropolis_pair_achieved": (BYTES_PER_UPDATE_FUSED * n_total // 2)
+                                        / (met_pair_ms * 1e-3) / 1e9,
             "site_per_thread_or_avg_launch_ms": or_ms_v0,
             # The whole timed sweep at its algorithmic bytes (exchanges,
             # bookkeeping and the replica groups' overlap included).

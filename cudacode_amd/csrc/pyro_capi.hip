@@ -60,8 +60,14 @@ static int enqueue_phases(pyro_ctx *ctx, bool record = false) {
     PYRO_TRY(launch_batch_sweep_body(ctx));
     return record ? launch_record_part(ctx) : 0;
   }
-  for (int p = 0; p < ctx->n_metro; ++p)
-    for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_metropolis_phase(ctx, c, p, nullptr));
+  for (int p = 0; p < ctx->n_metro; ++p) {
+    if (metro_fusable(ctx)) {
+      PYRO_TRY(launch_metropolis_pair(ctx, 0, p));
+      PYRO_TRY(launch_metropolis_pair(ctx, 2, p));
+    } else {
+      for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_metropolis_phase(ctx, c, p, nullptr));
+    }
+  }
   for (int p = 0; p < ctx->n_learned; ++p)
     for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_learned_phase(ctx, c, p));
   ctx->launch_series_done = 0;
@@ -1211,8 +1217,10 @@ int pyro_measure_fetch(pyro_ctx *ctx, double *rows, int *n_rows) {
 
 int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, double *span_ms) {
   if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
-  if (kind < 0 || kind > 3 || n_launches < 1 || n_launches > 100000)
+  if (kind < 0 || kind > 4 || n_launches < 1 || n_launches > 100000)
     return set_error(PYRO_EINVAL, "pyro_time_phase: kind %d / n_launches %d", kind, n_launches);
+  if (kind == 4 && !metro_fusable(ctx))
+    return set_error(PYRO_ESTATE, "pyro_time_phase: no fused Metropolis pair for this ctx");
   if (kind == 3 && !or_fusable(ctx))
     return set_error(PYRO_ESTATE, "pyro_time_phase: no fused over-relaxation for this ctx");
   PYRO_TRY(device_guard(ctx));
@@ -1231,6 +1239,8 @@ int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, dou
     if (kind == 3) {  // fused pairs (0,1) and (2,3) from d_spins into the second buffer:
                       // d_spins, the state, is left as it was
       rc = launch_overrelax_pair(ctx, 2 * (i & 1), ctx->d_spins, ctx->d_spins_alt);
+    } else if (kind == 4) {  // fused Metropolis pairs (0,1), (2,3), in place
+      rc = launch_metropolis_pair(ctx, 2 * (i & 1), 0);
     } else {
       rc = kind == 0   ? launch_metropolis_phase(ctx, i & 3, 0, nullptr)
            : kind == 1 ? launch_overrelax_phase(ctx, i & 3)

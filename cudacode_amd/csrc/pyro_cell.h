@@ -91,6 +91,36 @@ __device__ __forceinline__ double wave_sum_d(double v) {
   return v;
 }
 
+// One site's Metropolis step from its local-field sums (fx: FP32 sum in link
+// order, as the move uses it; gx: FP64 sum, for the energy books): Philox
+// draw keyed by (cell, site, pass, replica, sweep), cone proposal, test, and
+// optionally the rotation about the field. Returns the proposal (or the
+// rotated spin); wr tells whether the site is to be written, de gets the FP64
+// energy change of a written move, nacc counts the accepted ones.
+template <bool MAT>
+__device__ __forceinline__ f3 metropolis_site(f3 s, float fx, float fy, float fz, double gx,
+                                              double gy, double gz, int cell, int site, int colour,
+                                              int r, int pass, float beta_r, float cosc_r,
+                                              uint32_t sweep, uint32_t key0, uint32_t key1,
+                                              float jeff, const float *__restrict__ coup,
+                                              bool post_rotate, bool &wr, double &de,
+                                              unsigned &nacc) {
+  const u32x4 w = philox4x32_10(
+      rng_counter((uint32_t)cell, (uint32_t)site, (uint32_t)pass, kKindMetropolis, (uint32_t)r,
+                  sweep),
+      key0, key1);
+  const f3 h = field_h<MAT>(jeff, fx, fy, fz);
+  f3 sp = cone_proposal(s, cosc_r, u01(w.x), u01(w.y));
+  wr = metropolis_accept(h, s, sp, beta_r, u01(w.z));
+  nacc += wr ? 1u : 0u;
+  if (post_rotate) {  // uniform over the block: a kernel argument
+    sp = rotate_about(h, wr ? sp : s, u01(w.w));
+    wr = true;
+  }
+  if (wr) de += field_de<MAT>(jeff, coup, colour, s, sp, gx, gy, gz);
+  return sp;
+}
+
 // KIND 0: Metropolis with the cone proposal (MCMultipleTemperaturesPerSystem,
 // Pyrochlore.cu:829-885). KIND 1: over-relaxation by pi (OverRelaxationByPiStep,
 // Pyrochlore.cu:975-1007). TRACK: FP64 energy change also for KIND 1.
@@ -137,25 +167,15 @@ __device__ __forceinline__ void cell_update(float *__restrict__ Sx, int cell, in
   for (int k = 0; k < 4; ++k) {
     const int site = COLOUR + 4 * k;
     if (KIND == 0) {
-      const u32x4 w = philox4x32_10(
-          rng_counter((uint32_t)cell, (uint32_t)site, (uint32_t)pass, kKindMetropolis, (uint32_t)r,
-                      sweep),
-          key0, key1);
-      const f3 h = field_h<MAT>(jeff, fx[k], fy[k], fz[k]);
-      sp[k] = cone_proposal(s[k], cosc_r, u01(w.x), u01(w.y));
-      wr[k] = metropolis_accept(h, s[k], sp[k], beta_r, u01(w.z));
-      nacc += wr[k] ? 1u : 0u;
-      if (post_rotate) {  // uniform over the block: a kernel argument
-        sp[k] = rotate_about(h, wr[k] ? sp[k] : s[k], u01(w.w));
-        wr[k] = true;
-      }
+      sp[k] = metropolis_site<MAT>(s[k], fx[k], fy[k], fz[k], gx[k], gy[k], gz[k], cell, site,
+                                   COLOUR, r, pass, beta_r, cosc_r, sweep, key0, key1, jeff, coup,
+                                   post_rotate, wr[k], de, nacc);
     } else {
       const f3 f = {fx[k], fy[k], fz[k]};
       sp[k] = overrelax(f, s[k]);
       wr[k] = true;
+      if (TRACK) de += field_de<MAT>(jeff, coup, COLOUR, s[k], sp[k], gx[k], gy[k], gz[k]);
     }
-    if ((KIND == 0 || TRACK) && wr[k])
-      de += field_de<MAT>(jeff, coup, COLOUR, s[k], sp[k], gx[k], gy[k], gz[k]);
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) {

@@ -229,6 +229,10 @@ int launch_overrelax_phase(pyro_ctx *ctx, int colour);
 // launch between the spin buffers), and one fused pass: parity 0 reads
 // d_spins and writes d_spins_alt, parity 1 the reverse.
 bool or_fusable(const pyro_ctx *ctx);
+// Metropolis colour pairs (0,1), (2,3) in one launch each, in place: L = 16,
+// scalar model, cell kernels, no post-Metropolis rotation, fusion on.
+bool metro_fusable(const pyro_ctx *ctx);
+int launch_metropolis_pair(pyro_ctx *ctx, int ca, int pass);
 int launch_overrelax_pass_fused(pyro_ctx *ctx, int parity);
 int launch_overrelax_pair(pyro_ctx *ctx, int ca, const float *cur, float *oth);
 // Fills d_obs; with anchor, also re-anchors the incremental energy (anchor = E, dE = 0).

@@ -295,6 +295,21 @@ bool or_fusable(const pyro_ctx *ctx) {
   }
 }
 
+bool metro_fusable(const pyro_ctx *ctx) {
+  return ctx->fuse_or && !ctx->hetero && !ctx->mat && ctx->exec_mode == 0 &&
+         ctx->phase_variant == 1 && !ctx->post_rotate && ctx->L == 16;
+}
+
+int launch_metropolis_pair(pyro_ctx *ctx, int ca, int pass) {
+  if (!metro_fusable(ctx)) return set_error(PYRO_ESTATE, "fused Metropolis pair: not for this ctx");
+  if (ca == 0)
+    launch_metro_pair_t<0>(ctx, pass);
+  else
+    launch_metro_pair_t<2>(ctx, pass);
+  PYRO_HIP(hipGetLastError());
+  return 0;
+}
+
 int launch_overrelax_pair(pyro_ctx *ctx, int ca, const float *cur, float *oth) {
   if (!launch_or_pair(ctx, ca, cur, oth))
     return set_error(PYRO_ESTATE, "fused over-relaxation: no compiled copy for L=%d", ctx->L);

@@ -501,6 +501,192 @@ __global__ __launch_bounds__(256, or_plane_blocks(MODE)) void k_or_pair_plane(
 // VALU per wave, 94 VGPRs, 5 blocks per CU) 12.4 us (6.1 TB/s), 0.308 ms.
 constexpr int kOrPlaneMode = 3;
 
+// Two Metropolis colour phases (CA, CA + 1) at L = 16 in one launch, in
+// place, scalar model: one z-plane of a replica per block, staged as
+// k_or_pair_plane<CA, 3> stages it (the same 29 values a cell, one buffer),
+// then phase CA of the cell's 4 sites, the exchange of the foreign colour-CA
+// values between lanes, phase CA + 1, and the stores of the accepted moves.
+// Each site's step is metropolis_site with the Philox counters and field sums
+// of the one-colour kernel, so spins and acceptance counts equal two
+// one-colour launches bit for bit (the FP64 energy changes are summed per
+// block over both colours, a different order of the same terms). In place is
+// race-free: a block reads its own plane before its first write, and the
+// other planes it reads (z - 1 for colours 0/1, z + 1 for 2/3) only at sites
+// outside the pair, which no block of the launch writes. 36 instead of 60 B
+// per update. 2 waves per SIMD (171-190 VGPRs): at 3 (168) it spills, and
+// measured at L = 16 x 64 the pair launch takes 24.4-24.6 us against
+// 25.1 us at 3 and 2 x 14.1-14.4 us for two one-colour launches; sweep
+// 0.286-0.288 ms against 0.297 and 0.300 ms.
+template <int CA>
+__global__ __launch_bounds__(256, 2) void k_metro_pair_plane(
+    float *__restrict__ spins, int pass, const float *__restrict__ beta,
+    const float *__restrict__ cosc, const uint32_t *__restrict__ ctl, uint32_t key0,
+    uint32_t key1, float jeff, unsigned long long *__restrict__ acc, double *__restrict__ dE,
+    int r0, uint32_t sweep_delta) {
+  constexpr int LT = 16, NC = LT * LT * LT, P = 16 * NC, CB = CA + 1;
+  constexpr PairPlanes<CA, 3> pp{};
+  constexpr PairNeeds<CA, false> nd{};
+  constexpr int NS = pp.n, NN = nd.n, ROWS = 3 * pp.n_lds;
+  constexpr int NX = kPairExchange<CA>.n;
+  static_assert(NS == 20 && ROWS <= 72 && nd.nf == 3, "20 site-planes, 3 foreign neighbours");
+  __shared__ float4 sm[ROWS * 64];
+  __shared__ float xs[(NX > 0 ? NX : 1) * 3 * 256];
+  const int nb = gridDim.x;
+  const int b = blockIdx.x;
+  const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+  const int r = r0 + (lb >> 4);  // 16 planes per replica
+  const int z = lb & 15;
+  float *rs = spins + (size_t)r * 48 * NC;
+  const int t = threadIdx.x, x = t & 15, y = t >> 4;
+  const int lane = t & 63;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int cell = z * 256 + t;
+  float ox[NS], oy[NS], oz[NS];
+#pragma unroll
+  for (int sl = 0; sl < NS; ++sl) {
+    if (pp.kind[sl] == kPlaneLds) continue;
+    const uint32_t j = 4u * (uint32_t)(pp.site[sl] * NC + ((z + pp.dz[sl]) & 15) * 256 + t);
+    ox[sl] = ld_at(rs, j);
+    oy[sl] = ld_at(rs + P, j);
+    oz[sl] = ld_at(rs + 2 * P, j);
+  }
+  constexpr int NI = (ROWS + 3) / 4;
+  float4 st[NI];
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {  // every load before the first LDS write
+    const uint32_t e = kPairRows<CA, 3>[min(4 * i + w, ROWS - 1)];
+    const int zz = (z + (int)((e >> 20) & 3) - 1) & 15;
+    const uint32_t off = 4u * ((e & 0xFFFFFu) + (uint32_t)zz * 256u) + 16u * (uint32_t)lane;
+    st[i] = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(rs) + off);
+  }
+#pragma unroll
+  for (int i = 0; i < NI; ++i) sm[min(4 * i + w, ROWS - 1) * 64 + lane] = st[i];
+  __syncthreads();
+  const float *sf = reinterpret_cast<const float *>(sm);
+  const int X[3] = {(x + 15) & 15, x, (x + 1) & 15};
+  const int Y[3] = {((y + 15) & 15) * 16, y * 16, ((y + 1) & 15) * 16};
+  float vx[NN], vy[NN], vz[NN];
+#pragma unroll
+  for (int i = 0; i < NN; ++i) {
+    const PairNeed qn = nd.v[i];
+    const int sl = pp.slot[qn.site][qn.dz + 1];
+    if (pp.kind[sl] == kPlaneLds) {
+      const int at = Y[qn.dy + 1] + X[qn.dx + 1];
+      const int l = pp.lds[sl];
+      vx[i] = sf[(3 * l + 0) * 256 + at];
+      vy[i] = sf[(3 * l + 1) * 256 + at];
+      vz[i] = sf[(3 * l + 2) * 256 + at];
+    } else if (qn.dx == 0) {
+      vx[i] = ox[sl];
+      vy[i] = oy[sl];
+      vz[i] = oz[sl];
+    } else if (qn.dx < 0) {
+      vx[i] = row_neighbour<-1>(ox[sl]);
+      vy[i] = row_neighbour<-1>(oy[sl]);
+      vz[i] = row_neighbour<-1>(oz[sl]);
+    } else {
+      vx[i] = row_neighbour<1>(ox[sl]);
+      vy[i] = row_neighbour<1>(oy[sl]);
+      vz[i] = row_neighbour<1>(oz[sl]);
+    }
+  }
+  const float bt = beta[r], cc = cosc[r];
+  const uint32_t sweep = ctl[kCtlSweep] + sweep_delta;
+  double de = 0.0;
+  unsigned nacc = 0;
+  // Phase CA: na[k] = the new (or kept) spin of site CA + 4k.
+  f3 na[4];
+  bool wa[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    float fx, fy, fz;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    field_start<false>(nullptr, CA, fx, fy, fz);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      const int i = nd.a_nb[k][m];
+      field_add<false, true>(nullptr, CA, kLinks[CA + 4 * k][m].site & 3, vx[i], vy[i], vz[i], fx,
+                             fy, fz, gx, gy, gz);
+    }
+    const int is = nd.a_self[k];
+    const f3 s = {vx[is], vy[is], vz[is]};
+    const f3 sp = metropolis_site<false>(s, fx, fy, fz, gx, gy, gz, cell, CA + 4 * k, CA, r, pass,
+                                         bt, cc, sweep, key0, key1, jeff, nullptr, false, wa[k], de,
+                                         nacc);
+    na[k] = {wa[k] ? sp.x : s.x, wa[k] ? sp.y : s.y, wa[k] ? sp.z : s.z};  // component-wise: no stack copy
+  }
+  // The foreign colour-CA values from their owners (as k_or_pair_plane<CA, 3>).
+#pragma unroll
+  for (int e = 0; e < NX; ++e) {
+    const f3 v = na[kPairExchange<CA>.k[e]];
+    xs[(3 * e + 0) * 256 + t] = v.x;
+    xs[(3 * e + 1) * 256 + t] = v.y;
+    xs[(3 * e + 2) * 256 + t] = v.z;
+  }
+  __syncthreads();
+  f3 fo[3];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    const PairNeed q = nd.fv[f];
+    const f3 own = na[q.site >> 2];
+    if (q.dy == 0 && q.dx < 0) {
+      fo[f] = {row_neighbour<-1>(own.x), row_neighbour<-1>(own.y), row_neighbour<-1>(own.z)};
+    } else if (q.dy == 0) {
+      fo[f] = {row_neighbour<1>(own.x), row_neighbour<1>(own.y), row_neighbour<1>(own.z)};
+    } else {
+      const int e = kPairExchange<CA>.slot[q.site >> 2];
+      const int at = Y[q.dy + 1] + X[q.dx + 1];
+      fo[f] = {xs[(3 * e + 0) * 256 + at], xs[(3 * e + 1) * 256 + at], xs[(3 * e + 2) * 256 + at]};
+    }
+  }
+  // Phase CA + 1 from the old values, na and fo.
+  f3 nbv[4];
+  bool wb[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int tsite = CB + 4 * k;
+    float fx, fy, fz;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    field_start<false>(nullptr, CB, fx, fy, fz);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      const Link lk = kLinks[tsite][m];
+      f3 v;
+      if ((lk.site & 3) == CA) {
+        v = nd.f_idx[k][m] < 0 ? na[lk.site >> 2] : fo[nd.f_idx[k][m]];
+      } else {
+        const int i = nd.t_nb[k][m];
+        v = {vx[i], vy[i], vz[i]};
+      }
+      field_add<false, true>(nullptr, CB, lk.site & 3, v.x, v.y, v.z, fx, fy, fz, gx, gy, gz);
+    }
+    const int is = nd.t_self[k];
+    const f3 s = {vx[is], vy[is], vz[is]};
+    nbv[k] = metropolis_site<false>(s, fx, fy, fz, gx, gy, gz, cell, tsite, CB, r, pass, bt, cc,
+                                    sweep, key0, key1, jeff, nullptr, false, wb[k], de, nacc);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t ia = 4u * (uint32_t)((CA + 4 * k) * NC + cell);
+    const uint32_t ib = 4u * (uint32_t)((CB + 4 * k) * NC + cell);
+    if (wa[k]) {
+      st_at(rs, ia, na[k].x);
+      st_at(rs + P, ia, na[k].y);
+      st_at(rs + 2 * P, ia, na[k].z);
+    }
+    if (wb[k]) {
+      st_at(rs, ib, nbv[k].x);
+      st_at(rs + P, ib, nbv[k].y);
+      st_at(rs + 2 * P, ib, nbv[k].z);
+    }
+  }
+  block_sum<256>(de, nacc);
+  if (threadIdx.x == 0) {
+    if (de != 0.0) atomicAdd(&dE[r], de);
+    if (nacc) atomicAdd(&acc[r], (unsigned long long)nacc);
+  }
+}
+
 // Per-block FP64 partial sums of the energy and the sublattice-b magnetisation
 // (TotalEnergyGPU, Pyrochlore.cu:344-389; MagnetizationGPU, :277-342;
 // Measurement, :526-578). grid = (obs_blocks, 4, R): blockIdx.y is the basis
@@ -707,6 +893,23 @@ static void launch_or_pair_l(pyro_ctx *ctx, int ca, const float *cur, float *oth
     if (ca == 0) launch_or_pair_t<0, LT, false>(ctx, cur, oth);
     else launch_or_pair_t<2, LT, false>(ctx, cur, oth);
   }
+}
+
+// The fused Metropolis colour pair ca (0 or 2) at L = 16, in place on d_spins.
+template <int CA>
+static void launch_metro_pair_t(pyro_ctx *ctx, int pass) {
+  const int nrep = launch_nrep(ctx);
+  if (ctx->time_ev0) {
+    hipExtLaunchKernelGGL(k_metro_pair_plane<CA>, dim3(16 * nrep), dim3(256), 0,
+                          launch_stream(ctx), ctx->time_ev0, ctx->time_ev1, 0, ctx->d_spins, pass,
+                          (const float *)ctx->d_beta, (const float *)ctx->d_cosc,
+                          (const uint32_t *)ctx->d_ctl, key_lo(ctx), key_hi(ctx), ctx->jeff,
+                          ctx->d_acc, ctx->d_dE, launch_r0(ctx), ctx->launch_delta);
+    return;
+  }
+  k_metro_pair_plane<CA><<<16 * nrep, 256, 0, launch_stream(ctx)>>>(
+      ctx->d_spins, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl, key_lo(ctx), key_hi(ctx),
+      ctx->jeff, ctx->d_acc, ctx->d_dE, launch_r0(ctx), ctx->launch_delta);
 }
 
 // The fused over-relaxation colour pair ca (0 or 2) from cur into oth, for

@@ -617,7 +617,7 @@ class Pyrochlore:
     def time_phase(self, kind: int, n_launches: int):
         """(mean ms per launch, first-to-last ms) of n back-to-back phase launches
         (kind 0 Metropolis, 1 over-relaxation, 2 learned, 3 fused over-relaxation
-        pair), timed with HIP events on the ctx stream."""
+        pair, 4 fused Metropolis pair), timed with HIP events on the ctx stream."""
         avg, span = C.c_double(), C.c_double()
         _check(lib().pyro_time_phase(self._ctx, kind, n_launches, C.byref(avg), C.byref(span)))
         return avg.value, span.value

@@ -285,7 +285,10 @@ int pyro_get_streams(pyro_ctx *ctx, int *n);
  * spins from one buffer and writes the two updated colours to a second one
  * (allocated with the ctx), recomputing the neighbouring first-colour values
  * that the second colour needs, so a pass moves 36 N instead of 60 N bytes;
- * results are identical (no reference counterpart: a performance setting). */
+ * results are identical (no reference counterpart: a performance setting).
+ * The same switch fuses the Metropolis phases two colours per launch at
+ * L = 16 (scalar model, no post-Metropolis rotation), in place: identical
+ * spins and acceptance counts, energies to FP64 rounding. */
 int pyro_set_or_fusion(pyro_ctx *ctx, int on);
 
 /* Spin guard (no reference counterpart; the reference checks nothing): counts
@@ -371,11 +374,13 @@ int pyro_import_configurations(pyro_ctx *ctx, const char *prefix, int rep, int b
 /* Time n_launches back-to-back colour-phase launches of one kind (0 =
  * Metropolis, 1 = over-relaxation, 2 = learned proposal, which needs weights
  * set; colours cycle 0..3; 3 = fused over-relaxation pairs (0,1), (2,3)
- * alternating) on the ctx stream,
+ * alternating; 4 = fused Metropolis pairs (0,1), (2,3) alternating, where
+ * the sweeps use them) on the ctx stream,
  * each launch carrying its own start/stop HIP events in its dispatch
  * (hipExtLaunchKernelGGL). avg_ms: mean kernel time per launch; span_ms:
- * first start to last stop. Kinds 0-2 run for real (the state advances);
- * kind 3 writes only the second buffer and leaves the state as it was. The reference's counterpart is the ENABLE_TIMING cudaEvent block
+ * first start to last stop. Kinds 0-2 and 4 run for real (the state
+ * advances); kind 3 writes only the second buffer and leaves the state as it
+ * was. The reference's counterpart is the ENABLE_TIMING cudaEvent block
  * around Mc (MonteCarlo.cu:36-63). */
 int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, double *span_ms);
 

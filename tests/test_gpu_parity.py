@@ -525,3 +525,40 @@ def test_fused_overrelaxation_identical(L, n_or, track, streams, graph):
     o = orc(L, temps, **kw)
     o.run(20)
     assert np.array_equal(spins, o.spins)
+
+
+@pytest.mark.parametrize("streams,graph", [(1, True), (2, True), (2, False)])
+def test_fused_metropolis_identical(streams, graph):
+    """Metropolis phases fused two colours per launch in place (L=16, one
+    z-plane per block, the neighbours' first-colour values exchanged between
+    lanes) equal one colour per launch bit for bit: spins, rungs, acceptance
+    and counters over sweeps of 2 Metropolis + 1 over-relaxation passes with
+    exchanges; incremental energies to FP64 rounding (the block sums run over
+    both colours). Timing launches of the pair (kind 4) advance the state as
+    four one-colour launches do; with the post-Metropolis rotation on, the
+    sweeps fall back to one colour per launch."""
+    R = 6
+    temps = list(np.geomspace(0.05, 2.0, R))
+    kw = dict(n_metro=2, n_or=1, pt_every=2, cos_cone=0.4)
+    with gpu(16, temps, **kw) as a, gpu(16, temps, **kw) as b:
+        b.set_or_fusion(False)
+        for g in (a, b):
+            g.set_streams(streams)
+            g.Mc(9, use_graph=graph)
+        assert np.array_equal(a.get_spins(), b.get_spins())
+        assert np.array_equal(a.temperatures()[1], b.temperatures()[1])
+        assert np.array_equal(a.acceptance()[0], b.acceptance()[0])
+        assert a.acceptance()[0].sum() > 0
+        assert a.sweep_counters() == b.sweep_counters()
+        assert np.allclose(a.energy_incremental(), b.energy_incremental(), rtol=1e-11, atol=0)
+        assert np.allclose(a.energy_incremental(), a.TotalEnergyv2(), rtol=1e-10, atol=1e-8)
+        a.time_phase(4, 2)
+        b.time_phase(0, 4)
+        assert np.array_equal(a.get_spins(), b.get_spins())
+        assert np.array_equal(a.acceptance()[0], b.acceptance()[0])
+        for g in (a, b):
+            g.set_post_rotate(True)
+            g.Mc(3, use_graph=graph)
+        assert np.array_equal(a.get_spins(), b.get_spins())
+        with pytest.raises(RuntimeError):
+            a.time_phase(4, 2)
