@@ -20,6 +20,8 @@ namespace pyro {
 int launch_phase_mat(pyro_ctx *ctx, int kind, int colour, int pass) {
   if (kind == 0)
     launch_phase<0, true>(ctx, colour, pass);
+  else if (kind == 2)
+    launch_phase<2, true>(ctx, colour, pass);
   else
     launch_phase<1, true>(ctx, colour, pass);
   PYRO_HIP(hipGetLastError());

@@ -85,7 +85,7 @@ static int enqueue_phases(pyro_ctx *ctx, bool record = false) {
   for (; o < ctx->n_or; ++o)
     for (int c = 0; c < kColours; ++c) {
       ctx->launch_series = record && o == 0 && c == 0 ? ctx->d_series : nullptr;
-      const int rc = launch_overrelax_phase(ctx, c);
+      const int rc = launch_overrelax_phase(ctx, c, o);
       ctx->launch_series = nullptr;
       if (rc) return rc;
     }
@@ -832,6 +832,8 @@ int pyro_set_exec_mode(pyro_ctx *ctx, int mode, int block, int n_bins) {
     return set_error(PYRO_ESTATE, "exec mode %d: exchange matrices run in execution mode 0 only", mode);
   if (mode != 0 && ctx->post_rotate)
     return set_error(PYRO_ESTATE, "exec mode %d: the post-Metropolis rotation runs in execution mode 0 only", mode);
+  if (mode != 0 && (ctx->set_schedule || ctx->or_angle))
+    return set_error(PYRO_ESTATE, "exec mode %d: the update schedule options run in execution mode 0 only", mode);
   if (mode != 0 && !batch_size_supported(ctx->hetero ? 4 : ctx->L))
     return set_error(PYRO_EINVAL, "exec mode %d needs L in {4,8,12,16,24,32}", mode);
   if (mode != 0 && ctx->n_learned > 0)
@@ -877,6 +879,8 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
   if (ctx->mat) return set_error(PYRO_ESTATE, "pyro_autotune: exchange matrices run in execution mode 0 only");
   if (ctx->post_rotate)
     return set_error(PYRO_ESTATE, "pyro_autotune: the post-Metropolis rotation runs in execution mode 0 only");
+  if (ctx->set_schedule || ctx->or_angle)
+    return set_error(PYRO_ESTATE, "pyro_autotune: the update schedule options run in execution mode 0 only");
   PYRO_TRY(device_guard(ctx));
   struct Cand { int mode, block, variant, streams; };
   std::vector<Cand> cands;
@@ -1133,6 +1137,18 @@ int pyro_set_post_rotate(pyro_ctx *ctx, int on) {
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   drop_graph(ctx);  // the captured sweeps carry the old flag
   ctx->post_rotate = on != 0;
+  return 0;
+}
+
+int pyro_set_update_schedule(pyro_ctx *ctx, int independent_sets, int or_random_angle) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if ((independent_sets || or_random_angle) && (ctx->hetero || ctx->exec_mode != 0))
+    return set_error(PYRO_ESTATE, "pyro_set_update_schedule: execution mode 0 on a uniform lattice only");
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  drop_graph(ctx);  // the captured sweeps carry the old schedule
+  ctx->set_schedule = independent_sets != 0;
+  ctx->or_angle = or_random_angle != 0;
   return 0;
 }
 

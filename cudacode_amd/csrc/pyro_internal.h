@@ -157,6 +157,11 @@ struct pyro_ctx {
   double spin_len2 = 1.0;
   bool mat = false;
   bool post_rotate = false;  // rotate about h after each Metropolis test
+  // Update schedule: set_schedule draws each phase's independent set from the
+  // 24 (else colour classes); or_angle makes over-relaxation a rotation about
+  // h by a random angle (else by pi).
+  bool set_schedule = false;
+  bool or_angle = false;
 
   // Sweeps split into n_parts replica groups, each on its own stream (one
   // group's kernel ramps up while another's drains): group p holds replicas
@@ -224,7 +229,7 @@ int set_error(int code, const char *fmt, ...);
 // (launch_stream / launch_r0 / launch_nrep) while a split sweep sets one.
 int launch_init_spins(pyro_ctx *ctx);
 int launch_metropolis_phase(pyro_ctx *ctx, int colour, int pass, const uint32_t *sweep_override);
-int launch_overrelax_phase(pyro_ctx *ctx, int colour);
+int launch_overrelax_phase(pyro_ctx *ctx, int colour, int pass = 0);
 // Whether the over-relaxation passes of a sweep run fused (two colours per
 // launch between the spin buffers), and one fused pass: parity 0 reads
 // d_spins and writes d_spins_alt, parity 1 the reverse.
@@ -243,7 +248,8 @@ int launch_pt_swap(pyro_ctx *ctx, int parity, uint32_t round, int from_ctl, uint
 int launch_end_sweep(pyro_ctx *ctx, uint32_t n, int record_last);  // series row, sweep counter
 int launch_record_part(pyro_ctx *ctx);           // series entries of the launch range
 int launch_measure_row(pyro_ctx *ctx);           // append a measurement row from d_obs
-int launch_phase_mat(pyro_ctx *ctx, int kind, int colour, int pass);  // matrix field model
+// Matrix field model; kind 0 Metropolis, 1 over-relaxation by pi, 2 by a random angle.
+int launch_phase_mat(pyro_ctx *ctx, int kind, int colour, int pass);
 int reset_model_state(pyro_ctx *ctx);  // drop captured sweeps, re-anchor the energy
 int check_spins(pyro_ctx *ctx, double tol, unsigned long long *d_out);  // d_out: {count, first}
 int launch_observables_partials_mat(pyro_ctx *ctx);

@@ -287,7 +287,7 @@ int launch_metropolis_phase(pyro_ctx *ctx, int colour, int pass, const uint32_t 
 
 bool or_fusable(const pyro_ctx *ctx) {
   if (!ctx->fuse_or || !ctx->d_spins_alt || ctx->hetero || ctx->mat || ctx->exec_mode != 0 ||
-      ctx->phase_variant != 1)
+      ctx->phase_variant != 1 || ctx->set_schedule || ctx->or_angle)
     return false;
   switch (ctx->L) {
     case 4: case 8: case 12: case 16: case 24: case 32: case 64: return true;
@@ -297,7 +297,7 @@ bool or_fusable(const pyro_ctx *ctx) {
 
 bool metro_fusable(const pyro_ctx *ctx) {
   return ctx->fuse_or && !ctx->hetero && !ctx->mat && ctx->exec_mode == 0 &&
-         ctx->phase_variant == 1 && !ctx->post_rotate && ctx->L == 16;
+         ctx->phase_variant == 1 && !ctx->post_rotate && !ctx->set_schedule && ctx->L == 16;
 }
 
 int launch_metropolis_pair(pyro_ctx *ctx, int ca, int pass) {
@@ -326,10 +326,13 @@ int launch_overrelax_pass_fused(pyro_ctx *ctx, int parity) {
   return 0;
 }
 
-int launch_overrelax_phase(pyro_ctx *ctx, int colour) {
+int launch_overrelax_phase(pyro_ctx *ctx, int colour, int pass) {
   if (ctx->hetero) return set_error(PYRO_ESTATE, "single colour phases need a uniform lattice size");
-  if (ctx->mat) return launch_phase_mat(ctx, 1, colour, 0);
-  launch_phase<1, false>(ctx, colour, 0);
+  if (ctx->mat) return launch_phase_mat(ctx, ctx->or_angle ? 2 : 1, colour, pass);
+  if (ctx->or_angle)
+    launch_phase<2, false>(ctx, colour, pass);
+  else
+    launch_phase<1, false>(ctx, colour, pass);
   PYRO_HIP(hipGetLastError());
   return 0;
 }

@@ -115,6 +115,8 @@ enum : uint32_t {
   kKindSwap = 3u,
   kKindLearned = 4u,
   kKindLearnedInit = 5u,
+  kKindOrAngle = 6u,  // angle of the random-angle over-relaxation
+  kKindSets = 7u,     // independent set of a phase under the drawn-set schedule
 };
 PYRO_HD u32x4 rng_counter(uint32_t cell, uint32_t site, uint32_t pass, uint32_t kind,
                           uint32_t replica, uint32_t sweep) {

@@ -27,6 +27,7 @@ static inline uint32_t key_lo(const pyro_ctx *c) { return (uint32_t)c->seed; }
 static inline uint32_t key_hi(const pyro_ctx *c) { return (uint32_t)(c->seed >> 32); }
 
 static __constant__ Link c_links[16][6] = PYRO_LINKS16_INIT;
+static __constant__ int8_t c_sets[24][4] = PYRO_INDEPENDENT_SETS_INIT;
 
 __device__ __forceinline__ int wrap(int v, int L) {
   return v < 0 ? v + L : (v >= L ? v - L : v);
@@ -38,26 +39,46 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// One colour phase on every replica. KIND 0: Metropolis with the cone
-// proposal (MCMultipleTemperaturesPerSystem, Pyrochlore.cu:829-885).
-// KIND 1: over-relaxation by pi (OverRelaxationByPiStep,
-// Pyrochlore.cu:975-1007). grid = (ceil(ncell/64), R), block = 256.
-// LT > 0 fixes the lattice size at compile time (index arithmetic by
-// constants); LT = 0 reads it from L. MAT: the matrix field model (coup).
+// One phase on every replica. KIND 0: Metropolis with the cone proposal
+// (MCMultipleTemperaturesPerSystem, Pyrochlore.cu:829-885). KIND 1:
+// over-relaxation by pi (OverRelaxationByPiStep, Pyrochlore.cu:975-1007).
+// KIND 2: rotation about the local field by a uniform random angle
+// (OverRelaxationStep, Pyrochlore.cu:1009-1045). grid = (ceil(ncell/64), R),
+// block = 256. LT > 0 fixes the lattice size at compile time (index
+// arithmetic by constants); LT = 0 reads it from L. MAT: the matrix field
+// model (coup).
+//
+// Phase `colour` updates the colour class {c, c+4, c+8, c+12}; with
+// set_mode it updates one of the 24 independent sets instead, drawn from
+// Philox per (sweep, stage, pass, phase) and the same for every replica and
+// block (the reference's IndependentSubLattices[Update[0] % 24],
+// Pyrochlore.cu:848). A site may then be updated by two phases of one pass,
+// so the move draws of a drawn set carry the phase in their site field.
+// `pass` numbers the Metropolis or over-relaxation pass within the sweep.
 template <int KIND, int LT, bool MAT>
 __global__ __launch_bounds__(kPhaseBlock) void k_phase(
     float *__restrict__ spins, int L_rt, int ncell_rt, int colour, int pass,
     const float *__restrict__ beta, const float *__restrict__ cosc,
     const uint32_t *__restrict__ ctl, uint32_t key0, uint32_t key1, float jeff,
     unsigned long long *__restrict__ acc, double *__restrict__ dE, const float *__restrict__ coup,
-    int post_rotate, int r0, uint32_t sweep_delta) {
+    int post_rotate, int r0, uint32_t sweep_delta, int set_mode) {
   const int L = LT > 0 ? LT : L_rt;
   const int ncell = LT > 0 ? LT * LT * LT : ncell_rt;
   const int r = r0 + (int)blockIdx.y;
   const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int cell = blockIdx.x * kPhaseCells + lane;
-  const int site = colour + 4 * k;
+  const uint32_t sweep = ctl[kCtlSweep] + sweep_delta;
+  int site = colour + 4 * k;
+  uint32_t tag = (uint32_t)site;  // site field of this update's draws
+  if (set_mode) {
+    const u32x4 q = philox4x32_10(
+        rng_counter((uint32_t)colour, KIND == 0 ? 0u : 1u, (uint32_t)pass, kKindSets, 0u, sweep),
+        key0, key1);
+    site = __builtin_amdgcn_readfirstlane((int)c_sets[q.x % 24u][k]);
+    tag = (uint32_t)site | ((uint32_t)colour << 4);
+  }
+  const int b = site & 3;  // basis site (the matrix model's couplings)
   const size_t plane = (size_t)16 * ncell;
   float *Sx = spins + (size_t)r * 3 * plane;
   float *Sy = Sx + plane;
@@ -72,7 +93,7 @@ __global__ __launch_bounds__(kPhaseBlock) void k_phase(
     const int y = rem / L;
     const int x = rem - y * L;
     float fx, fy, fz;
-    field_start<MAT>(coup, colour, fx, fy, fz);
+    field_start<MAT>(coup, b, fx, fy, fz);
     double gx = 0.0, gy = 0.0, gz = 0.0;
 #pragma unroll
     for (int n = 0; n < 6; ++n) {
@@ -81,8 +102,8 @@ __global__ __launch_bounds__(kPhaseBlock) void k_phase(
       const int y1 = wrap(y + lk.dy, L);
       const int z1 = wrap(z + lk.dz, L);
       const int j = lk.site * ncell + (z1 * L + y1) * L + x1;
-      const float a = Sx[j], b = Sy[j], c = Sz[j];
-      field_add<MAT, true>(coup, colour, lk.site & 3, a, b, c, fx, fy, fz, gx, gy, gz);
+      const float sa = Sx[j], sb = Sy[j], sc = Sz[j];
+      field_add<MAT, true>(coup, b, lk.site & 3, sa, sb, sc, fx, fy, fz, gx, gy, gz);
     }
     const int i = site * ncell + cell;
     const f3 s = {Sx[i], Sy[i], Sz[i]};
@@ -90,8 +111,7 @@ __global__ __launch_bounds__(kPhaseBlock) void k_phase(
     bool write;
     if (KIND == 0) {
       const u32x4 w = philox4x32_10(
-          rng_counter((uint32_t)cell, (uint32_t)site, (uint32_t)pass, kKindMetropolis,
-                      (uint32_t)r, ctl[kCtlSweep] + sweep_delta),
+          rng_counter((uint32_t)cell, tag, (uint32_t)pass, kKindMetropolis, (uint32_t)r, sweep),
           key0, key1);
       const f3 h = field_h<MAT>(jeff, fx, fy, fz);
       sp = cone_proposal(s, cosc[r], u01(w.x), u01(w.y));
@@ -101,13 +121,19 @@ __global__ __launch_bounds__(kPhaseBlock) void k_phase(
         sp = rotate_about(h, write ? sp : s, u01(w.w));
         write = true;
       }
+    } else if (KIND == 2) {
+      const u32x4 w = philox4x32_10(
+          rng_counter((uint32_t)cell, tag, (uint32_t)pass, kKindOrAngle, (uint32_t)r, sweep),
+          key0, key1);
+      sp = rotate_about(field_h<MAT>(jeff, fx, fy, fz), s, u01(w.x));
+      write = true;
     } else {
       const f3 f = {fx, fy, fz};
       sp = overrelax(f, s);
       write = true;
     }
     if (write) {
-      de = field_de<MAT>(jeff, coup, colour, s, sp, gx, gy, gz);
+      de = field_de<MAT>(jeff, coup, b, s, sp, gx, gy, gz);
       Sx[i] = sp.x;
       Sy[i] = sp.y;
       Sz[i] = sp.z;
@@ -780,13 +806,13 @@ static void launch_phase_t(pyro_ctx *ctx, int colour, int pass) {
                           pass, (const float *)ctx->d_beta, (const float *)ctx->d_cosc,
                           (const uint32_t *)ctx->d_ctl, key_lo(ctx), key_hi(ctx), ctx->jeff,
                           ctx->d_acc, ctx->d_dE, (const float *)ctx->d_coup, (int)ctx->post_rotate,
-                          launch_r0(ctx), ctx->launch_delta);
+                          launch_r0(ctx), ctx->launch_delta, (int)ctx->set_schedule);
     return;
   }
   k_phase<KIND, LT, MAT><<<grid, kPhaseBlock, 0, launch_stream(ctx)>>>(
       ctx->d_spins, ctx->L, ctx->ncell, colour, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl,
       key_lo(ctx), key_hi(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE, ctx->d_coup,
-      (int)ctx->post_rotate, launch_r0(ctx), ctx->launch_delta);
+      (int)ctx->post_rotate, launch_r0(ctx), ctx->launch_delta, (int)ctx->set_schedule);
 }
 
 template <int KIND, int COLOUR, int LT, bool TRACK, bool MAT>
@@ -928,9 +954,14 @@ static inline bool launch_or_pair(pyro_ctx *ctx, int ca, const float *cur, float
 }
 
 // Lattice sizes of the benchmark configurations get a compile-time copy.
+// The cell-per-thread kernels cover the colour classes and the moves by pi;
+// the drawn-set schedule and the random-angle over-relaxation (KIND 2) run
+// in the wave-per-site kernel.
 template <int KIND, bool MAT>
 static void launch_phase(pyro_ctx *ctx, int colour, int pass) {
-  if (ctx->phase_variant == 1 && launch_cells<KIND, MAT>(ctx, colour, pass)) return;
+  if constexpr (KIND != 2)
+    if (ctx->phase_variant == 1 && !ctx->set_schedule && launch_cells<KIND, MAT>(ctx, colour, pass))
+      return;
   // The wave-per-site kernel always tracks the FP64 energy change.
   switch (ctx->L) {
     case 4: launch_phase_t<KIND, 4, MAT>(ctx, colour, pass); break;

@@ -34,7 +34,7 @@ EXPORTS = [
     "pyro_energy_incremental",
     "pyro_acceptance", "pyro_reset_acceptance", "pyro_adapt_cone", "pyro_record_start",
     "pyro_record_fetch", "pyro_set_couplings", "pyro_get_couplings", "pyro_exchange_matrices",
-    "pyro_zeeman_fields", "pyro_magnetization_g", "pyro_fold_magnetization_g", "pyro_set_streams", "pyro_get_streams", "pyro_set_or_fusion", "pyro_check_spins", "pyro_set_post_rotate", "pyro_anneal", "pyro_measure_start", "pyro_measure_stop", "pyro_measure_fetch",
+    "pyro_zeeman_fields", "pyro_magnetization_g", "pyro_fold_magnetization_g", "pyro_set_streams", "pyro_get_streams", "pyro_set_or_fusion", "pyro_check_spins", "pyro_set_post_rotate", "pyro_set_update_schedule", "pyro_anneal", "pyro_measure_start", "pyro_measure_stop", "pyro_measure_fetch",
     "pyro_set_passes", "pyro_set_kernel_options", "pyro_create_batch",
     "pyro_get_layout", "pyro_set_exec_mode", "pyro_get_exec_mode", "pyro_autotune",
     "pyro_checkpoint_save", "pyro_checkpoint_load", "pyro_export_configurations",
@@ -129,6 +129,7 @@ def lib():
         L.pyro_record_fetch.argtypes = [vp, vp, C.POINTER(C.c_int)]
         L.pyro_anneal.argtypes = [vp, C.c_double, C.c_int, C.c_int]
         L.pyro_set_post_rotate.argtypes = [vp, C.c_int]
+        L.pyro_set_update_schedule.argtypes = [vp, C.c_int, C.c_int]
         L.pyro_set_streams.argtypes = [vp, C.c_int]
         L.pyro_get_streams.argtypes = [vp, C.POINTER(C.c_int)]
         L.pyro_set_or_fusion.argtypes = [vp, C.c_int]
@@ -469,6 +470,15 @@ class Pyrochlore:
     def set_post_rotate(self, on: bool) -> None:
         """Rotate each spin about its field after the Metropolis test (Math.cu:168-197)."""
         _check(lib().pyro_set_post_rotate(self._ctx, 1 if on else 0))
+
+    def set_update_schedule(self, independent_sets: bool = False,
+                            or_random_angle: bool = False) -> None:
+        """Phases on independent sets drawn from the 24 (Pyrochlore.cu:848,
+        IndependentSubLattices[Update[0] % 24]) instead of the colour classes;
+        over-relaxation by a random angle about the field (OverRelaxationStep,
+        Pyrochlore.cu:1009-1045) instead of by pi."""
+        _check(lib().pyro_set_update_schedule(self._ctx, 1 if independent_sets else 0,
+                                              1 if or_random_angle else 0))
 
     def couplings(self):
         """(is_matrix, the 156 FP32 values the kernels use)."""

@@ -307,6 +307,23 @@ int pyro_check_spins(pyro_ctx *ctx, double tol, long long *n_bad, long long *fir
  * lattice only. Acceptance counts are those of the Metropolis test. */
 int pyro_set_post_rotate(pyro_ctx *ctx, int on);
 
+/* Update schedule of the reference's other kernels. independent_sets: each
+ * phase of a pass updates one of the 24 independent sets
+ * (pyro_independent_sets), drawn per (sweep, pass, phase) from the Philox
+ * stream and shared by every replica, as the reference launches
+ * MCMultipleTemperaturesPerSystem / OverRelaxationByPiStep on
+ * IndependentSubLattices[Update[0] % 24] (Pyrochlore.cu:848, :985, :1021;
+ * its Update buffer is never filled, Pyrochlore.cu:84). A set may repeat
+ * within a pass, so a pass need not touch every site; the Boltzmann law is
+ * kept. Default off: phase c updates {c, c+4, c+8, c+12}.
+ * or_random_angle: over-relaxation rotates each spin about its local field
+ * by a Philox-drawn angle uniform in [0, 2 pi) (OverRelaxationStep,
+ * Pyrochlore.cu:1009-1045) instead of by pi (OverRelaxationByPiStep,
+ * Pyrochlore.cu:975-1007, the default); both keep the energy. Either option
+ * runs the one-phase-per-launch kernels; execution mode 0 on a uniform
+ * lattice only. Learned passes keep the colour classes. */
+int pyro_set_update_schedule(pyro_ctx *ctx, int independent_sets, int or_random_angle);
+
 /* Kernel options (no reference counterpart; the reference's block shape is the
  * --BlockSpin option, Pyrochlore.cu:1152-1154). phase_variant 1 (default): one
  * thread per cell and the colour's 4 sites, for L in {4,8,12,16,24,32,64};

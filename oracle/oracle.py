@@ -62,6 +62,11 @@ def lib():
                               C.POINTER(C.c_int)]
         L.orc_set_update_order.argtypes = [C.c_void_p]
         L.orc_set_post_rotate.argtypes = [C.c_int]
+        L.orc_set_update_schedule.argtypes = [C.c_int, C.c_int]
+        L.orc_drawn_set.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_uint64]
+        L.orc_drawn_set.restype = C.c_int
+        L.orc_overrelax_phase_ex.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int,
+                                             C.c_float, C.c_uint64, C.c_void_p]
         L.orc_rotate_about.argtypes = [_f32p, _f32p, C.c_float, _f32p]
         L.orc_set_couplings.argtypes = [C.c_void_p]
         L.orc_magnetization_g.argtypes = [_f64p, C.c_double, C.c_double, _f64p]
@@ -226,6 +231,22 @@ def set_update_order(perm) -> None:
 
 def set_post_rotate(on: bool) -> None:
     lib().orc_set_post_rotate(1 if on else 0)
+
+
+def set_update_schedule(independent_sets: bool = False, or_random_angle: bool = False) -> None:
+    """Every following phase: independent sets drawn from the 24 (else colour
+    classes), over-relaxation by a random angle about h (else by pi)."""
+    lib().orc_set_update_schedule(1 if independent_sets else 0, 1 if or_random_angle else 0)
+
+
+def drawn_set(sweep, stage, pass_, phase, seed) -> int:
+    """Index (0..23) of the independent set of a phase under the drawn-set
+    schedule; stage 0 = Metropolis, 1 = over-relaxation."""
+    return lib().orc_drawn_set(sweep, stage, pass_, phase, seed)
+
+
+def overrelax_phase_ex(spins, L, R, phase, sweep, pass_, jeff, seed, dE=None) -> None:
+    lib().orc_overrelax_phase_ex(spins, L, R, phase, sweep, pass_, jeff, seed, _ptr(dE))
 
 
 def rotate_about(h, s, u) -> np.ndarray:

@@ -48,7 +48,7 @@ static const int8_t INDEP[24][4] = {
     {2, 10, 14, 6}, {2, 15, 11, 6}, {2, 15, 4, 9}, {2, 10, 4, 12}, {2, 5, 14, 9}, {2, 5, 11, 12},
     {3, 15, 11, 7}, {3, 15, 4, 8}, {3, 10, 14, 7}, {3, 10, 4, 13}, {3, 5, 14, 8}, {3, 5, 11, 13}};
 
-enum { KIND_METROPOLIS = 1, KIND_INIT = 2, KIND_SWAP = 3 };
+enum { KIND_METROPOLIS = 1, KIND_INIT = 2, KIND_SWAP = 3, KIND_OR_ANGLE = 6, KIND_SETS = 7 };
 
 void orc_links16(int8_t out[16][6][4]) { memcpy(out, LINKS, sizeof(LINKS)); }
 void orc_independent_sets(int8_t out[24][4]) { memcpy(out, INDEP, sizeof(INDEP)); }
@@ -316,6 +316,55 @@ void orc_rotate_about(const float h[3], const float s[3], float u, float out[3])
   renorm(out);
 }
 
+/* Update schedule. g_sets = 0: phase c of a pass updates the colour class
+ * {c, c+4, c+8, c+12}. g_sets = 1: it updates one of the 24 independent sets,
+ * drawn per (sweep, stage, pass, phase) and shared by every replica, as the
+ * reference picks IndependentSubLattices[Update[0] % 24] for each launch
+ * (Pyrochlore.cu:848, :985, :1021); its Update buffer is never filled
+ * (Pyrochlore.cu:84), so the draw here is a Philox word instead. A site can
+ * then be updated twice in one pass, so the move draws of a drawn-set phase
+ * carry the phase in bits 4-5 of their site field.
+ * g_or_angle = 0: over-relaxation by pi (OverRelaxationByPiStep,
+ * Pyrochlore.cu:975-1007); 1: rotation about the local field h by a uniform
+ * angle 2 pi u (OverRelaxationStep, Pyrochlore.cu:1009-1045), u from its own
+ * Philox stream keyed by (cell, site field, over-relaxation pass, replica,
+ * sweep). */
+static int g_sets = 0;
+static int g_or_angle = 0;
+
+void orc_set_update_schedule(int sets, int or_angle) {
+  g_sets = sets != 0;
+  g_or_angle = or_angle != 0;
+}
+
+/* The set drawn for phase `phase` of pass `pass` of a stage (0: Metropolis,
+ * 1: over-relaxation) of sweep `sweep`. */
+int orc_drawn_set(uint32_t sweep, int stage, int pass, int phase, uint64_t seed) {
+  uint32_t ctr[4] = {(uint32_t)phase, (uint32_t)stage | ((uint32_t)pass << 8) | ((uint32_t)KIND_SETS << 24),
+                     0u, sweep};
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t w[4];
+  orc_philox4x32_10(ctr, key, w);
+  return (int)(w[0] % 24u);
+}
+
+/* Target site k of phase `phase` and the site field of its draws. */
+static void phase_sites(uint32_t sweep, int stage, int pass, int phase, uint64_t seed, int sites[4],
+                        uint32_t tags[4]) {
+  if (g_sets) {
+    const int set = orc_drawn_set(sweep, stage, pass, phase, seed);
+    for (int k = 0; k < 4; k++) {
+      sites[k] = INDEP[set][k];
+      tags[k] = (uint32_t)sites[k] | ((uint32_t)phase << 4);
+    }
+  } else {
+    for (int k = 0; k < 4; k++) {
+      sites[k] = phase + 4 * k;
+      tags[k] = (uint32_t)sites[k];
+    }
+  }
+}
+
 /* Order in which a colour phase visits its N/4 updates (cell * 4 + k, target
  * site colour + 4 k): NULL = cell by cell. The updates of a phase are
  * independent, so any order must give the same spins; tests use a shuffled
@@ -368,6 +417,9 @@ void orc_metropolis_phase(float *spins, int L, int R, int colour, uint32_t sweep
                           const float *beta, const float *cos_cone, float jeff, uint64_t seed,
                           uint64_t *accepted, double *dE) {
   long nc = (long)L * L * L;
+  int sites[4];
+  uint32_t tags[4];
+  phase_sites(sweep, 0, pass, colour, seed, sites, tags);
   for (int r = 0; r < R; r++) {
     float *S = spins + (long)r * 48 * nc;
     float *Sx = S, *Sy = S + 16 * nc, *Sz = S + 32 * nc;
@@ -380,7 +432,7 @@ void orc_metropolis_phase(float *spins, int L, int R, int colour, uint32_t sweep
       int x = (int)(cell % L), y = (int)((cell / L) % L), z = (int)(cell / ((long)L * L));
       {
         int k = (int)(upd & 3);
-        int site = colour + 4 * k;
+        int site = sites[k];
         long i = (long)site * nc + cell;
         float s[3] = {Sx[i], Sy[i], Sz[i]};
         float f[3];
@@ -393,8 +445,7 @@ void orc_metropolis_phase(float *spins, int L, int R, int colour, uint32_t sweep
           h[2] = f[2];
         }
         uint32_t w[4];
-        draw((uint32_t)cell, (uint32_t)site, (uint32_t)pass, KIND_METROPOLIS, (uint32_t)r, sweep,
-             seed, w);
+        draw((uint32_t)cell, tags[k], (uint32_t)pass, KIND_METROPOLIS, (uint32_t)r, sweep, seed, w);
         float sp[3];
         orc_cone_proposal(s, cos_cone[r], u01(w[0]), u01(w[1]), sp);
         int take = orc_metropolis_accept(h, s, sp, beta[r], u01(w[2]));
@@ -418,9 +469,16 @@ void orc_metropolis_phase(float *spins, int L, int R, int colour, uint32_t sweep
   }
 }
 
-/* One over-relaxation colour phase: OverRelaxationByPiStep, Pyrochlore.cu:975-1007. */
-void orc_overrelax_phase(float *spins, int L, int R, int colour, float jeff, double *dE) {
+/* One over-relaxation phase: OverRelaxationByPiStep, Pyrochlore.cu:975-1007, or
+ * with g_or_angle the random-angle rotation of OverRelaxationStep,
+ * Pyrochlore.cu:1009-1045, about h (-jeff f, or f for the matrix model);
+ * the set of phase `phase` of over-relaxation pass `pass` by the schedule. */
+void orc_overrelax_phase_ex(float *spins, int L, int R, int phase, uint32_t sweep, int pass,
+                            float jeff, uint64_t seed, double *dE) {
   long nc = (long)L * L * L;
+  int sites[4];
+  uint32_t tags[4];
+  phase_sites(sweep, 1, pass, phase, seed, sites, tags);
   for (int r = 0; r < R; r++) {
     float *S = spins + (long)r * 48 * nc;
     float *Sx = S, *Sy = S + 16 * nc, *Sz = S + 32 * nc;
@@ -432,14 +490,26 @@ void orc_overrelax_phase(float *spins, int L, int R, int colour, float jeff, dou
       int x = (int)(cell % L), y = (int)((cell / L) % L), z = (int)(cell / ((long)L * L));
       {
         int k = (int)(upd & 3);
-        int site = colour + 4 * k;
+        int site = sites[k];
         long i = (long)site * nc + cell;
         float s[3] = {Sx[i], Sy[i], Sz[i]};
         float f[3];
         double d[3];
         nbr_sum(S, L, x, y, z, site, f, d);
         float sp[3];
-        orc_overrelax(f, s, sp);
+        if (g_or_angle) {
+          float h[3] = {-jeff * f[0], -jeff * f[1], -jeff * f[2]};
+          if (g_coup) {
+            h[0] = f[0];
+            h[1] = f[1];
+            h[2] = f[2];
+          }
+          uint32_t w[4];
+          draw((uint32_t)cell, tags[k], (uint32_t)pass, KIND_OR_ANGLE, (uint32_t)r, sweep, seed, w);
+          orc_rotate_about(h, s, u01(w[0]), sp);
+        } else {
+          orc_overrelax(f, s, sp);
+        }
         de += move_de(jeff, site & 3, s, sp, d);
         Sx[i] = sp[0];
         Sy[i] = sp[1];
@@ -448,6 +518,15 @@ void orc_overrelax_phase(float *spins, int L, int R, int colour, float jeff, dou
     }
     if (dE) dE[r] += de;
   }
+}
+
+/* The colour phase of the plain schedule (over-relaxation by pi). */
+void orc_overrelax_phase(float *spins, int L, int R, int colour, float jeff, double *dE) {
+  const int sets = g_sets, angle = g_or_angle;
+  g_sets = g_or_angle = 0;
+  orc_overrelax_phase_ex(spins, L, R, colour, 0, 0, jeff, 0, dE);
+  g_sets = sets;
+  g_or_angle = angle;
 }
 
 /* Energy: CPU TotalEnergy, Pyrochlore.cu:468-523 (every bond visited from both
@@ -569,8 +648,9 @@ int orc_pt_swap(int R, int parity, uint32_t round, uint64_t seed, const double *
 
 /* Mc, Pyrochlore.cu:74-133, plus the parallel-tempering cadence of
  * ParallelTempering (MonteCarlo.cu:429-470). Deviations: no 1000-sweep cap
- * (Pyrochlore.cu:83,131), colours in fixed order 0..3 rather than a set index
- * read from an unfilled buffer (Pyrochlore.cu:84,848). */
+ * (Pyrochlore.cu:83,131); by default colours in fixed order 0..3 rather than
+ * a set index read from an unfilled buffer (Pyrochlore.cu:84,848), drawn sets
+ * with orc_set_update_schedule. */
 void orc_run(float *spins, int L, int R, int n_sweeps, uint32_t sweep0, int n_metro, int n_or,
              int pt_every, uint32_t *pt_round, double *temps, float *beta, float *cos_cone,
              int *replica_at, float jeff, uint64_t seed, uint64_t *accepted, double *dE,
@@ -587,7 +667,7 @@ void orc_run(float *spins, int L, int R, int n_sweeps, uint32_t sweep0, int n_me
       for (int c = 0; c < 4; c++)
         orc_metropolis_phase(spins, L, R, c, sweep, p, beta, cos_cone, jeff, seed, accepted, dE);
     for (int o = 0; o < n_or; o++)
-      for (int c = 0; c < 4; c++) orc_overrelax_phase(spins, L, R, c, jeff, dE);
+      for (int c = 0; c < 4; c++) orc_overrelax_phase_ex(spins, L, R, c, sweep, o, jeff, seed, dE);
     if (pt_every > 0 && (sweep + 1) % (uint32_t)pt_every == 0) {
       orc_observables(spins, L, R, jeff, obs);
       for (int r = 0; r < R; r++) en[r] = obs[13 * r];

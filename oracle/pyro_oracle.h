@@ -47,8 +47,21 @@ void orc_metropolis_phase(float *spins, int L, int R, int colour, uint32_t sweep
                           const float *beta, const float *cos_cone, float jeff, uint64_t seed,
                           uint64_t *accepted, double *dE);
 
-/* One colour phase of over-relaxation on every replica. dE may be NULL. */
+/* One colour phase of over-relaxation by pi on every replica. dE may be NULL. */
 void orc_overrelax_phase(float *spins, int L, int R, int colour, float jeff, double *dE);
+
+/* Phase `phase` of over-relaxation pass `pass` of sweep `sweep` under the
+ * update schedule below (its set, and the rotation angle when random). */
+void orc_overrelax_phase_ex(float *spins, int L, int R, int phase, uint32_t sweep, int pass,
+                            float jeff, uint64_t seed, double *dE);
+
+/* Update schedule for every following phase: sets = 1 draws the independent
+ * set of each phase from the 24 (else phase c updates {c, c+4, c+8, c+12});
+ * or_angle = 1 makes over-relaxation a rotation about h by a uniform random
+ * angle (else by pi). orc_drawn_set: the set index of a phase (stage 0
+ * Metropolis, 1 over-relaxation). */
+void orc_set_update_schedule(int sets, int or_angle);
+int orc_drawn_set(uint32_t sweep, int stage, int pass, int phase, uint64_t seed);
 
 /* Per-replica FP64 observables: out[r*13 + 0] = energy, out[r*13 + 1 + 3*b + k]
  * = sum over the N/4 spins of sublattice b (= site % 4) of component k. */

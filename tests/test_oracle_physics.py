@@ -300,3 +300,100 @@ def test_cone_rule_converges_from_both_sides():
     r = ratio(o)
     assert abs(r[0] - 0.5) < 0.1, r
     assert r[1] > 0.5 and o.cosc[1] == -1.0, (r, o.cosc)
+
+
+def _schedule(sets, angle):
+    class _S:
+        def __enter__(self):
+            oracle.set_update_schedule(sets, angle)
+
+        def __exit__(self, *a):
+            oracle.set_update_schedule(False, False)
+    return _S()
+
+
+def test_drawn_sets_cover_all_24_uniformly():
+    """The drawn-set schedule picks each of the 24 independent sets about
+    equally often (chi-square over 24,000 phase draws), differently for the
+    Metropolis and over-relaxation stages, and the same for every replica."""
+    n = 24000
+    idx = np.array([oracle.drawn_set(s // 8, (s // 4) & 1, 0, s & 3, 77) for s in range(n)])
+    counts = np.bincount(idx, minlength=24)
+    chi2 = ((counts - n / 24) ** 2 / (n / 24)).sum()
+    assert chi2 < 60.0, (chi2, counts)  # 23 degrees of freedom: p ~ 4e-5
+    met = [oracle.drawn_set(s, 0, 0, 1, 77) for s in range(200)]
+    ore = [oracle.drawn_set(s, 1, 0, 1, 77) for s in range(200)]
+    assert met != ore
+
+
+def test_drawn_sets_are_independent_sets():
+    """Every set the schedule can draw holds four mutually non-adjacent sites
+    (no link of the 16-site cell, periodic images included, joins two)."""
+    links = oracle.links16()
+    for row in oracle.independent_sets():
+        members = set(int(v) for v in row)
+        assert len(members) == 4
+        for site in members:
+            assert not members & set(int(links[site, k, 3]) for k in range(6))
+
+
+def test_random_angle_overrelaxation_conserves_energy():
+    """Rotations about the local field keep every local energy, hence the
+    total (to FP32 rounding), keep spins unit, and move them differently from
+    the reflection by pi."""
+    L, R = 4, 2
+    for sets in (False, True):
+        s = fresh(L, R)
+        p = s.copy()
+        e0 = oracle.observables(s, L, R, 1.0)[:, 0]
+        with _schedule(sets, True):
+            for o in range(10):
+                for c in range(4):
+                    oracle.overrelax_phase_ex(s, L, R, c, 3, o, 1.0, 99)
+        for _ in range(10):
+            for c in range(4):
+                oracle.overrelax_phase(p, L, R, c, 1.0)
+        e1 = oracle.observables(s, L, R, 1.0)[:, 0]
+        assert np.allclose(e0, e1, rtol=0, atol=16 * L ** 3 * 1e-6), (sets, e0, e1)
+        n = np.sqrt((s.reshape(R, 3, -1) ** 2).sum(axis=1))
+        assert np.abs(n - 1).max() < 1e-6
+        assert not np.array_equal(s, p)
+
+
+@pytest.mark.parametrize("sets,angle", [(True, False), (False, True), (True, True)])
+def test_schedule_keeps_the_high_temperature_energy(sets, angle):
+    """Drawn sets and random-angle over-relaxation sample the same law:
+    E/N = -1/T at T = 40 (leading order; the next term is 2/(3T^2) = 4e-4)."""
+    T = 40.0
+    with _schedule(sets, angle):
+        e, err = equilibrium_energy(2, T, 2000, 100, None, -1.0, n_or=1 if angle else 0)
+    assert abs(e - (-1.0 / T)) < 4 * err + 1e-3, (e, err)
+
+
+def test_schedule_runs_are_resumable_and_order_independent():
+    """Counter-based draws: 16 sweeps equal 6 + 10 under the drawn-set,
+    random-angle schedule with PT; visiting each phase's updates in a shuffled
+    order gives the same spins (no update of a drawn set reads another's
+    write)."""
+    temps = [0.2, 0.5, 1.3]
+    kw = dict(n_metro=1, n_or=2, pt_every=4, cos_cone=0.3)
+    with _schedule(True, True):
+        a = oracle.OracleRun(3, temps, seed=8, **kw)
+        b = oracle.OracleRun(3, temps, seed=8, **kw)
+        a.run(16)
+        b.run(6)
+        b.run(10)
+        assert np.array_equal(a.spins, b.spins)
+        assert np.array_equal(a.replica_at, b.replica_at)
+        rng = np.random.default_rng(3)
+        oracle.set_update_order(rng.permutation(4 * 27))
+        try:
+            c = oracle.OracleRun(3, temps, seed=8, **kw)
+            c.run(16)
+        finally:
+            oracle.set_update_order(None)
+        assert np.array_equal(a.spins, c.spins)
+        assert np.array_equal(a.accepted, c.accepted)
+    d = oracle.OracleRun(3, temps, seed=8, **kw)
+    d.run(16)
+    assert not np.array_equal(a.spins, d.spins)
