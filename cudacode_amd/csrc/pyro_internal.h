@@ -76,6 +76,12 @@ __device__ __forceinline__ void st_at(float *__restrict__ base, uint32_t off, fl
   *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + off) = v;
 }
 
+// One block's partial sums of a learned phase (see k_learned_phase).
+struct LearnedSlot {
+  double de;
+  unsigned long long acc;
+};
+
 }  // namespace pyro
 
 struct pyro_ctx {
@@ -149,6 +155,11 @@ struct pyro_ctx {
   int adam_t = 0;
   bool mlp_ready = false;
   unsigned long long *d_acc_learned = nullptr;  // R, accepted learned proposals
+  // Per-block partial sums of the learned phases (energy change, accepted
+  // count): learned_nslots per replica and colour, moved into d_dE /
+  // d_acc_learned by launch_learned_fold after the learned passes of a sweep.
+  pyro::LearnedSlot *d_learned_slots = nullptr;
+  int learned_nslots = 0;
   std::vector<uint64_t> h_proposed_learned;
 
   // local-field model: scalar jeff, or (mat) exchange matrices and Zeeman
@@ -272,7 +283,11 @@ int launch_batch_sweeps(pyro_ctx *ctx, int n_sweeps, int record_last);
 void batch_free(pyro_ctx *ctx);
 
 // Learned proposal (pyro_learned.hip).
-int launch_learned_phase(pyro_ctx *ctx, int colour, int pass);
+// accumulate = 0: the launch's per-block slots of this colour are written
+// (the first learned pass since the last fold); 1: added to.
+int launch_learned_phase(pyro_ctx *ctx, int colour, int pass, int accumulate);
+// Adds the learned phases' per-block slots to d_dE and d_acc_learned (launch range).
+int launch_learned_fold(pyro_ctx *ctx);
 int mlp_alloc(pyro_ctx *ctx);
 int mlp_pack(pyro_ctx *ctx);
 void mlp_free(pyro_ctx *ctx);
