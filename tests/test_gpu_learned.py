@@ -393,3 +393,37 @@ def test_c5_workload_learned_pass_interleaved():
         assert a.sum() > 0.8 * p.sum()
         assert am.sum() > 0
         check_books(g, "c5")
+
+
+def test_learned_books_on_every_launch_path():
+    """The learned phases book their energy change and acceptance through
+    per-block slots that a fold moves into the books once per sweep. Every
+    path keeps the books exact to FP32 rounding of the moves and the
+    acceptance counts whole: two learned passes a sweep (the second pass adds
+    to the slots the first wrote), a single-phase call, and eager timed
+    launches (folded after the timed region). With over-relaxation tracked
+    in FP64 and no exchanges, the incremental energy equals the recomputed
+    one within the learned moves' FP32 rounding."""
+    temps = [0.05, 0.3, 1.0]
+    w = random_weights(5, out_scale=0.3).astype(np.float32)
+    with ctx(8, temps, n_metro=0, n_or=2, pt_every=0) as g:
+        g.set_kernel_options(1, True)
+        g.mlp_set_weights(w)
+        g.set_learned_passes(2)
+        g.set_spins(g.get_spins())  # re-anchor
+        g.Mc(6)
+        a1, p1 = g.acceptance_learned()
+        assert np.all(p1 == 6 * 2 * 16 * 8 ** 3)
+        assert np.all(a1 > 0) and np.all(a1 <= p1)
+        E, inc = g.TotalEnergyv2(), g.energy_incremental()
+        say(f"two passes: inc - E {inc - E}")
+        assert np.allclose(inc, E, rtol=1e-6, atol=1e-4)
+        g.learned_phase(2, 6)
+        avg_ms, _ = g.time_phase(2, 5)
+        assert avg_ms > 0
+        a2, p2 = g.acceptance_learned()
+        assert np.all(p2 == p1 + 6 * 4 * 8 ** 3)
+        assert np.all(a2 > a1)
+        E, inc = g.TotalEnergyv2(), g.energy_incremental()
+        say(f"single phase + timed launches: inc - E {inc - E}")
+        assert np.allclose(inc, E, rtol=1e-6, atol=1e-4)
