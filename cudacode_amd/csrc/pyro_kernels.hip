@@ -38,32 +38,39 @@ __global__ __launch_bounds__(256) void k_observables_final(const double *__restr
                                                            double *__restrict__ anchor,
                                                            double *__restrict__ dE) {
   const int r = blockIdx.x;
-  __shared__ double s_v[256];
+  const double *p = partials + (size_t)r * 4 * nblk * 4;
+  double v[kObs];
+#pragma unroll
+  for (int q = 0; q < kObs; ++q) v[q] = 0.0;
+  // Each thread sums its strided share of every sublattice's block partials
+  // (rows b * nblk + k), then the block reduces all 13 sums at once: wave
+  // butterflies and the 4 waves in order, a fixed order throughout.
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+    for (int k = threadIdx.x; k < nblk; k += 256) {
+      const double *row = p + ((size_t)b * nblk + k) * 4;
+      v[0] += row[0];
+      v[1 + 3 * b] += row[1];
+      v[2 + 3 * b] += row[2];
+      v[3 + 3 * b] += row[3];
+    }
+  __shared__ double s_v[4][kObs];
+  const int wv = threadIdx.x >> 6;
+#pragma unroll
   for (int q = 0; q < kObs; ++q) {
-    double t = 0.0;
-    if (q == 0) {
-      for (int k = threadIdx.x; k < 4 * nblk; k += 256)
-        t += partials[((size_t)r * 4 * nblk + k) * 4];
-    } else {
-      const int b = (q - 1) / 3, c = (q - 1) % 3;
-      for (int k = threadIdx.x; k < nblk; k += 256)
-        t += partials[(((size_t)r * 4 + b) * nblk + k) * 4 + 1 + c];
+    const double t = wave_sum(v[q]);
+    if ((threadIdx.x & 63) == 0) s_v[wv][q] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < kObs) {
+    const int q = threadIdx.x;
+    const double t = (s_v[0][q] + s_v[1][q]) + (s_v[2][q] + s_v[3][q]);
+    const double val = (q == 0) ? escale * t : t;
+    obs[(size_t)r * kObs + q] = val;
+    if (q == 0 && anchor != nullptr) {  // re-anchor the incremental energy here
+      anchor[r] = val;
+      dE[r] = 0.0;
     }
-    s_v[threadIdx.x] = t;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) s_v[threadIdx.x] += s_v[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      const double v = (q == 0) ? escale * s_v[0] : s_v[0];
-      obs[(size_t)r * kObs + q] = v;
-      if (q == 0 && anchor != nullptr) {  // re-anchor the incremental energy here
-        anchor[r] = v;
-        dE[r] = 0.0;
-      }
-    }
-    __syncthreads();
   }
 }
 
@@ -486,7 +493,7 @@ extern "C" int pyro_observables_noalloc(int device, const float *spins, int L, i
   if (rc == 0) {
     hipStream_t st = (hipStream_t)stream;
     // Every partial is written by its block, so nothing needs clearing.
-    k_observables<false><<<dim3(nblk, 4, R), kObsBlock, 0, st>>>(spins, L, ncell, part, nullptr);
+    launch_obs_partials<false>(spins, L, R, part, nullptr, st);
     k_observables_final<<<R, 256, 0, st>>>(part, nblk, 0.5 * (double)jeff, obs13, nullptr, nullptr);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess)

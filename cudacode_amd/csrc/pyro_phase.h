@@ -722,15 +722,23 @@ __global__ __launch_bounds__(256, 2) void k_metro_pair_plane(
 // q = (energy, mx, my, mz). The energy partial is sum_i s_i . g_i (scalar
 // model: g = neighbour sum) or sum_i s_i . (B_b + g_i / 2) (matrix model:
 // g = sum_n C s_n); the second stage scales it by jeff / 2 or -1.
-template <int B, bool MAT>
+// LT > 0: the lattice size at compile time (constant strides, one wrapped
+// coordinate triple per axis); LT = 0: L at run time. Spin offsets are 32-bit
+// byte offsets from the replica's planes.
+template <int B, bool MAT, int LT>
 __device__ __forceinline__ void obs_cell(const float *__restrict__ Sx, const float *__restrict__ Sy,
-                                         const float *__restrict__ Sz, int L, int ncell, int cell,
-                                         const float *__restrict__ coup, double v[4]) {
+                                         const float *__restrict__ Sz, int L_rt, int ncell_rt,
+                                         int cell, const float *__restrict__ coup, double v[4]) {
+  const int L = LT > 0 ? LT : L_rt;
+  const int ncell = LT > 0 ? LT * LT * LT : ncell_rt;
   const int LL = L * L;
   const int z = cell / LL;
   const int rem = cell - z * LL;
   const int y = rem / L;
   const int x = rem - y * L;
+  const int X[3] = {x == 0 ? L - 1 : x - 1, x, x == L - 1 ? 0 : x + 1};
+  const int Y[3] = {(y == 0 ? L - 1 : y - 1) * L, y * L, (y == L - 1 ? 0 : y + 1) * L};
+  const int Z[3] = {(z == 0 ? L - 1 : z - 1) * LL, z * LL, (z == L - 1 ? 0 : z + 1) * LL};
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int site = 4 * t + B;
@@ -740,14 +748,12 @@ __device__ __forceinline__ void obs_cell(const float *__restrict__ Sx, const flo
 #pragma unroll
     for (int n = 0; n < 6; ++n) {
       const Link lk = kLinks[site][n];
-      const int x1 = wrap(x + lk.dx, L);
-      const int y1 = wrap(y + lk.dy, L);
-      const int z1 = wrap(z + lk.dz, L);
-      const size_t j = (size_t)lk.site * ncell + (size_t)((z1 * L + y1) * L + x1);
-      field_add<MAT, true>(coup, B, lk.site & 3, Sx[j], Sy[j], Sz[j], fx, fy, fz, gx, gy, gz);
+      const uint32_t j = 4u * (uint32_t)(lk.site * ncell + Z[lk.dz + 1] + Y[lk.dy + 1] + X[lk.dx + 1]);
+      field_add<MAT, true>(coup, B, lk.site & 3, ld_at(Sx, j), ld_at(Sy, j), ld_at(Sz, j), fx, fy,
+                           fz, gx, gy, gz);
     }
-    const size_t i = (size_t)site * ncell + cell;
-    const double sx = Sx[i], sy = Sy[i], sz = Sz[i];
+    const uint32_t i = 4u * (uint32_t)(site * ncell + cell);
+    const double sx = ld_at(Sx, i), sy = ld_at(Sy, i), sz = ld_at(Sz, i);
     if constexpr (MAT)
       v[0] += sx * ((double)coup[144 + 3 * B] + 0.5 * gx) +
               sy * ((double)coup[145 + 3 * B] + 0.5 * gy) +
@@ -760,7 +766,7 @@ __device__ __forceinline__ void obs_cell(const float *__restrict__ Sx, const flo
   }
 }
 
-template <bool MAT>
+template <bool MAT, int LT>
 __global__ __launch_bounds__(kObsBlock) void k_observables(const float *__restrict__ spins,
                                                            int L, int ncell,
                                                            double *__restrict__ partials,
@@ -775,10 +781,10 @@ __global__ __launch_bounds__(kObsBlock) void k_observables(const float *__restri
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (cell < ncell) {
     switch (b) {
-      case 0: obs_cell<0, MAT>(Sx, Sy, Sz, L, ncell, cell, coup, v); break;
-      case 1: obs_cell<1, MAT>(Sx, Sy, Sz, L, ncell, cell, coup, v); break;
-      case 2: obs_cell<2, MAT>(Sx, Sy, Sz, L, ncell, cell, coup, v); break;
-      default: obs_cell<3, MAT>(Sx, Sy, Sz, L, ncell, cell, coup, v); break;
+      case 0: obs_cell<0, MAT, LT>(Sx, Sy, Sz, L, ncell, cell, coup, v); break;
+      case 1: obs_cell<1, MAT, LT>(Sx, Sy, Sz, L, ncell, cell, coup, v); break;
+      case 2: obs_cell<2, MAT, LT>(Sx, Sy, Sz, L, ncell, cell, coup, v); break;
+      default: obs_cell<3, MAT, LT>(Sx, Sy, Sz, L, ncell, cell, coup, v); break;
     }
   }
   __shared__ double s_v[4][4];
@@ -975,12 +981,34 @@ static void launch_phase(pyro_ctx *ctx, int colour, int pass) {
   }
 }
 
+// Partial sums of the observables (first stage) of R replicas of L^3 cells
+// into partials (R x 4 x ceil(L^3 / kObsBlock) x 4 doubles), on `stream`.
+template <bool MAT>
+static void launch_obs_partials(const float *spins, int L, int R, double *partials,
+                                const float *coup, hipStream_t stream) {
+  const int ncell = L * L * L;
+  const dim3 grid((ncell + kObsBlock - 1) / kObsBlock, 4, R);
+#define PYRO_OBS_CASE(LV)                                                                    \
+  case LV:                                                                                   \
+    k_observables<MAT, LV><<<grid, kObsBlock, 0, stream>>>(spins, L, ncell, partials, coup); \
+    return;
+  switch (L) {
+    PYRO_OBS_CASE(4)
+    PYRO_OBS_CASE(8)
+    PYRO_OBS_CASE(12)
+    PYRO_OBS_CASE(16)
+    PYRO_OBS_CASE(24)
+    PYRO_OBS_CASE(32)
+    PYRO_OBS_CASE(64)
+    default: k_observables<MAT, 0><<<grid, kObsBlock, 0, stream>>>(spins, L, ncell, partials, coup);
+  }
+#undef PYRO_OBS_CASE
+}
+
 // Partial sums of the observables (first stage) on the ctx stream.
 template <bool MAT>
 static void launch_observables_partials(pyro_ctx *ctx) {
-  dim3 grid(ctx->obs_blocks, 4, ctx->R);
-  k_observables<MAT><<<grid, kObsBlock, 0, ctx->stream>>>(ctx->d_spins, ctx->L, ctx->ncell,
-                                                          ctx->d_partials, ctx->d_coup);
+  launch_obs_partials<MAT>(ctx->d_spins, ctx->L, ctx->R, ctx->d_partials, ctx->d_coup, ctx->stream);
 }
 
 }  // namespace pyro
