@@ -1143,12 +1143,15 @@ int pyro_set_post_rotate(pyro_ctx *ctx, int on) {
 
 int pyro_set_update_schedule(pyro_ctx *ctx, int independent_sets, int or_random_angle) {
   if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (independent_sets < 0 || independent_sets > 2)
+    return set_error(PYRO_EINVAL, "pyro_set_update_schedule: independent_sets %d not 0/1/2",
+                     independent_sets);
   if ((independent_sets || or_random_angle) && (ctx->hetero || ctx->exec_mode != 0))
     return set_error(PYRO_ESTATE, "pyro_set_update_schedule: execution mode 0 on a uniform lattice only");
   PYRO_TRY(device_guard(ctx));
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   drop_graph(ctx);  // the captured sweeps carry the old schedule
-  ctx->set_schedule = independent_sets != 0;
+  ctx->set_schedule = independent_sets;
   ctx->or_angle = or_random_angle != 0;
   return 0;
 }

@@ -168,10 +168,11 @@ struct pyro_ctx {
   double spin_len2 = 1.0;
   bool mat = false;
   bool post_rotate = false;  // rotate about h after each Metropolis test
-  // Update schedule: set_schedule draws each phase's independent set from the
-  // 24 (else colour classes); or_angle makes over-relaxation a rotation about
+  // Update schedule: set_schedule 1 draws each phase's independent set from
+  // the 24, 2 runs the colour classes of each pass in a drawn order (0: the
+  // colour classes in order); or_angle makes over-relaxation a rotation about
   // h by a random angle (else by pi).
-  bool set_schedule = false;
+  int set_schedule = 0;
   bool or_angle = false;
 
   // Sweeps split into n_parts replica groups, each on its own stream (one

@@ -48,12 +48,15 @@ __device__ __forceinline__ double wave_sum(double v) {
 // arithmetic by constants); LT = 0 reads it from L. MAT: the matrix field
 // model (coup).
 //
-// Phase `colour` updates the colour class {c, c+4, c+8, c+12}; with
-// set_mode it updates one of the 24 independent sets instead, drawn from
-// Philox per (sweep, stage, pass, phase) and the same for every replica and
-// block (the reference's IndependentSubLattices[Update[0] % 24],
-// Pyrochlore.cu:848). A site may then be updated by two phases of one pass,
-// so the move draws of a drawn set carry the phase in their site field.
+// Phase `colour` updates the colour class {c, c+4, c+8, c+12}. set_mode 1:
+// it updates one of the 24 independent sets instead, drawn from Philox per
+// (sweep, stage, pass, phase) and the same for every replica and block (the
+// reference's IndependentSubLattices[Update[0] % 24], Pyrochlore.cu:848); a
+// site may then be updated by two phases of one pass, so the move draws of a
+// drawn set carry the phase in their site field. set_mode 2: it updates the
+// colour class perm[colour] of a permutation of the four drawn per (sweep,
+// stage, pass) by a Fisher-Yates shuffle on one Philox word (every site
+// still moves once a pass; the draws keep their counters).
 // `pass` numbers the Metropolis or over-relaxation pass within the sweep.
 template <int KIND, int LT, bool MAT>
 __global__ __launch_bounds__(kPhaseBlock) void k_phase(
@@ -71,12 +74,31 @@ __global__ __launch_bounds__(kPhaseBlock) void k_phase(
   const uint32_t sweep = ctl[kCtlSweep] + sweep_delta;
   int site = colour + 4 * k;
   uint32_t tag = (uint32_t)site;  // site field of this update's draws
-  if (set_mode) {
+  if (set_mode == 1) {
     const u32x4 q = philox4x32_10(
         rng_counter((uint32_t)colour, KIND == 0 ? 0u : 1u, (uint32_t)pass, kKindSets, 0u, sweep),
         key0, key1);
     site = __builtin_amdgcn_readfirstlane((int)c_sets[q.x % 24u][k]);
     tag = (uint32_t)site | ((uint32_t)colour << 4);
+  } else if (set_mode == 2) {
+    const u32x4 q = philox4x32_10(
+        rng_counter(0u, KIND == 0 ? 0u : 1u, (uint32_t)pass, kKindSets, 1u, sweep), key0, key1);
+    int perm[4] = {0, 1, 2, 3};
+    uint32_t v = q.x;
+#pragma unroll
+    for (int i = 3; i > 0; --i) {
+      const int j = (int)(v % (uint32_t)(i + 1));
+      v /= (uint32_t)(i + 1);
+      const int t = perm[i];
+      perm[i] = perm[j];
+      perm[j] = t;
+    }
+    int c = perm[0];
+#pragma unroll
+    for (int p = 1; p < 4; ++p)
+      if (colour == p) c = perm[p];
+    site = __builtin_amdgcn_readfirstlane(c + 4 * k);
+    tag = (uint32_t)site;
   }
   const int b = site & 3;  // basis site (the matrix model's couplings)
   const size_t plane = (size_t)16 * ncell;

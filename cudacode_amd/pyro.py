@@ -471,13 +471,14 @@ class Pyrochlore:
         """Rotate each spin about its field after the Metropolis test (Math.cu:168-197)."""
         _check(lib().pyro_set_post_rotate(self._ctx, 1 if on else 0))
 
-    def set_update_schedule(self, independent_sets: bool = False,
-                            or_random_angle: bool = False) -> None:
-        """Phases on independent sets drawn from the 24 (Pyrochlore.cu:848,
-        IndependentSubLattices[Update[0] % 24]) instead of the colour classes;
-        over-relaxation by a random angle about the field (OverRelaxationStep,
-        Pyrochlore.cu:1009-1045) instead of by pi."""
-        _check(lib().pyro_set_update_schedule(self._ctx, 1 if independent_sets else 0,
+    def set_update_schedule(self, independent_sets=0, or_random_angle: bool = False) -> None:
+        """independent_sets 1 (or True): phases on independent sets drawn from
+        the 24 (Pyrochlore.cu:848, IndependentSubLattices[Update[0] % 24]); 2:
+        the colour classes of each pass in a drawn order; 0 (or False): the
+        colour classes in order. or_random_angle: over-relaxation by a random
+        angle about the field (OverRelaxationStep, Pyrochlore.cu:1009-1045)
+        instead of by pi."""
+        _check(lib().pyro_set_update_schedule(self._ctx, int(independent_sets),
                                               1 if or_random_angle else 0))
 
     def couplings(self):

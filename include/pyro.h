@@ -315,7 +315,9 @@ int pyro_set_post_rotate(pyro_ctx *ctx, int on);
  * IndependentSubLattices[Update[0] % 24] (Pyrochlore.cu:848, :985, :1021;
  * its Update buffer is never filled, Pyrochlore.cu:84). A set may repeat
  * within a pass, so a pass need not touch every site; the Boltzmann law is
- * kept. Default off: phase c updates {c, c+4, c+8, c+12}.
+ * kept. independent_sets = 2: the four colour classes {c, c+4, c+8, c+12}
+ * run in an order drawn per (sweep, pass) from the Philox stream, so every
+ * site moves once a pass. Default 0: phase c updates {c, c+4, c+8, c+12}.
  * or_random_angle: over-relaxation rotates each spin about its local field
  * by a Philox-drawn angle uniform in [0, 2 pi) (OverRelaxationStep,
  * Pyrochlore.cu:1009-1045) instead of by pi (OverRelaxationByPiStep,

@@ -65,6 +65,8 @@ def lib():
         L.orc_set_update_schedule.argtypes = [C.c_int, C.c_int]
         L.orc_drawn_set.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_uint64]
         L.orc_drawn_set.restype = C.c_int
+        L.orc_drawn_colour.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_uint64]
+        L.orc_drawn_colour.restype = C.c_int
         L.orc_overrelax_phase_ex.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int,
                                              C.c_float, C.c_uint64, C.c_void_p]
         L.orc_rotate_about.argtypes = [_f32p, _f32p, C.c_float, _f32p]
@@ -233,10 +235,17 @@ def set_post_rotate(on: bool) -> None:
     lib().orc_set_post_rotate(1 if on else 0)
 
 
-def set_update_schedule(independent_sets: bool = False, or_random_angle: bool = False) -> None:
-    """Every following phase: independent sets drawn from the 24 (else colour
-    classes), over-relaxation by a random angle about h (else by pi)."""
-    lib().orc_set_update_schedule(1 if independent_sets else 0, 1 if or_random_angle else 0)
+def set_update_schedule(independent_sets=0, or_random_angle: bool = False) -> None:
+    """Every following phase: independent_sets 1 (or True) draws each phase's
+    set from the 24, 2 runs the colour classes in a drawn order per pass, 0
+    (or False) the colour classes in order; over-relaxation by a random angle
+    about h (else by pi)."""
+    lib().orc_set_update_schedule(int(independent_sets), 1 if or_random_angle else 0)
+
+
+def drawn_colour(sweep, stage, pass_, phase, seed) -> int:
+    """Colour of a phase under the drawn colour order (independent_sets=2)."""
+    return lib().orc_drawn_colour(sweep, stage, pass_, phase, seed)
 
 
 def drawn_set(sweep, stage, pass_, phase, seed) -> int:

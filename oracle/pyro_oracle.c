@@ -317,7 +317,11 @@ void orc_rotate_about(const float h[3], const float s[3], float u, float out[3])
 }
 
 /* Update schedule. g_sets = 0: phase c of a pass updates the colour class
- * {c, c+4, c+8, c+12}. g_sets = 1: it updates one of the 24 independent sets,
+ * {c, c+4, c+8, c+12}. g_sets = 2: phase c updates the colour class perm[c]
+ * of a permutation of the four drawn per (sweep, stage, pass) (the SURVEY's
+ * "Philox-drawn permutation per sweep"; every site still moves once a pass,
+ * so the move draws keep their colour-order counters). g_sets = 1: it
+ * updates one of the 24 independent sets,
  * drawn per (sweep, stage, pass, phase) and shared by every replica, as the
  * reference picks IndependentSubLattices[Update[0] % 24] for each launch
  * (Pyrochlore.cu:848, :985, :1021); its Update buffer is never filled
@@ -333,8 +337,29 @@ static int g_sets = 0;
 static int g_or_angle = 0;
 
 void orc_set_update_schedule(int sets, int or_angle) {
-  g_sets = sets != 0;
+  g_sets = (sets == 1 || sets == 2) ? sets : 0;
   g_or_angle = or_angle != 0;
+}
+
+/* The colour that phase `phase` of a pass updates under the drawn order: a
+ * Fisher-Yates shuffle of (0, 1, 2, 3) whose choices are the mixed-radix
+ * digits (base 4, 3, 2) of one Philox word, kind 7, replica field 1. */
+int orc_drawn_colour(uint32_t sweep, int stage, int pass, int phase, uint64_t seed) {
+  uint32_t ctr[4] = {0u, (uint32_t)stage | ((uint32_t)pass << 8) | ((uint32_t)KIND_SETS << 24), 1u,
+                     sweep};
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t w[4];
+  orc_philox4x32_10(ctr, key, w);
+  int perm[4] = {0, 1, 2, 3};
+  uint32_t v = w[0];
+  for (int i = 3; i > 0; i--) {
+    const int j = (int)(v % (uint32_t)(i + 1));
+    v /= (uint32_t)(i + 1);
+    const int t = perm[i];
+    perm[i] = perm[j];
+    perm[j] = t;
+  }
+  return perm[phase];
 }
 
 /* The set drawn for phase `phase` of pass `pass` of a stage (0: Metropolis,
@@ -351,7 +376,13 @@ int orc_drawn_set(uint32_t sweep, int stage, int pass, int phase, uint64_t seed)
 /* Target site k of phase `phase` and the site field of its draws. */
 static void phase_sites(uint32_t sweep, int stage, int pass, int phase, uint64_t seed, int sites[4],
                         uint32_t tags[4]) {
-  if (g_sets) {
+  if (g_sets == 2) {
+    const int c = orc_drawn_colour(sweep, stage, pass, phase, seed);
+    for (int k = 0; k < 4; k++) {
+      sites[k] = c + 4 * k;
+      tags[k] = (uint32_t)sites[k];
+    }
+  } else if (g_sets == 1) {
     const int set = orc_drawn_set(sweep, stage, pass, phase, seed);
     for (int k = 0; k < 4; k++) {
       sites[k] = INDEP[set][k];

@@ -56,12 +56,14 @@ void orc_overrelax_phase_ex(float *spins, int L, int R, int phase, uint32_t swee
                             float jeff, uint64_t seed, double *dE);
 
 /* Update schedule for every following phase: sets = 1 draws the independent
- * set of each phase from the 24 (else phase c updates {c, c+4, c+8, c+12});
+ * set of each phase from the 24, sets = 2 runs the colour classes of each
+ * pass in a drawn order (else phase c updates {c, c+4, c+8, c+12});
  * or_angle = 1 makes over-relaxation a rotation about h by a uniform random
- * angle (else by pi). orc_drawn_set: the set index of a phase (stage 0
- * Metropolis, 1 over-relaxation). */
+ * angle (else by pi). orc_drawn_set / orc_drawn_colour: the set index or
+ * colour of a phase (stage 0 Metropolis, 1 over-relaxation). */
 void orc_set_update_schedule(int sets, int or_angle);
 int orc_drawn_set(uint32_t sweep, int stage, int pass, int phase, uint64_t seed);
+int orc_drawn_colour(uint32_t sweep, int stage, int pass, int phase, uint64_t seed);
 
 /* Per-replica FP64 observables: out[r*13 + 0] = energy, out[r*13 + 1 + 3*b + k]
  * = sum over the N/4 spins of sublattice b (= site % 4) of component k. */

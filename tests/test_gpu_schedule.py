@@ -48,10 +48,11 @@ def gpu(L, temps, sets, angle, variant=1, track=True, **kw):
 
 
 @pytest.mark.parametrize("L,sets,angle,variant", [
-    (4, True, False, 1), (4, False, True, 1), (4, True, True, 0), (6, True, True, 1),
-    (8, True, True, 1), (16, True, False, 1)])
+    (4, 1, False, 1), (4, 0, True, 1), (4, 1, True, 0), (6, 1, True, 1),
+    (8, 1, True, 1), (16, 1, False, 1), (4, 2, False, 1), (6, 2, True, 1), (16, 2, True, 1)])
 def test_schedule_matches_oracle(L, sets, angle, variant):
-    """Drawn sets and/or random-angle over-relaxation with exchanges: the
+    """Drawn sets (1), drawn colour orders (2) and/or random-angle
+    over-relaxation with exchanges: the
     oracle's spins, rungs and acceptance bit for bit, FP64 energies to 1e-12
     and the incremental energy equal to the recomputed one, at compiled and
     runtime lattice sizes and from either kernel-variant setting."""
@@ -99,8 +100,8 @@ def test_schedule_matrix_model_matches_oracle():
     assert np.allclose(E, E_o, rtol=1e-12)
 
 
-@pytest.mark.parametrize("streams,graph", [(2, True), (3, False)])
-def test_schedule_split_streams_and_graphs_identical(streams, graph):
+@pytest.mark.parametrize("streams,graph,sets", [(2, True, 1), (3, False, 1), (3, True, 2)])
+def test_schedule_split_streams_and_graphs_identical(streams, graph, sets):
     """Replica groups on several streams draw the same set for the same
     phase: split sweeps equal one-stream sweeps bit for bit, captured or
     eager."""
@@ -108,7 +109,7 @@ def test_schedule_split_streams_and_graphs_identical(streams, graph):
     kw = dict(n_metro=1, n_or=3, pt_every=4, cos_cone=0.4)
     out = []
     for n_streams, use_graph in ((1, True), (streams, graph)):
-        with gpu(L, temps, True, True, **kw) as g:
+        with gpu(L, temps, sets, True, **kw) as g:
             g.set_streams(n_streams)
             g.Mc(12, use_graph=use_graph)
             out.append((g.get_spins(), g.acceptance()[0], g.temperatures()[1],
@@ -137,7 +138,10 @@ def test_random_angle_overrelaxation_full_size_properties():
 
 def test_schedule_refused_outside_execution_mode_0():
     from cudacode_amd import pyro
-    say("batch ctx refuses the schedule options")
+    say("batch ctx refuses the schedule options; bad schedule values refused")
     with pyro.Batch([4, 8], [0.5, 1.0], seed=SEED) as b:
         with pytest.raises(pyro.PyroError):
             b.set_update_schedule(True, False)
+    with pyro.Pyrochlore(4, [0.5], seed=SEED) as g:
+        with pytest.raises(pyro.PyroError):
+            g.set_update_schedule(3, False)

@@ -360,7 +360,21 @@ def test_random_angle_overrelaxation_conserves_energy():
         assert not np.array_equal(s, p)
 
 
-@pytest.mark.parametrize("sets,angle", [(True, False), (False, True), (True, True)])
+def test_drawn_colour_orders_are_uniform_permutations():
+    """The drawn colour order of a pass is a permutation of the four colour
+    classes, each of the 24 orders about equally often (chi-square over 4,800
+    passes), so every site moves exactly once a pass."""
+    import collections
+    n = 4800
+    c = collections.Counter(tuple(oracle.drawn_colour(s, s & 1, s >> 12, p, 31) for p in range(4))
+                            for s in range(n))
+    assert all(sorted(k) == [0, 1, 2, 3] for k in c)
+    assert len(c) == 24
+    chi2 = sum((v - n / 24) ** 2 / (n / 24) for v in c.values())
+    assert chi2 < 60.0, (chi2, c)
+
+
+@pytest.mark.parametrize("sets,angle", [(1, False), (False, True), (1, True), (2, False), (2, True)])
 def test_schedule_keeps_the_high_temperature_energy(sets, angle):
     """Drawn sets and random-angle over-relaxation sample the same law:
     E/N = -1/T at T = 40 (leading order; the next term is 2/(3T^2) = 4e-4)."""
@@ -377,7 +391,7 @@ def test_schedule_runs_are_resumable_and_order_independent():
     write)."""
     temps = [0.2, 0.5, 1.3]
     kw = dict(n_metro=1, n_or=2, pt_every=4, cos_cone=0.3)
-    with _schedule(True, True):
+    with _schedule(1, True):
         a = oracle.OracleRun(3, temps, seed=8, **kw)
         b = oracle.OracleRun(3, temps, seed=8, **kw)
         a.run(16)
