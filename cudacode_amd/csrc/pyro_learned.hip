@@ -845,14 +845,13 @@ int mlp_pack(pyro_ctx *ctx) {
 }
 
 // 64-cell groups per block. Several groups per wave share the weight
-// fragments a wave loads (22.5 KB) and, at 216 VGPRs (2 waves per SIMD), make
-// a launch that runs alone faster: 47.0 / 43.0 / 40.6 us per L=16 x 64 launch
-// for 1 / 4 / 8 groups, 1609 / 1249 / 1215 us at L=64 x 32, bit-identical
-// spins. Inside a sweep, where one replica group's learned launches overlap
-// the other groups' over-relaxation launches, they are not: 0.449 / 0.450 /
-// 0.458 ms per sweep at L=16 x 64, 15.66 / 15.95 / 15.98 ms at L=64 x 32 (two
-// 216-VGPR waves leave no room for an over-relaxation wave on their SIMD;
-// one group at 166 VGPRs does). One group.
+// fragments a wave loads (22.5 KB). With the per-block slots, 2 and 4 groups
+// spill at 168 VGPRs and run 45.9 and 47.9 us per L=16 x 64 launch against
+// 39.7 for one. (When every block still ended with two device-scope
+// atomics, fewer blocks also meant fewer atomics: 47.0 / 43.0 / 40.6 us for
+// 1 / 4 / 8 groups at 216 VGPRs, but slower sweeps, since two 216-VGPR waves
+// leave no room for another stream's over-relaxation wave on their SIMD.)
+// One group.
 constexpr int kLearnedGroups = 1;
 
 template <int LT>
