@@ -230,13 +230,15 @@ def learned_leg(g, args):
     g.Mc(args.therm_sweeps)
     log(f"learned: 2 x (plain, learned) legs of {args.series_sweeps} timed sweeps")
     plain_legs, learned_legs = learned.paired_legs(g, args.series_sweeps, rounds=2)
-    log("learned: phase timing, 200 learned launches")
-    learned_ms, _ = g.time_phase(2, 200)
     log(f"learned: observables, {args.obs_sweeps} measured sweeps per sampler")
     prev = learned.use_learned(g, 1)
     learned_obs = learned.observables_leg(g, args.obs_sweeps)
     learned.use_plain(g, prev)
     plain_obs = learned.observables_leg(g, args.obs_sweeps)
+    # Timed last: back-to-back launches of one phase reuse the sweep counter's
+    # draws, which would leave the chain out of equilibrium for a measurement.
+    log("learned: phase timing, 200 learned launches")
+    learned_ms, _ = g.time_phase(2, 200)
     n_spins = 16 * L ** 3
     rung_T = np.sort(pyro.geometric_ladder(T_MIN, T_MAX, R))[::-1]  # rungs: descending T
     verdict = learned.judge_paired(plain_legs, learned_legs, n_spins, rung_T, plain_obs,

@@ -259,6 +259,17 @@ def _obs_verdict(p: measure.Summary, q: measure.Summary) -> dict:
         "plain_tau_int_m_A2_sq_per_rung": [float(t) for t in p.tau_m2],
         "tau_int_m_A2_sq_median": float(np.nanmedian(q.tau_m2)),
         "plain_tau_int_m_A2_sq_median": float(np.nanmedian(p.tau_m2)),
+        # The lowest-temperature rung's estimates, (value, error) per sampler:
+        # E/N, C/N (-> 3/4 as T -> 0 for this model) and <|m_A2|^2>.
+        "lowest_T": {
+            "T": float(q.T[-1]),
+            "energy_per_spin": [float(q.e[-1, 0]), float(q.e[-1, 1])],
+            "plain_energy_per_spin": [float(p.e[-1, 0]), float(p.e[-1, 1])],
+            "specific_heat": [float(q.c[-1, 0]), float(q.c[-1, 1])],
+            "plain_specific_heat": [float(p.c[-1, 0]), float(p.c[-1, 1])],
+            "m_A2_sq": [float(q.m2[-1, 0, 0]), float(q.m2[-1, 0, 1])],
+            "plain_m_A2_sq": [float(p.m2[-1, 0, 0]), float(p.m2[-1, 0, 1])],
+        },
     }
 
 

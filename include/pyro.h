@@ -398,9 +398,12 @@ int pyro_import_configurations(pyro_ctx *ctx, const char *prefix, int rep, int b
  * each launch carrying its own start/stop HIP events in its dispatch
  * (hipExtLaunchKernelGGL). avg_ms: mean kernel time per launch; span_ms:
  * first start to last stop. Kinds 0-2 and 4 run for real (the state
- * advances); kind 3 writes only the second buffer and leaves the state as it
- * was. The reference's counterpart is the ENABLE_TIMING cudaEvent block
- * around Mc (MonteCarlo.cu:36-63). */
+ * advances) but all at the ctx's current sweep index, so a colour's launches
+ * reuse one set of random draws: the state they leave is not an equilibrium
+ * sample, and measurements should follow some sweeps later. Kind 3 writes
+ * only the second buffer and leaves the state as it was. The reference's
+ * counterpart is the ENABLE_TIMING cudaEvent block around Mc
+ * (MonteCarlo.cu:36-63). */
 int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, double *span_ms);
 
 /* Energies and sublattice sums of R replicas of L^3-cell spins that the caller

@@ -137,3 +137,27 @@ def test_choose_drops_a_slower_sampler():
     v = dict(v, samples_ratio_median=0.9)
     g = FakeCtx()
     assert learned.choose(g, v, (1, 10)) == "plain"
+
+
+def test_observables_verdict_reports_the_lowest_rung():
+    """The observables part of the verdict: z-scores of C/N and <|m_A2|^2>
+    between two summaries, and the lowest-temperature rung's estimates of
+    each sampler copied out unchanged (rungs by descending T)."""
+    from cudacode_amd import measure
+    rng = np.random.default_rng(5)
+    n, T = 2048, np.array([2.0, 1.0, 0.1])
+    rows = np.zeros((n, 3, 6))
+    for k in range(3):
+        rows[:, k, 0] = rng.normal(-500.0 * (k + 1), 4.0, n)
+        rows[:, k, 1:] = np.abs(rng.normal(0.1 * (k + 1), 0.01, (n, 5)))
+    p = measure.summarise(rows, T, 1024)
+    q = measure.summarise(rows[::-1].copy(), T, 1024)  # same samples, other order
+    v = learned._obs_verdict(p, q)
+    assert v["specific_heat_rungs_within_2sigma"] == 3
+    assert v["m_A2_sq_rungs_within_2sigma"] == 3
+    low = v["lowest_T"]
+    assert low["T"] == 0.1
+    assert low["plain_energy_per_spin"] == [p.e[-1, 0], p.e[-1, 1]]
+    assert low["energy_per_spin"][0] == pytest.approx(p.e[-1, 0], rel=1e-12)
+    assert low["specific_heat"] == [q.c[-1, 0], q.c[-1, 1]]
+    assert low["plain_m_A2_sq"] == [p.m2[-1, 0, 0], p.m2[-1, 0, 1]]
