@@ -552,6 +552,9 @@ def main():
         "host_inclusive_updates_per_s": (N_METRO + N_OR) * n_total / (
             (ms_per_step + get_ms + set_ms) * 1e-3),
     }
+    # The timed phase launches above reuse one sweep index's draws: sweep back
+    # to equilibrium before anything samples from the chain again.
+    g.Mc(200)
     # tau_int of the plain sampler from fixed-length legs after a fixed
     # thermalisation (a series as short as --steps cannot resolve it).
     learned = None
