@@ -318,9 +318,9 @@ void orc_rotate_about(const float h[3], const float s[3], float u, float out[3])
 
 /* Update schedule. g_sets = 0: phase c of a pass updates the colour class
  * {c, c+4, c+8, c+12}. g_sets = 2: phase c updates the colour class perm[c]
- * of a permutation of the four drawn per (sweep, stage, pass) (the SURVEY's
- * "Philox-drawn permutation per sweep"; every site still moves once a pass,
- * so the move draws keep their colour-order counters). g_sets = 1: it
+ * of a permutation of the four drawn per (sweep, stage, pass) (a random
+ * colour order; every site still moves once a pass, so the move draws keep
+ * their colour-order counters). g_sets = 1: it
  * updates one of the 24 independent sets,
  * drawn per (sweep, stage, pass, phase) and shared by every replica, as the
  * reference picks IndependentSubLattices[Update[0] % 24] for each launch
