@@ -490,6 +490,26 @@ def test_full_size_large_lattice_replicas_agree_with_oracle():
         assert np.array_equal(sr, o.spins.reshape(-1))
 
 
+def test_large_lattice_with_exchanges_agrees_with_oracle():
+    """L=64 (the c5 lattice) with parallel tempering on: 4 replicas of the c5
+    ladder's span, 1 Metropolis + 10 over-relaxation passes per sweep (the
+    fused default kernels), exchanges every 2 sweeps, 6 sweeps on two replica
+    groups: spins and rungs equal the oracle's bit for bit, energies to
+    1e-12."""
+    temps = [0.02, 0.06, 0.18, 0.5]
+    kw = dict(n_metro=1, n_or=10, pt_every=2, cos_cone=0.5)
+    with gpu(64, temps, track=False, **kw) as g:
+        g.set_streams(2)
+        g.Mc(6)
+        spins, E, at = g.get_spins(), g.TotalEnergyv2(), g.temperatures()[1]
+    say("L=64 x 4 with exchanges: 6 sweeps on the oracle")
+    o = orc(64, temps, **kw)
+    o.run(6)
+    assert np.array_equal(at, o.replica_at)
+    assert np.array_equal(spins, o.spins)
+    assert np.allclose(E, o.observables()[:, 0], rtol=1e-12)
+
+
 @pytest.mark.parametrize("L,n_or,track,streams,graph", [
     (4, 3, False, 1, True), (8, 4, True, 1, False), (8, 5, False, 2, True),
     (12, 2, True, 2, True), (16, 10, False, 2, False), (24, 3, False, 3, True),
