@@ -73,6 +73,24 @@ def test_schedule_matches_oracle(L, sets, angle, variant):
     assert np.allclose(inc, E, rtol=1e-11, atol=1e-9)
 
 
+@pytest.mark.parametrize("sets", [1, 2])
+def test_schedule_with_two_metropolis_passes_matches_oracle(sets):
+    """Two Metropolis and three over-relaxation passes a sweep: the pass index
+    keys the set or colour-order draws of both stages; spins, rungs and
+    acceptance equal the oracle's bit for bit."""
+    L, temps = 6, [0.1, 0.4, 1.5]
+    kw = dict(n_metro=2, n_or=3, pt_every=4, cos_cone=0.4)
+    with gpu(L, temps, sets, True, **kw) as g:
+        g.Mc(8)
+        spins, acc, at = g.get_spins(), g.acceptance()[0], g.temperatures()[1]
+    with schedule(sets, True):
+        o = oracle.OracleRun(L, temps, seed=SEED, **kw)
+        o.run(8)
+    assert np.array_equal(spins, o.spins)
+    assert np.array_equal(acc, o.accepted)
+    assert np.array_equal(at, o.replica_at)
+
+
 def test_schedule_matrix_model_matches_oracle():
     """The same with the anisotropic exchange in a field (each drawn site
     takes its own basis couplings and Zeeman term)."""
