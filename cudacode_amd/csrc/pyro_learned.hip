@@ -809,18 +809,24 @@ int mlp_alloc(pyro_ctx *ctx) {
       hipMalloc((void **)&ctx->d_mlp_v, nb) != hipSuccess ||
       hipMalloc((void **)&ctx->d_grad, nb + sizeof(float)) != hipSuccess ||
       hipMalloc((void **)&ctx->d_mlp_pack, sizeof(uint16_t) * kPackSize) != hipSuccess ||
-      hipMalloc((void **)&ctx->d_acc_learned, sizeof(unsigned long long) * ctx->R) != hipSuccess)
+      hipMalloc((void **)&ctx->d_acc_learned, sizeof(unsigned long long) * ctx->R) != hipSuccess) {
+    mlp_free(ctx);  // all or nothing: a later call allocates afresh
     return set_error(PYRO_ENOMEM, "learned proposal: device allocation failed");
+  }
   ctx->learned_nslots = (ctx->ncell + kPhaseCells - 1) / kPhaseCells;
   const size_t slot_bytes =
       sizeof(LearnedSlot) * (size_t)ctx->R * kColours * (size_t)ctx->learned_nslots;
-  if (hipMalloc((void **)&ctx->d_learned_slots, slot_bytes) != hipSuccess)
+  if (hipMalloc((void **)&ctx->d_learned_slots, slot_bytes) != hipSuccess) {
+    mlp_free(ctx);
     return set_error(PYRO_ENOMEM, "learned proposal: device allocation failed");
-  PYRO_HIP(hipMemset(ctx->d_learned_slots, 0, slot_bytes));
-  PYRO_HIP(hipMemset(ctx->d_mlp_w, 0, nb));
-  PYRO_HIP(hipMemset(ctx->d_mlp_m, 0, nb));
-  PYRO_HIP(hipMemset(ctx->d_mlp_v, 0, nb));
-  PYRO_HIP(hipMemset(ctx->d_acc_learned, 0, sizeof(unsigned long long) * ctx->R));
+  }
+  if (hipMemset(ctx->d_learned_slots, 0, slot_bytes) != hipSuccess ||
+      hipMemset(ctx->d_mlp_w, 0, nb) != hipSuccess || hipMemset(ctx->d_mlp_m, 0, nb) != hipSuccess ||
+      hipMemset(ctx->d_mlp_v, 0, nb) != hipSuccess ||
+      hipMemset(ctx->d_acc_learned, 0, sizeof(unsigned long long) * ctx->R) != hipSuccess) {
+    mlp_free(ctx);
+    return set_error(PYRO_EHIP, "learned proposal: clearing the device state failed");
+  }
   ctx->h_proposed_learned.assign(ctx->R, 0);
   return 0;
 }
@@ -874,7 +880,8 @@ static void launch_learned_t(pyro_ctx *ctx, int colour, int pass, int accumulate
 }
 
 int launch_learned_phase(pyro_ctx *ctx, int colour, int pass, int accumulate) {
-  if (!ctx->mlp_ready) return set_error(PYRO_ESTATE, "learned proposal: no weights set");
+  if (!ctx->mlp_ready || !ctx->d_learned_slots)
+    return set_error(PYRO_ESTATE, "learned proposal: no weights set");
   if (ctx->hetero) return set_error(PYRO_ESTATE, "learned proposal: needs a uniform lattice size");
   if (ctx->mat) return set_error(PYRO_ESTATE, "learned proposal: it is for the scalar model");
   switch (ctx->L) {
