@@ -185,16 +185,17 @@ __global__ __launch_bounds__(BLOCK) void k_batch_sweep(
 }
 
 // --- tile-list phases --------------------------------------------------------
-// grid = n_tiles (padded to a multiple of 8 with r = -1), block = 256.
-template <int KIND, int COLOUR, bool TRACK>
-__global__ __launch_bounds__(256) void k_batch_phase(
+// grid = n_tiles (padded to a multiple of 8 with r = -1), block = TILE, the
+// cells of a tile (128, 256 or 512).
+template <int KIND, int COLOUR, bool TRACK, int TILE>
+__global__ __launch_bounds__(TILE) void k_batch_phase(
     float *__restrict__ spins, const long long *__restrict__ off, const int *__restrict__ Ls,
     const int *__restrict__ tiles, int pass, const float *__restrict__ beta,
     const float *__restrict__ cosc, const uint32_t *__restrict__ ctl, uint32_t key0, uint32_t key1,
     float jeff, unsigned long long *__restrict__ acc, double *__restrict__ dE, uint32_t sweep_delta) {
   const int r = tiles[2 * blockIdx.x];
   if (r < 0) return;  // padding tile: the whole block leaves together
-  const int cell = tiles[2 * blockIdx.x + 1] * 256 + threadIdx.x;
+  const int cell = tiles[2 * blockIdx.x + 1] * TILE + threadIdx.x;
   const int L = Ls[r];
   float *Sx = spins + off[r];
   const float b = KIND == 0 ? beta[r] : 0.0f, c = KIND == 0 ? cosc[r] : 0.0f;
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(256) void k_batch_phase(
     }
   }
   if (KIND == 0 || TRACK) {
-    block_sum<256>(de, nacc);
+    block_sum<TILE>(de, nacc);
     if (threadIdx.x == 0) {
       if (de != 0.0) atomicAdd(&dE[r], de);
       if (KIND == 0 && nacc) atomicAdd(&acc[r], (unsigned long long)nacc);
@@ -222,15 +223,15 @@ __global__ __launch_bounds__(256) void k_batch_phase(
 }
 
 // Fused over-relaxation pair (colours CA, CA + 1, or_pair_cell) over the tile
-// list, between the two spin buffers. grid = n_tiles, block = 256.
-template <int CA, bool TRACK>
-__global__ __launch_bounds__(256) void k_batch_or_pair(
+// list, between the two spin buffers. grid = n_tiles, block = TILE.
+template <int CA, bool TRACK, int TILE>
+__global__ __launch_bounds__(TILE) void k_batch_or_pair(
     const float *__restrict__ cur, float *__restrict__ oth, const long long *__restrict__ off,
     const int *__restrict__ Ls, const int *__restrict__ tiles, float jeff,
     double *__restrict__ dE) {
   const int r = tiles[2 * blockIdx.x];
   if (r < 0) return;  // padding tile: the whole block leaves together
-  const int cell = tiles[2 * blockIdx.x + 1] * 256 + threadIdx.x;
+  const int cell = tiles[2 * blockIdx.x + 1] * TILE + threadIdx.x;
   const int L = Ls[r];
   const float *c = cur + off[r];
   float *o = oth + off[r];
@@ -248,7 +249,7 @@ __global__ __launch_bounds__(256) void k_batch_or_pair(
   }
   if (TRACK) {
     unsigned nacc = 0;
-    block_sum<256>(de, nacc);
+    block_sum<TILE>(de, nacc);
     if (threadIdx.x == 0 && de != 0.0) atomicAdd(&dE[r], de);
   }
 }
@@ -381,20 +382,32 @@ void batch_free(pyro_ctx *ctx) {
     }
   for (auto &row : ctx->n_tiles_part)
     for (int &n : row) n = 0;
+  for (int x = 0; x < 2; ++x) {
+    if (ctx->d_tiles_x[x]) (void)hipFree(ctx->d_tiles_x[x]);
+    ctx->d_tiles_x[x] = nullptr;
+    ctx->n_tiles_x[x] = 0;
+    for (int n = 0; n <= kMaxParts; ++n)
+      for (int p = 0; p < kMaxParts; ++p) {
+        if (ctx->d_tiles_part_x[x][n][p]) (void)hipFree(ctx->d_tiles_part_x[x][n][p]);
+        ctx->d_tiles_part_x[x][n][p] = nullptr;
+        ctx->n_tiles_part_x[x][n][p] = 0;
+      }
+  }
   ctx->d_L = nullptr;
   ctx->d_off = nullptr;
   ctx->d_order = ctx->d_bin_start = ctx->d_tiles = ctx->d_part_off = nullptr;
   ctx->d_tile_part = nullptr;
 }
 
-// Tiles of replicas [r_begin, r_end) in replica order, dealt to the 8 XCDs in
-// contiguous chunks: hardware block b runs on XCD b % 8, so list slot b takes
-// logical tile (b % 8) * chunk + b / 8. Padded to a multiple of 8 with r = -1.
-static std::vector<int> tile_list(const pyro_ctx *ctx, int r_begin, int r_end) {
+// Tiles of `tile` cells of replicas [r_begin, r_end) in replica order, dealt
+// to the 8 XCDs in contiguous chunks: hardware block b runs on XCD b % 8, so
+// list slot b takes logical tile (b % 8) * chunk + b / 8. Padded to a
+// multiple of 8 with r = -1.
+static std::vector<int> tile_list(const pyro_ctx *ctx, int r_begin, int r_end, int tile = 256) {
   std::vector<int> logical;
   for (int r = r_begin; r < r_end; ++r) {
     const int nc = ctx->h_L[r] * ctx->h_L[r] * ctx->h_L[r];
-    for (int t = 0; t < (nc + 255) / 256; ++t) {
+    for (int t = 0; t < (nc + tile - 1) / tile; ++t) {
       logical.push_back(r);
       logical.push_back(t);
     }
@@ -466,6 +479,23 @@ int batch_setup(pyro_ctx *ctx, int n_bins) {
                          hipMemcpyHostToDevice));
       ctx->n_tiles_part[n][p] = (int)t.size() / 2;
     }
+  // The lists of 128- and 512-cell tiles, whole and per replica group.
+  for (int x = 0; x < 2; ++x) {
+    const int tile = x == 0 ? 128 : 512;
+    const std::vector<int> all = tile_list(ctx, 0, R, tile);
+    PYRO_HIP(hipMalloc((void **)&ctx->d_tiles_x[x], sizeof(int) * all.size()));
+    PYRO_HIP(hipMemcpy(ctx->d_tiles_x[x], all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice));
+    ctx->n_tiles_x[x] = (int)all.size() / 2;
+    for (int n = 2; n <= kMaxParts && n <= R; ++n)
+      for (int p = 0; p < n; ++p) {
+        const std::vector<int> t = tile_list(ctx, p * R / n, (p + 1) * R / n, tile);
+        if (t.empty()) continue;
+        PYRO_HIP(hipMalloc((void **)&ctx->d_tiles_part_x[x][n][p], sizeof(int) * t.size()));
+        PYRO_HIP(hipMemcpy(ctx->d_tiles_part_x[x][n][p], t.data(), sizeof(int) * t.size(),
+                           hipMemcpyHostToDevice));
+        ctx->n_tiles_part_x[x][n][p] = (int)t.size() / 2;
+      }
+  }
   ctx->n_bins = n_bins;
   ctx->n_tiles = (int)tiles.size() / 2;
   return 0;
@@ -524,15 +554,32 @@ static int launch_persistent_t(pyro_ctx *ctx, int n_sweeps, int record_last) {
   return 0;
 }
 
-// All tiles, or the list of one replica group while a split sweep has set a
-// launch range (always group launch_part of n_parts).
+// All tiles of the current tile size, or the list of one replica group while
+// a split sweep has set a launch range (always group launch_part of n_parts).
 static void launch_tile_range(const pyro_ctx *ctx, const int *&tiles, int &n) {
-  tiles = ctx->d_tiles;
-  n = ctx->n_tiles;
-  if (ctx->launch_nrep) {
-    tiles = ctx->d_tiles_part[ctx->n_parts][ctx->launch_part];
-    n = ctx->n_tiles_part[ctx->n_parts][ctx->launch_part];
+  const int x = ctx->tile_cells == 128 ? 0 : (ctx->tile_cells == 512 ? 1 : -1);
+  if (x < 0) {
+    tiles = ctx->d_tiles;
+    n = ctx->n_tiles;
+    if (ctx->launch_nrep) {
+      tiles = ctx->d_tiles_part[ctx->n_parts][ctx->launch_part];
+      n = ctx->n_tiles_part[ctx->n_parts][ctx->launch_part];
+    }
+    return;
   }
+  tiles = ctx->d_tiles_x[x];
+  n = ctx->n_tiles_x[x];
+  if (ctx->launch_nrep) {
+    tiles = ctx->d_tiles_part_x[x][ctx->n_parts][ctx->launch_part];
+    n = ctx->n_tiles_part_x[x][ctx->n_parts][ctx->launch_part];
+  }
+}
+
+template <int KIND, int COLOUR, bool TRACK, int TILE>
+static void launch_tiles_tt(pyro_ctx *ctx, int pass, const int *tiles, int n) {
+  k_batch_phase<KIND, COLOUR, TRACK, TILE><<<n, TILE, 0, launch_stream(ctx)>>>(
+      ctx->d_spins, ctx->d_off, ctx->d_L, tiles, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl,
+      bk0(ctx), bk1(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE, ctx->launch_delta);
 }
 
 template <int KIND, int COLOUR, bool TRACK>
@@ -541,9 +588,20 @@ static void launch_tiles_t(pyro_ctx *ctx, int pass) {
   int n;
   launch_tile_range(ctx, tiles, n);
   if (n == 0) return;
-  k_batch_phase<KIND, COLOUR, TRACK><<<n, 256, 0, launch_stream(ctx)>>>(
-      ctx->d_spins, ctx->d_off, ctx->d_L, tiles, pass, ctx->d_beta, ctx->d_cosc, ctx->d_ctl,
-      bk0(ctx), bk1(ctx), ctx->jeff, ctx->d_acc, ctx->d_dE, ctx->launch_delta);
+  if (ctx->tile_cells == 128)
+    launch_tiles_tt<KIND, COLOUR, TRACK, 128>(ctx, pass, tiles, n);
+  else if (ctx->tile_cells == 512)
+    launch_tiles_tt<KIND, COLOUR, TRACK, 512>(ctx, pass, tiles, n);
+  else
+    launch_tiles_tt<KIND, COLOUR, TRACK, 256>(ctx, pass, tiles, n);
+}
+
+template <bool TRACK, int TILE>
+static void launch_tiles_or_tt(pyro_ctx *ctx, const float *cur, float *oth, const int *tiles, int n) {
+  k_batch_or_pair<0, TRACK, TILE><<<n, TILE, 0, launch_stream(ctx)>>>(cur, oth, ctx->d_off, ctx->d_L,
+                                                                     tiles, ctx->jeff, ctx->d_dE);
+  k_batch_or_pair<2, TRACK, TILE><<<n, TILE, 0, launch_stream(ctx)>>>(cur, oth, ctx->d_off, ctx->d_L,
+                                                                     tiles, ctx->jeff, ctx->d_dE);
 }
 
 // The fused over-relaxation pass `parity` (0: d_spins -> second buffer,
@@ -556,10 +614,12 @@ static void launch_tiles_or_pass(pyro_ctx *ctx, int parity) {
   if (n == 0) return;
   const float *cur = parity ? ctx->d_spins_alt : ctx->d_spins;
   float *oth = parity ? ctx->d_spins : ctx->d_spins_alt;
-  k_batch_or_pair<0, TRACK><<<n, 256, 0, launch_stream(ctx)>>>(cur, oth, ctx->d_off, ctx->d_L,
-                                                                 tiles, ctx->jeff, ctx->d_dE);
-  k_batch_or_pair<2, TRACK><<<n, 256, 0, launch_stream(ctx)>>>(cur, oth, ctx->d_off, ctx->d_L,
-                                                                 tiles, ctx->jeff, ctx->d_dE);
+  if (ctx->tile_cells == 128)
+    launch_tiles_or_tt<TRACK, 128>(ctx, cur, oth, tiles, n);
+  else if (ctx->tile_cells == 512)
+    launch_tiles_or_tt<TRACK, 512>(ctx, cur, oth, tiles, n);
+  else
+    launch_tiles_or_tt<TRACK, 256>(ctx, cur, oth, tiles, n);
 }
 
 template <int KIND, bool TRACK>

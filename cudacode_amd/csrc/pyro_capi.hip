@@ -841,19 +841,22 @@ int pyro_set_exec_mode(pyro_ctx *ctx, int mode, int block, int n_bins) {
     return set_error(PYRO_ESTATE, "learned passes run only in exec mode 0");
   if (mode == 1 && block != 256 && block != 512 && block != 1024)
     return set_error(PYRO_EINVAL, "persistent block size %d not 256/512/1024", block);
+  if (mode == 2 && block != 0 && block != 128 && block != 256 && block != 512)
+    return set_error(PYRO_EINVAL, "tile size %d not 128/256/512 (0: 256)", block);
   PYRO_TRY(device_guard(ctx));
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   drop_graph(ctx);
   if (mode == 1 && n_bins != ctx->n_bins) PYRO_TRY(batch_setup(ctx, n_bins));
   ctx->exec_mode = mode;
   if (mode == 1) ctx->batch_block = block;
+  if (mode == 2) ctx->tile_cells = block ? block : 256;
   return 0;
 }
 
 int pyro_get_exec_mode(pyro_ctx *ctx, int *mode, int *block, int *n_bins) {
   if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
   if (mode) *mode = ctx->exec_mode;
-  if (block) *block = ctx->batch_block;
+  if (block) *block = ctx->exec_mode == 2 ? ctx->tile_cells : ctx->batch_block;
   if (n_bins) *n_bins = ctx->n_bins;
   return 0;
 }
@@ -896,8 +899,9 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
       cands.push_back({1, 512, 1, 1});
       cands.push_back({1, 1024, 1, 1});
     }
-    for (int n : {1, 2, 3})
-      if (n <= ctx->n_streams) cands.push_back({2, 256, 1, n});
+    for (int tile : {256, 128, 512})
+      for (int n : {1, 2, 3})
+        if (n <= ctx->n_streams) cands.push_back({2, tile, 1, n});
   }
   if (cands.empty()) return 0;
   // Snapshot of everything a sweep changes (an exchange or measurement
@@ -932,9 +936,9 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
   const uint32_t h_sweep = ctx->h_sweep, h_round = ctx->h_pt_round;
   const std::vector<uint64_t> h_prop = ctx->h_proposed;
   const int mode0 = ctx->exec_mode, block0 = ctx->batch_block, variant0 = ctx->phase_variant;
-  const int parts0 = ctx->n_parts;
+  const int parts0 = ctx->n_parts, tile0 = ctx->tile_cells;
   double best = 1e30;
-  Cand best_c = {mode0, block0, variant0, parts0};
+  Cand best_c = {mode0, mode0 == 2 ? tile0 : block0, variant0, parts0};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     rc = set_error(PYRO_EHIP, "pyro_autotune: event creation failed");
@@ -945,7 +949,8 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
     if (c.mode == 1 && ctx->n_bins != R) rc = batch_setup(ctx, R);
     if (rc) break;
     ctx->exec_mode = c.mode;
-    ctx->batch_block = c.block ? c.block : block0;
+    ctx->batch_block = c.mode == 1 ? c.block : block0;
+    ctx->tile_cells = c.mode == 2 ? c.block : tile0;
     ctx->phase_variant = c.variant;
     ctx->n_parts = c.streams;
     // Time the candidate as pyro_sweep runs it: `sweeps` sweeps captured in
@@ -997,13 +1002,15 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
   (void)hipFree(s_spins);
   drop_graph(ctx);
   ctx->exec_mode = best_c.mode;
-  ctx->batch_block = best_c.block ? best_c.block : block0;
+  ctx->batch_block = best_c.mode == 1 ? best_c.block : block0;
+  ctx->tile_cells = best_c.mode == 2 ? best_c.block : tile0;
   ctx->phase_variant = best_c.variant;
   ctx->n_parts = best_c.streams;
   if (n_results) *n_results = nres;
   if (rc) {
     ctx->exec_mode = mode0;
     ctx->batch_block = block0;
+    ctx->tile_cells = tile0;
     ctx->phase_variant = variant0;
     ctx->n_parts = parts0;
   }

@@ -130,6 +130,7 @@ struct pyro_ctx {
   // kernels over a work list of 256-cell tiles of all replicas.
   int exec_mode = 0;
   int batch_block = 256;            // block size of the persistent kernel
+  int tile_cells = 256;             // cells per tile (= threads per block) of the tile lists
   int *d_order = nullptr;           // replicas in bin order
   int *d_bin_start = nullptr;       // n_bins + 1 offsets into d_order
   int n_bins = 0;
@@ -141,6 +142,12 @@ struct pyro_ctx {
   // d_tiles_part[n][p] covers group p of n (replicas [p R / n, (p + 1) R / n)).
   int *d_tiles_part[pyro::kMaxParts + 1][pyro::kMaxParts] = {};
   int n_tiles_part[pyro::kMaxParts + 1][pyro::kMaxParts] = {};
+  // The same lists with 128-cell (index 0) and 512-cell (index 1) tiles, for
+  // the phases of the tile-list execution (the observables keep 256).
+  int *d_tiles_x[2] = {};
+  int n_tiles_x[2] = {};
+  int *d_tiles_part_x[2][pyro::kMaxParts + 1][pyro::kMaxParts] = {};
+  int n_tiles_part_x[2][pyro::kMaxParts + 1][pyro::kMaxParts] = {};
 
   // learned proposal
   int n_learned = 0;                    // learned passes per sweep (after Metropolis)

@@ -512,8 +512,12 @@ class Pyrochlore:
     def set_kernel_options(self, phase_variant: int = 1, track_or_energy: bool = False) -> None:
         _check(lib().pyro_set_kernel_options(self._ctx, phase_variant, 1 if track_or_energy else 0))
 
-    def set_exec_mode(self, mode: int, block: int = 1024, n_bins: int = 0) -> None:
-        """0: per-phase kernels; 1: one persistent kernel per sweep; 2: tile-list phases."""
+    def set_exec_mode(self, mode: int, block: int = None, n_bins: int = 0) -> None:
+        """0: per-phase kernels; 1: one persistent kernel per sweep (block:
+        256/512/1024 threads, default 1024); 2: tile-list phases (block: cells
+        per tile, 128/256/512, default 256)."""
+        if block is None:
+            block = 1024 if mode == 1 else (256 if mode == 2 else 0)
         _check(lib().pyro_set_exec_mode(self._ctx, mode, block, n_bins))
 
     def exec_mode(self):
@@ -524,7 +528,7 @@ class Pyrochlore:
     def autotune(self, sweeps: int = 5) -> np.ndarray:
         """Try every execution, keep the fastest; rows (mode, block, variant,
         streams, ms/sweep)."""
-        res = np.zeros(5 * 16, np.float64)
+        res = np.zeros(5 * 24, np.float64)
         n = C.c_int()
         _check(lib().pyro_autotune(self._ctx, sweeps, _p(res), C.byref(n)))
         return res[: 5 * n.value].reshape(n.value, 5)

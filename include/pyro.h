@@ -91,9 +91,12 @@ int pyro_get_layout(pyro_ctx *ctx, int *L, long long *offset);
  * in which a block of `block` threads (256, 512, 1024) sweeps whole replicas
  * through every phase with block barriers only (replicas of L <= 8 staged in
  * LDS), replicas packed into n_bins balanced bins (<= 0: one per block);
- * mode 2: per-phase kernels over a list of 256-cell tiles of all replicas.
+ * mode 2: per-phase kernels over a list of tiles of `block` cells (128, 256
+ * or 512; 0 means 256; a block of threads per tile) of all replicas.
  * Modes 1 and 2 need L in {4,8,12,16,24,32}. Results are identical in every
- * mode. Default: 0 for a uniform ctx, 2 for a batch. */
+ * mode and tile size. Default: 0 for a uniform ctx, 2 (256-cell tiles) for a
+ * batch. pyro_get_exec_mode returns the persistent block size in mode 1 and
+ * the tile size in mode 2. */
 int pyro_set_exec_mode(pyro_ctx *ctx, int mode, int block, int n_bins);
 int pyro_get_exec_mode(pyro_ctx *ctx, int *mode, int *block, int *n_bins);
 
@@ -102,10 +105,11 @@ int pyro_get_exec_mode(pyro_ctx *ctx, int *mode, int *block, int *n_bins);
  * at 256/512/1024 threads when every replica has L <= 8 and is staged in LDS
  * -- a block owns whole replicas, so larger ones serialise: 5.9-12 ms per
  * sweep for 256 mixed replicas of L up to 24 against 1.58 ms as tile lists
- * -- and tile list on 1, 2 or 3 streams), captured in one graph as pyro_sweep
- * replays them, restoring the state after each trial, and keep the fastest.
- * results (optional, 5 doubles per candidate, at most 16 candidates): mode,
- * block, phase variant, streams, ms per sweep. */
+ * -- and tile lists of 128, 256 and 512 cells on 1, 2 or 3 streams), captured
+ * in one graph as pyro_sweep replays them, restoring the state after each
+ * trial, and keep the fastest. results (optional, 5 doubles per candidate, at
+ * most 24 candidates): mode, block (mode 1) or tile size (mode 2), phase
+ * variant, streams, ms per sweep. */
 int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results);
 
 /* Lattice tables: links16 (Pyrochlore.cu:152-167) and IndependentSubLattices

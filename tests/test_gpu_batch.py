@@ -42,7 +42,7 @@ def oracle_replica(L, T, r, sweeps, **kw):
     return o
 
 
-MODES = [(1, 256), (1, 512), (1, 1024), (2, 256)]
+MODES = [(1, 256), (1, 512), (1, 1024), (2, 256), (2, 128), (2, 512)]
 
 
 @pytest.mark.parametrize("mode,block", MODES)
@@ -62,12 +62,14 @@ def test_batch_replicas_match_oracle(mode, block):
         assert a.sum() > 0
 
 
-def test_batch_parallel_tempering_within_groups():
+@pytest.mark.parametrize("tile", [256, 128, 512])
+def test_batch_parallel_tempering_within_groups(tile):
     Ls = [4, 4, 4, 4, 8, 8, 8]
     temps = [0.2, 0.35, 0.5, 0.8, 0.25, 0.4, 0.9]
     kw = dict(n_metro=1, n_or=2, pt_every=2, cos_cone=0.3)
     with batch(Ls, temps, **kw) as g:
-        g.set_exec_mode(2, 256)
+        g.set_exec_mode(2, tile)
+        assert g.exec_mode()[:2] == (2, tile)
         g.Mc(12)
         T, at = g.temperatures()
         s = g.get_spins()
@@ -136,10 +138,14 @@ def test_autotune_keeps_the_state():
         before = g.get_spins()
         inc_before = g.energy_incremental()
         res = g.autotune(3)
-        # tile lists on 1, 2 and 3 streams; no persistent trials, replicas of
-        # L > 8 being too large for one block
-        assert len(res) == 3 and np.all(res[:, 4] > 0)
-        assert res[:, 3].tolist() == [1, 2, 3] and res[:, 0].tolist() == [2, 2, 2]
+        # tile lists of 256, 128 and 512 cells on 1, 2 and 3 streams; no
+        # persistent trials, replicas of L > 8 being too large for one block
+        assert len(res) == 9 and np.all(res[:, 4] > 0)
+        assert res[:, 3].tolist() == [1, 2, 3] * 3 and res[:, 0].tolist() == [2] * 9
+        assert res[:, 1].tolist() == [256] * 3 + [128] * 3 + [512] * 3
+        mode, tile, _ = g.exec_mode()
+        best = res[np.argmin(res[:, 4])]
+        assert (mode, tile) == (2, int(best[1]))
         say(f"autotune results (mode, block, variant, streams, ms/sweep): {res.tolist()}")
         assert np.array_equal(g.get_spins(), before)
         assert g.sweep_counters() == h.sweep_counters()
@@ -211,8 +217,9 @@ def test_persistent_single_small_lattice_equals_phases():
     assert np.array_equal(spins, o.spins)
 
 
-@pytest.mark.parametrize("graph,streams", [(True, 2), (False, 2), (True, 4), (False, 3)])
-def test_tile_list_split_streams_identical(graph, streams):
+@pytest.mark.parametrize("graph,streams,tile", [(True, 2, 256), (False, 2, 256), (True, 4, 256),
+                                               (False, 3, 256), (True, 3, 128), (False, 2, 512)])
+def test_tile_list_split_streams_identical(graph, streams, tile):
     """Mixed lattice sizes in the tile-list execution with the replica groups
     on 2, 3 or 4 streams (each group its own XCD-dealt tile list, joined at
     exchanges and measurements) equal one stream: spins, rungs, acceptance,
@@ -224,7 +231,7 @@ def test_tile_list_split_streams_identical(graph, streams):
     kw = dict(n_metro=1, n_or=2, pt_every=3, cos_cone=0.4)
     with batch(Ls, temps, **kw) as a, batch(Ls, temps, **kw) as b:
         for g, n in ((a, streams), (b, 1)):
-            g.set_exec_mode(2, 256)
+            g.set_exec_mode(2, tile if g is a else 256)
             g.set_streams(n)
             g.record_start(30)
             g.measure_start(4, 8)
@@ -316,7 +323,7 @@ def test_autotune_tries_persistent_when_replicas_fit_lds():
         res = g.autotune(2)
         say(f"autotune results (mode, block, variant, streams, ms/sweep): {res.tolist()}")
         assert res[res[:, 0] == 1, 1].tolist() == [256, 512, 1024]
-        assert res[res[:, 0] == 2, 3].tolist() == [1, 2, 3]
+        assert res[res[:, 0] == 2, 3].tolist() == [1, 2, 3] * 3
         g.Mc(5)
         h.Mc(5)
         assert np.array_equal(g.get_spins(), h.get_spins())
