@@ -1124,10 +1124,11 @@ int pyro_set_streams(pyro_ctx *ctx, int n) {
 
 int pyro_set_or_fusion(pyro_ctx *ctx, int on) {
   if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (on < 0 || on > 2) return set_error(PYRO_EINVAL, "pyro_set_or_fusion: level %d not 0/1/2", on);
   PYRO_TRY(device_guard(ctx));
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   drop_graph(ctx);
-  ctx->fuse_or = on ? 1 : 0;
+  ctx->fuse_or = on;
   return 0;
 }
 
@@ -1244,12 +1245,14 @@ int pyro_measure_fetch(pyro_ctx *ctx, double *rows, int *n_rows) {
 
 int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, double *span_ms) {
   if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
-  if (kind < 0 || kind > 4 || n_launches < 1 || n_launches > 100000)
+  if (kind < 0 || kind > 5 || n_launches < 1 || n_launches > 100000)
     return set_error(PYRO_EINVAL, "pyro_time_phase: kind %d / n_launches %d", kind, n_launches);
   if (kind == 4 && !metro_fusable(ctx))
     return set_error(PYRO_ESTATE, "pyro_time_phase: no fused Metropolis pair for this ctx");
   if (kind == 3 && !or_fusable(ctx))
     return set_error(PYRO_ESTATE, "pyro_time_phase: no fused over-relaxation for this ctx");
+  if (kind == 5 && !or_whole_pass(ctx))
+    return set_error(PYRO_ESTATE, "pyro_time_phase: no whole over-relaxation pass for this ctx");
   PYRO_TRY(device_guard(ctx));
   std::vector<hipEvent_t> ev(2 * (size_t)n_launches, nullptr);
   int rc = 0;
@@ -1266,6 +1269,8 @@ int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, dou
     if (kind == 3) {  // fused pairs (0,1) and (2,3) from d_spins into the second buffer:
                       // d_spins, the state, is left as it was
       rc = launch_overrelax_pair(ctx, 2 * (i & 1), ctx->d_spins, ctx->d_spins_alt);
+    } else if (kind == 5) {  // whole passes from d_spins into the second buffer
+      rc = launch_overrelax_pass_whole(ctx, ctx->d_spins, ctx->d_spins_alt);
     } else if (kind == 4) {  // fused Metropolis pairs (0,1), (2,3), in place
       rc = launch_metropolis_pair(ctx, 2 * (i & 1), 0);
     } else {

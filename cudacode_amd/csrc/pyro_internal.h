@@ -209,8 +209,10 @@ struct pyro_ctx {
   // Second spin buffer for the fused over-relaxation passes (k_or_pair): a
   // pass reads one buffer and writes the other, so passes alternate and an
   // even number of them ends back in d_spins. nullptr: no fusion.
+  // fuse_or: 0 one colour per launch, 1 two, 2 a whole pass where a kernel
+  // for it exists (L = 16, untracked over-relaxation), else two.
   float *d_spins_alt = nullptr;
-  int fuse_or = 1;
+  int fuse_or = 2;
   float *d_coup = nullptr;
   float h_coup[pyro::kCoupFloats] = {};
 
@@ -258,6 +260,10 @@ bool or_fusable(const pyro_ctx *ctx);
 bool metro_fusable(const pyro_ctx *ctx);
 int launch_metropolis_pair(pyro_ctx *ctx, int ca, int pass);
 int launch_overrelax_pass_fused(pyro_ctx *ctx, int parity);
+// A whole over-relaxation pass per launch (k_or_pass_plane): fusion level 2
+// on an or_fusable ctx at L = 16 without over-relaxation energy tracking.
+bool or_whole_pass(const pyro_ctx *ctx);
+int launch_overrelax_pass_whole(pyro_ctx *ctx, const float *cur, float *oth);
 int launch_overrelax_pair(pyro_ctx *ctx, int ca, const float *cur, float *oth);
 // Fills d_obs; with anchor, also re-anchors the incremental energy (anchor = E, dE = 0).
 int launch_observables(pyro_ctx *ctx, bool anchor = false);

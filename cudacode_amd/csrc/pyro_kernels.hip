@@ -324,9 +324,21 @@ int launch_overrelax_pair(pyro_ctx *ctx, int ca, const float *cur, float *oth) {
   return 0;
 }
 
+bool or_whole_pass(const pyro_ctx *ctx) {
+  return ctx->fuse_or == 2 && or_fusable(ctx) && ctx->L == 16 && !ctx->track_or;
+}
+
+int launch_overrelax_pass_whole(pyro_ctx *ctx, const float *cur, float *oth) {
+  if (!or_whole_pass(ctx)) return set_error(PYRO_ESTATE, "whole over-relaxation pass: not for this ctx");
+  launch_or_pass(ctx, cur, oth);
+  PYRO_HIP(hipGetLastError());
+  return 0;
+}
+
 int launch_overrelax_pass_fused(pyro_ctx *ctx, int parity) {
   const float *cur = parity ? ctx->d_spins_alt : ctx->d_spins;
   float *oth = parity ? ctx->d_spins : ctx->d_spins_alt;
+  if (or_whole_pass(ctx)) return launch_overrelax_pass_whole(ctx, cur, oth);
   if (!launch_or_pair(ctx, 0, cur, oth) || !launch_or_pair(ctx, 2, cur, oth))
     return set_error(PYRO_ESTATE, "fused over-relaxation: no compiled copy for L=%d", ctx->L);
   PYRO_HIP(hipGetLastError());

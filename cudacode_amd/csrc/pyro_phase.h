@@ -549,6 +549,226 @@ __global__ __launch_bounds__(256, or_plane_blocks(MODE)) void k_or_pair_plane(
 // VALU per wave, 94 VGPRs, 5 blocks per CU) 12.4 us (6.1 TB/s), 0.308 ms.
 constexpr int kOrPlaneMode = 3;
 
+// A whole over-relaxation pass (colours 0, 1, 2, 3 in turn) of one z-plane of
+// an L = 16 replica in one launch, from cur into oth (all 16 sites written):
+// 24 instead of 36 B per update. The colour-2 and -3 sites of plane z link to
+// colour-0 and -1 sites of plane z + 1 (10, 11, 14, 15 to 0, 1, 4, 5), whose
+// new values the block recomputes from planes z + 1 and z with their owners'
+// float expressions in their owners' order (over-relaxation is deterministic),
+// so they equal what the block of plane z + 1 computes bit for bit; the
+// colour-0 and -1 sites link down to plane z - 1, read there before anything
+// of it changes. PassPlan lists the site-planes read (plane index 0, 1, 2 =
+// z - 1, z, z + 1) and holds each as the pair kernels do: in LDS when read at
+// a y offset (kPlaneLds), in registers when read only at the own position
+// (kPlaneOwn), in registers rotated along the DPP row when read only at x
+// offsets (kPlaneRow). A new value replaces the old one where it is held: no
+// site is read by another thread while its own colour is being updated, and a
+// barrier separates a colour's writes from the first read of them elsewhere.
+struct PassPlan {
+  int8_t kind[16][3] = {}, slot[16][3] = {};
+  bool redo[16] = {};  // colour-0/1 sites recomputed on plane z + 1
+  // LDS row k = 3 * slot + component: the row's float offset in a replica's
+  // buffer without its z term (component * 16 N + site * N, N = 4096) and
+  // the plane index << 20.
+  uint32_t row[72] = {};
+  int n_lds = 0, n_reg = 0;
+  constexpr void use(bool (&u)[16][3], bool (&ux)[16][3], bool (&uy)[16][3], int s, int p, int dx,
+                     int dy) {
+    u[s][p] = true;
+    ux[s][p] = ux[s][p] || dx != 0;
+    uy[s][p] = uy[s][p] || dy != 0;
+  }
+  constexpr PassPlan() {
+    bool u[16][3] = {}, ux[16][3] = {}, uy[16][3] = {};
+    for (int s = 0; s < 16; ++s) {
+      use(u, ux, uy, s, 1, 0, 0);
+      for (int m = 0; m < 6; ++m)
+        use(u, ux, uy, kLinks[s][m].site, 1 + kLinks[s][m].dz, kLinks[s][m].dx, kLinks[s][m].dy);
+      if ((s & 3) >= 2)
+        for (int m = 0; m < 6; ++m)
+          if (kLinks[s][m].dz == 1) redo[kLinks[s][m].site] = true;
+    }
+    for (int s = 0; s < 16; ++s)
+      if (redo[s]) {
+        use(u, ux, uy, s, 2, 0, 0);
+        for (int m = 0; m < 6; ++m)
+          use(u, ux, uy, kLinks[s][m].site, 2 + kLinks[s][m].dz, kLinks[s][m].dx, kLinks[s][m].dy);
+      }
+    for (int p = 0; p < 3; ++p)
+      for (int s = 0; s < 16; ++s) {
+        kind[s][p] = -1;
+        slot[s][p] = -1;
+        if (!u[s][p]) continue;
+        if (uy[s][p]) {
+          kind[s][p] = kPlaneLds;
+          for (int comp = 0; comp < 3; ++comp)
+            row[3 * n_lds + comp] = (uint32_t)(comp * 16 * 4096 + s * 4096) | (uint32_t)p << 20;
+          slot[s][p] = (int8_t)n_lds++;
+        } else {
+          kind[s][p] = ux[s][p] ? kPlaneRow : kPlaneOwn;
+          slot[s][p] = (int8_t)n_reg++;
+        }
+      }
+  }
+};
+constexpr PassPlan kPassPlan{};
+static_assert(kPassPlan.n_lds == 12 && kPassPlan.n_reg == 16,
+              "plane z - 1: 2 LDS + 2 own; z: 8 LDS + 8 register; z + 1: 2 LDS + 6 own");
+
+__constant__ constexpr uint32_t kPassRows[36] = {
+#define PYRO_ROW(k) kPassPlan.row[k]
+    PYRO_ROW(0),  PYRO_ROW(1),  PYRO_ROW(2),  PYRO_ROW(3),  PYRO_ROW(4),  PYRO_ROW(5),
+    PYRO_ROW(6),  PYRO_ROW(7),  PYRO_ROW(8),  PYRO_ROW(9),  PYRO_ROW(10), PYRO_ROW(11),
+    PYRO_ROW(12), PYRO_ROW(13), PYRO_ROW(14), PYRO_ROW(15), PYRO_ROW(16), PYRO_ROW(17),
+    PYRO_ROW(18), PYRO_ROW(19), PYRO_ROW(20), PYRO_ROW(21), PYRO_ROW(22), PYRO_ROW(23),
+    PYRO_ROW(24), PYRO_ROW(25), PYRO_ROW(26), PYRO_ROW(27), PYRO_ROW(28), PYRO_ROW(29),
+    PYRO_ROW(30), PYRO_ROW(31), PYRO_ROW(32), PYRO_ROW(33), PYRO_ROW(34), PYRO_ROW(35),
+#undef PYRO_ROW
+};
+
+// Value of site s on plane p at cell offset (dx, dy) from where PassPlan
+// holds it (all arguments compile-time constants after unrolling).
+__device__ __forceinline__ f3 pass_get(const float *sf, const float (&rx)[16],
+                                       const float (&ry)[16], const float (&rz)[16], int s, int p,
+                                       int dx, int dy, const int (&X)[3], const int (&Y)[3]) {
+  const int k = kPassPlan.kind[s][p], sl = kPassPlan.slot[s][p];
+  if (k == kPlaneLds) {
+    const int at = Y[dy + 1] + X[dx + 1];
+    return {sf[(3 * sl + 0) * 256 + at], sf[(3 * sl + 1) * 256 + at], sf[(3 * sl + 2) * 256 + at]};
+  }
+  if (dx < 0) return {row_neighbour<-1>(rx[sl]), row_neighbour<-1>(ry[sl]), row_neighbour<-1>(rz[sl])};
+  if (dx > 0) return {row_neighbour<1>(rx[sl]), row_neighbour<1>(ry[sl]), row_neighbour<1>(rz[sl])};
+  return {rx[sl], ry[sl], rz[sl]};
+}
+
+// Over-relaxation of site s on plane p from its 6 links in link order (the
+// sum and the move of every over-relaxation kernel).
+__device__ __forceinline__ f3 pass_relax(const float *sf, const float (&rx)[16],
+                                         const float (&ry)[16], const float (&rz)[16], int s, int p,
+                                         const int (&X)[3], const int (&Y)[3]) {
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    const Link lk = kLinks[s][m];
+    const f3 v = pass_get(sf, rx, ry, rz, lk.site, p + lk.dz, lk.dx, lk.dy, X, Y);
+    fx = fx + v.x;
+    fy = fy + v.y;
+    fz = fz + v.z;
+  }
+  return overrelax({fx, fy, fz}, pass_get(sf, rx, ry, rz, s, p, 0, 0, X, Y));
+}
+
+// The new value of site s on plane p replaces the old one where it is held.
+__device__ __forceinline__ void pass_put(float *sf, float (&rx)[16], float (&ry)[16],
+                                         float (&rz)[16], int s, int p, f3 v, int t) {
+  const int sl = kPassPlan.slot[s][p];
+  if (kPassPlan.kind[s][p] == kPlaneLds) {
+    sf[(3 * sl + 0) * 256 + t] = v.x;
+    sf[(3 * sl + 1) * 256 + t] = v.y;
+    sf[(3 * sl + 2) * 256 + t] = v.z;
+  } else {
+    rx[sl] = v.x;
+    ry[sl] = v.y;
+    rz[sl] = v.z;
+  }
+}
+
+// grid = 16 planes x replicas (XCD-dealt as k_or_pair_plane), block = 256
+// (one cell of the plane per thread). LDS 36 KB: 4 blocks per CU.
+template <int LT>
+__global__ __launch_bounds__(256, 4) void k_or_pass_plane(
+    const float *__restrict__ cur, float *__restrict__ oth, int r0,
+    const uint32_t *__restrict__ ctl, uint32_t sweep_delta, double *__restrict__ series,
+    int R_total, const int *__restrict__ replica_at, const double *__restrict__ anchor,
+    const double *__restrict__ dE) {
+  static_assert(LT == 16, "a z-plane is one block of 256 cells, a row one DPP row");
+  constexpr int NC = LT * LT * LT, P = 16 * NC;
+  constexpr int ROWS = 3 * kPassPlan.n_lds;
+  __shared__ float4 sm[ROWS * 64];
+  const int nb = gridDim.x;
+  const int b = blockIdx.x;
+  const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+  const int r = r0 + (lb >> 4);  // 16 planes per replica
+  const int z = lb & 15;
+  if (series != nullptr && b == 0) {
+    const uint32_t row = ctl[kCtlRecordRow] + sweep_delta;
+    if (ctl[kCtlRecordOn] && row < ctl[kCtlRecordCap]) {
+      const int r1 = r0 + (nb >> 4);
+      for (int k = threadIdx.x; k < R_total; k += 256) {
+        const int rep = replica_at[k];
+        if (rep >= r0 && rep < r1) series[(size_t)row * R_total + k] = anchor[rep] + dE[rep];
+      }
+    }
+  }
+  const float *rc = cur + (size_t)r * 48 * NC;
+  float *ro = oth + (size_t)r * 48 * NC;
+  const int t = threadIdx.x, x = t & 15, y = t >> 4;
+  const int lane = t & 63;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  // The site-planes held in registers: this cell's value.
+  float rx[16], ry[16], rz[16];
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int k = kPassPlan.kind[s][p];
+      if (k != kPlaneOwn && k != kPlaneRow) continue;
+      const int sl = kPassPlan.slot[s][p];
+      const uint32_t j = 4u * (uint32_t)(s * NC + ((z + p - 1) & 15) * 256 + t);
+      rx[sl] = ld_at(rc, j);
+      ry[sl] = ld_at(rc + P, j);
+      rz[sl] = ld_at(rc + 2 * P, j);
+    }
+  // LDS rows: wave w copies rows w, w + 4, ... with 16-byte loads, all issued
+  // before the first LDS write (as k_or_pair_plane).
+  constexpr int NI = ROWS / 4;
+  static_assert(ROWS % 4 == 0, "whole rounds of rows");
+  float4 st[NI];
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const uint32_t e = kPassRows[4 * i + w];
+    const int zz = (z + (int)(e >> 20) - 1) & 15;
+    const uint32_t off = 4u * ((e & 0xFFFFFu) + (uint32_t)zz * 256u) + 16u * (uint32_t)lane;
+    st[i] = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(rc) + off);
+  }
+#pragma unroll
+  for (int i = 0; i < NI; ++i) sm[(4 * i + w) * 64 + lane] = st[i];
+  __syncthreads();
+  float *sf = reinterpret_cast<float *>(sm);
+  const int X[3] = {(x + 15) & 15, x, (x + 1) & 15};
+  const int Y[3] = {((y + 15) & 15) * 16, y * 16, ((y + 1) & 15) * 16};
+  // Plane z + 1: its colour-0, then colour-1 sites that plane z's colours 2
+  // and 3 read (they read plane z's colour 2 and 3 sites, still old).
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    f3 nv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (kPassPlan.redo[c + 4 * k]) nv[k] = pass_relax(sf, rx, ry, rz, c + 4 * k, 2, X, Y);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (kPassPlan.redo[c + 4 * k]) pass_put(sf, rx, ry, rz, c + 4 * k, 2, nv[k], t);
+    if (c == 0) __syncthreads();
+  }
+  // Plane z, colours 0 to 3; each colour's new values go to oth at once.
+  float *oy = ro + P, *oz = ro + 2 * P;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    f3 nv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) nv[k] = pass_relax(sf, rx, ry, rz, c + 4 * k, 1, X, Y);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t i = 4u * (uint32_t)((c + 4 * k) * NC + z * 256 + t);
+      st_at(ro, i, nv[k].x);
+      st_at(oy, i, nv[k].y);
+      st_at(oz, i, nv[k].z);
+      if (c < 3) pass_put(sf, rx, ry, rz, c + 4 * k, 1, nv[k], t);
+    }
+    if (c < 3) __syncthreads();
+  }
+}
+
 // Two Metropolis colour phases (CA, CA + 1) at L = 16 in one launch, in
 // place, scalar model: one z-plane of a replica per block, staged as
 // k_or_pair_plane<CA, 3> stages it (the same 29 values a cell, one buffer),
@@ -947,6 +1167,27 @@ static void launch_or_pair_l(pyro_ctx *ctx, int ca, const float *cur, float *oth
     if (ca == 0) launch_or_pair_t<0, LT, false>(ctx, cur, oth);
     else launch_or_pair_t<2, LT, false>(ctx, cur, oth);
   }
+}
+
+// A whole over-relaxation pass at L = 16 from cur into oth (one launch).
+static inline void launch_or_pass(pyro_ctx *ctx, const float *cur, float *oth) {
+  const int nrep = launch_nrep(ctx);
+  double *series = nullptr;
+  if (ctx->launch_series) {
+    series = ctx->launch_series;
+    ctx->launch_series_done = 1;
+  }
+  if (ctx->time_ev0) {
+    hipExtLaunchKernelGGL(k_or_pass_plane<16>, dim3(16 * nrep), dim3(256), 0, launch_stream(ctx),
+                          ctx->time_ev0, ctx->time_ev1, 0, cur, oth, launch_r0(ctx),
+                          (const uint32_t *)ctx->d_ctl, ctx->launch_delta, series, ctx->R,
+                          (const int *)ctx->d_replica_at, (const double *)ctx->d_anchor,
+                          (const double *)ctx->d_dE);
+    return;
+  }
+  k_or_pass_plane<16><<<16 * nrep, 256, 0, launch_stream(ctx)>>>(
+      cur, oth, launch_r0(ctx), ctx->d_ctl, ctx->launch_delta, series, ctx->R, ctx->d_replica_at,
+      ctx->d_anchor, ctx->d_dE);
 }
 
 // The fused Metropolis colour pair ca (0 or 2) at L = 16, in place on d_spins.

@@ -452,9 +452,11 @@ class Pyrochlore:
         (identical results)."""
         _check(lib().pyro_set_streams(self._ctx, int(n)))
 
-    def set_or_fusion(self, on: bool) -> None:
-        """Two over-relaxation colours per launch (default) or one (identical results)."""
-        _check(lib().pyro_set_or_fusion(self._ctx, 1 if on else 0))
+    def set_or_fusion(self, level) -> None:
+        """Over-relaxation colours per launch, identical results: 0 (or False)
+        one, 1 (or True) two, 2 (default) a whole pass where a kernel for it
+        exists (L = 16, untracked over-relaxation), else two."""
+        _check(lib().pyro_set_or_fusion(self._ctx, int(level)))
 
     def streams(self) -> int:
         n = C.c_int()
@@ -632,7 +634,8 @@ class Pyrochlore:
     def time_phase(self, kind: int, n_launches: int):
         """(mean ms per launch, first-to-last ms) of n back-to-back phase launches
         (kind 0 Metropolis, 1 over-relaxation, 2 learned, 3 fused over-relaxation
-        pair, 4 fused Metropolis pair), timed with HIP events on the ctx stream."""
+        pair, 4 fused Metropolis pair, 5 whole over-relaxation pass), timed with
+        HIP events on the ctx stream."""
         avg, span = C.c_double(), C.c_double()
         _check(lib().pyro_time_phase(self._ctx, kind, n_launches, C.byref(avg), C.byref(span)))
         return avg.value, span.value

@@ -282,17 +282,19 @@ int pyro_set_passes(pyro_ctx *ctx, int n_metro, int n_or);
 int pyro_set_streams(pyro_ctx *ctx, int n);
 int pyro_get_streams(pyro_ctx *ctx, int *n);
 
-/* Over-relaxation passes fused two colours per launch (on = 1, default where
- * it applies: execution mode 0 with the cell-per-thread kernels on a uniform
- * lattice, or the tile-list execution mode 2; scalar model) or one colour per
- * launch (on = 0). A fused launch reads the
- * spins from one buffer and writes the two updated colours to a second one
- * (allocated with the ctx), recomputing the neighbouring first-colour values
- * that the second colour needs, so a pass moves 36 N instead of 60 N bytes;
- * results are identical (no reference counterpart: a performance setting).
- * The same switch fuses the Metropolis phases two colours per launch at
- * L = 16 (scalar model, no post-Metropolis rotation), in place: identical
- * spins and acceptance counts, energies to FP64 rounding. */
+/* Over-relaxation colours per launch: on = 0 one, on = 1 two, on = 2
+ * (default) a whole pass where a kernel for it exists (L = 16 without
+ * over-relaxation energy tracking), two elsewhere. Fusion applies to
+ * execution mode 0 with the cell-per-thread kernels on a uniform lattice, or
+ * the tile-list execution mode 2; scalar model. A fused launch reads the
+ * spins from one buffer and writes the updated colours to a second one
+ * (allocated with the ctx), recomputing the neighbouring values of earlier
+ * colours that later ones need, so a pass moves 36 N (two colours a launch)
+ * or 24 N (a whole pass) instead of 60 N bytes; results are identical (no
+ * reference counterpart: a performance setting). Any on > 0 also fuses the
+ * Metropolis phases two colours per launch at L = 16 (scalar model, no
+ * post-Metropolis rotation), in place: identical spins and acceptance
+ * counts, energies to FP64 rounding. PYRO_EINVAL for on outside 0..2. */
 int pyro_set_or_fusion(pyro_ctx *ctx, int on);
 
 /* Spin guard (no reference counterpart; the reference checks nothing): counts
@@ -398,14 +400,15 @@ int pyro_import_configurations(pyro_ctx *ctx, const char *prefix, int rep, int b
  * Metropolis, 1 = over-relaxation, 2 = learned proposal, which needs weights
  * set; colours cycle 0..3; 3 = fused over-relaxation pairs (0,1), (2,3)
  * alternating; 4 = fused Metropolis pairs (0,1), (2,3) alternating, where
- * the sweeps use them) on the ctx stream,
+ * the sweeps use them; 5 = whole over-relaxation passes, where the sweeps
+ * use them) on the ctx stream,
  * each launch carrying its own start/stop HIP events in its dispatch
  * (hipExtLaunchKernelGGL). avg_ms: mean kernel time per launch; span_ms:
  * first start to last stop. Kinds 0-2 and 4 run for real (the state
  * advances) but all at the ctx's current sweep index, so a colour's launches
  * reuse one set of random draws: the state they leave is not an equilibrium
- * sample, and measurements should follow some sweeps later. Kind 3 writes
- * only the second buffer and leaves the state as it was. The reference's
+ * sample, and measurements should follow some sweeps later. Kinds 3 and 5
+ * write only the second buffer and leave the state as it was. The reference's
  * counterpart is the ENABLE_TIMING cudaEvent block around Mc
  * (MonteCarlo.cu:36-63). */
 int pyro_time_phase(pyro_ctx *ctx, int kind, int n_launches, double *avg_ms, double *span_ms);

@@ -547,6 +547,61 @@ def test_fused_overrelaxation_identical(L, n_or, track, streams, graph):
     assert np.array_equal(spins, o.spins)
 
 
+@pytest.mark.parametrize("n_or,streams,graph,R", [
+    (10, 3, False, 6), (4, 2, True, 4), (3, 1, True, 3), (5, 3, True, 7)])
+def test_whole_pass_overrelaxation_identical(n_or, streams, graph, R):
+    """At L=16, over-relaxation passes of one launch each (colours 0-3 of a
+    z-plane per block, plane z + 1's first two colours recomputed where plane
+    z's last two read them) equal passes of two launches and of four, bit for
+    bit: spins, rungs, acceptance, counters, measurement rows; the series and
+    incremental energies to FP64 rounding; and the oracle. Timing launches
+    (kind 5) leave the state as it was; with over-relaxation energy tracking
+    on, or at another size, the whole pass is not used and kind 5 is refused."""
+    from cudacode_amd import pyro
+    temps = list(np.geomspace(0.02, 2.0, R))
+    kw = dict(n_metro=1, n_or=n_or, pt_every=2, cos_cone=0.4)
+    runs = []
+    for level in (2, 1, 0):
+        with gpu(16, temps, track=False, **kw) as g:
+            g.set_or_fusion(level)
+            g.set_streams(streams)
+            g.record_start(12)
+            g.measure_start(3, 4)
+            g.Mc(2, use_graph=graph)
+            g.Mc(10, use_graph=graph)
+            runs.append((g.get_spins(), g.temperatures()[1], g.acceptance()[0],
+                         g.sweep_counters(), g.measure_fetch(), g.record_fetch(),
+                         g.energy_incremental()))
+            if level == 2:
+                spins = g.get_spins()
+                avg, _ = g.time_phase(5, 4)
+                assert avg > 0 and np.array_equal(g.get_spins(), spins)
+            else:
+                with pytest.raises(pyro.PyroError):
+                    g.time_phase(5, 1)
+    (s2, t2, a2, c2, m2, r2, e2) = runs[0]
+    for (s, t, a, c, m, r, e) in runs[1:]:
+        assert np.array_equal(s2, s)
+        assert np.array_equal(t2, t)
+        assert np.array_equal(a2, a)
+        assert c2 == c
+        assert np.array_equal(m2, m)
+        assert np.allclose(r2, r, rtol=1e-11, atol=0)
+        assert np.allclose(e2, e, rtol=1e-11, atol=0)
+    o = orc(16, temps, **kw)
+    o.run(12)
+    assert np.array_equal(s2, o.spins)
+    assert np.array_equal(t2, o.replica_at)
+    with gpu(16, temps, track=True, **kw) as g:
+        with pytest.raises(pyro.PyroError):
+            g.time_phase(5, 1)
+    with gpu(8, temps, track=False, **kw) as g:
+        with pytest.raises(pyro.PyroError):
+            g.time_phase(5, 1)
+        with pytest.raises(pyro.PyroError):
+            g.set_or_fusion(3)
+
+
 @pytest.mark.parametrize("streams,graph", [(1, True), (2, True), (2, False)])
 def test_fused_metropolis_identical(streams, graph):
     """Metropolis phases fused two colours per launch in place (L=16, one
