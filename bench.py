@@ -35,6 +35,9 @@ BYTES_PER_UPDATE = 60  # 12 N B read + 3 N B written per colour phase of N/4 upd
 # Fused over-relaxation (two colours per launch between two spin buffers):
 # 12 N B read + 6 N B written for N/2 updates.
 BYTES_PER_UPDATE_FUSED = 36
+# A whole over-relaxation pass per launch (L=16): 12 N B read + 12 N B
+# written for N updates.
+BYTES_PER_UPDATE_PASS = 24
 HBM_PEAK_GBS = 8000.0
 FLOP_PER_FORWARD = 2 * (6 * 64 + 64 * 64 + 64 * 4)  # MLP 6 -> 64 -> 64 -> 4
 BF16_PEAK_TFLOPS = 2500.0  # dense bf16 MFMA
@@ -42,6 +45,7 @@ BF16_PEAK_TFLOPS = 2500.0  # dense bf16 MFMA
 
 OR_KERNEL = "k_phase_cells<1, colour, 16, false, false>"
 OR_PAIR_KERNEL = "k_or_pair_plane<pair, 3>"
+OR_PASS_KERNEL = "k_or_pass_plane<16>"
 OR_PAIR_KERNEL_C5 = "k_or_pair<pair, 64, false>"
 OR_KERNEL_C5 = "k_phase_cells<1, colour, 64, false, false>"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -284,7 +288,8 @@ def phase_only(args):
                         n_or=N_OR, pt_every=PT_EVERY, cos_cone=0.5)
     g.CalculateSolidAngle(target=0.5, rounds=10, sweeps_per_round=10)
     g.Mc(args.warmup)
-    kind = {"learned": 2, "or_pair": 3, "metropolis": 0, "metropolis_pair": 4}[args.phase_only]
+    kind = {"learned": 2, "or_pair": 3, "metropolis": 0, "metropolis_pair": 4,
+            "or_pass": 5}[args.phase_only]
     if kind == 2:
         learned.train(g, steps=args.train_steps, init_seed=SEED)
     avg_ms, span_ms = g.time_phase(kind, args.phase_launches)
@@ -298,7 +303,8 @@ def phase_only(args):
                     "achieved_tflops": flop / (avg_ms * 1e-3) / 1e12,
                     "frac_bf16_peak": flop / (avg_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS})
     else:
-        b = (BYTES_PER_UPDATE_FUSED * n_total // 2) if kind in (3, 4) else BYTES_PER_UPDATE * n_total // 4
+        b = {3: BYTES_PER_UPDATE_FUSED * n_total // 2, 4: BYTES_PER_UPDATE_FUSED * n_total // 2,
+             5: BYTES_PER_UPDATE_PASS * n_total}.get(kind, BYTES_PER_UPDATE * n_total // 4)
         out.update({"bytes_per_launch": b, "achieved_gbs": b / (avg_ms * 1e-3) / 1e9,
                     "frac_hbm_peak": b / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
     return out
@@ -474,7 +480,8 @@ def main():
                     help="plain sweeps before the tau_int legs (independent of --warmup)")
     ap.add_argument("--series-sweeps", type=int, default=3000,
                     help="timed sweeps per tau_int leg (independent of --steps)")
-    ap.add_argument("--phase-only", choices=["learned", "or_pair", "metropolis", "metropolis_pair"],
+    ap.add_argument("--phase-only", choices=["learned", "or_pair", "metropolis", "metropolis_pair",
+                                             "or_pass"],
                     help="only time N launches of one colour-phase kernel on the L=16 x 64 "
                          "workload, eagerly (no captured sweeps): the command the kernel-trace "
                          "summaries of these kernels are made with")
@@ -518,7 +525,8 @@ def main():
 
     sweeps_per_s = args.steps / dt
 
-    log(f"phase timing: {args.phase_launches} fused and single over-relaxation launches")
+    log(f"phase timing: {args.phase_launches} whole-pass, pair and single over-relaxation launches")
+    pass_ms, _ = g.time_phase(5, args.phase_launches)
     pair_ms, _ = g.time_phase(3, args.phase_launches)
     or_ms, _ = g.time_phase(1, args.phase_launches)
     met_ms, _ = g.time_phase(0, args.phase_launches // 4)
@@ -530,10 +538,12 @@ def main():
     achieved = bytes_per_launch / (or_ms * 1e-3) / 1e9
     pair_bytes = BYTES_PER_UPDATE_FUSED * n_total // 2
     pair_achieved = pair_bytes / (pair_ms * 1e-3) / 1e9
-    # Algorithmic bytes of the whole sweep as it runs: 60 B per Metropolis
-    # update, 36 B per fused over-relaxation update.
-    # L=16: Metropolis and over-relaxation both run two colours per launch
-    sweep_bytes = BYTES_PER_UPDATE_FUSED * (N_METRO + N_OR) * n_total
+    pass_bytes = BYTES_PER_UPDATE_PASS * n_total
+    pass_achieved = pass_bytes / (pass_ms * 1e-3) / 1e9
+    # Algorithmic bytes of the whole sweep as it runs at L=16: 36 B per
+    # Metropolis update (two colours per launch, in place), 24 B per
+    # over-relaxation update (a whole pass per launch).
+    sweep_bytes = (BYTES_PER_UPDATE_FUSED * N_METRO + BYTES_PER_UPDATE_PASS * N_OR) * n_total
     # A caller that hands host buffers over every sweep pays the PCIe copies
     # too (pyro_set_spins re-anchors the energy as well): the host-inclusive
     # rate, reported beside the HBM-resident value.
@@ -604,19 +614,24 @@ def main():
         "host_transfer": host_transfer,
         "roofline": {
             "bound": "hbm",
-            "kernel": OR_PAIR_KERNEL + " (two over-relaxation colours per launch, one thread "
-                                       "per cell, one z-plane of a replica per block: the "
-                                       "planes read at y offsets staged in LDS, x offsets from "
-                                       "the DPP row, the neighbours' first-colour values "
-                                       "exchanged; 20 of 24 launches per sweep and replica "
-                                       "group)",
-            "achieved": pair_achieved,
+            "kernel": OR_PASS_KERNEL + " (a whole over-relaxation pass per launch, one thread "
+                                       "per cell, one z-plane of a replica per block, plane "
+                                       "z+1's first two colours recomputed where the last two "
+                                       "read them: planes read at y offsets staged in LDS, x "
+                                       "offsets from the DPP row; 10 of 12 launches per sweep "
+                                       "and replica group)",
+            "achieved": pass_achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": pair_achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(OR_PAIR_KERNEL),
-            "avg_launch_ms": pair_ms,
-            "bytes_per_launch": pair_bytes,
+            "frac": pass_achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(OR_PASS_KERNEL),
+            "avg_launch_ms": pass_ms,
+            "bytes_per_launch": pass_bytes,
+            "pair": {
+                "kernel": OR_PAIR_KERNEL + " (two over-relaxation colours per launch)",
+                "achieved": pair_achieved, "frac": pair_achieved / HBM_PEAK_GBS,
+                "traffic": measured_traffic(OR_PAIR_KERNEL), "avg_launch_ms": pair_ms,
+                "bytes_per_launch": pair_bytes},
             "single_phase": {
                 "kernel": OR_KERNEL + " (one over-relaxation colour per launch)",
                 "achieved": achieved, "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
