@@ -378,6 +378,7 @@ def workload_c3(args):
     g.Mc(args.warmup)
     tune = g.autotune(5)
     mode = g.exec_mode()
+    streams, groups = g.streams(), g.group_blocking()
     g.sync()
     steps = max(args.steps // 4, 10)
     t0 = time.perf_counter()
@@ -389,7 +390,8 @@ def workload_c3(args):
     return line(steps * 11 * n_spins / dt, steps, args.warmup, dt,
                 "256 replicas, L from the seed over {8,12,16,24}, 32 temperatures 0.05..1.0, "
                 "1 Metropolis + 10 over-relaxation, swaps every 10 within equal-size groups",
-                {"spins": n_spins, "exec_mode": {"mode": mode[0], "block": mode[1], "bins": mode[2]},
+                {"spins": n_spins, "exec_mode": {"mode": mode[0], "block": mode[1], "bins": mode[2],
+                                                 "streams": streams, "group_blocking": groups},
                  "autotune_ms_per_sweep": tune.tolist(),
                  "L_counts": {int(k): int((Ls == k).sum()) for k in (8, 12, 16, 24)}})
 
@@ -404,6 +406,7 @@ def workload_c5(args):
     cfg = presets.preset("c5")
     g = cfg.build()
     g.Mc(max(args.warmup // 20, 2))
+    tune = g.autotune(3)
     steps = max(args.steps // 40, 10)
     g.sync()
     t0 = time.perf_counter()
@@ -428,7 +431,9 @@ def workload_c5(args):
                               "traffic": measured_traffic(OR_KERNEL_C5), "avg_launch_ms": or_ms,
                               "bytes_per_launch": BYTES_PER_UPDATE * n_total // 4},
                           "sweep_achieved": sweep_bytes * steps / dt / 1e9,
-                          "sweep_frac": sweep_bytes * steps / dt / 1e9 / HBM_PEAK_GBS}}
+                          "sweep_frac": sweep_bytes * steps / dt / 1e9 / HBM_PEAK_GBS},
+             "streams": g.streams(), "group_blocking": g.group_blocking(),
+             "autotune_ms_per_sweep": tune.tolist()}
     if not args.no_learned:
         log("c5: training the learned proposal on this ensemble")
         info = learned.train(g, snapshots=16, per_replica=2048, steps=args.train_steps,

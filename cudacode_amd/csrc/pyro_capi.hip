@@ -96,9 +96,47 @@ static int enqueue_phases(pyro_ctx *ctx, bool record = false) {
   return 0;
 }
 
+// Whether the sweeps run cache-blocked (block_groups): execution mode 0 on a
+// uniform lattice with up to R groups, or the tile lists (mode 2) with up to
+// kMaxParts groups (one prebuilt tile list each).
+static bool group_blocked(const pyro_ctx *ctx) {
+  const int G = ctx->block_groups;
+  if (G < 2 || G > ctx->R) return false;
+  return (ctx->exec_mode == 0 && !ctx->hetero) || (ctx->exec_mode == 2 && G <= kMaxParts);
+}
+
 static bool split_sweep(const pyro_ctx *ctx) {
-  return ctx->n_parts > 1 && ctx->n_parts <= ctx->n_streams && ctx->n_parts <= ctx->R &&
-         (ctx->exec_mode == 2 || (ctx->exec_mode == 0 && !ctx->hetero));
+  return !group_blocked(ctx) && ctx->n_parts > 1 && ctx->n_parts <= ctx->n_streams &&
+         ctx->n_parts <= ctx->R && (ctx->exec_mode == 2 || (ctx->exec_mode == 0 && !ctx->hetero));
+}
+
+// The n sweeps of one interval, cache-blocked: group p runs all n before
+// group p + 1 starts, all on the ctx stream. The draws of sweep d of the
+// interval are keyed on ctl[kCtlSweep] + d (launch_delta), as in a split
+// sweep, so spins equal an unblocked run's bit for bit. Every sweep's series
+// entries are written by the phase launches, except the last one's when an
+// exchange or measurement ends the interval (the interval end writes it).
+static int enqueue_blocked_interval(pyro_ctx *ctx, uint32_t n, bool joined_end) {
+  const int G = ctx->block_groups, parts0 = ctx->n_parts;
+  if (ctx->exec_mode == 2) ctx->n_parts = G;  // the tile list of group p of G
+  int rc = 0;
+  for (int p = 0; p < G && rc == 0; ++p) {
+    const int r0 = p * ctx->R / G, r1 = (p + 1) * ctx->R / G;
+    ctx->launch_stream = ctx->stream;
+    ctx->launch_part = p;
+    ctx->launch_r0 = r0;
+    ctx->launch_nrep = r1 - r0;
+    for (uint32_t d = 0; d < n && rc == 0; ++d) {
+      ctx->launch_delta = d;
+      rc = enqueue_phases(ctx, ctx->d_series != nullptr && !(joined_end && d + 1 == n));
+    }
+  }
+  ctx->n_parts = parts0;
+  ctx->launch_stream = nullptr;
+  ctx->launch_nrep = 0;
+  ctx->launch_part = 0;
+  ctx->launch_delta = 0;
+  return rc;
 }
 
 // The other groups' streams join the ctx stream (no booking).
@@ -206,6 +244,23 @@ static int enqueue_sweep(pyro_ctx *ctx, uint32_t sweep_index) {
 // interval up to an exchange or measurement (or to the end) in one launch;
 // the others go sweep by sweep.
 static int enqueue_sweeps(pyro_ctx *ctx, uint32_t first, uint32_t count) {
+  if (group_blocked(ctx)) {
+    uint32_t i = 0;
+    while (i < count) {
+      uint32_t j = i;
+      while (j + 1 < count && !exchange_after(ctx, first + j) && !measure_after(ctx, first + j)) ++j;
+      const bool exchange = exchange_after(ctx, first + j), measure = measure_after(ctx, first + j);
+      const uint32_t n = j - i + 1;
+      PYRO_TRY(enqueue_blocked_interval(ctx, n, exchange || measure));
+      if (exchange || measure) {
+        PYRO_TRY(enqueue_interval_end(ctx, exchange, measure, n));
+      } else {
+        PYRO_TRY(launch_end_sweep(ctx, n, 0));
+      }
+      i = j + 1;
+    }
+    return 0;
+  }
   if (ctx->exec_mode != 1) {
     for (uint32_t i = 0; i < count; ++i) PYRO_TRY(enqueue_sweep(ctx, first + i));
     return 0;
@@ -886,24 +941,30 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
   if (ctx->set_schedule || ctx->or_angle)
     return set_error(PYRO_ESTATE, "pyro_autotune: the update schedule options run in execution mode 0 only");
   PYRO_TRY(device_guard(ctx));
-  struct Cand { int mode, block, variant, streams; };
+  struct Cand { int mode, block, variant, streams, groups; };
   std::vector<Cand> cands;
   if (!ctx->hetero) {
     for (int v : {1, 0})
       for (int n : {1, 2, 3})
-        if (n <= ctx->n_streams) cands.push_back({0, 0, v, n});
+        if (n <= ctx->n_streams) cands.push_back({0, 0, v, n, 0});
+    for (int G : {2, 4, 8, 16})
+      if (G <= ctx->R) cands.push_back({0, 0, 1, 1, G});
   }
   if (batch_size_supported(ctx->hetero ? 4 : ctx->L)) {
     if (persistent_candidate(ctx)) {
-      cands.push_back({1, 256, 1, 1});
-      cands.push_back({1, 512, 1, 1});
-      cands.push_back({1, 1024, 1, 1});
+      cands.push_back({1, 256, 1, 1, 0});
+      cands.push_back({1, 512, 1, 1, 0});
+      cands.push_back({1, 1024, 1, 1, 0});
     }
     for (int tile : {256, 128, 512})
       for (int n : {1, 2, 3})
-        if (n <= ctx->n_streams) cands.push_back({2, tile, 1, n});
+        if (n <= ctx->n_streams) cands.push_back({2, tile, 1, n, 0});
+    for (int G : {2, 3, 4})
+      if (G <= ctx->R) cands.push_back({2, 256, 1, 1, G});
   }
   if (cands.empty()) return 0;
+  if ((int)cands.size() > PYRO_AUTOTUNE_MAX)
+    return set_error(PYRO_EINVAL, "pyro_autotune: %d candidates", (int)cands.size());
   // Snapshot of everything a sweep changes (an exchange or measurement
   // inside a trial also re-anchors the incremental energy: anchor and dE).
   const int R = ctx->R;
@@ -936,9 +997,9 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
   const uint32_t h_sweep = ctx->h_sweep, h_round = ctx->h_pt_round;
   const std::vector<uint64_t> h_prop = ctx->h_proposed;
   const int mode0 = ctx->exec_mode, block0 = ctx->batch_block, variant0 = ctx->phase_variant;
-  const int parts0 = ctx->n_parts, tile0 = ctx->tile_cells;
+  const int parts0 = ctx->n_parts, tile0 = ctx->tile_cells, groups0 = ctx->block_groups;
   double best = 1e30;
-  Cand best_c = {mode0, mode0 == 2 ? tile0 : block0, variant0, parts0};
+  Cand best_c = {mode0, mode0 == 2 ? tile0 : block0, variant0, parts0, groups0};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     rc = set_error(PYRO_EHIP, "pyro_autotune: event creation failed");
@@ -953,6 +1014,7 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
     ctx->tile_cells = c.mode == 2 ? c.block : tile0;
     ctx->phase_variant = c.variant;
     ctx->n_parts = c.streams;
+    ctx->block_groups = c.groups;
     // Time the candidate as pyro_sweep runs it: `sweeps` sweeps captured in
     // one graph, replayed once to warm up (code load, caches) and once timed.
     hipGraph_t graph = nullptr;
@@ -981,11 +1043,12 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
     if (rc) break;
     const double per = ms / sweeps;
     if (results) {
-      results[5 * nres + 0] = c.mode;
-      results[5 * nres + 1] = c.block;
-      results[5 * nres + 2] = c.variant;
-      results[5 * nres + 3] = c.streams;
-      results[5 * nres + 4] = per;
+      results[6 * nres + 0] = c.mode;
+      results[6 * nres + 1] = c.block;
+      results[6 * nres + 2] = c.variant;
+      results[6 * nres + 3] = c.streams;
+      results[6 * nres + 4] = c.groups;
+      results[6 * nres + 5] = per;
     }
     ++nres;
     if (per < best) {
@@ -1006,6 +1069,7 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
   ctx->tile_cells = best_c.mode == 2 ? best_c.block : tile0;
   ctx->phase_variant = best_c.variant;
   ctx->n_parts = best_c.streams;
+  ctx->block_groups = best_c.groups;
   if (n_results) *n_results = nres;
   if (rc) {
     ctx->exec_mode = mode0;
@@ -1013,6 +1077,7 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
     ctx->tile_cells = tile0;
     ctx->phase_variant = variant0;
     ctx->n_parts = parts0;
+    ctx->block_groups = groups0;
   }
   return rc;
 }
@@ -1129,6 +1194,27 @@ int pyro_set_or_fusion(pyro_ctx *ctx, int on) {
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   drop_graph(ctx);
   ctx->fuse_or = on;
+  return 0;
+}
+
+int pyro_set_group_blocking(pyro_ctx *ctx, int groups) {
+  if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
+  if (groups < 0 || groups > ctx->R)
+    return set_error(PYRO_EINVAL, "pyro_set_group_blocking: %d groups (0 to %d)", groups, ctx->R);
+  if (groups >= 2 && ctx->exec_mode == 1)
+    return set_error(PYRO_ESTATE, "pyro_set_group_blocking: not in the persistent execution");
+  if (groups > kMaxParts && (ctx->exec_mode == 2 || ctx->hetero))
+    return set_error(PYRO_EINVAL, "pyro_set_group_blocking: at most %d groups of tile lists", kMaxParts);
+  PYRO_TRY(device_guard(ctx));
+  PYRO_HIP(hipStreamSynchronize(ctx->stream));
+  drop_graph(ctx);
+  ctx->block_groups = groups >= 2 ? groups : 0;
+  return 0;
+}
+
+int pyro_get_group_blocking(pyro_ctx *ctx, int *groups) {
+  if (!ctx || !groups) return set_error(PYRO_EINVAL, "null argument");
+  *groups = ctx->block_groups;
   return 0;
 }
 

@@ -190,6 +190,12 @@ struct pyro_ctx {
   // around each group and is empty otherwise.
   int n_parts = 1;
   int n_streams = 1;
+  // Cache blocking: with block_groups >= 2 the sweeps of each exchange or
+  // measurement interval run group after group on the ctx stream, group p
+  // (replicas [p R / G, (p + 1) R / G)) through all of the interval's sweeps
+  // before group p + 1 starts, so that one group's two spin buffers stay in
+  // the Infinity Cache across the interval. 0: off (n_parts decides).
+  int block_groups = 0;
   hipStream_t part_stream[pyro::kMaxParts] = {};
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_join[pyro::kMaxParts] = {};

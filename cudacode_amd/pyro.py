@@ -34,7 +34,7 @@ EXPORTS = [
     "pyro_energy_incremental",
     "pyro_acceptance", "pyro_reset_acceptance", "pyro_adapt_cone", "pyro_record_start",
     "pyro_record_fetch", "pyro_set_couplings", "pyro_get_couplings", "pyro_exchange_matrices",
-    "pyro_zeeman_fields", "pyro_magnetization_g", "pyro_fold_magnetization_g", "pyro_set_streams", "pyro_get_streams", "pyro_set_or_fusion", "pyro_check_spins", "pyro_set_post_rotate", "pyro_set_update_schedule", "pyro_anneal", "pyro_measure_start", "pyro_measure_stop", "pyro_measure_fetch",
+    "pyro_zeeman_fields", "pyro_magnetization_g", "pyro_fold_magnetization_g", "pyro_set_streams", "pyro_get_streams", "pyro_set_group_blocking", "pyro_get_group_blocking", "pyro_set_or_fusion", "pyro_check_spins", "pyro_set_post_rotate", "pyro_set_update_schedule", "pyro_anneal", "pyro_measure_start", "pyro_measure_stop", "pyro_measure_fetch",
     "pyro_set_passes", "pyro_set_kernel_options", "pyro_create_batch",
     "pyro_get_layout", "pyro_set_exec_mode", "pyro_get_exec_mode", "pyro_autotune",
     "pyro_checkpoint_save", "pyro_checkpoint_load", "pyro_export_configurations",
@@ -133,6 +133,8 @@ def lib():
         L.pyro_set_streams.argtypes = [vp, C.c_int]
         L.pyro_get_streams.argtypes = [vp, C.POINTER(C.c_int)]
         L.pyro_set_or_fusion.argtypes = [vp, C.c_int]
+        L.pyro_set_group_blocking.argtypes = [vp, C.c_int]
+        L.pyro_get_group_blocking.argtypes = [vp, C.POINTER(C.c_int)]
         L.pyro_check_spins.argtypes = [vp, C.c_double, C.POINTER(C.c_longlong),
                                        C.POINTER(C.c_longlong)]
         L.pyro_set_couplings.argtypes = [vp, vp, vp]
@@ -463,6 +465,16 @@ class Pyrochlore:
         _check(lib().pyro_get_streams(self._ctx, C.byref(n)))
         return n.value
 
+    def set_group_blocking(self, groups: int) -> None:
+        """Cache-blocked sweeps: each exchange interval runs replica group after
+        group (0 or 1: off). Identical results."""
+        _check(lib().pyro_set_group_blocking(self._ctx, int(groups)))
+
+    def group_blocking(self) -> int:
+        n = C.c_int()
+        _check(lib().pyro_get_group_blocking(self._ctx, C.byref(n)))
+        return n.value
+
     def check_spins(self, tol: float = 1e-5):
         """(number of non-finite or non-unit spins, first such index or -1)."""
         n, first = C.c_longlong(), C.c_longlong()
@@ -527,13 +539,15 @@ class Pyrochlore:
         _check(lib().pyro_get_exec_mode(self._ctx, C.byref(m), C.byref(b), C.byref(n)))
         return m.value, b.value, n.value
 
+    AUTOTUNE_MAX = 40  # PYRO_AUTOTUNE_MAX
+
     def autotune(self, sweeps: int = 5) -> np.ndarray:
         """Try every execution, keep the fastest; rows (mode, block, variant,
-        streams, ms/sweep)."""
-        res = np.zeros(5 * 24, np.float64)
+        streams, cache-blocking groups, ms/sweep)."""
+        res = np.zeros(6 * self.AUTOTUNE_MAX, np.float64)
         n = C.c_int()
         _check(lib().pyro_autotune(self._ctx, sweeps, _p(res), C.byref(n)))
-        return res[: 5 * n.value].reshape(n.value, 5)
+        return res[: 6 * n.value].reshape(n.value, 6)
 
     # -- checkpoint / configuration files ----------------------------------
     def save_checkpoint(self, path: str) -> None:

@@ -435,9 +435,10 @@ def test_stream_default_and_autotune_on_uniform_lattice():
         g.Mc(2)
         h.Mc(2)
         res = g.autotune(2)
-        say(f"autotune (mode, block, variant, streams, ms/sweep): {res.tolist()}")
+        say(f"autotune (mode, block, variant, streams, groups, ms/sweep): {res.tolist()}")
         assert set(res[res[:, 0] == 0, 3].tolist()) == {1.0, 2.0, 3.0}
-        assert np.all(res[:, 4] > 0)
+        assert res[(res[:, 0] == 0) & (res[:, 4] > 0), 4].tolist() == [2, 4, 8, 16]
+        assert np.all(res[:, 5] > 0)
         assert np.array_equal(g.get_spins(), h.get_spins())
         g.Mc(4)
         h.Mc(4)
@@ -637,3 +638,53 @@ def test_fused_metropolis_identical(streams, graph):
         assert np.array_equal(a.get_spins(), b.get_spins())
         with pytest.raises(RuntimeError):
             a.time_phase(4, 2)
+
+
+@pytest.mark.parametrize("L,groups,graph,track", [
+    (8, 2, True, True), (8, 3, False, False), (16, 4, True, False), (8, 8, True, False)])
+def test_group_blocked_sweeps_identical(L, groups, graph, track):
+    """Cache-blocked sweeps (each exchange or measurement interval run replica
+    group after group on one stream, the draws of the interval's sweeps keyed
+    as in an unblocked run) equal unblocked sweeps bit for bit: spins, rungs,
+    acceptance, counters and measurement rows; the series and incremental
+    energies to FP64 rounding; and the oracle. Calls that end inside an
+    interval and graph blocks both cut intervals."""
+    from cudacode_amd import pyro
+    R = 8
+    temps = list(np.geomspace(0.05, 2.0, R))
+    kw = dict(n_metro=1, n_or=3, pt_every=3, cos_cone=0.4)
+    out = []
+    for G in (groups, 0):
+        with gpu(L, temps, track=track, **kw) as g:
+            g.set_streams(1)
+            g.set_group_blocking(G)
+            assert g.group_blocking() == (G if G >= 2 else 0)
+            g.record_start(20)
+            g.measure_start(4, 8)
+            g.Mc(5, use_graph=graph)
+            g.Mc(15, use_graph=graph)
+            out.append((g.get_spins(), g.temperatures()[1], g.acceptance()[0], g.sweep_counters(),
+                        g.measure_fetch(), g.record_fetch(), g.energy_incremental()))
+    (s1, t1, a1, c1, m1, r1, e1), (s0, t0, a0, c0, m0, r0, e0) = out
+    assert np.array_equal(s1, s0)
+    assert np.array_equal(t1, t0)
+    assert np.array_equal(a1, a0)
+    assert c1 == c0 == (20, 6)
+    assert np.array_equal(m1, m0)
+    assert r1.shape == (20, R) and np.allclose(r1, r0, rtol=1e-11, atol=0)
+    assert np.allclose(e1, e0, rtol=1e-11, atol=0)
+    o = orc(L, temps, **kw)
+    o.run(20)
+    assert np.array_equal(s1, o.spins)
+    assert np.array_equal(t1, o.replica_at)
+
+
+def test_group_blocking_refusals():
+    from cudacode_amd import pyro
+    with gpu(4, [0.5, 1.0, 2.0]) as g:
+        with pytest.raises(pyro.PyroError):
+            g.set_group_blocking(4)  # more groups than replicas
+        with pytest.raises(pyro.PyroError):
+            g.set_group_blocking(-1)
+        g.set_group_blocking(1)
+        assert g.group_blocking() == 0
