@@ -386,8 +386,8 @@ void batch_free(pyro_ctx *ctx) {
     if (ctx->d_tiles_x[x]) (void)hipFree(ctx->d_tiles_x[x]);
     ctx->d_tiles_x[x] = nullptr;
     ctx->n_tiles_x[x] = 0;
-    for (int n = 0; n <= kMaxParts; ++n)
-      for (int p = 0; p < kMaxParts; ++p) {
+    for (int n = 0; n <= kMaxGroups; ++n)
+      for (int p = 0; p < kMaxGroups; ++p) {
         if (ctx->d_tiles_part_x[x][n][p]) (void)hipFree(ctx->d_tiles_part_x[x][n][p]);
         ctx->d_tiles_part_x[x][n][p] = nullptr;
         ctx->n_tiles_part_x[x][n][p] = 0;
@@ -451,9 +451,9 @@ int batch_setup(pyro_ctx *ctx, int n_bins) {
   std::vector<int> part_off(R + 1, 0);
   for (int r = 0; r < R; ++r)
     part_off[r + 1] = part_off[r] + (ctx->h_L[r] * ctx->h_L[r] * ctx->h_L[r] + 255) / 256;
-  // parts[n][p]: group p of a split into n groups (n = 2 .. kMaxParts, n <= R).
-  std::vector<int> parts[kMaxParts + 1][kMaxParts];
-  for (int n = 2; n <= kMaxParts && n <= R; ++n)
+  // parts[n][p]: group p of a split into n groups (n = 2 .. kMaxGroups, n <= R).
+  std::vector<int> parts[kMaxGroups + 1][kMaxGroups];
+  for (int n = 2; n <= kMaxGroups && n <= R; ++n)
     for (int p = 0; p < n; ++p) parts[n][p] = tile_list(ctx, p * R / n, (p + 1) * R / n);
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   batch_free(ctx);
@@ -470,7 +470,7 @@ int batch_setup(pyro_ctx *ctx, int n_bins) {
   PYRO_HIP(hipMalloc((void **)&ctx->d_part_off, sizeof(int) * (R + 1)));
   PYRO_HIP(hipMemcpy(ctx->d_part_off, part_off.data(), sizeof(int) * (R + 1), hipMemcpyHostToDevice));
   PYRO_HIP(hipMalloc((void **)&ctx->d_tile_part, sizeof(double) * kObs * (size_t)part_off[R]));
-  for (int n = 2; n <= kMaxParts; ++n)
+  for (int n = 2; n <= kMaxGroups; ++n)
     for (int p = 0; p < n; ++p) {
       const std::vector<int> &t = parts[n][p];
       if (t.empty()) continue;
@@ -486,7 +486,7 @@ int batch_setup(pyro_ctx *ctx, int n_bins) {
     PYRO_HIP(hipMalloc((void **)&ctx->d_tiles_x[x], sizeof(int) * all.size()));
     PYRO_HIP(hipMemcpy(ctx->d_tiles_x[x], all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice));
     ctx->n_tiles_x[x] = (int)all.size() / 2;
-    for (int n = 2; n <= kMaxParts && n <= R; ++n)
+    for (int n = 2; n <= kMaxGroups && n <= R; ++n)
       for (int p = 0; p < n; ++p) {
         const std::vector<int> t = tile_list(ctx, p * R / n, (p + 1) * R / n, tile);
         if (t.empty()) continue;

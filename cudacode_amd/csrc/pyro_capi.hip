@@ -98,11 +98,11 @@ static int enqueue_phases(pyro_ctx *ctx, bool record = false) {
 
 // Whether the sweeps run cache-blocked (block_groups): execution mode 0 on a
 // uniform lattice with up to R groups, or the tile lists (mode 2) with up to
-// kMaxParts groups (one prebuilt tile list each).
+// kMaxGroups groups (one prebuilt tile list each).
 static bool group_blocked(const pyro_ctx *ctx) {
   const int G = ctx->block_groups;
   if (G < 2 || G > ctx->R) return false;
-  return (ctx->exec_mode == 0 && !ctx->hetero) || (ctx->exec_mode == 2 && G <= kMaxParts);
+  return (ctx->exec_mode == 0 && !ctx->hetero) || (ctx->exec_mode == 2 && G <= kMaxGroups);
 }
 
 static bool split_sweep(const pyro_ctx *ctx) {
@@ -111,18 +111,27 @@ static bool split_sweep(const pyro_ctx *ctx) {
 }
 
 // The n sweeps of one interval, cache-blocked: group p runs all n before
-// group p + 1 starts, all on the ctx stream. The draws of sweep d of the
-// interval are keyed on ctl[kCtlSweep] + d (launch_delta), as in a split
-// sweep, so spins equal an unblocked run's bit for bit. Every sweep's series
-// entries are written by the phase launches, except the last one's when an
-// exchange or measurement ends the interval (the interval end writes it).
+// the next group of its stream starts. With S = n_parts streams (1 when
+// n_parts does not name a usable split), stream s takes groups s, s + S, ...
+// in turn, so S groups are in flight (and in the cache) at a time; the
+// streams fork from and join the ctx stream around the interval. The draws
+// of sweep d of the interval are keyed on ctl[kCtlSweep] + d (launch_delta),
+// as in a split sweep, so spins equal an unblocked run's bit for bit. Every
+// sweep's series entries are written by the phase launches, except the last
+// one's when an exchange or measurement ends the interval (the interval end
+// writes it).
 static int enqueue_blocked_interval(pyro_ctx *ctx, uint32_t n, bool joined_end) {
   const int G = ctx->block_groups, parts0 = ctx->n_parts;
+  const int S = parts0 >= 2 && parts0 <= ctx->n_streams ? std::min(parts0, G) : 1;
+  if (S > 1) {
+    PYRO_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+    for (int s = 1; s < S; ++s) PYRO_HIP(hipStreamWaitEvent(ctx->part_stream[s], ctx->ev_fork, 0));
+  }
   if (ctx->exec_mode == 2) ctx->n_parts = G;  // the tile list of group p of G
   int rc = 0;
   for (int p = 0; p < G && rc == 0; ++p) {
     const int r0 = p * ctx->R / G, r1 = (p + 1) * ctx->R / G;
-    ctx->launch_stream = ctx->stream;
+    ctx->launch_stream = ctx->part_stream[p % S];
     ctx->launch_part = p;
     ctx->launch_r0 = r0;
     ctx->launch_nrep = r1 - r0;
@@ -136,6 +145,11 @@ static int enqueue_blocked_interval(pyro_ctx *ctx, uint32_t n, bool joined_end) 
   ctx->launch_nrep = 0;
   ctx->launch_part = 0;
   ctx->launch_delta = 0;
+  for (int s = 1; s < S; ++s) {  // joined even after an error: no stream left running ahead
+    if (hipEventRecord(ctx->ev_join[s], ctx->part_stream[s]) != hipSuccess ||
+        hipStreamWaitEvent(ctx->stream, ctx->ev_join[s], 0) != hipSuccess)
+      rc = rc ? rc : set_error(PYRO_EHIP, "blocked interval: stream join failed");
+  }
   return rc;
 }
 
@@ -947,8 +961,11 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
     for (int v : {1, 0})
       for (int n : {1, 2, 3})
         if (n <= ctx->n_streams) cands.push_back({0, 0, v, n, 0});
-    for (int G : {2, 4, 8, 16})
-      if (G <= ctx->R) cands.push_back({0, 0, 1, 1, G});
+    // cache-blocked: G groups, S of them in flight at a time (S = G is the
+    // stream split above)
+    for (int S : {1, 2, 3})
+      for (int G : {2, 4, 8, 16, 32})
+        if (G > S && G <= ctx->R && S <= ctx->n_streams) cands.push_back({0, 0, 1, S, G});
   }
   if (batch_size_supported(ctx->hetero ? 4 : ctx->L)) {
     if (persistent_candidate(ctx)) {
@@ -959,8 +976,9 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
     for (int tile : {256, 128, 512})
       for (int n : {1, 2, 3})
         if (n <= ctx->n_streams) cands.push_back({2, tile, 1, n, 0});
-    for (int G : {2, 3, 4})
-      if (G <= ctx->R) cands.push_back({2, 256, 1, 1, G});
+    for (int S : {1, 2, 3})
+      for (int G : {2, 3, 4, 6, 8})
+        if (G > S && G <= ctx->R && S <= ctx->n_streams) cands.push_back({2, 256, 1, S, G});
   }
   if (cands.empty()) return 0;
   if ((int)cands.size() > PYRO_AUTOTUNE_MAX)
@@ -1203,8 +1221,8 @@ int pyro_set_group_blocking(pyro_ctx *ctx, int groups) {
     return set_error(PYRO_EINVAL, "pyro_set_group_blocking: %d groups (0 to %d)", groups, ctx->R);
   if (groups >= 2 && ctx->exec_mode == 1)
     return set_error(PYRO_ESTATE, "pyro_set_group_blocking: not in the persistent execution");
-  if (groups > kMaxParts && (ctx->exec_mode == 2 || ctx->hetero))
-    return set_error(PYRO_EINVAL, "pyro_set_group_blocking: at most %d groups of tile lists", kMaxParts);
+  if (groups > kMaxGroups && (ctx->exec_mode == 2 || ctx->hetero))
+    return set_error(PYRO_EINVAL, "pyro_set_group_blocking: at most %d groups of tile lists", kMaxGroups);
   PYRO_TRY(device_guard(ctx));
   PYRO_HIP(hipStreamSynchronize(ctx->stream));
   drop_graph(ctx);

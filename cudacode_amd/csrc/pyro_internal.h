@@ -34,6 +34,9 @@ constexpr int kCoupFloats = 156;  // matrix field model: 16 x 9 exchange + 4 x 3
 // group's launch must span for the split to be chosen by default.
 constexpr int kMaxParts = 4;
 constexpr long long kSplitMinBlocks = 256;
+// Replica groups of a cache-blocked sweep in the tile-list execution (one
+// prebuilt tile list per group; any number up to R in execution mode 0).
+constexpr int kMaxGroups = 8;
 // Measurement row, per rung: E, |m_A2|, |m_E|, |m_T1|, |m_F| (= |M| / N), |m_T2|.
 constexpr int kMeasQ = 6;  // per observables block: 4 bases x (energy, mx, my, mz)
 constexpr int kPhaseBlock = 256;   // 4 waves: one per site of the colour
@@ -138,16 +141,16 @@ struct pyro_ctx {
   int n_tiles = 0;
   int *d_part_off = nullptr;        // R + 1: first observables partial of each replica
   double *d_tile_part = nullptr;    // kObs doubles per tile: observables partials
-  // The same lists for each replica group of a split sweep:
+  // The same lists for each replica group of a split or cache-blocked sweep:
   // d_tiles_part[n][p] covers group p of n (replicas [p R / n, (p + 1) R / n)).
-  int *d_tiles_part[pyro::kMaxParts + 1][pyro::kMaxParts] = {};
-  int n_tiles_part[pyro::kMaxParts + 1][pyro::kMaxParts] = {};
+  int *d_tiles_part[pyro::kMaxGroups + 1][pyro::kMaxGroups] = {};
+  int n_tiles_part[pyro::kMaxGroups + 1][pyro::kMaxGroups] = {};
   // The same lists with 128-cell (index 0) and 512-cell (index 1) tiles, for
   // the phases of the tile-list execution (the observables keep 256).
   int *d_tiles_x[2] = {};
   int n_tiles_x[2] = {};
-  int *d_tiles_part_x[2][pyro::kMaxParts + 1][pyro::kMaxParts] = {};
-  int n_tiles_part_x[2][pyro::kMaxParts + 1][pyro::kMaxParts] = {};
+  int *d_tiles_part_x[2][pyro::kMaxGroups + 1][pyro::kMaxGroups] = {};
+  int n_tiles_part_x[2][pyro::kMaxGroups + 1][pyro::kMaxGroups] = {};
 
   // learned proposal
   int n_learned = 0;                    // learned passes per sweep (after Metropolis)

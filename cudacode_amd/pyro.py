@@ -539,7 +539,7 @@ class Pyrochlore:
         _check(lib().pyro_get_exec_mode(self._ctx, C.byref(m), C.byref(b), C.byref(n)))
         return m.value, b.value, n.value
 
-    AUTOTUNE_MAX = 40  # PYRO_AUTOTUNE_MAX
+    AUTOTUNE_MAX = 64  # PYRO_AUTOTUNE_MAX
 
     def autotune(self, sweeps: int = 5) -> np.ndarray:
         """Try every execution, keep the fastest; rows (mode, block, variant,

@@ -102,17 +102,19 @@ int pyro_get_exec_mode(pyro_ctx *ctx, int *mode, int *block, int *n_bins);
 
 /* Autotuner: time `sweeps` sweeps of each candidate execution (per-phase
  * cell- and site-per-thread kernels on 1, 2 or 3 streams, and cache-blocked
- * in 2, 4, 8 or 16 replica groups; persistent kernel at 256/512/1024 threads
+ * in 2, 4, 8, 16 or 32 replica groups, 1, 2 or 3 of them at a time;
+ * persistent kernel at 256/512/1024 threads
  * when every replica has L <= 8 and is staged in LDS -- a block owns whole
  * replicas, so larger ones serialise: 5.9-12 ms per sweep for 256 mixed
  * replicas of L up to 24 against 1.58 ms as tile lists -- and tile lists of
  * 128, 256 and 512 cells on 1, 2 or 3 streams, and of 256 cells
- * cache-blocked in 2, 3 or 4 groups; pyro_set_group_blocking), captured in
+ * cache-blocked in 2, 3, 4, 6 or 8 groups, 1, 2 or 3 at a time;
+ * pyro_set_group_blocking), captured in
  * one graph as pyro_sweep replays them, restoring the state after each trial,
  * and keep the fastest. results (optional, 6 doubles per candidate, at most
  * PYRO_AUTOTUNE_MAX candidates): mode, block (mode 1) or tile size (mode 2),
  * phase variant, streams, cache-blocking groups (0: none), ms per sweep. */
-#define PYRO_AUTOTUNE_MAX 40
+#define PYRO_AUTOTUNE_MAX 64
 int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results);
 
 /* Lattice tables: links16 (Pyrochlore.cu:152-167) and IndependentSubLattices
@@ -290,10 +292,10 @@ int pyro_get_streams(pyro_ctx *ctx, int *n);
  * group p ([p R / groups, (p + 1) R / groups)) through all of the interval's
  * sweeps before group p + 1 starts, so that a group whose two spin buffers
  * fit the 256 MB Infinity Cache keeps them there across the interval.
- * groups = 0 (or 1): off, the default (pyro_set_streams decides). Up to R
- * groups in execution mode 0, up to 4 in the tile-list execution (mode 2);
- * PYRO_ESTATE in the persistent execution. While on, it replaces the stream
- * split. Spins, rungs, counters and measurement rows are identical to an
+ * groups = 0 (or 1): off, the default. Up to R groups in execution mode 0,
+ * up to 8 in the tile-list execution (mode 2); PYRO_ESTATE in the persistent
+ * execution. With n > 1 streams (pyro_set_streams), stream s takes groups
+ * s, s + n, ... in turn, so n groups are in flight at a time. Spins, rungs, counters and measurement rows are identical to an
  * unblocked run; energies agree to FP64 rounding (no reference counterpart: a
  * performance setting; the reference sweeps every replica in each launch). */
 int pyro_set_group_blocking(pyro_ctx *ctx, int groups);

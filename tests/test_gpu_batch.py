@@ -139,12 +139,13 @@ def test_autotune_keeps_the_state():
         inc_before = g.energy_incremental()
         res = g.autotune(3)
         # tile lists of 256, 128 and 512 cells on 1, 2 and 3 streams, and of
-        # 256 cells cache-blocked in 2, 3 and 4 groups; no persistent trials,
-        # replicas of L > 8 being too large for one block
-        assert len(res) == 12 and np.all(res[:, 5] > 0)
-        assert res[:, 3].tolist() == [1, 2, 3] * 3 + [1] * 3 and res[:, 0].tolist() == [2] * 12
-        assert res[:, 1].tolist() == [256] * 3 + [128] * 3 + [512] * 3 + [256] * 3
-        assert res[:, 4].tolist() == [0] * 9 + [2, 3, 4]
+        # 256 cells cache-blocked in 2 to 8 groups, 1, 2 or 3 at a time; no
+        # persistent trials, replicas of L > 8 being too large for one block
+        assert len(res) == 21 and np.all(res[:, 5] > 0)
+        assert res[:, 0].tolist() == [2] * 21
+        assert res[:, 3].tolist() == [1, 2, 3] * 3 + [1] * 5 + [2] * 4 + [3] * 3
+        assert res[:, 1].tolist() == [256] * 3 + [128] * 3 + [512] * 3 + [256] * 12
+        assert res[:, 4].tolist() == [0] * 9 + [2, 3, 4, 6, 8] + [3, 4, 6, 8] + [4, 6, 8]
         mode, tile, _ = g.exec_mode()
         best = res[np.argmin(res[:, 5])]
         assert (mode, tile, g.group_blocking()) == (2, int(best[1]), int(best[4]))
@@ -325,21 +326,24 @@ def test_autotune_tries_persistent_when_replicas_fit_lds():
         res = g.autotune(2)
         say(f"autotune results (mode, block, variant, streams, groups, ms/sweep): {res.tolist()}")
         assert res[res[:, 0] == 1, 1].tolist() == [256, 512, 1024]
-        assert res[res[:, 0] == 2, 3].tolist() == [1, 2, 3] * 3 + [1] * 3
-        assert res[res[:, 0] == 2, 4].tolist() == [0] * 9 + [2, 3, 4]
+        assert res[res[:, 0] == 2, 3].tolist() == [1, 2, 3] * 3 + [1] * 3 + [2] * 2 + [3]
+        assert res[res[:, 0] == 2, 4].tolist() == [0] * 9 + [2, 3, 4] + [3, 4] + [4]
         g.Mc(5)
         h.Mc(5)
         assert np.array_equal(g.get_spins(), h.get_spins())
 
 
-@pytest.mark.parametrize("groups,graph,tile", [(2, True, 256), (4, False, 128), (3, True, 512)])
-def test_tile_list_group_blocked_identical(groups, graph, tile):
+@pytest.mark.parametrize("groups,graph,tile,streams", [
+    (2, True, 256, 1), (4, False, 128, 1), (3, True, 512, 1), (4, True, 256, 2), (3, False, 256, 3),
+    (8, True, 256, 2), (6, False, 512, 3)])
+def test_tile_list_group_blocked_identical(groups, graph, tile, streams):
     """Mixed lattice sizes in the tile-list execution, cache-blocked (each
     exchange interval run replica group after group, each group on its own
     XCD-dealt tile list) equal unblocked sweeps: spins, rungs, acceptance,
     counters and measurement rows bit for bit, the energy series to FP64
-    rounding. The persistent execution refuses blocking; tile lists take at
-    most 4 groups."""
+    rounding, with one group in flight at a time or one per stream. The
+    persistent execution refuses blocking; tile lists take at most 8
+    groups."""
     from cudacode_amd import pyro
     Ls = [4, 4, 4, 8, 8, 8, 12, 12]
     temps = [0.2, 0.5, 1.1, 0.3, 0.9, 1.5, 0.4, 0.8]
@@ -347,7 +351,7 @@ def test_tile_list_group_blocked_identical(groups, graph, tile):
     with batch(Ls, temps, **kw) as a, batch(Ls, temps, **kw) as b:
         for g, G in ((a, groups), (b, 0)):
             g.set_exec_mode(2, tile)
-            g.set_streams(1)
+            g.set_streams(streams if G else 1)
             g.set_group_blocking(G)
             g.record_start(30)
             g.measure_start(4, 8)
@@ -360,7 +364,7 @@ def test_tile_list_group_blocked_identical(groups, graph, tile):
         assert np.array_equal(a.measure_fetch(), b.measure_fetch())
         assert np.allclose(a.record_fetch(), b.record_fetch(), rtol=1e-11, atol=0)
         with pytest.raises(pyro.PyroError):
-            a.set_group_blocking(5)
+            a.set_group_blocking(9)
         a.set_group_blocking(0)
         a.set_exec_mode(1, 256)
         with pytest.raises(pyro.PyroError):

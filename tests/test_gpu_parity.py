@@ -437,7 +437,9 @@ def test_stream_default_and_autotune_on_uniform_lattice():
         res = g.autotune(2)
         say(f"autotune (mode, block, variant, streams, groups, ms/sweep): {res.tolist()}")
         assert set(res[res[:, 0] == 0, 3].tolist()) == {1.0, 2.0, 3.0}
-        assert res[(res[:, 0] == 0) & (res[:, 4] > 0), 4].tolist() == [2, 4, 8, 16]
+        blocked = res[(res[:, 0] == 0) & (res[:, 4] > 0)]
+        assert blocked[:, 4].tolist() == [2, 4, 8, 16, 32] + [4, 8, 16, 32] * 2
+        assert blocked[:, 3].tolist() == [1] * 5 + [2] * 4 + [3] * 4
         assert np.all(res[:, 5] > 0)
         assert np.array_equal(g.get_spins(), h.get_spins())
         g.Mc(4)
@@ -640,15 +642,17 @@ def test_fused_metropolis_identical(streams, graph):
             a.time_phase(4, 2)
 
 
-@pytest.mark.parametrize("L,groups,graph,track", [
-    (8, 2, True, True), (8, 3, False, False), (16, 4, True, False), (8, 8, True, False)])
-def test_group_blocked_sweeps_identical(L, groups, graph, track):
+@pytest.mark.parametrize("L,groups,graph,track,streams", [
+    (8, 2, True, True, 1), (8, 3, False, False, 1), (16, 4, True, False, 1),
+    (8, 8, True, False, 1), (8, 4, True, False, 2), (16, 8, False, False, 3)])
+def test_group_blocked_sweeps_identical(L, groups, graph, track, streams):
     """Cache-blocked sweeps (each exchange or measurement interval run replica
     group after group on one stream, the draws of the interval's sweeps keyed
     as in an unblocked run) equal unblocked sweeps bit for bit: spins, rungs,
     acceptance, counters and measurement rows; the series and incremental
-    energies to FP64 rounding; and the oracle. Calls that end inside an
-    interval and graph blocks both cut intervals."""
+    energies to FP64 rounding; and the oracle. One group in flight at a time,
+    or one per stream. Calls that end inside an interval and graph blocks
+    both cut intervals."""
     from cudacode_amd import pyro
     R = 8
     temps = list(np.geomspace(0.05, 2.0, R))
@@ -656,7 +660,7 @@ def test_group_blocked_sweeps_identical(L, groups, graph, track):
     out = []
     for G in (groups, 0):
         with gpu(L, temps, track=track, **kw) as g:
-            g.set_streams(1)
+            g.set_streams(streams if G else 1)
             g.set_group_blocking(G)
             assert g.group_blocking() == (G if G >= 2 else 0)
             g.record_start(20)
