@@ -365,6 +365,17 @@ def workload_c1(args):
                 extra)
 
 
+def fixed_exec(g, spec, mode2_tile=None):
+    """Set the execution named by "STREAMS,GROUPS" (--exec) in place of the
+    autotuner's choice; returns an empty result table."""
+    streams, groups = (int(v) for v in spec.split(","))
+    if mode2_tile:
+        g.set_exec_mode(2, mode2_tile)
+    g.set_streams(streams)
+    g.set_group_blocking(groups)
+    return np.zeros((0, 6))
+
+
 def workload_c3(args):
     """256 replicas, L drawn from the seed over {8,12,16,24}, 32 temperatures
     from 0.05 to 1.0 (8 replicas each), 1 Metropolis + 10 over-relaxation,
@@ -376,7 +387,7 @@ def workload_c3(args):
     g = cfg.build()
     g.CalculateSolidAngle(0.5, 5, 10)
     g.Mc(args.warmup)
-    tune = g.autotune(5)
+    tune = fixed_exec(g, args.exec, mode2_tile=256) if args.exec else g.autotune(5)
     mode = g.exec_mode()
     streams, groups = g.streams(), g.group_blocking()
     g.sync()
@@ -406,7 +417,7 @@ def workload_c5(args):
     cfg = presets.preset("c5")
     g = cfg.build()
     g.Mc(max(args.warmup // 20, 2))
-    tune = g.autotune(3)
+    tune = fixed_exec(g, args.exec) if args.exec else g.autotune(3)
     steps = max(args.steps // 40, 10)
     g.sync()
     t0 = time.perf_counter()
@@ -474,6 +485,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--phase-launches", type=int, default=400)
+    ap.add_argument("--exec", default=None, metavar="STREAMS,GROUPS",
+                    help="c3 and c5: run this execution instead of the autotuner's choice "
+                         "(replica groups in flight, cache-blocking groups; c3 on 256-cell tiles), "
+                         "e.g. for a profiler run, whose per-launch cost would sway the autotuner")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5"],
                     help="c2 (default): the L=16 x 64 parallel-tempering line; others print "
