@@ -261,12 +261,14 @@ def test_adam_step_closed_form_and_descent():
         assert loss1 < 0.8 * loss0
 
 
-@pytest.mark.parametrize("graph,streams", [(True, 2), (False, 2), (True, 4)])
-def test_learned_sweeps_on_split_streams_identical(graph, streams):
-    """Learned sweeps whose replica groups run on 2 or 4 streams equal
-    one-stream learned sweeps bit for bit (spins, learned acceptance,
-    incremental energies to FP64 rounding: per-block atomic sums), with
-    exchanges."""
+@pytest.mark.parametrize("graph,streams,groups", [
+    (True, 2, 0), (False, 2, 0), (True, 4, 0), (True, 1, 3), (False, 2, 5)])
+def test_learned_sweeps_on_split_streams_identical(graph, streams, groups):
+    """Learned sweeps whose replica groups run on 2 or 4 streams, or
+    cache-blocked (each exchange interval group after group, one or two in
+    flight), equal one-stream learned sweeps bit for bit (spins, learned
+    acceptance, incremental energies to FP64 rounding: per-block atomic
+    sums), with exchanges."""
     from cudacode_amd import pyro
     temps = [0.05, 0.2, 0.5, 1.0, 2.0]
     kw = dict(n_metro=0, n_or=2, pt_every=3, cos_cone=0.5)
@@ -274,6 +276,7 @@ def test_learned_sweeps_on_split_streams_identical(graph, streams):
     with ctx(8, temps, **kw) as a, ctx(8, temps, **kw) as b:
         for g, n in ((a, streams), (b, 1)):
             g.set_streams(n)
+            g.set_group_blocking(groups if g is a else 0)
             g.mlp_set_weights(w)
             g.set_learned_passes(1)
         a.Mc(12, use_graph=graph)
