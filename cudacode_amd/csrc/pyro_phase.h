@@ -556,8 +556,9 @@ constexpr int kOrPlaneMode = 3;
 // new values the block recomputes from planes z + 1 and z with their owners'
 // float expressions in their owners' order (over-relaxation is deterministic),
 // so they equal what the block of plane z + 1 computes bit for bit; the
-// colour-0 and -1 sites link down to plane z - 1, read there before anything
-// of it changes. PassPlan lists the site-planes read (plane index 0, 1, 2 =
+// colour-0 and -1 sites link down to plane z - 1, whose old colour-2 and -3
+// values they need: every read is from cur, which no block of the launch
+// writes. PassPlan lists the site-planes read (plane index 0, 1, 2 =
 // z - 1, z, z + 1) and holds each as the pair kernels do: in LDS when read at
 // a y offset (kPlaneLds), in registers when read only at the own position
 // (kPlaneOwn), in registers rotated along the DPP row when read only at x
