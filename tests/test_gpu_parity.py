@@ -516,7 +516,7 @@ def test_large_lattice_with_exchanges_agrees_with_oracle():
 @pytest.mark.parametrize("L,n_or,track,streams,graph", [
     (4, 3, False, 1, True), (8, 4, True, 1, False), (8, 5, False, 2, True),
     (12, 2, True, 2, True), (16, 10, False, 2, False), (24, 3, False, 3, True),
-    (32, 2, True, 2, False)])
+    (32, 2, True, 2, False), (64, 3, False, 2, True), (64, 2, True, 1, False)])
 def test_fused_overrelaxation_identical(L, n_or, track, streams, graph):
     """Over-relaxation passes fused two colours per launch (alternating
     between the two spin buffers, an odd last pass in place) equal one colour
