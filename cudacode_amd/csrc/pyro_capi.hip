@@ -733,6 +733,12 @@ static int run_sweeps(pyro_ctx *ctx, int n_sweeps, int use_graph) {
   return 0;
 }
 
+int pyro_get_graph_block(pyro_ctx *ctx, int *sweeps) {
+  if (!ctx || !sweeps) return set_error(PYRO_EINVAL, "null argument");
+  *sweeps = (int)graph_block(ctx);
+  return 0;
+}
+
 int pyro_get_sweep(pyro_ctx *ctx, uint32_t *sweep, uint32_t *pt_round) {
   if (!ctx) return set_error(PYRO_EINVAL, "null ctx");
   PYRO_TRY(device_guard(ctx));

@@ -29,7 +29,7 @@ EXPORTS = [
     "pyro_independent_sets", "pyro_philox4x32_10", "pyro_init_random_spins", "pyro_set_spins",
     "pyro_get_spins", "pyro_spins_device_ptr", "pyro_set_temperatures", "pyro_get_temperatures",
     "pyro_set_cos_cone", "pyro_get_cos_cone", "pyro_metropolis_phase", "pyro_overrelax_phase",
-    "pyro_sweep", "pyro_get_sweep", "pyro_set_sweep", "pyro_pt_swap", "pyro_energy",
+    "pyro_sweep", "pyro_get_sweep", "pyro_get_graph_block", "pyro_set_sweep", "pyro_pt_swap", "pyro_energy",
     "pyro_magnetization", "pyro_order_params", "pyro_sublattice_sums", "pyro_fold_order_params",
     "pyro_energy_incremental",
     "pyro_acceptance", "pyro_reset_acceptance", "pyro_adapt_cone", "pyro_record_start",
@@ -116,6 +116,7 @@ def lib():
         L.pyro_overrelax_phase.argtypes = [vp, C.c_int]
         L.pyro_sweep.argtypes = [vp, C.c_int, C.c_int]
         L.pyro_get_sweep.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.pyro_get_graph_block.argtypes = [vp, C.POINTER(C.c_int)]
         L.pyro_set_sweep.argtypes = [vp, C.c_uint32, C.c_uint32]
         L.pyro_pt_swap.argtypes = [vp, C.c_int, C.c_uint32, C.POINTER(C.c_int)]
         for name in ("pyro_energy", "pyro_magnetization", "pyro_order_params",
@@ -373,6 +374,13 @@ class Pyrochlore:
         s, r = C.c_uint32(), C.c_uint32()
         _check(lib().pyro_get_sweep(self._ctx, C.byref(s), C.byref(r)))
         return s.value, r.value
+
+    def graph_block(self) -> int:
+        """Sweeps per captured graph: Mc replays one graph per whole block that
+        starts at a multiple of it."""
+        n = C.c_int()
+        _check(lib().pyro_get_graph_block(self._ctx, C.byref(n)))
+        return n.value
 
     def set_sweep_counters(self, sweep: int, pt_round: int) -> None:
         _check(lib().pyro_set_sweep(self._ctx, sweep, pt_round))

@@ -160,6 +160,13 @@ int pyro_overrelax_phase(pyro_ctx *ctx, int colour);
  * cap. use_graph != 0 replays a captured hipGraph per block of sweeps. */
 int pyro_sweep(pyro_ctx *ctx, int n_sweeps, int use_graph);
 int pyro_get_sweep(pyro_ctx *ctx, uint32_t *sweep, uint32_t *pt_round);
+
+/* Sweeps per captured graph of pyro_sweep: a whole number of exchange and
+ * measurement intervals, at least 16. A call replays the graph for each whole
+ * block that starts at a multiple of it, and runs the sweeps before the first
+ * such start and after the last block one launch at a time (no reference
+ * counterpart). */
+int pyro_get_graph_block(pyro_ctx *ctx, int *sweeps);
 int pyro_set_sweep(pyro_ctx *ctx, uint32_t sweep, uint32_t pt_round);
 
 /* One parallel-tempering exchange with the given parity on the current

@@ -692,3 +692,22 @@ def test_group_blocking_refusals():
             g.set_group_blocking(-1)
         g.set_group_blocking(1)
         assert g.group_blocking() == 0
+
+
+def test_graph_block_sizes():
+    """Sweeps per captured graph: a whole number of exchange and measurement
+    intervals, at least 16; a call that starts on a block and spans whole
+    blocks equals the same sweeps run eagerly."""
+    with gpu(4, [0.5, 1.0], n_metro=1, n_or=2, pt_every=10) as g:
+        assert g.graph_block() == 20
+        g.measure_start(4, 8)
+        assert g.graph_block() == 20
+    with gpu(4, [0.5, 1.0], n_metro=1, n_or=2, pt_every=3) as g:
+        g.measure_start(4, 8)
+        assert g.graph_block() == 24
+    with gpu(4, [0.5, 1.0], n_metro=1, n_or=2, pt_every=0) as g, \
+            gpu(4, [0.5, 1.0], n_metro=1, n_or=2, pt_every=0) as h:
+        assert g.graph_block() == 16
+        g.Mc(32, use_graph=True)
+        h.Mc(32, use_graph=False)
+        assert np.array_equal(g.get_spins(), h.get_spins())
