@@ -542,6 +542,7 @@ def main():
     value = updates / dt
     ms_per_step = 1e3 * dt / args.steps
     acc = float(np.mean(g.CalculateRatio()))
+    timed_streams = g.streams()
 
     sweeps_per_s = args.steps / dt
 
@@ -618,6 +619,7 @@ def main():
                         "swaps every 10 sweeps, FP32 spins, FP64 reductions",
             "L": L, "replicas": R, "n_metro": N_METRO, "n_or": N_OR, "pt_every": PT_EVERY,
             "seed": SEED, "parallelism": "replicas resident on one GPU",
+            "streams": timed_streams,
         },
         "indep_samples_per_s": {
             # sweeps/s of the timed sweeps / (2 tau_int), tau_int per rung of

@@ -452,7 +452,11 @@ static int create_ctx(const std::vector<int> &Ls, const double *temps, uint64_t 
     // the plane-staged fused pair, 0.3115-0.3125 ms on one box and
     // 0.43-0.45 ms on another, against 0.319-0.325 for three on both. So the
     // default is the larger of 3 and 2 groups whose smallest group spans
-    // kSplitMinBlocks; pyro_set_streams allows 4.
+    // kSplitMinBlocks; pyro_set_streams allows 4. With the whole
+    // over-relaxation pass at L=16 x 64, four groups ran 0.2041-0.2054 ms per
+    // sweep against 0.2122-0.2164 for three on three boxes, and 0.2375-0.2440
+    // against 0.2188-0.2249 (20 sweeps after the cone adaptation) on a
+    // fourth, so the default stays 3 and pyro_autotune tries 4 there.
     for (int n : {3, 2}) {
       if (n > ctx->n_streams) continue;
       long long blocks = 0;  // group 0: the first R / n replicas (the fewest)
@@ -967,6 +971,7 @@ int pyro_autotune(pyro_ctx *ctx, int sweeps, double *results, int *n_results) {
     for (int v : {1, 0})
       for (int n : {1, 2, 3})
         if (n <= ctx->n_streams) cands.push_back({0, 0, v, n, 0});
+    if (or_whole_pass(ctx) && ctx->n_streams >= 4) cands.push_back({0, 0, 1, 4, 0});
     // cache-blocked: G groups, S of them in flight at a time (S = G is the
     // stream split above)
     for (int S : {1, 2, 3})

@@ -288,9 +288,11 @@ int pyro_set_passes(pyro_ctx *ctx, int n_metro, int n_or);
  * performance setting; each group's kernels fill the GPU while another's
  * drain, hiding part of each short phase kernel's ramp). Default: 3 when
  * group 0 of 3 spans >= 256 workgroups of 256 cells, else 2 when the first
- * half does, else 1; pyro_autotune tries 1, 2 and 3 (4 streams share the
- * default four hardware queues with the runtime and time erratically). Learned
- * passes take the same split. */
+ * half does, else 1; pyro_autotune tries 1, 2 and 3, and 4 where the whole
+ * over-relaxation pass runs (L = 16, untracked over-relaxation, fusion level
+ * 2). Four streams share the default four hardware queues with the runtime
+ * and time erratically: faster than three on some GPUs, slower on others.
+ * Learned passes take the same split. */
 int pyro_set_streams(pyro_ctx *ctx, int n);
 int pyro_get_streams(pyro_ctx *ctx, int *n);
 

@@ -448,6 +448,37 @@ def test_stream_default_and_autotune_on_uniform_lattice():
         assert np.allclose(g.energy_incremental(), h.energy_incremental(), rtol=1e-11, atol=0)
 
 
+def test_autotune_tries_four_streams_with_the_whole_pass():
+    """Where the whole over-relaxation pass runs (L=16, untracked), the
+    autotuner also tries four replica groups; the count it keeps stays, and
+    the sweeps after it equal an untuned ctx's."""
+    temps = list(np.geomspace(0.05, 1.0, 64))
+    kw = dict(n_metro=1, n_or=3, pt_every=2, cos_cone=0.4)
+    with gpu(16, temps, track=False, **kw) as g, gpu(16, temps, track=False, **kw) as h:
+        assert g.streams() == 3 and h.streams() == 3
+        g.Mc(3)
+        h.Mc(3)
+        res = g.autotune(2)
+        say(f"autotune (mode, block, variant, streams, groups, ms/sweep): {res.tolist()}")
+        plain = res[(res[:, 0] == 0) & (res[:, 4] == 0)]
+        assert plain[:, 3].tolist() == [1, 2, 3, 1, 2, 3, 4]
+        assert plain[:, 2].tolist() == [1, 1, 1, 0, 0, 0, 1]
+        best = res[np.argmin(res[:, 5])]
+        assert (g.streams(), g.group_blocking()) == (int(best[3]), int(best[4]))
+        g.Mc(4)
+        h.Mc(4)
+        assert np.array_equal(g.get_spins(), h.get_spins())
+        assert g.sweep_counters() == h.sweep_counters()
+        g.set_streams(4)
+        h.set_streams(4)
+        g.Mc(3)
+        h.Mc(3)
+        assert np.array_equal(g.get_spins(), h.get_spins())
+    with gpu(16, temps, track=True, **kw) as t:
+        res = t.autotune(2)
+        assert res[(res[:, 0] == 0) & (res[:, 4] == 0), 3].tolist() == [1, 2, 3, 1, 2, 3]
+
+
 def test_full_size_c2_spins_agree_with_oracle():
     """The L=16 x 64 parallel-tempering workload at its full size (T 0.01..1,
     1 Metropolis + 10 over-relaxation, swaps every 10 sweeps, the fast
