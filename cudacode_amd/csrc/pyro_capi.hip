@@ -449,12 +449,12 @@ static int create_ctx(const std::vector<int> &Ls, const double *temps, uint64_t 
     // 0.342 in two groups, 0.335 in three; L=4 x 64 2.2x slower in two).
     // Four groups ran 0.328-0.363 ms from one captured graph to the next
     // (four streams of this ctx on the default four hardware queues); with
-    // the plane-staged fused pair, 0.3115-0.3125 ms on one box and
+    // the plane-staged fused pair, 0.3115-0.3125 ms on one MI355X and
     // 0.43-0.45 ms on another, against 0.319-0.325 for three on both. So the
     // default is the larger of 3 and 2 groups whose smallest group spans
     // kSplitMinBlocks; pyro_set_streams allows 4. With the whole
     // over-relaxation pass at L=16 x 64, four groups ran 0.2041-0.2054 ms per
-    // sweep against 0.2122-0.2164 for three on three boxes, and 0.2375-0.2440
+    // sweep against 0.2122-0.2164 for three on three GPUs, and 0.2375-0.2440
     // against 0.2188-0.2249 (20 sweeps after the cone adaptation) on a
     // fourth, so the default stays 3 and pyro_autotune tries 4 there.
     for (int n : {3, 2}) {

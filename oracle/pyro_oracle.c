@@ -625,8 +625,8 @@ void orc_magnetization_g(const double sub12[12], double gxy, double gz, double o
  * 0.9 while the ratio is below the target, never enlarges it, and stops when
  * replica 0 reaches the target; its acceptance counters are never filled by
  * the production kernels (MonteCarlo.cu:346-367 reads DevResults, which
- * MCMultipleTemperaturesPerSystem leaves alone). This build's rule, restated
- * here: every replica's theta is multiplied by 0.9 below the target and
+ * MCMultipleTemperaturesPerSystem leaves alone). The library's rule
+ * (pyro_adapt_cone), restated here: every replica's theta is multiplied by 0.9 below the target and
  * divided by 0.9 at or above it, clamped to [1e-4, pi], and cos_cone =
  * (float)cos(theta), kept below 1. */
 void orc_adapt_cone_step(int R, double target, const uint64_t *accepted,
