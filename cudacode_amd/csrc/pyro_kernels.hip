@@ -325,7 +325,7 @@ int launch_overrelax_pair(pyro_ctx *ctx, int ca, const float *cur, float *oth) {
 }
 
 bool or_whole_pass(const pyro_ctx *ctx) {
-  return ctx->fuse_or == 2 && or_fusable(ctx) && ctx->L == 16 && !ctx->track_or;
+  return ctx->fuse_or == 2 && or_fusable(ctx) && (ctx->L == 16 || ctx->L == 64) && !ctx->track_or;
 }
 
 int launch_overrelax_pass_whole(pyro_ctx *ctx, const float *cur, float *oth) {

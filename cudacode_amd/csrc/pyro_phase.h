@@ -770,6 +770,337 @@ __global__ __launch_bounds__(256, 4) void k_or_pass_plane(
   }
 }
 
+// ---- A whole over-relaxation pass per launch at L = 64 ----
+//
+// A block updates a strip of 4 rows of 64 cells (y0 .. y0 + 3) of plane z,
+// one row per wave: a row is periodic in x within the wave, x neighbours come
+// from the adjacent lanes. Along z the colours are ordered (links up go to
+// lower colours), so as at L = 16 the block recomputes the new values of
+// plane z + 1 that plane z's later colours read. Along y they are not: links
+// reach lower colours in both directions, so the block also recomputes what
+// its rows read of the row below (y0 - 1) and the row above (y0 + 4) — a fifth
+// wave does those two rows. StripPlan finds the nodes (plane, row, site) to
+// compute as the closure of the strip's own updates over reads of lower
+// colours. Rows rho of a strip are indexed rho + 2 (-2 .. 5), planes q = 0, 1,
+// 2 for z - 1, z, z + 1. A site-plane read at a y offset by any node is staged
+// in LDS over the rows read; one read only at the row of the reading node is
+// held in the registers of the wave computing that row.
+constexpr int kStripMaxRowComp = 3 * 16 * 8;
+struct StripRowTable {
+  // staged row-component k: (component * 16 + site) | q << 8 | (rho + 2) << 12,
+  // and its float offset in LDS
+  uint32_t ent[kStripMaxRowComp] = {}, lds[kStripMaxRowComp] = {};
+};
+struct StripPlan {
+  bool comp[3][8][16] = {};
+  int8_t lds_slot[3][16] = {}, lds_r0[3][16] = {}, lds_nr[3][16] = {};
+  int16_t lds_base[3][16] = {};  // in units of 64 floats
+  int8_t reg_s[3][16] = {}, reg_lo[3][16] = {}, reg_hi[3][16] = {};
+  StripRowTable rows{};
+  int n_lds = 0, n_rowcomp = 0, n_lds_floats = 0, n_reg_s = 0, n_reg_lo = 0, n_reg_hi = 0;
+  bool ok = true;
+  constexpr StripPlan() {
+    for (int r = 2; r < 6; ++r)
+      for (int s = 0; s < 16; ++s) comp[1][r][s] = true;
+    for (bool changed = true; changed;) {
+      changed = false;
+      for (int q = 0; q < 3; ++q)
+        for (int r = 0; r < 8; ++r)
+          for (int s = 0; s < 16; ++s) {
+            if (!comp[q][r][s]) continue;
+            for (int m = 0; m < 6; ++m) {
+              const Link lk = kLinks[s][m];
+              if ((lk.site & 3) >= (s & 3)) continue;
+              const int q2 = q + lk.dz, r2 = r + lk.dy;
+              if (q2 < 0 || q2 > 2 || r2 < 0 || r2 > 7) {
+                ok = false;
+                continue;
+              }
+              if (!comp[q2][r2][lk.site]) {
+                comp[q2][r2][lk.site] = true;
+                changed = true;
+              }
+            }
+          }
+    }
+    bool dy[3][16] = {}, ns[3][16] = {}, nlo[3][16] = {}, nhi[3][16] = {};
+    int lo[3][16] = {}, hi[3][16] = {};
+    for (int q = 0; q < 3; ++q)
+      for (int s = 0; s < 16; ++s) {
+        lo[q][s] = 99;
+        hi[q][s] = -99;
+      }
+    for (int q = 0; q < 3; ++q)
+      for (int r = 0; r < 8; ++r)
+        for (int s = 0; s < 16; ++s) {
+          if (!comp[q][r][s]) continue;
+          // computed rows: the halo rows 1 and 6 and the strip rows 2..5,
+          // the strip rows all alike
+          if (r < 1 || r > 6) ok = false;
+          if (r >= 2 && r <= 5 && comp[q][r][s] != comp[q][2][s]) ok = false;
+          for (int m = -1; m < 6; ++m) {
+            const Link lk = m < 0 ? Link{0, 0, 0, (int8_t)s} : kLinks[s][m];
+            const int q2 = q + lk.dz, r2 = r + lk.dy, s2 = lk.site;
+            if (q2 < 0 || q2 > 2 || r2 < 0 || r2 > 7) {
+              ok = false;
+              continue;
+            }
+            dy[q2][s2] = dy[q2][s2] || lk.dy != 0;
+            lo[q2][s2] = r2 < lo[q2][s2] ? r2 : lo[q2][s2];
+            hi[q2][s2] = r2 > hi[q2][s2] ? r2 : hi[q2][s2];
+            if (lk.dy == 0) {
+              if (r >= 2 && r <= 5) ns[q2][s2] = true;
+              if (r == 1) nlo[q2][s2] = true;
+              if (r == 6) nhi[q2][s2] = true;
+            }
+          }
+        }
+    for (int q = 0; q < 3; ++q)
+      for (int s = 0; s < 16; ++s) {
+        lds_slot[q][s] = reg_s[q][s] = reg_lo[q][s] = reg_hi[q][s] = -1;
+        if (dy[q][s]) {
+          const int nr = hi[q][s] - lo[q][s] + 1;
+          lds_slot[q][s] = (int8_t)n_lds++;
+          lds_r0[q][s] = (int8_t)lo[q][s];
+          lds_nr[q][s] = (int8_t)nr;
+          lds_base[q][s] = (int16_t)(n_lds_floats / 64);
+          for (int c = 0; c < 3; ++c)
+            for (int rr = 0; rr < nr; ++rr) {
+              rows.ent[n_rowcomp] =
+                  (uint32_t)(c * 16 + s) | (uint32_t)q << 8 | (uint32_t)(lo[q][s] + rr) << 12;
+              rows.lds[n_rowcomp] = (uint32_t)(n_lds_floats + (c * nr + rr) * 64);
+              ++n_rowcomp;
+            }
+          n_lds_floats += 3 * nr * 64;
+        } else {
+          if (ns[q][s]) reg_s[q][s] = (int8_t)n_reg_s++;
+          if (nlo[q][s]) reg_lo[q][s] = (int8_t)n_reg_lo++;
+          if (nhi[q][s]) reg_hi[q][s] = (int8_t)n_reg_hi++;
+        }
+      }
+  }
+};
+constexpr StripPlan kStripPlan{};
+static_assert(kStripPlan.ok, "the closure stays within planes z .. z + 1 and rows y0 - 1 .. y0 + 4");
+static_assert(kStripPlan.n_lds == 12 && kStripPlan.n_rowcomp == 180 &&
+                  kStripPlan.n_reg_s == 16 && kStripPlan.n_reg_lo == 14 && kStripPlan.n_reg_hi == 7,
+              "12 site-planes of 5 rows in LDS (45 KB); 16, 14 and 7 register planes");
+constexpr bool strip_halo_ok() {
+  for (int q = 0; q < 3; ++q)
+    for (int s = 0; s < 16; ++s) {
+      if ((s & 3) == 3 && (kStripPlan.comp[q][1][s] || kStripPlan.comp[q][6][s])) return false;
+      if ((s & 3) == 3 && kStripPlan.comp[2][2][s]) return false;
+    }
+  return true;
+}
+static_assert(strip_halo_ok(), "colour 3 runs on the strip's own plane only");
+__constant__ constexpr StripRowTable kStripRows = kStripPlan.rows;
+constexpr int kStripRegs = kStripPlan.n_reg_s > kStripPlan.n_reg_lo + kStripPlan.n_reg_hi
+                               ? kStripPlan.n_reg_s
+                               : kStripPlan.n_reg_lo + kStripPlan.n_reg_hi;
+
+// Value of the cell x + DX (DX = -1 or 1) of the wave's 64-cell row:
+// wave_ror:1 gives lane i the value of lane i - 1, wave_rol:1 that of lane
+// i + 1, periodic over the 64 lanes.
+template <int DX>
+__device__ __forceinline__ float wave_neighbour(float v) {
+  constexpr int ctrl = DX < 0 ? 0x13C : 0x134;
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), ctrl, 0xF, 0xF, false));
+}
+
+// Register slot of site-plane (q, s) for the wave's role: 0 a strip row, 1
+// the row below the strip, 2 the row above.
+template <int ROLE>
+__device__ __forceinline__ int strip_reg(int q, int s) {
+  if (ROLE == 0) return kStripPlan.reg_s[q][s];
+  if (ROLE == 1) return kStripPlan.reg_lo[q][s];
+  return kStripPlan.n_reg_lo + kStripPlan.reg_hi[q][s];
+}
+
+// Value of site s on plane q at row index rr (rho + 2) and x offset dx, for a
+// node of a wave of role ROLE (all but rr compile-time after unrolling).
+template <int ROLE>
+__device__ __forceinline__ f3 strip_get(const float *sf, const float (&vx)[kStripRegs],
+                                        const float (&vy)[kStripRegs],
+                                        const float (&vz)[kStripRegs], int q, int s, int rr, int dx,
+                                        int x) {
+  if (kStripPlan.lds_slot[q][s] >= 0) {
+    const int nr = kStripPlan.lds_nr[q][s];
+    const int at = (kStripPlan.lds_base[q][s] + rr - kStripPlan.lds_r0[q][s]) * 64 + ((x + dx) & 63);
+    return {sf[at], sf[at + nr * 64], sf[at + 2 * nr * 64]};
+  }
+  const int sl = strip_reg<ROLE>(q, s);
+  if (dx < 0) return {wave_neighbour<-1>(vx[sl]), wave_neighbour<-1>(vy[sl]), wave_neighbour<-1>(vz[sl])};
+  if (dx > 0) return {wave_neighbour<1>(vx[sl]), wave_neighbour<1>(vy[sl]), wave_neighbour<1>(vz[sl])};
+  return {vx[sl], vy[sl], vz[sl]};
+}
+
+// Over-relaxation of node (q, rr, s) from its 6 links in link order; the new
+// value replaces the old one where it is held (no node reads another of its
+// own colour, so in place within a colour step).
+template <int ROLE>
+__device__ __forceinline__ f3 strip_relax(float *sf, float (&vx)[kStripRegs],
+                                          float (&vy)[kStripRegs], float (&vz)[kStripRegs], int q,
+                                          int s, int rr, int x) {
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    const Link lk = kLinks[s][m];
+    const f3 v = strip_get<ROLE>(sf, vx, vy, vz, q + lk.dz, lk.site, rr + lk.dy, lk.dx, x);
+    fx = fx + v.x;
+    fy = fy + v.y;
+    fz = fz + v.z;
+  }
+  const f3 nv = overrelax({fx, fy, fz}, strip_get<ROLE>(sf, vx, vy, vz, q, s, rr, 0, x));
+  if (kStripPlan.lds_slot[q][s] >= 0) {
+    const int nr = kStripPlan.lds_nr[q][s];
+    const int at = (kStripPlan.lds_base[q][s] + rr - kStripPlan.lds_r0[q][s]) * 64 + x;
+    sf[at] = nv.x;
+    sf[at + nr * 64] = nv.y;
+    sf[at + 2 * nr * 64] = nv.z;
+  } else {
+    const int sl = strip_reg<ROLE>(q, s);
+    vx[sl] = nv.x;
+    vy[sl] = nv.y;
+    vz[sl] = nv.z;
+  }
+  return nv;
+}
+
+// Colour C of a strip row (row index rr = wave + 2): plane z's sites, stored
+// to the second buffer, then plane z + 1's recomputed ones.
+template <int C>
+__device__ __forceinline__ void strip_step(float *sf, float (&vx)[kStripRegs],
+                                           float (&vy)[kStripRegs], float (&vz)[kStripRegs],
+                                           int rr, int x, float *ro, uint32_t cell) {
+  constexpr int NC = 64 * 64 * 64, P = 16 * NC;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int s = C + 4 * k;
+    const f3 nv = strip_relax<0>(sf, vx, vy, vz, 1, s, rr, x);
+    const uint32_t i = 4u * (uint32_t)(s * NC) + cell;
+    st_at(ro, i, nv.x);
+    st_at(ro + P, i, nv.y);
+    st_at(ro + 2 * P, i, nv.z);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (kStripPlan.comp[2][2][C + 4 * k]) (void)strip_relax<0>(sf, vx, vy, vz, 2, C + 4 * k, rr, x);
+}
+
+// Colour C of the rows below (rr = 1) and above (rr = 6) the strip.
+template <int C>
+__device__ __forceinline__ void strip_halo_step(float *sf, float (&vx)[kStripRegs],
+                                                float (&vy)[kStripRegs], float (&vz)[kStripRegs],
+                                                int x) {
+#pragma unroll
+  for (int q = 1; q < 3; ++q)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (kStripPlan.comp[q][1][C + 4 * k]) (void)strip_relax<1>(sf, vx, vy, vz, q, C + 4 * k, 1, x);
+      if (kStripPlan.comp[q][6][C + 4 * k]) (void)strip_relax<2>(sf, vx, vy, vz, q, C + 4 * k, 6, x);
+    }
+}
+
+template <int ROLE>
+__device__ __forceinline__ void strip_load_regs(const float *__restrict__ rc, int z, int y, int x,
+                                                float (&vx)[kStripRegs], float (&vy)[kStripRegs],
+                                                float (&vz)[kStripRegs]) {
+  constexpr int NC = 64 * 64 * 64, P = 16 * NC;
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int raw = ROLE == 0 ? kStripPlan.reg_s[q][s]
+                                : (ROLE == 1 ? kStripPlan.reg_lo[q][s] : kStripPlan.reg_hi[q][s]);
+      if (raw < 0) continue;
+      const int sl = strip_reg<ROLE>(q, s);
+      const uint32_t j = 4u * (uint32_t)(s * NC + ((z + q - 1) & 63) * 4096 + y * 64 + x);
+      vx[sl] = ld_at(rc, j);
+      vy[sl] = ld_at(rc + P, j);
+      vz[sl] = ld_at(rc + 2 * P, j);
+    }
+}
+
+// grid = 1,024 strips (16 per plane, 64 planes) x replicas (XCD-dealt), block
+// = 320 (waves 0-3 the strip's rows, wave 4 the rows around it). LDS 45 KB,
+// held to 128 VGPRs (a few spill) for 3 blocks per CU: at 134 VGPRs, 2 blocks
+// per CU ran 990 against 950 us per launch over 32 replicas.
+template <int LT>
+__global__ __launch_bounds__(320) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_or_pass_strip(
+    const float *__restrict__ cur, float *__restrict__ oth, int r0,
+    const uint32_t *__restrict__ ctl, uint32_t sweep_delta, double *__restrict__ series,
+    int R_total, const int *__restrict__ replica_at, const double *__restrict__ anchor,
+    const double *__restrict__ dE) {
+  static_assert(LT == 64, "a row is one wave of 64 cells");
+  constexpr int NC = LT * LT * LT;
+  constexpr int NITEM = kStripPlan.n_rowcomp * 16;  // 16-byte pieces of the staged rows
+  constexpr int NI = (NITEM + 319) / 320;
+  __shared__ float4 sm[kStripPlan.n_lds_floats / 4];
+  const int nb = gridDim.x;
+  const int b = blockIdx.x;
+  const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+  const int r = r0 + (lb >> 10);
+  const int z = (lb >> 4) & 63;
+  const int y0 = (lb & 15) * 4;
+  if (series != nullptr && b == 0) {
+    const uint32_t row = ctl[kCtlRecordRow] + sweep_delta;
+    if (ctl[kCtlRecordOn] && row < ctl[kCtlRecordCap]) {
+      const int r1 = r0 + (nb >> 10);
+      for (int k = threadIdx.x; k < R_total; k += 320) {
+        const int rep = replica_at[k];
+        if (rep >= r0 && rep < r1) series[(size_t)row * R_total + k] = anchor[rep] + dE[rep];
+      }
+    }
+  }
+  const float *rc = cur + (size_t)r * 48 * NC;
+  float *ro = oth + (size_t)r * 48 * NC;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // 0..4
+  const int x = threadIdx.x & 63;
+  float4 st[NI];
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int item = min((int)threadIdx.x + 320 * i, NITEM - 1);
+    const uint32_t e = kStripRows.ent[item >> 4];
+    const int zz = (z + (int)((e >> 8) & 3u) - 1) & 63;
+    const int yy = (y0 + (int)((e >> 12) & 7u) - 2) & 63;
+    const uint32_t off =
+        4u * ((e & 0xFFu) * (uint32_t)NC + (uint32_t)(zz * 4096 + yy * 64)) + 16u * (uint32_t)(item & 15);
+    st[i] = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(rc) + off);
+  }
+  float vx[kStripRegs], vy[kStripRegs], vz[kStripRegs];
+  if (w < 4) {
+    strip_load_regs<0>(rc, z, y0 + w, x, vx, vy, vz);
+  } else {
+    strip_load_regs<1>(rc, z, (y0 + 63) & 63, x, vx, vy, vz);
+    strip_load_regs<2>(rc, z, (y0 + 4) & 63, x, vx, vy, vz);
+  }
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int item = (int)threadIdx.x + 320 * i;
+    if (NITEM % 320 == 0 || item < NITEM)
+      sm[(kStripRows.lds[item >> 4] >> 2) + (item & 15)] = st[i];
+  }
+  __syncthreads();
+  float *sf = reinterpret_cast<float *>(sm);
+  const int rr = w + 2;
+  const uint32_t cell = 4u * (uint32_t)(z * 4096 + (y0 + w) * 64 + x);
+#define PYRO_STRIP_STEP(C)                                  \
+  if (w < 4)                                                \
+    strip_step<C>(sf, vx, vy, vz, rr, x, ro, cell);         \
+  else                                                      \
+    strip_halo_step<C>(sf, vx, vy, vz, x);
+  PYRO_STRIP_STEP(0)
+  __syncthreads();
+  PYRO_STRIP_STEP(1)
+  __syncthreads();
+  PYRO_STRIP_STEP(2)
+  __syncthreads();
+  if (w < 4) strip_step<3>(sf, vx, vy, vz, rr, x, ro, cell);
+#undef PYRO_STRIP_STEP
+}
+
 // Two Metropolis colour phases (CA, CA + 1) at L = 16 in one launch, in
 // place, scalar model: one z-plane of a replica per block, staged as
 // k_or_pair_plane<CA, 3> stages it (the same 29 values a cell, one buffer),
@@ -1177,6 +1508,20 @@ static inline void launch_or_pass(pyro_ctx *ctx, const float *cur, float *oth) {
   if (ctx->launch_series) {
     series = ctx->launch_series;
     ctx->launch_series_done = 1;
+  }
+  if (ctx->L == 64) {
+    if (ctx->time_ev0) {
+      hipExtLaunchKernelGGL(k_or_pass_strip<64>, dim3(1024 * nrep), dim3(320), 0,
+                            launch_stream(ctx), ctx->time_ev0, ctx->time_ev1, 0, cur, oth,
+                            launch_r0(ctx), (const uint32_t *)ctx->d_ctl, ctx->launch_delta, series,
+                            ctx->R, (const int *)ctx->d_replica_at, (const double *)ctx->d_anchor,
+                            (const double *)ctx->d_dE);
+      return;
+    }
+    k_or_pass_strip<64><<<1024 * nrep, 320, 0, launch_stream(ctx)>>>(
+        cur, oth, launch_r0(ctx), ctx->d_ctl, ctx->launch_delta, series, ctx->R,
+        ctx->d_replica_at, ctx->d_anchor, ctx->d_dE);
+    return;
   }
   if (ctx->time_ev0) {
     hipExtLaunchKernelGGL(k_or_pass_plane<16>, dim3(16 * nrep), dim3(256), 0, launch_stream(ctx),

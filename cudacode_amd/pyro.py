@@ -465,7 +465,7 @@ class Pyrochlore:
     def set_or_fusion(self, level) -> None:
         """Over-relaxation colours per launch, identical results: 0 (or False)
         one, 1 (or True) two, 2 (default) a whole pass where a kernel for it
-        exists (L = 16, untracked over-relaxation), else two."""
+        exists (L = 16 or 64, untracked over-relaxation), else two."""
         _check(lib().pyro_set_or_fusion(self._ctx, int(level)))
 
     def streams(self) -> int:

@@ -311,8 +311,8 @@ int pyro_set_group_blocking(pyro_ctx *ctx, int groups);
 int pyro_get_group_blocking(pyro_ctx *ctx, int *groups);
 
 /* Over-relaxation colours per launch: on = 0 one, on = 1 two, on = 2
- * (default) a whole pass where a kernel for it exists (L = 16 without
- * over-relaxation energy tracking), two elsewhere. Fusion applies to
+ * (default) a whole pass where a kernel for it exists (L = 16 and L = 64
+ * without over-relaxation energy tracking), two elsewhere. Fusion applies to
  * execution mode 0 with the cell-per-thread kernels on a uniform lattice, or
  * the tile-list execution mode 2; scalar model. A fused launch reads the
  * spins from one buffer and writes the updated colours to a second one

@@ -636,6 +636,39 @@ def test_whole_pass_overrelaxation_identical(n_or, streams, graph, R):
             g.set_or_fusion(3)
 
 
+@pytest.mark.parametrize("R,pt_every,graph,streams", [
+    (2, 0, False, 1), (4, 2, True, 2), (3, 2, False, 3)])
+def test_whole_pass_at_l64_identical(R, pt_every, graph, streams):
+    """At L=64 the whole over-relaxation pass runs in strips of 4 rows of 64
+    cells (a row per wave; the rows below and above and plane z + 1's first
+    two colours recomputed where the strip reads them): spins, rungs and
+    counters equal two colours per launch bit for bit, the series and the
+    incremental energies to FP64 rounding; a timing launch (kind 5) leaves
+    the state as it was."""
+    temps = list(np.geomspace(0.05, 1.5, R))
+    kw = dict(n_metro=1, n_or=3, pt_every=pt_every, cos_cone=0.4)
+    runs = []
+    for level in (2, 1):
+        with gpu(64, temps, track=False, **kw) as g:
+            g.set_or_fusion(level)
+            g.set_streams(streams)
+            g.record_start(6)
+            g.Mc(2, use_graph=graph)
+            if level == 2:
+                spins = g.get_spins()
+                avg, _ = g.time_phase(5, 2)
+                assert avg > 0 and np.array_equal(g.get_spins(), spins)
+            g.Mc(4, use_graph=graph)
+            runs.append((g.get_spins(), g.temperatures()[1], g.sweep_counters(), g.record_fetch(),
+                         g.energy_incremental(), g.TotalEnergyv2()))
+    (s2, t2, c2, r2, e2, f2), (s1, t1, c1, r1, e1, _) = runs
+    assert np.array_equal(s2, s1)
+    assert np.array_equal(t2, t1) and c2 == c1
+    assert np.allclose(r2, r1, rtol=1e-11, atol=0)
+    assert np.allclose(e2, e1, rtol=1e-11, atol=0)
+    assert np.allclose(e2, f2, rtol=1e-8, atol=0)
+
+
 @pytest.mark.parametrize("streams,graph", [(1, True), (2, True), (2, False)])
 def test_fused_metropolis_identical(streams, graph):
     """Metropolis phases fused two colours per launch in place (L=16, one
