@@ -46,6 +46,7 @@ BF16_PEAK_TFLOPS = 2500.0  # dense bf16 MFMA
 OR_KERNEL = "k_phase_cells<1, colour, 16, false, false>"
 OR_PAIR_KERNEL = "k_or_pair_plane<pair, 3>"
 OR_PASS_KERNEL = "k_or_pass_plane<16>"
+OR_PASS_KERNEL_C5 = "k_or_pass_strip<64>"
 OR_PAIR_KERNEL_C5 = "k_or_pair<pair, 64, false>"
 OR_KERNEL_C5 = "k_phase_cells<1, colour, 64, false, false>"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -413,7 +414,7 @@ def workload_c5(args):
     over-relaxation passes give the line's value; then the learned proposal,
     trained on 1,048,576 environments of this ensemble, is interleaved
     (1 Metropolis + 1 learned + 10 over-relaxation per sweep)."""
-    from cudacode_amd import learned, presets
+    from cudacode_amd import learned, presets, pyro
     cfg = presets.preset("c5")
     g = cfg.build()
     g.Mc(max(args.warmup // 20, 2))
@@ -424,18 +425,32 @@ def workload_c5(args):
     g.Mc(steps)
     g.sync()
     dt = time.perf_counter() - t0
+    # the sweeps' over-relaxation kernel: a whole pass per launch where the
+    # tuned execution runs one (the strips of k_or_pass_strip), else the pair
+    try:
+        pass_ms, _ = g.time_phase(5, 20)
+    except pyro.PyroError:
+        pass_ms = None
     pair_ms, _ = g.time_phase(3, 40)
     or_ms, _ = g.time_phase(1, 40)
     n_total = 16 * 64 ** 3 * 32
     achieved = BYTES_PER_UPDATE * n_total / 4 / (or_ms * 1e-3) / 1e9
     pair_achieved = BYTES_PER_UPDATE_FUSED * n_total / 2 / (pair_ms * 1e-3) / 1e9
-    sweep_bytes = (BYTES_PER_UPDATE + BYTES_PER_UPDATE_FUSED * 10) * n_total
-    extra = {"roofline": {"bound": "hbm", "kernel": OR_PAIR_KERNEL_C5,
-                          "achieved": pair_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": pair_achieved / HBM_PEAK_GBS,
-                          "traffic": measured_traffic(OR_PAIR_KERNEL_C5),
-                          "avg_launch_ms": pair_ms,
-                          "bytes_per_launch": BYTES_PER_UPDATE_FUSED * n_total // 2,
+    pair = {"kernel": OR_PAIR_KERNEL_C5, "achieved": pair_achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": pair_achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(OR_PAIR_KERNEL_C5), "avg_launch_ms": pair_ms,
+            "bytes_per_launch": BYTES_PER_UPDATE_FUSED * n_total // 2}
+    if pass_ms is not None:
+        pass_achieved = BYTES_PER_UPDATE_PASS * n_total / (pass_ms * 1e-3) / 1e9
+        main = {"kernel": OR_PASS_KERNEL_C5, "achieved": pass_achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": pass_achieved / HBM_PEAK_GBS,
+                "traffic": measured_traffic(OR_PASS_KERNEL_C5), "avg_launch_ms": pass_ms,
+                "bytes_per_launch": BYTES_PER_UPDATE_PASS * n_total, "pair": pair}
+        or_bytes = BYTES_PER_UPDATE_PASS
+    else:
+        main, or_bytes = pair, BYTES_PER_UPDATE_FUSED
+    sweep_bytes = (BYTES_PER_UPDATE + or_bytes * 10) * n_total
+    extra = {"roofline": {"bound": "hbm", **main,
                           "single_phase": {
                               "kernel": OR_KERNEL_C5, "achieved": achieved,
                               "frac": achieved / HBM_PEAK_GBS,
